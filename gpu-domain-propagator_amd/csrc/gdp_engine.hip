@@ -1,0 +1,411 @@
+// gdp_engine.hip - host driver of the MI355X propagation engine.
+//
+// Reference counterpart: propagateConstraintsGPUAtomic<T>
+// (src/propagators/GPU_propagator.cuh:98-268) and the row blocking of
+// fill_row_blocks (src/kernels/util_kernels.cu:12-62).  Differences by
+// design: the matrix is laid out once per Problem (rows grouped by tree
+// width, integrality folded into the column index), rounds are
+// round-synchronous through two bound buffers, and GPU_LOOP is a captured HIP
+// graph of round launches that stop themselves on the device.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "gdp_arith.h"
+#include "gdp_device.h"
+#include "gdp_engine.h"
+
+namespace gdp {
+
+__global__ void round_kernel(DevMatrix M, DevState D, int round, int guard);
+__global__ void fixup_kernel(DevMatrix M, DevState D, int round);
+__global__ void mega_kernel(DevMatrix M, DevState D, int max_rounds);
+__global__ void init_kernel(DevMatrix M, DevState D, const double *lbs, const double *ubs);
+__global__ void unpack_kernel(DevMatrix M, const double2 *bnd, double *lbs, double *ubs);
+
+namespace {
+
+thread_local std::string g_last_error;
+
+void check(hipError_t e, const char *what) {
+    if (e != hipSuccess) {
+        throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+    }
+}
+
+int next_pow2_log(int len) {
+    int lg = 0;
+    while ((1 << lg) < len) lg++;
+    return lg;
+}
+
+template <typename T>
+T *dev_alloc(size_t count, std::vector<void *> &owned) {
+    void *p = nullptr;
+    if (count == 0) count = 1;
+    check(hipMalloc(&p, count * sizeof(T)), "hipMalloc");
+    owned.push_back(p);
+    return static_cast<T *>(p);
+}
+
+}  // namespace
+
+std::string last_error() { return g_last_error; }
+void set_last_error(const std::string &msg) { g_last_error = msg; }
+
+struct Problem {
+    int m = 0, n = 0, nnz = 0;
+    Tuning tuning;
+    ProblemStats stats;
+    DevMatrix M{};
+    DevState D{};
+    std::vector<void *> owned;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int device = 0;
+    int cu_count = 0;
+    int mega_grid = 0;
+    // captured GPU_LOOP graphs, keyed by the first round they cover
+    std::vector<hipGraphExec_t> graphs;
+    std::vector<int> graph_last;
+    std::mutex run_mutex;
+
+    ~Problem() {
+        for (auto g : graphs)
+            if (g) (void)hipGraphExecDestroy(g);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+        for (void *p : owned) (void)hipFree(p);
+    }
+};
+
+Problem *problem_create(int m, int n, int nnz, const int *col, const int *row_ptr,
+                        const double *val, const double *lhs, const double *rhs,
+                        const int *vartype, const Tuning *tuning) {
+    if (m <= 0 || n <= 0 || nnz <= 0) throw std::runtime_error("empty problem");
+    if (!col || !row_ptr || !val || !lhs || !rhs || !vartype)
+        throw std::runtime_error("null input array");
+    if (row_ptr[0] != 0 || row_ptr[m] != nnz) throw std::runtime_error("row_ptrs do not span nnz");
+    for (int i = 0; i < m; i++)
+        if (row_ptr[i + 1] < row_ptr[i]) throw std::runtime_error("row_ptrs not monotone");
+    for (int k = 0; k < nnz; k++)
+        if (col[k] < 0 || col[k] >= n) throw std::runtime_error("column index out of range");
+
+    auto *P = new Problem();
+    try {
+        P->m = m;
+        P->n = n;
+        P->nnz = nnz;
+        if (tuning) P->tuning = *tuning;
+        const Tuning &T = P->tuning;
+        if (T.block_threads % 64 != 0 || T.block_threads < 64 || T.block_threads > 256)
+            throw std::runtime_error("block_threads must be 64, 128, 192 or 256");
+
+        // ---- group rows by tree width: long rows first, then widest bins ----
+        // bin b < 10 holds rows with next_pow2(len) == 2^b; bin 10 rows longer
+        // than kSegLen.  Long rows lead so that their waves start first.
+        std::vector<int> bin_of(m);
+        std::vector<std::vector<int>> rows_in_bin(11);
+        for (int i = 0; i < m; i++) {
+            const int len = row_ptr[i + 1] - row_ptr[i];
+            if (len == 0) { bin_of[i] = -1; continue; }
+            const int b = len > kSegLen ? 10 : next_pow2_log(len);
+            bin_of[i] = b;
+            rows_in_bin[b].push_back(i);
+            P->stats.bin_rows[b]++;
+            P->stats.bin_nnz[b] += len;
+        }
+        std::vector<int> order;
+        order.reserve(m);
+        for (int b = 10; b >= 0; b--)
+            for (int i : rows_in_bin[b]) order.push_back(i);
+        const int mp = static_cast<int>(order.size());
+
+        std::vector<int> h_rp(mp + 1);
+        std::vector<double2> h_sides(mp);
+        std::vector<int> h_col(nnz);
+        std::vector<double> h_val(nnz);
+        int pos = 0;
+        for (int r = 0; r < mp; r++) {
+            const int i = order[r];
+            h_rp[r] = pos;
+            h_sides[r] = make_double2(lhs[i], rhs[i]);
+            for (int k = row_ptr[i]; k < row_ptr[i + 1]; k++) {
+                const int j = col[k];
+                h_col[pos] = j | (vartype[j] == kVarContinuous ? 0 : static_cast<int>(0x80000000u));
+                h_val[pos] = val[k];
+                pos++;
+            }
+        }
+        h_rp[mp] = pos;
+
+        // ---- work items ----
+        std::vector<int4> h_items;
+        int r = 0;
+        for (int b = 10; b >= 0; b--) {
+            const int cnt = static_cast<int>(rows_in_bin[b].size());
+            if (b == 10) {
+                for (int q = 0; q < cnt; q++) h_items.push_back(make_int4(kItemLong, r + q, 1, 0));
+                P->stats.n_long_rows = cnt;
+            } else if (cnt > 0) {
+                const int width = 1 << b;
+                int logg = std::min(T.group_log[b], b);
+                logg = std::max(logg, b - 3);  // at most 8 entries per lane
+                logg = std::min(logg, 6);
+                const int logk = b - logg;
+                const int per_pass = 64 >> logg;
+                int rows_per_item = std::max(1, T.item_nnz / width);
+                rows_per_item = ((rows_per_item + per_pass - 1) / per_pass) * per_pass;
+                for (int q = 0; q < cnt; q += rows_per_item) {
+                    const int nr = std::min(rows_per_item, cnt - q);
+                    h_items.push_back(make_int4(kItemGroup | (logg << 4) | (logk << 8), r + q, nr, 0));
+                }
+            }
+            r += cnt;
+        }
+
+        P->stats.m = m;
+        P->stats.n = n;
+        P->stats.nnz = nnz;
+        P->stats.m_nonempty = mp;
+        P->stats.n_items = static_cast<int>(h_items.size());
+
+        // ---- device ----
+        check(hipGetDevice(&P->device), "hipGetDevice");
+        hipDeviceProp_t prop;
+        check(hipGetDeviceProperties(&prop, P->device), "hipGetDeviceProperties");
+        P->cu_count = prop.multiProcessorCount;
+        check(hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking), "hipStreamCreate");
+        check(hipEventCreate(&P->ev0), "hipEventCreate");
+        check(hipEventCreate(&P->ev1), "hipEventCreate");
+
+        const int nwords = (n + 31) / 32;
+        DevMatrix &M = P->M;
+        M.m = mp;
+        M.n = n;
+        M.nnz = nnz;
+        M.nwords = nwords;
+        M.n_items = static_cast<int>(h_items.size());
+        M.n_fix_items = (nwords + 63) / 64;
+        int *d_rp = dev_alloc<int>(mp + 1, P->owned);
+        double2 *d_sides = dev_alloc<double2>(mp, P->owned);
+        int *d_col = dev_alloc<int>(nnz, P->owned);
+        double *d_val = dev_alloc<double>(nnz, P->owned);
+        int4 *d_items = dev_alloc<int4>(h_items.size(), P->owned);
+        check(hipMemcpyAsync(d_rp, h_rp.data(), (mp + 1) * sizeof(int), hipMemcpyHostToDevice, P->stream), "H2D");
+        check(hipMemcpyAsync(d_sides, h_sides.data(), mp * sizeof(double2), hipMemcpyHostToDevice, P->stream), "H2D");
+        check(hipMemcpyAsync(d_col, h_col.data(), nnz * sizeof(int), hipMemcpyHostToDevice, P->stream), "H2D");
+        check(hipMemcpyAsync(d_val, h_val.data(), nnz * sizeof(double), hipMemcpyHostToDevice, P->stream), "H2D");
+        if (!h_items.empty())
+            check(hipMemcpyAsync(d_items, h_items.data(), h_items.size() * sizeof(int4), hipMemcpyHostToDevice, P->stream), "H2D");
+        M.row_ptr = d_rp;
+        M.sides = d_sides;
+        M.col = d_col;
+        M.val = d_val;
+        M.items = d_items;
+
+        DevState &D = P->D;
+        D.bnd0 = dev_alloc<double2>(n, P->owned);
+        D.bnd1 = dev_alloc<double2>(n, P->owned);
+        D.bm0 = dev_alloc<unsigned>(nwords, P->owned);
+        D.bm1 = dev_alloc<unsigned>(nwords, P->owned);
+        D.status = dev_alloc<int>(kStatusRows * kStatusWords, P->owned);
+        D.barrier = dev_alloc<unsigned>(64, P->owned);
+
+        int per_cu = 0;
+        check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(mega_kernel),
+                                                           256, 0),
+              "occupancy");
+        if (T.mega_blocks_per_cu > 0) per_cu = std::min(per_cu, T.mega_blocks_per_cu);
+        P->mega_grid = std::max(1, per_cu) * P->cu_count;
+        check(hipStreamSynchronize(P->stream), "upload");
+    } catch (...) {
+        delete P;
+        throw;
+    }
+    return P;
+}
+
+void problem_destroy(Problem *p) { delete p; }
+
+ProblemStats problem_stats(const Problem *p) { return p->stats; }
+
+namespace {
+
+int round_grid(const Problem &P, int items) {
+    const int waves = 256 / 64;
+    return std::max(1, (items + waves - 1) / waves);
+}
+
+// Host-side outcome logic shared by CPU_LOOP and GPU_LOOP: walks the status
+// rows exactly as the persistent kernel does.
+void outcome_from_status(const std::vector<int> &st, int max_rounds, RunResult &res, int &result_buf,
+                         bool &need_final_check) {
+    res.rounds = 0;
+    res.infeasible = 0;
+    res.capped = 0;
+    result_buf = 0;
+    need_final_check = false;
+    for (int round = 1; round <= max_rounds; round++) {
+        const int *row = st.data() + kStatusWords * round;
+        if (row[kStatusInfeasPrev]) {
+            res.rounds = round - 1;
+            result_buf = (round - 1) & 1;
+            res.infeasible = 1;
+            return;
+        }
+        res.rounds = round;
+        result_buf = round & 1;
+        if (!row[kStatusChanged]) return;
+        if (round == max_rounds) need_final_check = true;
+    }
+}
+
+}  // namespace
+
+RunResult problem_propagate_device(Problem *P, double *d_lbs, double *d_ubs, int sync_type,
+                                   int max_rounds) {
+    if (sync_type < CPU_LOOP_ID || sync_type > MEGAKERNEL_ID) throw std::runtime_error("Unknown sync type");
+    if (max_rounds < 1 || max_rounds > kMaxRounds) throw std::runtime_error("max_rounds out of range");
+    std::lock_guard<std::mutex> lock(P->run_mutex);
+    const DevMatrix &M = P->M;
+    const DevState &D = P->D;
+    hipStream_t s = P->stream;
+    const int n = P->n;
+    RunResult res;
+
+    // initial state: both bound buffers hold the input, bitmaps and flags clear
+    const int fill_grid = std::min(4096, std::max(1, (n + 255) / 256));
+    hipLaunchKernelGGL(init_kernel, dim3(fill_grid), dim3(256), 0, s, M, D, d_lbs, d_ubs);
+    check(hipGetLastError(), "init_kernel launch");
+
+    const int total_items = M.n_items + M.n_fix_items;
+    const int grid = round_grid(*P, total_items);
+    std::vector<int> st(kStatusRows * kStatusWords, 0);
+    int result_buf = 0;
+    bool need_final = false;
+
+    check(hipEventRecord(P->ev0, s), "event");
+    if (sync_type == CPU_LOOP_ID) {
+        // one launch per round; the host reads the round's flags back
+        for (int round = 1; round <= max_rounds; round++) {
+            hipLaunchKernelGGL(round_kernel, dim3(grid), dim3(256), 0, s, M, D, round, 0);
+            check(hipGetLastError(), "round_kernel launch");
+            int flags[kStatusWords];
+            check(hipMemcpyAsync(flags, D.status + kStatusWords * round, sizeof(flags), hipMemcpyDeviceToHost, s),
+                  "flags D2H");
+            check(hipStreamSynchronize(s), "round sync");
+            std::memcpy(st.data() + kStatusWords * round, flags, sizeof(flags));
+            if (flags[kStatusInfeasPrev] || !flags[kStatusChanged]) break;
+        }
+    } else if (sync_type == GPU_LOOP_ID) {
+        // captured chunks of round launches that stop themselves on the device
+        const int chunk = std::max(1, P->tuning.graph_rounds);
+        for (int first = 1, ci = 0; first <= max_rounds; first += chunk, ci++) {
+            const int last = std::min(max_rounds, first + chunk - 1);
+            if (static_cast<int>(P->graphs.size()) <= ci) P->graphs.resize(ci + 1, nullptr);
+            if (P->graphs[ci] && P->graph_last[ci] != last) {
+                (void)hipGraphExecDestroy(P->graphs[ci]);
+                P->graphs[ci] = nullptr;
+            }
+            if (!P->graphs[ci]) {
+                hipGraph_t graph;
+                check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "capture");
+                for (int round = first; round <= last; round++)
+                    hipLaunchKernelGGL(round_kernel, dim3(grid), dim3(256), 0, s, M, D, round, 1);
+                check(hipStreamEndCapture(s, &graph), "end capture");
+                hipGraphExec_t exec;
+                check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "instantiate");
+                (void)hipGraphDestroy(graph);
+                P->graphs[ci] = exec;
+                if (static_cast<int>(P->graph_last.size()) <= ci) P->graph_last.resize(ci + 1, 0);
+                P->graph_last[ci] = last;
+            }
+            check(hipGraphLaunch(P->graphs[ci], s), "graph launch");
+            int flags[kStatusWords];
+            check(hipMemcpyAsync(flags, D.status + kStatusWords * last, sizeof(flags), hipMemcpyDeviceToHost, s),
+                  "flags D2H");
+            check(hipStreamSynchronize(s), "graph sync");
+            // the chunk ran on if and only if its last round has a status set
+            // or the run ended inside it; read all rows once at the end
+            if (flags[kStatusInfeasPrev] || !flags[kStatusChanged]) break;
+        }
+        check(hipMemcpyAsync(st.data(), D.status, st.size() * sizeof(int), hipMemcpyDeviceToHost, s), "status D2H");
+        check(hipStreamSynchronize(s), "status sync");
+    } else {
+        void *args[] = {const_cast<DevMatrix *>(&M), const_cast<DevState *>(&D), &max_rounds};
+        check(hipLaunchCooperativeKernel(reinterpret_cast<const void *>(mega_kernel), dim3(P->mega_grid), dim3(256),
+                                         args, 0, s),
+              "mega_kernel launch");
+    }
+    check(hipEventRecord(P->ev1, s), "event");
+
+    if (sync_type == MEGAKERNEL_ID) {
+        check(hipMemcpyAsync(st.data(), D.status, kStatusWords * sizeof(int), hipMemcpyDeviceToHost, s), "outcome D2H");
+        check(hipStreamSynchronize(s), "mega sync");
+        if (st[kGlobalError] || st[kOutRounds] < 0) throw std::runtime_error("device barrier timed out");
+        res.rounds = st[kOutRounds];
+        res.infeasible = st[kOutInfeasible];
+        res.capped = st[kOutCapped];
+        result_buf = st[kOutResultBuf];
+    } else {
+        outcome_from_status(st, max_rounds, res, result_buf, need_final);
+        if (need_final) {
+            // the cap stopped a run that still changed: check its result
+            const int fgrid = round_grid(*P, M.n_fix_items);
+            hipLaunchKernelGGL(fixup_kernel, dim3(fgrid), dim3(256), 0, s, M, D, max_rounds + 1);
+            check(hipGetLastError(), "fixup launch");
+            int flags[kStatusWords];
+            check(hipMemcpyAsync(flags, D.status + kStatusWords * (max_rounds + 1), sizeof(flags),
+                                 hipMemcpyDeviceToHost, s), "flags D2H");
+            check(hipStreamSynchronize(s), "fixup sync");
+            res.infeasible = flags[kStatusInfeasPrev] != 0;
+            res.capped = 1;
+        }
+        check(hipStreamSynchronize(s), "sync");
+    }
+    float ms = 0.f;
+    check(hipEventElapsedTime(&ms, P->ev0, P->ev1), "elapsed");
+    res.kernel_ms = ms;
+
+    const double2 *out = result_buf ? D.bnd1 : D.bnd0;
+    hipLaunchKernelGGL(unpack_kernel, dim3(fill_grid), dim3(256), 0, s, M, out, d_lbs, d_ubs);
+    check(hipGetLastError(), "unpack_kernel launch");
+    check(hipStreamSynchronize(s), "unpack sync");
+    return res;
+}
+
+RunResult problem_propagate(Problem *P, double *lbs, double *ubs, int sync_type, int max_rounds) {
+    const int n = P->n;
+    // an input with crossed bounds is infeasible before any round
+    for (int j = 0; j < n; j++)
+        if (eps_gt(lbs[j], ubs[j])) {
+            RunResult r;
+            r.infeasible = 1;
+            return r;
+        }
+    double *d_b = nullptr;
+    check(hipMalloc(&d_b, 2 * sizeof(double) * static_cast<size_t>(n)), "hipMalloc bounds");
+    RunResult res;
+    try {
+        check(hipMemcpy(d_b, lbs, n * sizeof(double), hipMemcpyHostToDevice), "H2D lbs");
+        check(hipMemcpy(d_b + n, ubs, n * sizeof(double), hipMemcpyHostToDevice), "H2D ubs");
+        res = problem_propagate_device(P, d_b, d_b + n, sync_type, max_rounds);
+        check(hipMemcpy(lbs, d_b, n * sizeof(double), hipMemcpyDeviceToHost), "D2H lbs");
+        check(hipMemcpy(ubs, d_b + n, n * sizeof(double), hipMemcpyDeviceToHost), "D2H ubs");
+    } catch (...) {
+        (void)hipFree(d_b);
+        throw;
+    }
+    (void)hipFree(d_b);
+    return res;
+}
+
+}  // namespace gdp

@@ -1,0 +1,343 @@
+// gdp_interface.cpp - the C ABI of libGpuProp.so: the eleven entry points of
+// the reference's include/interface.h plus the extension API of gdp_ext.h.
+//
+// Error behaviour follows src/interface.cu:7-16 (an exception becomes
+// GDP_ERROR and a message on stderr), except that device errors never
+// terminate the process (the reference exit()s, src/cuda_def.cuh:8-15).
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <exception>
+#include <string>
+#include <vector>
+
+#include "../../include/gdp_ext.h"
+#include "../../include/interface.h"
+#include "gdp_engine.h"
+#include "gdp_host.h"
+
+using gdp::HostResult;
+using gdp::RunResult;
+
+struct gdp_problem_s {
+    gdp::Problem *p;
+};
+
+namespace {
+
+bool empty_problem(int m, int n, int nnz) { return m == 0 || n == 0 || nnz == 0; }
+
+GDP_RETCODE fail(const char *where, const std::exception &e) {
+    std::string msg = std::string(where) + ": " + e.what();
+    gdp::set_last_error(msg);
+    std::fprintf(stderr, "Error happened. Exception:\n%s\n", msg.c_str());
+    return GDP_ERROR;
+}
+
+bool sizes_ok(int m, int n, int nnz) { return m >= 0 && n >= 0 && nnz >= 0; }
+
+void fill_info(gdp_run_info *info, int rounds, int infeasible, int capped, double ms) {
+    if (!info) return;
+    info->rounds = rounds;
+    info->infeasible = infeasible;
+    info->capped = capped;
+    info->kernel_ms = ms;
+}
+
+// Runs the FP64 GPU path on one call's arrays (create, propagate, destroy).
+GDP_RETCODE gpu_double(int m, int n, int nnz, const int *col, const int *rp, const double *val,
+                       const double *lhs, const double *rhs, double *lb, double *ub, const int *vt,
+                       int sync_type, gdp_run_info *info) {
+    if (!sizes_ok(m, n, nnz)) {
+        gdp::set_last_error("negative size");
+        return GDP_ERROR;
+    }
+    if (sync_type < CPU_LOOP || sync_type > MEGAKERNEL) {
+        gdp::set_last_error("Unknown sync type");
+        std::fprintf(stderr, "Error happened. Exception:\nUnknown sync type\n");
+        return GDP_ERROR;
+    }
+    if (empty_problem(m, n, nnz)) return GDP_OKAY;
+    gdp::Problem *p = nullptr;
+    try {
+        p = gdp::problem_create(m, n, nnz, col, rp, val, lhs, rhs, vt, nullptr);
+        std::vector<double> l(lb, lb + n), u(ub, ub + n);
+        const RunResult r = gdp::problem_propagate(p, l.data(), u.data(), sync_type, MAX_NUM_ROUNDS);
+        std::memcpy(lb, l.data(), n * sizeof(double));
+        std::memcpy(ub, u.data(), n * sizeof(double));
+        fill_info(info, r.rounds, r.infeasible, r.capped, r.kernel_ms);
+        gdp::problem_destroy(p);
+        return GDP_OKAY;
+    } catch (const std::exception &e) {
+        if (p) gdp::problem_destroy(p);
+        return fail("gpu_atomic", e);
+    }
+}
+
+// Float entry points compute in FP64; a bound that moved is rounded outward
+// to float (lower bounds down, upper bounds up) so that it stays valid.
+float outward_lower(double v) {
+    float f = static_cast<float>(v);
+    if (static_cast<double>(f) > v) f = std::nextafter(f, -INFINITY);
+    return f;
+}
+float outward_upper(double v) {
+    float f = static_cast<float>(v);
+    if (static_cast<double>(f) < v) f = std::nextafter(f, INFINITY);
+    return f;
+}
+
+template <typename Fn>
+GDP_RETCODE via_double(int m, int n, int nnz, const float *val, const float *lhs, const float *rhs, float *lb,
+                       float *ub, Fn &&run) {
+    if (!sizes_ok(m, n, nnz)) return GDP_ERROR;
+    if (empty_problem(m, n, nnz)) return GDP_OKAY;
+    std::vector<double> v(val, val + nnz), l(lhs, lhs + m), r(rhs, rhs + m), lo(lb, lb + n), hi(ub, ub + n);
+    const GDP_RETCODE rc = run(v.data(), l.data(), r.data(), lo.data(), hi.data());
+    if (rc != GDP_OKAY) return rc;
+    for (int j = 0; j < n; j++) {
+        if (lo[j] != static_cast<double>(lb[j])) lb[j] = outward_lower(lo[j]);
+        if (hi[j] != static_cast<double>(ub[j])) ub[j] = outward_upper(hi[j]);
+    }
+    return GDP_OKAY;
+}
+
+GDP_RETCODE host_double(int which, int m, int n, int nnz, const int *col, const int *rp, const double *val,
+                        const double *lhs, const double *rhs, double *lb, double *ub, const int *vt,
+                        int max_rounds, int threads, gdp_run_info *info) {
+    if (!sizes_ok(m, n, nnz)) return GDP_ERROR;
+    if (empty_problem(m, n, nnz)) return GDP_OKAY;
+    try {
+        HostResult r;
+        if (which == 0)
+            r = gdp::seq_propagate(m, n, col, rp, val, lhs, rhs, lb, ub, vt, max_rounds);
+        else if (which == 1)
+            r = gdp::seq_dis_propagate(m, n, col, rp, val, lhs, rhs, lb, ub, vt, max_rounds);
+        else
+            r = gdp::omp_propagate(m, n, col, rp, val, lhs, rhs, lb, ub, vt, max_rounds, threads);
+        fill_info(info, r.rounds, r.infeasible, r.capped, 0.0);
+        return GDP_OKAY;
+    } catch (const std::exception &e) {
+        return fail("host propagator", e);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+// ---- reference entry points (include/interface.h) ----
+
+GDP_RETCODE propagateConstraintsGPUReductionDouble(const int n_cons, const int n_vars, const int nnz,
+                                                   const int *csr_col_indices, const int *csr_row_ptrs,
+                                                   const double *csr_vals, const double *lhss,
+                                                   const double *rhss, double *lbs, double *ubs,
+                                                   const int *vartypes) {
+    return gpu_double(n_cons, n_vars, nnz, csr_col_indices, csr_row_ptrs, csr_vals, lhss, rhss, lbs, ubs,
+                      vartypes, CPU_LOOP, nullptr);
+}
+
+GDP_RETCODE propagateConstraintsGPUReductionFloat(const int n_cons, const int n_vars, const int nnz,
+                                                  const int *csr_col_indices, const int *csr_row_ptrs,
+                                                  const float *csr_vals, const float *lhss, const float *rhss,
+                                                  float *lbs, float *ubs, const int *vartypes) {
+    return via_double(n_cons, n_vars, nnz, csr_vals, lhss, rhss, lbs, ubs,
+                      [&](double *v, double *l, double *r, double *lo, double *hi) {
+                          return gpu_double(n_cons, n_vars, nnz, csr_col_indices, csr_row_ptrs, v, l, r, lo, hi,
+                                            vartypes, CPU_LOOP, nullptr);
+                      });
+}
+
+GDP_RETCODE propagateConstraintsGPUAtomicDouble(const int n_cons, const int n_vars, const int nnz,
+                                                const int *csr_col_indices, const int *csr_row_ptrs,
+                                                const double *csr_vals, const double *lhss, const double *rhss,
+                                                double *lbs, double *ubs, const int *vartypes,
+                                                const int sync_type) {
+    return gpu_double(n_cons, n_vars, nnz, csr_col_indices, csr_row_ptrs, csr_vals, lhss, rhss, lbs, ubs,
+                      vartypes, sync_type, nullptr);
+}
+
+GDP_RETCODE propagateConstraintsGPUAtomicFloat(const int n_cons, const int n_vars, const int nnz,
+                                               const int *csr_col_indices, const int *csr_row_ptrs,
+                                               const float *csr_vals, const float *lhss, const float *rhss,
+                                               float *lbs, float *ubs, const int *vartypes, const int sync_type) {
+    if (sync_type < CPU_LOOP || sync_type > MEGAKERNEL) {
+        gdp::set_last_error("Unknown sync type");
+        std::fprintf(stderr, "Error happened. Exception:\nUnknown sync type\n");
+        return GDP_ERROR;
+    }
+    return via_double(n_cons, n_vars, nnz, csr_vals, lhss, rhss, lbs, ubs,
+                      [&](double *v, double *l, double *r, double *lo, double *hi) {
+                          return gpu_double(n_cons, n_vars, nnz, csr_col_indices, csr_row_ptrs, v, l, r, lo, hi,
+                                            vartypes, sync_type, nullptr);
+                      });
+}
+
+GDP_RETCODE propagateConstraintsSequentialDouble(const int n_cons, const int n_vars, const int nnz,
+                                                 const int *col_indices, const int *row_indices,
+                                                 const double *vals, const double *lhss, const double *rhss,
+                                                 double *lbs, double *ubs, const int *vartypes) {
+    return host_double(0, n_cons, n_vars, nnz, col_indices, row_indices, vals, lhss, rhss, lbs, ubs, vartypes,
+                       MAX_NUM_ROUNDS, 0, nullptr);
+}
+
+GDP_RETCODE propagateConstraintsSequentialFloat(const int n_cons, const int n_vars, const int nnz,
+                                                const int *col_indices, const int *row_indices, const float *vals,
+                                                const float *lhss, const float *rhss, float *lbs, float *ubs,
+                                                const int *vartypes) {
+    return via_double(n_cons, n_vars, nnz, vals, lhss, rhss, lbs, ubs,
+                      [&](double *v, double *l, double *r, double *lo, double *hi) {
+                          return host_double(0, n_cons, n_vars, nnz, col_indices, row_indices, v, l, r, lo, hi,
+                                             vartypes, MAX_NUM_ROUNDS, 0, nullptr);
+                      });
+}
+
+GDP_RETCODE propagateConstraintsFullOMPDouble(const int n_cons, const int n_vars, const int nnz,
+                                              const int *col_indices, const int *row_indices, const double *vals,
+                                              const double *lhss, const double *rhss, double *lbs, double *ubs,
+                                              const int *vartypes) {
+    // cpu_omp stops after MAX_NUM_ROUNDS - 1 rounds (OMP_propagator.h:195)
+    return host_double(2, n_cons, n_vars, nnz, col_indices, row_indices, vals, lhss, rhss, lbs, ubs, vartypes,
+                       MAX_NUM_ROUNDS - 1, 0, nullptr);
+}
+
+GDP_RETCODE propagateConstraintsFullOMPFloat(const int n_cons, const int n_vars, const int nnz,
+                                             const int *col_indices, const int *row_indices, const float *vals,
+                                             const float *lhss, const float *rhss, float *lbs, float *ubs,
+                                             const int *vartypes) {
+    return via_double(n_cons, n_vars, nnz, vals, lhss, rhss, lbs, ubs,
+                      [&](double *v, double *l, double *r, double *lo, double *hi) {
+                          return host_double(2, n_cons, n_vars, nnz, col_indices, row_indices, v, l, r, lo, hi,
+                                             vartypes, MAX_NUM_ROUNDS - 1, 0, nullptr);
+                      });
+}
+
+GDP_RETCODE propagateConstraintsSequentialDisjointDouble(const int n_cons, const int n_vars, const int nnz,
+                                                         const int *col_indices, const int *row_indices,
+                                                         const double *vals, const double *lhss,
+                                                         const double *rhss, double *lbs, double *ubs,
+                                                         const int *vartypes) {
+    return host_double(1, n_cons, n_vars, nnz, col_indices, row_indices, vals, lhss, rhss, lbs, ubs, vartypes,
+                       MAX_NUM_ROUNDS, 0, nullptr);
+}
+
+GDP_RETCODE sequentialPropagateWithMeasureDouble(const int, const int, const int, const int *, const int *,
+                                                 const double *, const double *, const double *, double *,
+                                                 double *, const int *) {
+    gdp::set_last_error("progress-measure propagation is not provided");
+    return GDP_NOTIMPLEMENTED;
+}
+
+GDP_RETCODE atomicPropagateWithMeasureDouble(const int, const int, const int, const int *, const int *,
+                                             const double *, const double *, const double *, double *, double *,
+                                             const int *) {
+    gdp::set_last_error("progress-measure propagation is not provided");
+    return GDP_NOTIMPLEMENTED;
+}
+
+// ---- extension API (gdp_ext.h) ----
+
+void gdpTuningDefaults(gdp_tuning *t) {
+    if (!t) return;
+    const gdp::Tuning d;
+    t->item_nnz = d.item_nnz;
+    t->mega_blocks_per_cu = d.mega_blocks_per_cu;
+    t->graph_rounds = d.graph_rounds;
+    for (int b = 0; b < 10; b++) t->group_log[b] = d.group_log[b];
+}
+
+GDP_RETCODE gdpProblemCreate(const int n_cons, const int n_vars, const int nnz, const int *col_indices,
+                             const int *row_ptrs, const double *vals, const double *lhss, const double *rhss,
+                             const int *vartypes, const gdp_tuning *tuning, gdp_problem *out) {
+    if (!out) return GDP_ERROR;
+    *out = nullptr;
+    try {
+        gdp::Tuning T;
+        if (tuning) {
+            if (tuning->item_nnz > 0) T.item_nnz = tuning->item_nnz;
+            if (tuning->mega_blocks_per_cu > 0) T.mega_blocks_per_cu = tuning->mega_blocks_per_cu;
+            if (tuning->graph_rounds > 0) T.graph_rounds = tuning->graph_rounds;
+            for (int b = 0; b < 10; b++)
+                if (tuning->group_log[b] >= 0) T.group_log[b] = tuning->group_log[b];
+        }
+        gdp::Problem *p = gdp::problem_create(n_cons, n_vars, nnz, col_indices, row_ptrs, vals, lhss, rhss,
+                                              vartypes, &T);
+        *out = new gdp_problem_s{p};
+        return GDP_OKAY;
+    } catch (const std::exception &e) {
+        return fail("gdpProblemCreate", e);
+    }
+}
+
+GDP_RETCODE gdpProblemDestroy(gdp_problem p) {
+    if (!p) return GDP_OKAY;
+    gdp::problem_destroy(p->p);
+    delete p;
+    return GDP_OKAY;
+}
+
+GDP_RETCODE gdpProblemGetStats(gdp_problem p, gdp_problem_stats *out) {
+    if (!p || !out) return GDP_ERROR;
+    const gdp::ProblemStats s = gdp::problem_stats(p->p);
+    out->n_cons = s.m;
+    out->n_vars = s.n;
+    out->nnz = s.nnz;
+    out->n_cons_nonempty = s.m_nonempty;
+    out->n_items = s.n_items;
+    out->n_long_rows = s.n_long_rows;
+    for (int b = 0; b < 11; b++) {
+        out->bin_rows[b] = s.bin_rows[b];
+        out->bin_nnz[b] = s.bin_nnz[b];
+    }
+    return GDP_OKAY;
+}
+
+GDP_RETCODE gdpPropagate(gdp_problem p, double *lbs, double *ubs, const int sync_type, const int max_rounds,
+                         gdp_run_info *info) {
+    if (!p || !lbs || !ubs) return GDP_ERROR;
+    try {
+        const RunResult r = gdp::problem_propagate(p->p, lbs, ubs, sync_type, max_rounds);
+        fill_info(info, r.rounds, r.infeasible, r.capped, r.kernel_ms);
+        return GDP_OKAY;
+    } catch (const std::exception &e) {
+        return fail("gdpPropagate", e);
+    }
+}
+
+GDP_RETCODE gdpPropagateDevice(gdp_problem p, double *d_lbs, double *d_ubs, const int sync_type,
+                               const int max_rounds, gdp_run_info *info) {
+    if (!p || !d_lbs || !d_ubs) return GDP_ERROR;
+    try {
+        const RunResult r = gdp::problem_propagate_device(p->p, d_lbs, d_ubs, sync_type, max_rounds);
+        fill_info(info, r.rounds, r.infeasible, r.capped, r.kernel_ms);
+        return GDP_OKAY;
+    } catch (const std::exception &e) {
+        return fail("gdpPropagateDevice", e);
+    }
+}
+
+GDP_RETCODE gdpSequentialPropagate(const int n_cons, const int n_vars, const int nnz, const int *col_indices,
+                                   const int *row_ptrs, const double *vals, const double *lhss, const double *rhss,
+                                   double *lbs, double *ubs, const int *vartypes, const int max_rounds,
+                                   gdp_run_info *info) {
+    return host_double(0, n_cons, n_vars, nnz, col_indices, row_ptrs, vals, lhss, rhss, lbs, ubs, vartypes,
+                       max_rounds, 0, info);
+}
+
+GDP_RETCODE gdpOMPPropagate(const int n_cons, const int n_vars, const int nnz, const int *col_indices,
+                            const int *row_ptrs, const double *vals, const double *lhss, const double *rhss,
+                            double *lbs, double *ubs, const int *vartypes, const int max_rounds, const int threads,
+                            gdp_run_info *info) {
+    return host_double(2, n_cons, n_vars, nnz, col_indices, row_ptrs, vals, lhss, rhss, lbs, ubs, vartypes,
+                       max_rounds, threads, info);
+}
+
+const char *gdpLastError(void) {
+    thread_local std::string s;
+    s = gdp::last_error();
+    return s.c_str();
+}
+
+const char *gdpVersion(void) { return "gdp-amd 0.1 gfx950 fp64"; }
+
+}  // extern "C"

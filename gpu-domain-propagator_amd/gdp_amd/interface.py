@@ -1,0 +1,334 @@
+"""ctypes binding of libGpuProp.so.
+
+The functions ``propagateGPUAtomic``, ``propagateGPUReduction``,
+``propagateSequential``, ``propagateFullOMP`` and ``propagateSequentialDisjoint``
+mirror the reference's Python interface (fileReader/GPUDomPropInterface.py:35-145,
+:195-247, :330-383, :385-429): same names, same argument order (note: n_vars
+before n_cons), same return value ``(lbs, ubs)`` as lists.  The ``datatype``
+argument selects the Double or Float entry point.
+
+The numpy-level helpers (``gpu_propagate``, ``Problem`` ...) use the extension
+API of include/gdp_ext.h and also return the round count and infeasibility.
+
+Loading never falls back to anything: if the library is missing, importing
+functions that need it raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import POINTER, c_double, c_float, c_int, c_longlong, c_void_p
+from dataclasses import dataclass
+from typing import List, Optional, Sequence, Tuple
+
+import numpy as np
+
+_PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.environ.get("GDP_LIBRARY", os.path.join(_PKG_DIR, "lib", "libGpuProp.so"))
+
+GDP_OKAY = 1
+GDP_ERROR = 0
+GDP_NOTIMPLEMENTED = -18
+GDP_BINARY, GDP_INTEGER, GDP_CONTINUOUS = 0, 1, 3
+CPU_LOOP, GPU_LOOP, MEGAKERNEL = 0, 1, 2
+MAX_NUM_ROUNDS = 100
+GDP_INF = 1e20
+GDP_EPS = 1e-9
+
+#: every symbol include/interface.h and include/gdp_ext.h declare
+LEGACY_SYMBOLS = (
+    "propagateConstraintsGPUReductionDouble",
+    "propagateConstraintsGPUReductionFloat",
+    "propagateConstraintsGPUAtomicDouble",
+    "propagateConstraintsGPUAtomicFloat",
+    "propagateConstraintsSequentialDouble",
+    "propagateConstraintsSequentialFloat",
+    "propagateConstraintsFullOMPDouble",
+    "propagateConstraintsFullOMPFloat",
+    "propagateConstraintsSequentialDisjointDouble",
+    "sequentialPropagateWithMeasureDouble",
+    "atomicPropagateWithMeasureDouble",
+)
+EXT_SYMBOLS = (
+    "gdpTuningDefaults",
+    "gdpProblemCreate",
+    "gdpProblemDestroy",
+    "gdpProblemGetStats",
+    "gdpPropagate",
+    "gdpPropagateDevice",
+    "gdpSequentialPropagate",
+    "gdpOMPPropagate",
+    "gdpLastError",
+    "gdpVersion",
+)
+
+
+class RunInfo(ctypes.Structure):
+    _fields_ = [("rounds", c_int), ("infeasible", c_int), ("capped", c_int), ("kernel_ms", c_double)]
+
+
+class Tuning(ctypes.Structure):
+    _fields_ = [
+        ("item_nnz", c_int),
+        ("mega_blocks_per_cu", c_int),
+        ("graph_rounds", c_int),
+        ("group_log", c_int * 10),
+    ]
+
+
+class ProblemStats(ctypes.Structure):
+    _fields_ = [
+        ("n_cons", c_int),
+        ("n_vars", c_int),
+        ("nnz", c_int),
+        ("n_cons_nonempty", c_int),
+        ("n_items", c_int),
+        ("n_long_rows", c_int),
+        ("bin_rows", c_longlong * 11),
+        ("bin_nnz", c_longlong * 11),
+    ]
+
+
+_lib = None
+
+
+def load_library(path: Optional[str] = None) -> ctypes.CDLL:
+    """Loads libGpuProp.so (once).  Raises if it has not been built."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise RuntimeError(f"libGpuProp.so not found at {p}; run __graft_entry__.build() first")
+    lib = ctypes.CDLL(p)
+    vp = c_void_p
+    for name in ("propagateConstraintsGPUAtomicDouble", "propagateConstraintsGPUAtomicFloat"):
+        getattr(lib, name).argtypes = [c_int, c_int, c_int, vp, vp, vp, vp, vp, vp, vp, vp, c_int]
+        getattr(lib, name).restype = c_int
+    for name in LEGACY_SYMBOLS:
+        if "GPUAtomic" in name:
+            continue
+        getattr(lib, name).argtypes = [c_int, c_int, c_int, vp, vp, vp, vp, vp, vp, vp, vp]
+        getattr(lib, name).restype = c_int
+    lib.gdpTuningDefaults.argtypes = [POINTER(Tuning)]
+    lib.gdpTuningDefaults.restype = None
+    lib.gdpProblemCreate.argtypes = [c_int, c_int, c_int, vp, vp, vp, vp, vp, vp, POINTER(Tuning), POINTER(vp)]
+    lib.gdpProblemCreate.restype = c_int
+    lib.gdpProblemDestroy.argtypes = [vp]
+    lib.gdpProblemDestroy.restype = c_int
+    lib.gdpProblemGetStats.argtypes = [vp, POINTER(ProblemStats)]
+    lib.gdpProblemGetStats.restype = c_int
+    lib.gdpPropagate.argtypes = [vp, vp, vp, c_int, c_int, POINTER(RunInfo)]
+    lib.gdpPropagate.restype = c_int
+    lib.gdpPropagateDevice.argtypes = [vp, vp, vp, c_int, c_int, POINTER(RunInfo)]
+    lib.gdpPropagateDevice.restype = c_int
+    lib.gdpSequentialPropagate.argtypes = [c_int, c_int, c_int, vp, vp, vp, vp, vp, vp, vp, vp, c_int,
+                                           POINTER(RunInfo)]
+    lib.gdpSequentialPropagate.restype = c_int
+    lib.gdpOMPPropagate.argtypes = [c_int, c_int, c_int, vp, vp, vp, vp, vp, vp, vp, vp, c_int, c_int,
+                                    POINTER(RunInfo)]
+    lib.gdpOMPPropagate.restype = c_int
+    lib.gdpLastError.argtypes = []
+    lib.gdpLastError.restype = ctypes.c_char_p
+    lib.gdpVersion.argtypes = []
+    lib.gdpVersion.restype = ctypes.c_char_p
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def last_error() -> str:
+    return load_library().gdpLastError().decode()
+
+
+# --------------------------------------------------------------------------
+# reference-style list interface (fileReader/GPUDomPropInterface.py)
+# --------------------------------------------------------------------------
+
+def check_input(n_vars, n_cons, nnz, csr_col_indices, csr_row_ptrs, csr_vals, lhss, rhss, lbs, ubs, vartypes):
+    """Same contract as GPUDomPropInterface.py:12-32."""
+    assert len(csr_col_indices) == nnz
+    assert len(csr_row_ptrs) == n_cons + 1
+    assert len(csr_vals) == nnz
+    assert len(lhss) == n_cons
+    assert len(rhss) == n_cons
+    assert len(lbs) == n_vars
+    assert len(ubs) == n_vars
+    assert len(vartypes) == n_vars
+
+
+def _call_list(fname_double, fname_float, n_vars, n_cons, nnz, csr_col_indices, csr_row_ptrs, csr_vals, lhss,
+               rhss, lbs, ubs, vartypes, datatype, extra=()):
+    lib = load_library()
+    check_input(n_vars, n_cons, nnz, csr_col_indices, csr_row_ptrs, csr_vals, lhss, rhss, lbs, ubs, vartypes)
+    if datatype == c_double:
+        fun = getattr(lib, fname_double)
+    elif datatype == c_float and fname_float is not None:
+        fun = getattr(lib, fname_float)
+    else:
+        raise Exception("unsupported datatype")
+    c_col = (c_int * max(nnz, 1))(*csr_col_indices)
+    c_rp = (c_int * (n_cons + 1))(*csr_row_ptrs)
+    c_vals = (datatype * max(nnz, 1))(*csr_vals)
+    c_lhss = (datatype * max(n_cons, 1))(*lhss)
+    c_rhss = (datatype * max(n_cons, 1))(*rhss)
+    c_lbs = (datatype * max(n_vars, 1))(*lbs)
+    c_ubs = (datatype * max(n_vars, 1))(*ubs)
+    c_vt = (c_int * max(n_vars, 1))(*vartypes)
+    rc = fun(n_cons, n_vars, nnz, c_col, c_rp, c_vals, c_lhss, c_rhss, c_lbs, c_ubs, c_vt, *extra)
+    if rc != GDP_OKAY:
+        raise RuntimeError(f"{fun.__name__} returned {rc}: {last_error()}")
+    return list(c_lbs)[:n_vars], list(c_ubs)[:n_vars]
+
+
+def propagateGPUAtomic(n_vars: int, n_cons: int, nnz: int, csr_col_indices: List[int], csr_row_ptrs: List[int],
+                       csr_vals: List[float], lhss: List[float], rhss: List[float], lbs: List[float],
+                       ubs: List[float], vartypes: List[int], synctype: int = 0,
+                       datatype=c_double) -> Tuple[List[float], List[float]]:
+    return _call_list("propagateConstraintsGPUAtomicDouble", "propagateConstraintsGPUAtomicFloat", n_vars, n_cons,
+                      nnz, csr_col_indices, csr_row_ptrs, csr_vals, lhss, rhss, lbs, ubs, vartypes, datatype,
+                      extra=(c_int(synctype),))
+
+
+def propagateGPUReduction(n_vars, n_cons, nnz, csr_col_indices, csr_row_ptrs, csr_vals, lhss, rhss, lbs, ubs,
+                          vartypes, datatype=c_double):
+    return _call_list("propagateConstraintsGPUReductionDouble", "propagateConstraintsGPUReductionFloat", n_vars,
+                      n_cons, nnz, csr_col_indices, csr_row_ptrs, csr_vals, lhss, rhss, lbs, ubs, vartypes, datatype)
+
+
+def propagateSequential(n_vars, n_cons, nnz, csr_col_indices, csr_row_ptrs, csr_vals, lhss, rhss, lbs, ubs,
+                        vartypes, datatype=c_double):
+    return _call_list("propagateConstraintsSequentialDouble", "propagateConstraintsSequentialFloat", n_vars, n_cons,
+                      nnz, csr_col_indices, csr_row_ptrs, csr_vals, lhss, rhss, lbs, ubs, vartypes, datatype)
+
+
+def propagateFullOMP(n_vars, n_cons, nnz, csr_col_indices, csr_row_ptrs, csr_vals, lhss, rhss, lbs, ubs,
+                     vartypes, datatype=c_double):
+    return _call_list("propagateConstraintsFullOMPDouble", "propagateConstraintsFullOMPFloat", n_vars, n_cons, nnz,
+                      csr_col_indices, csr_row_ptrs, csr_vals, lhss, rhss, lbs, ubs, vartypes, datatype)
+
+
+def propagateSequentialDisjoint(n_vars, n_cons, nnz, csr_col_indices, csr_row_ptrs, csr_vals, lhss, rhss, lbs, ubs,
+                                vartypes, datatype=c_double):
+    return _call_list("propagateConstraintsSequentialDisjointDouble", None, n_vars, n_cons, nnz, csr_col_indices,
+                      csr_row_ptrs, csr_vals, lhss, rhss, lbs, ubs, vartypes, datatype)
+
+
+# --------------------------------------------------------------------------
+# numpy interface over the extension API
+# --------------------------------------------------------------------------
+
+@dataclass
+class Outcome:
+    lbs: np.ndarray
+    ubs: np.ndarray
+    rounds: int
+    infeasible: bool
+    capped: bool
+    kernel_ms: float
+
+
+def _arr(a, dt):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+def _ptr(a: np.ndarray):
+    return c_void_p(a.ctypes.data)
+
+
+class Problem:
+    """A constraint matrix resident in device memory (gdpProblemCreate)."""
+
+    def __init__(self, inst, tuning: Optional[dict] = None):
+        self.lib = load_library()
+        self.m, self.n = int(inst.m), int(inst.n)
+        self.col = _arr(inst.col, np.int32)
+        self.row_ptr = _arr(inst.row_ptr, np.int32)
+        self.val = _arr(inst.val, np.float64)
+        self.lhs = _arr(inst.lhs, np.float64)
+        self.rhs = _arr(inst.rhs, np.float64)
+        self.vartype = _arr(inst.vartype, np.int32)
+        t = Tuning()
+        self.lib.gdpTuningDefaults(ctypes.byref(t))
+        for k, v in (tuning or {}).items():
+            if k == "group_log":
+                for b, g in enumerate(v):
+                    t.group_log[b] = int(g)
+            else:
+                setattr(t, k, int(v))
+        h = c_void_p()
+        rc = self.lib.gdpProblemCreate(self.m, self.n, len(self.col), _ptr(self.col), _ptr(self.row_ptr),
+                                       _ptr(self.val), _ptr(self.lhs), _ptr(self.rhs), _ptr(self.vartype),
+                                       ctypes.byref(t), ctypes.byref(h))
+        if rc != GDP_OKAY:
+            raise RuntimeError(f"gdpProblemCreate failed: {last_error()}")
+        self.handle = h
+
+    def stats(self) -> ProblemStats:
+        s = ProblemStats()
+        self.lib.gdpProblemGetStats(self.handle, ctypes.byref(s))
+        return s
+
+    def propagate(self, lbs, ubs, sync_type: int = CPU_LOOP, max_rounds: int = MAX_NUM_ROUNDS) -> Outcome:
+        lb = np.array(lbs, dtype=np.float64, copy=True)
+        ub = np.array(ubs, dtype=np.float64, copy=True)
+        info = RunInfo()
+        rc = self.lib.gdpPropagate(self.handle, _ptr(lb), _ptr(ub), sync_type, max_rounds, ctypes.byref(info))
+        if rc != GDP_OKAY:
+            raise RuntimeError(f"gdpPropagate failed: {last_error()}")
+        return Outcome(lb, ub, info.rounds, bool(info.infeasible), bool(info.capped), info.kernel_ms)
+
+    def propagate_device(self, d_lbs_ptr: int, d_ubs_ptr: int, sync_type: int = CPU_LOOP,
+                         max_rounds: int = MAX_NUM_ROUNDS) -> RunInfo:
+        """Bounds already in device memory (e.g. torch.Tensor.data_ptr())."""
+        info = RunInfo()
+        rc = self.lib.gdpPropagateDevice(self.handle, c_void_p(d_lbs_ptr), c_void_p(d_ubs_ptr), sync_type,
+                                         max_rounds, ctypes.byref(info))
+        if rc != GDP_OKAY:
+            raise RuntimeError(f"gdpPropagateDevice failed: {last_error()}")
+        return info
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.gdpProblemDestroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def gpu_propagate(inst, sync_type: int = CPU_LOOP, max_rounds: int = MAX_NUM_ROUNDS,
+                  tuning: Optional[dict] = None) -> Outcome:
+    p = Problem(inst, tuning)
+    try:
+        return p.propagate(inst.lb, inst.ub, sync_type, max_rounds)
+    finally:
+        p.close()
+
+
+def host_propagate(inst, method: str = "seq", max_rounds: int = MAX_NUM_ROUNDS, threads: int = 0) -> Outcome:
+    """The library's own host propagators (cpu_seq or cpu_omp) with run info."""
+    lib = load_library()
+    col = _arr(inst.col, np.int32)
+    rp = _arr(inst.row_ptr, np.int32)
+    val = _arr(inst.val, np.float64)
+    lhs = _arr(inst.lhs, np.float64)
+    rhs = _arr(inst.rhs, np.float64)
+    vt = _arr(inst.vartype, np.int32)
+    lb = np.array(inst.lb, dtype=np.float64, copy=True)
+    ub = np.array(inst.ub, dtype=np.float64, copy=True)
+    info = RunInfo()
+    args = (int(inst.m), int(inst.n), len(col), _ptr(col), _ptr(rp), _ptr(val), _ptr(lhs), _ptr(rhs), _ptr(lb),
+            _ptr(ub), _ptr(vt), max_rounds)
+    if method == "seq":
+        rc = lib.gdpSequentialPropagate(*args, ctypes.byref(info))
+    elif method == "omp":
+        rc = lib.gdpOMPPropagate(*args, threads, ctypes.byref(info))
+    else:
+        raise ValueError(method)
+    if rc != GDP_OKAY:
+        raise RuntimeError(f"host propagation failed: {last_error()}")
+    return Outcome(lb, ub, info.rounds, bool(info.infeasible), bool(info.capped), info.kernel_ms)

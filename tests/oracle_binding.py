@@ -1,0 +1,125 @@
+"""ctypes binding of the oracle (oracle/gdp_oracle.c).
+
+Test infrastructure: the parity tests use it as the checker of the library.
+"""
+import ctypes
+import os
+from ctypes import POINTER, c_double, c_int, c_void_p
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(REPO, "oracle", "lib", "libgdp_oracle.so")
+SEG_LEN = 512  # canonical segment length shared with the kernels
+
+
+class Result(ctypes.Structure):
+    _fields_ = [("rounds", c_int), ("infeasible", c_int), ("capped", c_int)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            raise RuntimeError(f"oracle not built: {LIB}")
+        L = ctypes.CDLL(LIB)
+        vp = c_void_p
+        L.oracle_seq_propagate.argtypes = [c_int, c_int, c_int, vp, vp, vp, vp, vp, vp, vp, vp, c_int, c_int,
+                                           POINTER(Result)]
+        L.oracle_jacobi_propagate.argtypes = [c_int, c_int, c_int, vp, vp, vp, vp, vp, vp, vp, vp, c_int, c_int,
+                                              c_int, POINTER(Result)]
+        L.oracle_seq_disjoint_propagate.argtypes = [c_int, c_int, c_int, vp, vp, vp, vp, vp, vp, vp, vp, c_int,
+                                                    POINTER(Result)]
+        for f in ("oracle_adjust_upper", "oracle_adjust_lower"):
+            getattr(L, f).argtypes = [c_int, c_double]
+            getattr(L, f).restype = c_double
+        for f in ("oracle_is_lb_better", "oracle_is_ub_better"):
+            getattr(L, f).argtypes = [c_double, c_double, c_double]
+            getattr(L, f).restype = c_int
+        for f in ("oracle_tighten_var_upper", "oracle_tighten_var_lower"):
+            getattr(L, f).argtypes = [c_double, c_double, c_double, c_int, c_double, c_double, c_int,
+                                      POINTER(c_double)]
+            getattr(L, f).restype = c_int
+        L.oracle_can_cons_be_tightened.argtypes = [c_double, c_double, c_int, c_int, c_double, c_double, c_double]
+        L.oracle_is_cons_infeasible.argtypes = [c_double, c_double, c_double, c_double]
+        L.oracle_canonical_sum.argtypes = [vp, c_int, c_int]
+        L.oracle_canonical_sum.restype = c_double
+        L.oracle_row_activities_seq.argtypes = [c_int, vp, vp, vp, vp, vp, vp, vp]
+        L.oracle_row_activities_canon.argtypes = [c_int, vp, vp, vp, vp, vp, c_int, vp, vp]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return c_void_p(a.ctypes.data)
+
+
+def _arrays(inst):
+    return (np.ascontiguousarray(inst.col, np.int32), np.ascontiguousarray(inst.row_ptr, np.int32),
+            np.ascontiguousarray(inst.val, np.float64), np.ascontiguousarray(inst.lhs, np.float64),
+            np.ascontiguousarray(inst.rhs, np.float64), np.ascontiguousarray(inst.vartype, np.int32))
+
+
+def seq(inst, max_rounds=100, detect_infeasible=True, lb=None, ub=None):
+    """cpu_seq fixpoint: returns (lb, ub, rounds, infeasible, capped)."""
+    col, rp, val, lhs, rhs, vt = _arrays(inst)
+    lb = np.array(inst.lb if lb is None else lb, np.float64)
+    ub = np.array(inst.ub if ub is None else ub, np.float64)
+    r = Result()
+    lib().oracle_seq_propagate(inst.m, inst.n, len(col), _p(col), _p(rp), _p(val), _p(lhs), _p(rhs), _p(lb),
+                               _p(ub), _p(vt), max_rounds, int(detect_infeasible), ctypes.byref(r))
+    return lb, ub, r.rounds, bool(r.infeasible), bool(r.capped)
+
+
+def jacobi(inst, max_rounds=100, detect_infeasible=True, seg_len=SEG_LEN, lb=None, ub=None):
+    """Round-synchronous fixpoint in the kernels' summation order."""
+    col, rp, val, lhs, rhs, vt = _arrays(inst)
+    lb = np.array(inst.lb if lb is None else lb, np.float64)
+    ub = np.array(inst.ub if ub is None else ub, np.float64)
+    r = Result()
+    lib().oracle_jacobi_propagate(inst.m, inst.n, len(col), _p(col), _p(rp), _p(val), _p(lhs), _p(rhs), _p(lb),
+                                  _p(ub), _p(vt), max_rounds, int(detect_infeasible), seg_len, ctypes.byref(r))
+    return lb, ub, r.rounds, bool(r.infeasible), bool(r.capped)
+
+
+def seq_disjoint(inst, max_rounds=100):
+    col, rp, val, lhs, rhs, vt = _arrays(inst)
+    lb = np.array(inst.lb, np.float64)
+    ub = np.array(inst.ub, np.float64)
+    r = Result()
+    lib().oracle_seq_disjoint_propagate(inst.m, inst.n, len(col), _p(col), _p(rp), _p(val), _p(lhs), _p(rhs),
+                                        _p(lb), _p(ub), _p(vt), max_rounds, ctypes.byref(r))
+    return lb, ub, r.rounds, bool(r.infeasible), bool(r.capped)
+
+
+def canonical_sum(x, seg_len=SEG_LEN):
+    x = np.ascontiguousarray(x, np.float64)
+    return lib().oracle_canonical_sum(_p(x), len(x), seg_len)
+
+
+def row_activities(inst, row, canonical=False, lb=None, ub=None, seg_len=SEG_LEN):
+    col, rp, val, _, _, _ = _arrays(inst)
+    lb = np.ascontiguousarray(inst.lb if lb is None else lb, np.float64)
+    ub = np.ascontiguousarray(inst.ub if ub is None else ub, np.float64)
+    out = np.zeros(3)
+    inf = np.zeros(2, np.int32)
+    if canonical:
+        lib().oracle_row_activities_canon(row, _p(col), _p(rp), _p(val), _p(lb), _p(ub), seg_len, _p(out), _p(inf))
+    else:
+        lib().oracle_row_activities_seq(row, _p(col), _p(rp), _p(val), _p(lb), _p(ub), _p(out), _p(inf))
+    return out, inf
+
+
+def tighten_upper(coeff, slack, surplus, num_inf, lb, ub, is_cont):
+    nb = c_double()
+    t = lib().oracle_tighten_var_upper(coeff, slack, surplus, num_inf, lb, ub, int(is_cont), ctypes.byref(nb))
+    return bool(t), nb.value
+
+
+def tighten_lower(coeff, slack, surplus, num_inf, lb, ub, is_cont):
+    nb = c_double()
+    t = lib().oracle_tighten_var_lower(coeff, slack, surplus, num_inf, lb, ub, int(is_cont), ctypes.byref(nb))
+    return bool(t), nb.value

@@ -1,0 +1,153 @@
+"""GPU parity: libGpuProp.so's HIP path against the oracle, through the C ABI.
+
+Bar (per the propagation contract):
+  * against the round-synchronous oracle (same arithmetic, same summation
+    order): bounds identical bit for bit (compared as doubles, so -0.0 ==
+    +0.0), identical round count, infeasibility flag and cap flag, for every
+    sync type and for every round prefix (max_rounds = 1, 2, ...);
+  * against cpu_seq (the reference's Gauss-Seidel fixpoint): integer bounds
+    identical, continuous bounds within 1e-9 relative or 1e-9 absolute, same
+    infeasibility verdict (round counts differ by design and are reported).
+"""
+import numpy as np
+import pytest
+
+import fixtures
+from gdp_amd import interface as I
+from gdp_amd.instances import INF, powerlaw_mip, small_mip
+
+pytestmark = pytest.mark.gpu
+
+CONT_TOL = 1e-9
+
+
+def _case(name):
+    kind, _, arg = name.partition(":")
+    if kind == "analytic":
+        return fixtures.analytic(arg)
+    if kind == "small":
+        return small_mip(seed=int(arg))
+    if kind == "small_int":
+        return small_mip(seed=int(arg), int_frac=1.0)
+    if kind == "varied":
+        return fixtures.varied(int(arg))
+    if kind == "gcfp":
+        return fixtures.generalized_cfp(int(arg))
+    if kind == "infeasible":
+        return fixtures.infeasible_instance()
+    if kind == "tiny":
+        return fixtures.tiny_integer(int(arg))
+    if kind == "powerlaw":
+        return powerlaw_mip(seed=int(arg), m=20000, n=10000, len_hi=5000, s=1.8)
+    raise KeyError(name)
+
+
+CASES = ([f"analytic:{n}" for n in fixtures.ANALYTIC] + [f"small:{s}" for s in range(3)] +
+         ["small_int:7", "varied:0", "varied:1", "varied:2", "gcfp:30", "infeasible", "tiny:3", "tiny:11",
+          "powerlaw:5"])
+
+
+def assert_same(got, exp, label=""):
+    lb, ub, rounds, infeasible, capped = exp
+    np.testing.assert_array_equal(got.lbs, lb, err_msg=f"{label} lb")
+    np.testing.assert_array_equal(got.ubs, ub, err_msg=f"{label} ub")
+    assert (got.rounds, got.infeasible, got.capped) == (rounds, infeasible, capped), label
+
+
+@pytest.mark.parametrize("sync", [I.CPU_LOOP, I.GPU_LOOP, I.MEGAKERNEL])
+@pytest.mark.parametrize("case", CASES)
+def test_gpu_matches_round_synchronous_oracle(gpu, oracle, case, sync):
+    inst = _case(case)
+    exp = oracle.jacobi(inst)
+    got = I.gpu_propagate(inst, sync)
+    assert_same(got, exp, f"{case} sync={sync}")
+
+
+@pytest.mark.parametrize("case", ["analytic:achterberg", "analytic:two_lines", "small:0", "varied:1", "powerlaw:5"])
+def test_every_round_prefix_matches(gpu, oracle, case):
+    inst = _case(case)
+    full = oracle.jacobi(inst)
+    p = I.Problem(inst)
+    try:
+        for k in sorted({1, 2, 3, 5, max(1, full[2] - 1), full[2]}):
+            exp = oracle.jacobi(inst, max_rounds=k)
+            for sync in (I.CPU_LOOP, I.MEGAKERNEL, I.GPU_LOOP):
+                got = p.propagate(inst.lb, inst.ub, sync, max_rounds=k)
+                assert_same(got, exp, f"{case} rounds<={k} sync={sync}")
+    finally:
+        p.close()
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_gpu_matches_cpu_seq_fixpoint(gpu, oracle, case):
+    inst = _case(case)
+    seq = oracle.seq(inst)
+    got = I.gpu_propagate(inst, I.MEGAKERNEL)
+    assert got.infeasible == seq[3]
+    if seq[3] or seq[4] or got.capped:
+        return
+    is_int = inst.vartype != 3
+    np.testing.assert_array_equal(got.lbs[is_int], seq[0][is_int])
+    np.testing.assert_array_equal(got.ubs[is_int], seq[1][is_int])
+    for g, s in ((got.lbs, seq[0]), (got.ubs, seq[1])):
+        d = np.abs(g - s)[~is_int]
+        bound = np.maximum(CONT_TOL, CONT_TOL * np.abs(s[~is_int]))
+        assert np.all((d <= bound) | (g[~is_int] == s[~is_int])), f"{case}: max diff {d.max(initial=0)}"
+
+
+def test_problem_reuse_many_bound_vectors(gpu, oracle):
+    inst = small_mip(seed=11)
+    rng = np.random.default_rng(0)
+    p = I.Problem(inst)
+    try:
+        for t in range(6):
+            lb, ub = inst.lb.copy(), inst.ub.copy()
+            pick = rng.choice(inst.n, size=30, replace=False)
+            for j in pick:
+                if inst.vartype[j] != 3 and lb[j] > -INF and ub[j] < INF and ub[j] - lb[j] >= 1:
+                    ub[j] = np.floor((lb[j] + ub[j]) / 2)
+            exp = oracle.jacobi(inst.with_bounds(lb, ub))
+            got = p.propagate(lb, ub, [I.CPU_LOOP, I.MEGAKERNEL, I.GPU_LOOP][t % 3])
+            assert_same(got, exp, f"vector {t}")
+    finally:
+        p.close()
+
+
+def test_legacy_entry_points(gpu, oracle):
+    inst = fixtures.analytic("achterberg")
+    args = (inst.n, inst.m, inst.nnz, inst.col.tolist(), inst.row_ptr.tolist(), inst.val.tolist(), inst.lhs.tolist(),
+            inst.rhs.tolist(), inst.lb.tolist(), inst.ub.tolist(), inst.vartype.tolist())
+    for st in (0, 1, 2):
+        lbs, ubs = I.propagateGPUAtomic(*args, synctype=st)
+        assert ubs == inst.meta["ub_expected"].tolist() and lbs == inst.meta["lb_expected"].tolist()
+    lbs, ubs = I.propagateGPUReduction(*args)
+    assert ubs == inst.meta["ub_expected"].tolist()
+    from ctypes import c_float
+    lbs, ubs = I.propagateGPUAtomic(*args, synctype=2, datatype=c_float)
+    assert ubs == inst.meta["ub_expected"].tolist()
+    tl = fixtures.analytic("two_lines")
+    exp = oracle.jacobi(tl)
+    targs = (tl.n, tl.m, tl.nnz, tl.col.tolist(), tl.row_ptr.tolist(), tl.val.tolist(), tl.lhs.tolist(),
+             tl.rhs.tolist(), tl.lb.tolist(), tl.ub.tolist(), tl.vartype.tolist())
+    lbs, ubs = I.propagateGPUAtomic(*targs, synctype=0, datatype=c_float)
+    assert np.all(np.array(lbs, np.float64) <= exp[0]) and np.all(np.array(ubs, np.float64) >= exp[1])
+
+
+def test_ieee_infinity(gpu, oracle):
+    for name in ("savelsbergh1", "savelsbergh_cflp"):
+        inst = fixtures.analytic(name)
+        ieee = inst.with_bounds(np.where(inst.lb <= -INF, -np.inf, inst.lb), np.where(inst.ub >= INF, np.inf, inst.ub))
+        ieee.lhs = np.where(inst.lhs <= -INF, -np.inf, inst.lhs)
+        ieee.rhs = np.where(inst.rhs >= INF, np.inf, inst.rhs)
+        exp = oracle.jacobi(ieee)
+        for sync in (0, 2):
+            got = I.gpu_propagate(ieee, sync)
+            assert_same(got, exp, name)
+
+
+def test_input_with_crossed_bounds(gpu):
+    inst = fixtures.analytic("achterberg")
+    lb = inst.lb.copy()
+    lb[3] = 2.0  # above ub = 1
+    got = I.Problem(inst).propagate(lb, inst.ub, I.MEGAKERNEL)
+    assert got.infeasible and got.rounds == 0

@@ -1,0 +1,290 @@
+"""Pins the oracle before it is trusted as the checker.
+
+Each check is chosen so that a plausible restatement mistake (a dropped
+term, a swapped side, a wrong rounding direction, a different summation
+order) fails at least one of them:
+  * known-answer tests of the scalar arithmetic (reference
+    test/testCases/test_propagator.cuh:4-249, tests/golden/known_answers.json);
+  * the analytic fixpoints of the reference's end-to-end fixtures
+    (test/test_setups.cuh, tests/golden/analytic.json) for both drivers;
+  * the closed form of the generalized facility-location family;
+  * closed-form residual-activity bounds of single-row systems;
+  * brute-force soundness on tiny integer systems (no integer point of the
+    box that satisfies all rows is cut off, and single rows are tight);
+  * invariants: the Gauss-Seidel and round-synchronous fixpoints of integer
+    systems coincide, a fixpoint is idempotent, and row/column permutations
+    do not change it;
+  * the canonical summation order against an independent bottom-up
+    implementation and against inputs where the order decides the result.
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+import fixtures
+from gdp_amd.instances import INF, from_rows, small_mip
+
+KAT = fixtures.load_json("known_answers.json")
+
+
+# ---- scalar arithmetic ----------------------------------------------------
+
+def test_adjust_bounds(oracle):
+    L = oracle.lib()
+    for c in KAT["adjust_upper"]:
+        assert L.oracle_adjust_upper(c["is_cont"], c["bound"]) == pytest.approx(c["expected"], abs=1e-6)
+    for c in KAT["adjust_lower"]:
+        assert L.oracle_adjust_lower(c["is_cont"], c["bound"]) == pytest.approx(c["expected"], abs=1e-6)
+    # integer rounding absorbs the tolerance in the right direction
+    assert L.oracle_adjust_upper(0, 3.0 - 5e-10) == 3.0
+    assert L.oracle_adjust_upper(0, 3.0 - 5e-9) == 2.0
+    assert L.oracle_adjust_lower(0, 3.0 + 5e-10) == 3.0
+    assert L.oracle_adjust_lower(0, 3.0 + 5e-9) == 4.0
+
+
+def test_better_predicates(oracle):
+    L = oracle.lib()
+    for c in KAT["is_ub_better"]:
+        assert bool(L.oracle_is_ub_better(c["lb"], c["ub"], c["new"])) == c["expected"]
+    for c in KAT["is_lb_better"]:
+        assert bool(L.oracle_is_lb_better(c["lb"], c["ub"], c["new"])) == c["expected"]
+    # improvements within the tolerance are not improvements
+    assert not L.oracle_is_ub_better(0.0, 1.0, 1.0 - 5e-10)
+    assert not L.oracle_is_lb_better(0.0, 1.0, 5e-10)
+
+
+def test_row_predicates(oracle):
+    L = oracle.lib()
+    for c in KAT["can_cons_be_tightened"]:
+        got = L.oracle_can_cons_be_tightened(c["minact"], c["maxact"], c["nmin"], c["nmax"], c["lhs"], c["rhs"],
+                                             c["delta"])
+        assert bool(got) == c["expected"]
+    for c in KAT["is_cons_infeasible"]:
+        assert bool(L.oracle_is_cons_infeasible(c["minact"], c["maxact"], c["rhs"], c["lhs"])) == c["expected"]
+    # two infinite contributions on both sides: nothing can be derived
+    assert not L.oracle_can_cons_be_tightened(0.0, 0.0, 2, 2, -1.0, 1.0, 5.0)
+    # an infinite range always passes the gate
+    assert L.oracle_can_cons_be_tightened(0.0, 0.0, 0, 0, -1.0, 1.0, INF)
+
+
+def test_tighten_known_answers(oracle):
+    for c in KAT["tighten_upper"]:
+        t, nb = oracle.tighten_upper(c["coeff"], c["slack"], c["surplus"], c["ninf"], c["lb"], c["ub"], c["is_cont"])
+        assert t == c["tightened"]
+        assert nb == pytest.approx(c["expected"], abs=1e-6)
+    for c in KAT["tighten_lower"]:
+        t, nb = oracle.tighten_lower(c["coeff"], c["slack"], c["surplus"], c["ninf"], c["lb"], c["ub"], c["is_cont"])
+        assert t == c["tightened"]
+        assert nb == pytest.approx(c["expected"], abs=1e-6)
+
+
+def test_tighten_infinite_contributor(oracle):
+    # x0 + x1 <= 4, x0 in [-inf, 3], x1 in [1, 10]: min activity has one
+    # infinite contribution (x0's).  x0 is that contributor: its bound is
+    # derived from the finite rest (4 - 1 = 3, no change); x1 cannot be
+    # bounded.
+    t, nb = oracle.tighten_upper(1.0, 4.0 - 1.0, 0.0, 1, -INF, 3.0, True)
+    assert not t and nb == 3.0
+    t, nb = oracle.tighten_upper(1.0, 4.0 - 1.0, 0.0, 1, -INF, 5.0, True)
+    assert t and nb == 3.0
+    t, nb = oracle.tighten_upper(1.0, 4.0 - 1.0, 0.0, 1, 1.0, 10.0, True)
+    assert not t
+
+
+# ---- fixpoints with known answers ------------------------------------------
+
+@pytest.mark.parametrize("name", fixtures.ANALYTIC)
+@pytest.mark.parametrize("driver", ["seq", "jacobi", "seq_disjoint"])
+def test_analytic_fixpoints(oracle, name, driver):
+    inst = fixtures.analytic(name)
+    lb, ub, rounds, infeasible, capped = getattr(oracle, driver)(inst)
+    assert not infeasible and not capped
+    np.testing.assert_allclose(lb, inst.meta["lb_expected"], atol=1e-4, rtol=0)
+    np.testing.assert_allclose(ub, inst.meta["ub_expected"], atol=1e-4, rtol=0)
+
+
+def test_gs_rounds_fewer_or_equal(oracle):
+    # Gauss-Seidel sees this round's changes; round-synchronous does not.
+    for name in fixtures.ANALYTIC:
+        inst = fixtures.analytic(name)
+        assert oracle.seq(inst)[2] <= oracle.jacobi(inst)[2]
+
+
+@pytest.mark.parametrize("k", [3, 17, 40])
+def test_generalized_cfp_closed_form(oracle, k):
+    inst = fixtures.generalized_cfp(k)
+    for driver in (oracle.seq, oracle.jacobi):
+        lb, ub, rounds, infeasible, capped = driver(inst)
+        assert not infeasible and not capped
+        np.testing.assert_array_equal(ub, inst.meta["ub_expected"])
+        np.testing.assert_array_equal(lb, inst.meta["lb_expected"])
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_single_row_closed_form(oracle, seed):
+    """One row sum_j a_j x_j <= b with finite bounds: the fixpoint is the
+    textbook residual bound, reached in one round (plus the no-change round)."""
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(2, 9))
+    a = rng.choice([-4.0, -2.0, -1.0, 1.0, 3.0, 5.0], size=n)
+    lb = rng.integers(-5, 1, size=n).astype(float)
+    ub = lb + rng.integers(1, 8, size=n)
+    is_int = rng.random(n) < 0.5
+    minact = float(np.sum(np.where(a > 0, a * lb, a * ub)))
+    b = minact + float(rng.integers(0, 6))
+    inst = from_rows(1, n, [list(range(n))], [a.tolist()], [-INF], [b], lb, ub, np.where(is_int, 1, 3))
+    exp_lb, exp_ub = lb.copy(), ub.copy()
+    for j in range(n):
+        resid = minact - (a[j] * lb[j] if a[j] > 0 else a[j] * ub[j])
+        if a[j] > 0:
+            cand = (b - resid) / a[j]
+            cand = np.floor(cand + 1e-9) if is_int[j] else cand
+            if cand < ub[j] - 1e-9:
+                exp_ub[j] = cand
+        else:
+            cand = (b - resid) / a[j]
+            cand = np.ceil(cand - 1e-9) if is_int[j] else cand
+            if cand > lb[j] + 1e-9:
+                exp_lb[j] = cand
+    for driver in (oracle.seq, oracle.jacobi):
+        got_lb, got_ub, rounds, infeasible, _ = driver(inst)
+        np.testing.assert_allclose(got_lb, exp_lb, rtol=0, atol=1e-12)
+        np.testing.assert_allclose(got_ub, exp_ub, rtol=0, atol=1e-12)
+        changed = np.any(exp_lb != lb) or np.any(exp_ub != ub)
+        assert rounds == (2 if changed else 1)
+
+
+@pytest.mark.parametrize("seed", range(40))
+def test_bruteforce_soundness(oracle, seed):
+    inst = fixtures.tiny_integer(seed)
+    boxes = [range(int(inst.lb[j]), int(inst.ub[j]) + 1) for j in range(inst.n)]
+    feasible = []
+    rp, col, val = inst.row_ptr, inst.col, inst.val
+    for x in itertools.product(*boxes):
+        ok = True
+        for i in range(inst.m):
+            act = sum(val[k] * x[col[k]] for k in range(rp[i], rp[i + 1]))
+            if act < inst.lhs[i] - 1e-9 or act > inst.rhs[i] + 1e-9:
+                ok = False
+                break
+        if ok:
+            feasible.append(x)
+    for driver in (oracle.seq, oracle.jacobi):
+        lb, ub, rounds, infeasible, capped = driver(inst)
+        if not feasible:
+            continue  # propagation need not detect every infeasible box
+        assert not infeasible
+        pts = np.array(feasible, float)
+        assert np.all(pts >= lb - 1e-9) and np.all(pts <= ub + 1e-9), "a feasible point was cut off"
+
+
+def test_infeasible_detected(oracle):
+    inst = fixtures.infeasible_instance()
+    for driver in (oracle.seq, oracle.jacobi):
+        lb, ub, rounds, infeasible, capped = driver(inst)
+        assert infeasible and rounds == 1
+        assert np.any(lb > ub + 1e-9)
+    # without detection the reference's behaviour (run on) is kept
+    lb, ub, rounds, infeasible, capped = oracle.seq(inst, detect_infeasible=False)
+    assert not infeasible
+
+
+# ---- invariants -------------------------------------------------------------
+
+@pytest.mark.parametrize("seed", range(6))
+def test_gs_and_jacobi_agree_on_integer_systems(oracle, seed):
+    inst = small_mip(seed=seed, m=400, n=200, int_frac=1.0)
+    a = oracle.seq(inst)
+    b = oracle.jacobi(inst)
+    assert a[3] == b[3]
+    if not a[3] and not a[4] and not b[4]:
+        np.testing.assert_array_equal(a[0], b[0])
+        np.testing.assert_array_equal(a[1], b[1])
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_fixpoint_idempotent(oracle, seed):
+    inst = small_mip(seed=seed)
+    for driver in (oracle.seq, oracle.jacobi):
+        lb, ub, rounds, infeasible, capped = driver(inst)
+        if infeasible or capped:
+            continue
+        lb2, ub2, rounds2, _, _ = driver(inst, lb=lb, ub=ub)
+        assert rounds2 == 1
+        np.testing.assert_array_equal(lb2, lb)
+        np.testing.assert_array_equal(ub2, ub)
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_fixpoint_permutation_invariant(oracle, seed):
+    inst = small_mip(seed=100 + seed, m=500, n=250, int_frac=1.0)
+    rng = np.random.default_rng(seed)
+    rperm = rng.permutation(inst.m)
+    cperm = rng.permutation(inst.n)       # new column c' = cperm[c]
+    rows, vals = [], []
+    for i in rperm:
+        s, e = inst.row_ptr[i], inst.row_ptr[i + 1]
+        rows.append(cperm[inst.col[s:e]].tolist())
+        vals.append(inst.val[s:e].tolist())
+    inv = np.empty_like(cperm)
+    inv[cperm] = np.arange(inst.n)
+    p = from_rows(inst.m, inst.n, rows, vals, inst.lhs[rperm], inst.rhs[rperm], inst.lb[inv], inst.ub[inv],
+                  inst.vartype[inv])
+    a = oracle.seq(inst)
+    b = oracle.seq(p)
+    assert a[3] == b[3]
+    if not a[3]:
+        np.testing.assert_array_equal(a[0], b[0][cperm])
+        np.testing.assert_array_equal(a[1], b[1][cperm])
+
+
+# ---- canonical summation order -------------------------------------------
+
+def _bottom_up(x, seg):
+    """Independent statement of the canonical order: pad each segment to a
+    power of two with zeros, add neighbours level by level, then add the
+    segment sums left to right starting from +0.0."""
+    acc = 0.0
+    for s in range(0, len(x), seg):
+        part = list(x[s:s + seg])
+        w = 1
+        while w < len(part):
+            w *= 2
+        part += [0.0] * (w - len(part))
+        while len(part) > 1:
+            part = [part[i] + part[i + 1] for i in range(0, len(part), 2)]
+        acc = acc + part[0]
+    return acc
+
+
+def test_canonical_sum_bottom_up(oracle):
+    rng = np.random.default_rng(7)
+    for n in list(range(1, 70)) + [511, 512, 513, 1000, 1024, 1025, 3000]:
+        x = rng.standard_normal(n) * 10 ** rng.uniform(-3, 12, size=n)
+        for seg in (512, 64, 7):
+            assert oracle.canonical_sum(x, seg) == _bottom_up(x, seg)
+
+
+def test_canonical_sum_order_matters(oracle):
+    # pairwise: (1e16 + 1) + (-1e16 + 1) = 0; left to right it would be 1
+    x = [1e16, 1.0, -1e16, 1.0]
+    assert oracle.canonical_sum(x) == 0.0
+    assert ((x[0] + x[1]) + x[2]) + x[3] == 1.0
+    # segments are added left to right: with seg=2 the result is 1e16 + ... = 0 too,
+    # with seg=3 the first segment is (1e16+1)+(-1e16) = 0 padded, then + 1
+    assert oracle.canonical_sum(x, 3) == 1.0
+    # integers: exact whatever the order
+    ints = np.arange(-500, 700, dtype=float)
+    assert oracle.canonical_sum(ints) == float(ints.sum())
+    assert oracle.canonical_sum([]) == 0.0
+    assert np.signbit(oracle.canonical_sum([-0.0])) == False  # noqa: E712  (canonical +0.0)
+
+
+def test_canonical_activities_match_sequential_on_integers(oracle):
+    inst = small_mip(seed=3)
+    for i in range(0, inst.m, 37):
+        a, ia = oracle.row_activities(inst, i, canonical=False)
+        b, ib = oracle.row_activities(inst, i, canonical=True)
+        np.testing.assert_array_equal(a, b)
+        np.testing.assert_array_equal(ia, ib)
