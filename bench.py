@@ -77,7 +77,6 @@ def main():
         sys.exit(2)
 
     import numpy as np
-    import torch
 
     import gdp_amd
     from gdp_amd import interface as I
@@ -88,28 +87,26 @@ def main():
     inst = powerlaw_mip(seed=args.seed)
     print(f"bench: m={inst.m} n={inst.n} nnz={inst.nnz}", file=sys.stderr, flush=True)
 
-    dev = torch.device("cuda:0")
-    root_lb = torch.tensor(inst.lb, dtype=torch.float64, device=dev)
-    root_ub = torch.tensor(inst.ub, dtype=torch.float64, device=dev)
-    lb = root_lb.clone()
-    ub = root_ub.clone()
+    # Device buffers come from the library's own HIP runtime (a framework
+    # with a bundled ROCm runtime cannot share the process with it).
+    print(f"bench: device {I.device_name()}", file=sys.stderr, flush=True)
+    root_lb = I.DeviceArray(inst.lb)
+    root_ub = I.DeviceArray(inst.ub)
+    lb = I.DeviceArray(inst.lb)
+    ub = I.DeviceArray(inst.ub)
     prob = I.Problem(inst)
-    torch.cuda.synchronize()
 
     def step():
-        lb.copy_(root_lb)
-        ub.copy_(root_ub)
-        torch.cuda.synchronize()
-        return prob.propagate_device(lb.data_ptr(), ub.data_ptr(), args.sync)
+        lb.copy_from(root_lb)   # synchronous device-to-device copies
+        ub.copy_from(root_ub)
+        return prob.propagate_device(lb.ptr, ub.ptr, args.sync)  # returns after the device finished
 
     for w in range(args.warmup):
         info = step()
         print(f"bench: warmup {w}: rounds={info.rounds} kernel_ms={info.kernel_ms:.3f}", file=sys.stderr, flush=True)
 
-    torch.cuda.synchronize()
     t0 = time.perf_counter()
     infos = [step() for _ in range(args.steps)]
-    torch.cuda.synchronize()
     t1 = time.perf_counter()
 
     rounds = infos[-1].rounds
@@ -123,7 +120,7 @@ def main():
           f"{achieved:.1f} GB/s algorithmic", file=sys.stderr, flush=True)
 
     # parity spot check of the timed result against the oracle (cheap: one run)
-    out_lb, out_ub = lb.cpu().numpy(), ub.cpu().numpy()
+    out_lb, out_ub = lb.numpy(), ub.numpy()
 
     result = {
         "metric": METRIC,

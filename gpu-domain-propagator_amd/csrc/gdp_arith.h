@@ -29,7 +29,7 @@ constexpr int kVarContinuous = 3;
 // segment (pairwise tree over the entries of a segment in natural order,
 // segment sums added left to right from +0.0).  A format constant shared by
 // every kernel and the oracle.
-constexpr int kSegLen = 512;
+constexpr int kSegLen = 256;
 
 GDP_HD bool eps_le(double x, double y) { return (x - y) <= kEps; }
 GDP_HD bool eps_lt(double x, double y) { return (x - y) < -kEps; }

@@ -19,7 +19,14 @@ namespace gdp {
 
 constexpr int kColMask = 0x7fffffff;
 constexpr int kItemGroup = 0;
-constexpr int kItemLong = 1;
+constexpr int kItemLong = 1;   // one row, one wave, all segments (very long rows)
+constexpr int kItemSeg = 2;    // one segment of a split row
+
+// Rows longer than kSegLen are split into one item per segment when they
+// have at most this many segments; longer rows go to single-wave items.
+// Segment waves of one row wait for each other, which is deadlock-free as
+// long as this bound is below the number of co-resident waves.
+constexpr int kMaxSplitSegs = 512;
 
 // status words: row 0 holds the outcome, row r >= 1 the flags of round r
 constexpr int kStatusWords = 8;
@@ -36,6 +43,18 @@ constexpr int kStatusRows = 128;  // rounds 1..101 fit
 // wall_clock64 runs at 100 MHz on CDNA; give up a barrier wait after 1 s
 constexpr long long kBarrierTimeoutTicks = 100000000LL;
 
+// Partial activity of one segment of a split row.
+struct SegPart {
+    double smin, smax, dmax;
+    int mininf, maxinf;
+};
+
+// Activity and gate of a split row, published by the last segment wave.
+struct SegState {
+    double minact, maxact;
+    int mininf, maxinf, gate, pad;
+};
+
 struct DevMatrix {
     int m;            // nonempty rows
     int n;            // columns
@@ -48,6 +67,9 @@ struct DevMatrix {
     const int *col;
     const double *val;
     const int4 *items;
+    const int4 *seg_rows;  // per split row: {row, first partial, segments, 0}
+    int n_seg_rows;
+    int n_seg_parts;
 };
 
 struct DevState {
@@ -57,6 +79,10 @@ struct DevState {
     unsigned *bm1;
     int *status;        // kStatusRows * kStatusWords
     unsigned *barrier;  // [count, generation]
+    SegPart *seg_parts;    // [n_seg_parts]
+    SegState *seg_state;   // [n_seg_rows]
+    unsigned *seg_count;   // [n_seg_rows] arrivals in the current round
+    int *seg_flag;         // [n_seg_rows] round whose state is published
 };
 
 }  // namespace gdp

@@ -25,8 +25,10 @@ struct Tuning {
     int block_threads = 256;  // threads per workgroup (multiple of 64)
     int mega_blocks_per_cu = 0;  // 0: occupancy limit
     int graph_rounds = 8;     // rounds per captured graph for GPU_LOOP
+    int max_split_segs = 256; // rows with more segments run on one wave (0: never split)
     // per tree-width bin (width 2^b, b = 0..9): lanes per row as log2
-    int group_log[10] = {0, 1, 2, 3, 4, 5, 6, 6, 6, 6};
+    // default: K = min(width, 4) consecutive entries per lane
+    int group_log[10] = {0, 0, 0, 1, 2, 3, 4, 5, 6, 6};
 };
 
 struct RunResult {

@@ -109,14 +109,16 @@ Problem *problem_create(int m, int n, int nnz, const int *col, const int *row_pt
             throw std::runtime_error("block_threads must be 64, 128, 192 or 256");
 
         // ---- group rows by tree width: long rows first, then widest bins ----
-        // bin b < 10 holds rows with next_pow2(len) == 2^b; bin 10 rows longer
-        // than kSegLen.  Long rows lead so that their waves start first.
+        // bin b < kLongBin holds rows with next_pow2(len) == 2^b; bin kLongBin
+        // rows longer than kSegLen.  Long rows lead so that their waves start
+        // first.
+        constexpr int kLongBin = 10;
         std::vector<int> bin_of(m);
         std::vector<std::vector<int>> rows_in_bin(11);
         for (int i = 0; i < m; i++) {
             const int len = row_ptr[i + 1] - row_ptr[i];
             if (len == 0) { bin_of[i] = -1; continue; }
-            const int b = len > kSegLen ? 10 : next_pow2_log(len);
+            const int b = len > kSegLen ? kLongBin : next_pow2_log(len);
             bin_of[i] = b;
             rows_in_bin[b].push_back(i);
             P->stats.bin_rows[b]++;
@@ -147,17 +149,37 @@ Problem *problem_create(int m, int n, int nnz, const int *col, const int *row_pt
         h_rp[mp] = pos;
 
         // ---- work items ----
+        // Order: segments of split rows, whole very long rows, then the
+        // tree-width bins from wide to narrow, so that long work starts first.
         std::vector<int4> h_items;
+        std::vector<int4> h_seg_rows;
+        int n_parts = 0;
         int r = 0;
-        for (int b = 10; b >= 0; b--) {
+        {
+            const int cnt = static_cast<int>(rows_in_bin[10].size());
+            std::vector<int> giant;
+            for (int q = 0; q < cnt; q++) {
+                const int len = h_rp[r + q + 1] - h_rp[r + q];
+                const int nseg = (len + kSegLen - 1) / kSegLen;
+                if (nseg <= std::min(kMaxSplitSegs, T.max_split_segs)) {
+                    const int lr = static_cast<int>(h_seg_rows.size());
+                    h_seg_rows.push_back(make_int4(r + q, n_parts, nseg, 0));
+                    for (int sg = 0; sg < nseg; sg++) h_items.push_back(make_int4(kItemSeg, lr, sg, 0));
+                    n_parts += nseg;
+                } else {
+                    giant.push_back(r + q);
+                }
+            }
+            for (int row : giant) h_items.push_back(make_int4(kItemLong, row, 1, 0));
+            P->stats.n_long_rows = cnt;
+            r += cnt;
+        }
+        for (int b = 9; b >= 0; b--) {
             const int cnt = static_cast<int>(rows_in_bin[b].size());
-            if (b == 10) {
-                for (int q = 0; q < cnt; q++) h_items.push_back(make_int4(kItemLong, r + q, 1, 0));
-                P->stats.n_long_rows = cnt;
-            } else if (cnt > 0) {
+            if (cnt > 0) {
                 const int width = 1 << b;
                 int logg = std::min(T.group_log[b], b);
-                logg = std::max(logg, b - 3);  // at most 8 entries per lane
+                logg = std::max(logg, b - 2);  // at most 4 entries per lane
                 logg = std::min(logg, 6);
                 const int logk = b - logg;
                 const int per_pass = 64 >> logg;
@@ -210,6 +232,13 @@ Problem *problem_create(int m, int n, int nnz, const int *col, const int *row_pt
         M.col = d_col;
         M.val = d_val;
         M.items = d_items;
+        M.n_seg_rows = static_cast<int>(h_seg_rows.size());
+        M.n_seg_parts = n_parts;
+        int4 *d_seg_rows = dev_alloc<int4>(h_seg_rows.size(), P->owned);
+        if (!h_seg_rows.empty())
+            check(hipMemcpyAsync(d_seg_rows, h_seg_rows.data(), h_seg_rows.size() * sizeof(int4),
+                                 hipMemcpyHostToDevice, P->stream), "H2D");
+        M.seg_rows = d_seg_rows;
 
         DevState &D = P->D;
         D.bnd0 = dev_alloc<double2>(n, P->owned);
@@ -218,6 +247,10 @@ Problem *problem_create(int m, int n, int nnz, const int *col, const int *row_pt
         D.bm1 = dev_alloc<unsigned>(nwords, P->owned);
         D.status = dev_alloc<int>(kStatusRows * kStatusWords, P->owned);
         D.barrier = dev_alloc<unsigned>(64, P->owned);
+        D.seg_parts = dev_alloc<SegPart>(n_parts, P->owned);
+        D.seg_state = dev_alloc<SegState>(h_seg_rows.size(), P->owned);
+        D.seg_count = dev_alloc<unsigned>(h_seg_rows.size(), P->owned);
+        D.seg_flag = dev_alloc<int>(h_seg_rows.size(), P->owned);
 
         int per_cu = 0;
         check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(mega_kernel),
@@ -225,6 +258,10 @@ Problem *problem_create(int m, int n, int nnz, const int *col, const int *row_pt
               "occupancy");
         if (T.mega_blocks_per_cu > 0) per_cu = std::min(per_cu, T.mega_blocks_per_cu);
         P->mega_grid = std::max(1, per_cu) * P->cu_count;
+        // split rows wait for their sibling segments: all of them must be able
+        // to be resident at once, with room to spare
+        if (P->mega_grid * 4 < 4 * kMaxSplitSegs)
+            throw std::runtime_error("device too small for split-row items");
         check(hipStreamSynchronize(P->stream), "upload");
     } catch (...) {
         delete P;
@@ -298,10 +335,12 @@ RunResult problem_propagate_device(Problem *P, double *d_lbs, double *d_ubs, int
         for (int round = 1; round <= max_rounds; round++) {
             hipLaunchKernelGGL(round_kernel, dim3(grid), dim3(256), 0, s, M, D, round, 0);
             check(hipGetLastError(), "round_kernel launch");
-            int flags[kStatusWords];
+            int flags[kStatusWords], head[kStatusWords];
             check(hipMemcpyAsync(flags, D.status + kStatusWords * round, sizeof(flags), hipMemcpyDeviceToHost, s),
                   "flags D2H");
+            check(hipMemcpyAsync(head, D.status, sizeof(head), hipMemcpyDeviceToHost, s), "flags D2H");
             check(hipStreamSynchronize(s), "round sync");
+            if (head[kGlobalError]) throw std::runtime_error("device wait timed out");
             std::memcpy(st.data() + kStatusWords * round, flags, sizeof(flags));
             if (flags[kStatusInfeasPrev] || !flags[kStatusChanged]) break;
         }
@@ -339,6 +378,7 @@ RunResult problem_propagate_device(Problem *P, double *d_lbs, double *d_ubs, int
         }
         check(hipMemcpyAsync(st.data(), D.status, st.size() * sizeof(int), hipMemcpyDeviceToHost, s), "status D2H");
         check(hipStreamSynchronize(s), "status sync");
+        if (st[kGlobalError]) throw std::runtime_error("device wait timed out");
     } else {
         void *args[] = {const_cast<DevMatrix *>(&M), const_cast<DevState *>(&D), &max_rounds};
         check(hipLaunchCooperativeKernel(reinterpret_cast<const void *>(mega_kernel), dim3(P->mega_grid), dim3(256),

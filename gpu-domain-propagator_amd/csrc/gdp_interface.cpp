@@ -11,6 +11,8 @@
 #include <string>
 #include <vector>
 
+#include <hip/hip_runtime_api.h>
+
 #include "../../include/gdp_ext.h"
 #include "../../include/interface.h"
 #include "gdp_engine.h"
@@ -243,6 +245,7 @@ void gdpTuningDefaults(gdp_tuning *t) {
     t->item_nnz = d.item_nnz;
     t->mega_blocks_per_cu = d.mega_blocks_per_cu;
     t->graph_rounds = d.graph_rounds;
+    t->max_split_segs = d.max_split_segs;
     for (int b = 0; b < 10; b++) t->group_log[b] = d.group_log[b];
 }
 
@@ -257,6 +260,7 @@ GDP_RETCODE gdpProblemCreate(const int n_cons, const int n_vars, const int nnz, 
             if (tuning->item_nnz > 0) T.item_nnz = tuning->item_nnz;
             if (tuning->mega_blocks_per_cu > 0) T.mega_blocks_per_cu = tuning->mega_blocks_per_cu;
             if (tuning->graph_rounds > 0) T.graph_rounds = tuning->graph_rounds;
+            if (tuning->max_split_segs >= 0) T.max_split_segs = tuning->max_split_segs;
             for (int b = 0; b < 10; b++)
                 if (tuning->group_log[b] >= 0) T.group_log[b] = tuning->group_log[b];
         }
@@ -330,6 +334,59 @@ GDP_RETCODE gdpOMPPropagate(const int n_cons, const int n_vars, const int nnz, c
                             gdp_run_info *info) {
     return host_double(2, n_cons, n_vars, nnz, col_indices, row_ptrs, vals, lhss, rhss, lbs, ubs, vartypes,
                        max_rounds, threads, info);
+}
+
+namespace {
+GDP_RETCODE hip_rc(hipError_t e, const char *what) {
+    if (e == hipSuccess) return GDP_OKAY;
+    gdp::set_last_error(std::string(what) + ": " + hipGetErrorString(e));
+    return GDP_ERROR;
+}
+}  // namespace
+
+GDP_RETCODE gdpDeviceCount(int *count) {
+    if (!count) return GDP_ERROR;
+    *count = 0;
+    const hipError_t e = hipGetDeviceCount(count);
+    if (e == hipErrorNoDevice) { *count = 0; return GDP_OKAY; }
+    return hip_rc(e, "hipGetDeviceCount");
+}
+
+GDP_RETCODE gdpDeviceMalloc(void **ptr, const size_t bytes) {
+    if (!ptr) return GDP_ERROR;
+    return hip_rc(hipMalloc(ptr, bytes ? bytes : 1), "hipMalloc");
+}
+
+GDP_RETCODE gdpDeviceFree(void *ptr) { return hip_rc(hipFree(ptr), "hipFree"); }
+
+// hipMemcpy may return before a device-to-device copy has finished; these
+// helpers promise completion, since the propagation runs on its own stream.
+GDP_RETCODE gdpMemcpyHtoD(void *dst, const void *src, const size_t bytes) {
+    const GDP_RETCODE rc = hip_rc(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice), "hipMemcpy H2D");
+    return rc == GDP_OKAY ? hip_rc(hipDeviceSynchronize(), "hipDeviceSynchronize") : rc;
+}
+
+GDP_RETCODE gdpMemcpyDtoH(void *dst, const void *src, const size_t bytes) {
+    const GDP_RETCODE rc = hip_rc(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+    return rc == GDP_OKAY ? hip_rc(hipDeviceSynchronize(), "hipDeviceSynchronize") : rc;
+}
+
+GDP_RETCODE gdpMemcpyDtoD(void *dst, const void *src, const size_t bytes) {
+    const GDP_RETCODE rc = hip_rc(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice), "hipMemcpy D2D");
+    return rc == GDP_OKAY ? hip_rc(hipDeviceSynchronize(), "hipDeviceSynchronize") : rc;
+}
+
+GDP_RETCODE gdpDeviceName(char *buf, const int len) {
+    if (!buf || len <= 0) return GDP_ERROR;
+    int dev = 0;
+    hipDeviceProp_t prop;
+    GDP_RETCODE rc = hip_rc(hipGetDevice(&dev), "hipGetDevice");
+    if (rc != GDP_OKAY) return rc;
+    rc = hip_rc(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
+    if (rc != GDP_OKAY) return rc;
+    std::snprintf(buf, static_cast<size_t>(len), "%s %s cu=%d", prop.name, prop.gcnArchName,
+                  prop.multiProcessorCount);
+    return GDP_OKAY;
 }
 
 const char *gdpLastError(void) {

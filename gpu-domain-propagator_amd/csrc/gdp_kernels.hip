@@ -47,6 +47,30 @@ __device__ __forceinline__ void atomic_or_i(int *p, int v) {
     __hip_atomic_fetch_or(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Device-coherent plain accesses.  Relaxed agent-scope atomic loads and
+// stores are single memory operations marked device-coherent (no whole-cache
+// write-back or invalidate, unlike acquire/release fences at agent scope,
+// which on this target maintain the entire L2).  Data exchanged between
+// waves of one launch uses these; ordering comes from waiting for the
+// wave's outstanding memory operations (mem_wait) before the operation that
+// publishes them.
+template <typename T>
+__device__ __forceinline__ T coherent_load(const T *p) {
+    return __hip_atomic_load(const_cast<T *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ void coherent_store(T *p, T v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double coherent_load_d(const double *p) {
+    return __longlong_as_double(coherent_load(reinterpret_cast<const long long *>(p)));
+}
+__device__ __forceinline__ void coherent_store_d(double *p, double v) {
+    coherent_store(reinterpret_cast<long long *>(p), __double_as_longlong(v));
+}
+// waits until every memory operation this wave issued has completed
+__device__ __forceinline__ void mem_wait() { __builtin_amdgcn_s_waitcnt(0); }
+
 // Row gate with the sequential propagator's epsilon-maximum of |a|(u-l).
 // That maximum m depends on the order of the entries but always lies in
 // [lo, M] with M the exact maximum (computed in parallel, order-free) and
@@ -244,20 +268,165 @@ __device__ bool long_item(const DevMatrix &M, const RoundCtx &c, int r) {
     return moved;
 }
 
+// One segment of a split row.  The wave reduces its kSegLen entries exactly
+// like a 512-wide group row, stores the partial, and counts itself in.  The
+// last wave of the row to arrive adds the partials in segment order, decides
+// the row gate and publishes the row's state; the others wait for it (their
+// entries stay in registers) and then apply their candidates.
+__device__ bool seg_item(const DevMatrix &M, const DevState &D, const RoundCtx &c, int lr, int s, int round) {
+    constexpr int K = kSegLen / 64;
+    const int lane = threadIdx.x & 63;
+    const int4 info = M.seg_rows[lr];
+    const int r = info.x;
+    const int start = M.row_ptr[r];
+    const int len = M.row_ptr[r + 1] - start;
+    const int s0 = s * kSegLen;
+    int jp[K];
+    double a[K], l[K], u[K], smin[K], smax[K];
+    int mininf = 0, maxinf = 0;
+    double dmax = 0.0;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int e = s0 + lane * K + k;
+        smin[k] = 0.0;
+        smax[k] = 0.0;
+        jp[k] = 0;
+        a[k] = 0.0;
+        l[k] = 0.0;
+        u[k] = 0.0;
+        if (e < len) {
+            jp[k] = M.col[start + e];
+            a[k] = M.val[start + e];
+            const double2 b = c.S[jp[k] & kColMask];
+            l[k] = b.x;
+            u[k] = b.y;
+            const Contribution q = contribution(a[k], l[k], u[k]);
+            smin[k] = q.min_part;
+            smax[k] = q.max_part;
+            mininf += q.min_inf;
+            maxinf += q.max_inf;
+            dmax = fmax(dmax, q.delta);
+        }
+    }
+#pragma unroll
+    for (int w = 1; w < K; w <<= 1) {
+#pragma unroll
+        for (int k = 0; k < K; k += 2 * w) {
+            smin[k] = smin[k] + smin[k + w];
+            smax[k] = smax[k] + smax[k + w];
+        }
+    }
+    double tmin = smin[0], tmax = smax[0];
+#pragma unroll
+    for (int h = 1; h < 64; h <<= 1) {
+        tmin = tmin + shfl_xor_d(tmin, h);
+        tmax = tmax + shfl_xor_d(tmax, h);
+        mininf += __shfl_xor(mininf, h, 64);
+        maxinf += __shfl_xor(maxinf, h, 64);
+        dmax = fmax(dmax, shfl_xor_d(dmax, h));
+    }
+    int ok = 1;
+    unsigned old = 0;
+    SegPart *part = &D.seg_parts[info.y + s];
+    if (lane == 0) {
+        coherent_store_d(&part->smin, tmin);
+        coherent_store_d(&part->smax, tmax);
+        coherent_store_d(&part->dmax, dmax);
+        coherent_store(&part->mininf, mininf);
+        coherent_store(&part->maxinf, maxinf);
+        mem_wait();
+        old = __hip_atomic_fetch_add(&D.seg_count[lr], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    old = __shfl(old, 0, 64);
+    SegState *stp = &D.seg_state[lr];
+    if (old == static_cast<unsigned>(info.z - 1)) {
+        // last arrival: the lanes load 64 partials at a time (independent
+        // loads), which are then added in segment order
+        double am = 0.0, ax = 0.0, dm = 0.0;
+        int ni = 0, xi = 0;
+        for (int base = 0; base < info.z; base += 64) {
+            const int q = base + lane;
+            double pmin = 0.0, pmax = 0.0;
+            if (q < info.z) {
+                const SegPart *pq = &D.seg_parts[info.y + q];
+                pmin = coherent_load_d(&pq->smin);
+                pmax = coherent_load_d(&pq->smax);
+                dm = fmax(dm, coherent_load_d(&pq->dmax));
+                ni += coherent_load(&pq->mininf);
+                xi += coherent_load(&pq->maxinf);
+            }
+            const int cnt = min(64, info.z - base);
+            for (int t = 0; t < cnt; t++) {
+                am = am + __shfl(pmin, t, 64);
+                ax = ax + __shfl(pmax, t, 64);
+            }
+        }
+#pragma unroll
+        for (int h = 1; h < 64; h <<= 1) {
+            ni += __shfl_xor(ni, h, 64);
+            xi += __shfl_xor(xi, h, 64);
+            dm = fmax(dm, shfl_xor_d(dm, h));
+        }
+        if (lane == 0) {
+            const double2 sd = M.sides[r];
+            const int gate = decide_gate(M, c.S, start, len, am, ax, ni, xi, sd.x, sd.y, dm) ? 1 : 0;
+            coherent_store_d(&stp->minact, am);
+            coherent_store_d(&stp->maxact, ax);
+            coherent_store(&stp->mininf, ni);
+            coherent_store(&stp->maxinf, xi);
+            coherent_store(&stp->gate, gate);
+            coherent_store(&D.seg_count[lr], 0u);
+            mem_wait();
+            coherent_store(&D.seg_flag[lr], round);
+        }
+    } else if (lane == 0) {
+        const long long t0 = wall_clock64();
+        while (coherent_load(&D.seg_flag[lr]) != round) {
+            __builtin_amdgcn_s_sleep(2);
+            if (coherent_load(&D.status[kGlobalError]) || wall_clock64() - t0 > kBarrierTimeoutTicks) {
+                atomic_or_i(&D.status[kGlobalError], 1);
+                ok = 0;
+                break;
+            }
+        }
+    }
+    ok = __shfl(ok, 0, 64);
+    if (!ok) return false;
+    SegState st;
+    st.minact = coherent_load_d(&stp->minact);
+    st.maxact = coherent_load_d(&stp->maxact);
+    st.mininf = coherent_load(&stp->mininf);
+    st.maxinf = coherent_load(&stp->maxinf);
+    st.gate = coherent_load(&stp->gate);
+    bool moved = false;
+    if (st.gate) {
+        const double2 sd = M.sides[r];
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            if (s0 + lane * K + k < len)
+                moved |= apply_entry(c, jp[k], a[k], l[k], u[k], sd.x, sd.y, st.minact, st.maxact, st.mininf,
+                                     st.maxinf);
+        }
+    }
+    return moved;
+}
+
 // Folds the previous round's result into the accumulator for the variables
 // it changed, checks them for crossed bounds and clears the bitmap words.
 __device__ void fixup_item(const DevMatrix &M, const RoundCtx &c, int f) {
     const int w = f * 64 + (threadIdx.x & 63);
     if (w >= M.nwords) return;
-    unsigned bits = c.bm_prev[w];
+    unsigned bits = coherent_load(&c.bm_prev[w]);
     if (bits == 0) return;
-    c.bm_prev[w] = 0u;
+    coherent_store(&c.bm_prev[w], 0u);
     bool crossed = false;
     while (bits) {
         const int b = __builtin_ctz(bits);
         bits &= bits - 1;
         const int j = w * 32 + b;
-        const double2 s = c.S[j];
+        double2 s;
+        s.x = coherent_load_d(&c.S[j].x);
+        s.y = coherent_load_d(&c.S[j].y);
         crossed |= eps_gt(s.x, s.y);
         atomic_max_d(&c.A[j].x, s.x);
         atomic_min_d(&c.A[j].y, s.y);
@@ -277,21 +446,21 @@ __device__ __forceinline__ RoundCtx round_ctx(const DevState &D, int round) {
 }
 
 // Processes work item `it` of a round (row items, then fixup items).
-__device__ bool process_item(const DevMatrix &M, const RoundCtx &c, int it, bool rows) {
-    if (rows && it < M.n_items) {
+__device__ bool process_item(const DevMatrix &M, const DevState &D, const RoundCtx &c, int it, int round) {
+    if (it < M.n_items) {
         const int4 d = M.items[it];
         const int kind = d.x & 15;
+        if (kind == kItemSeg) return seg_item(M, D, c, d.y, d.z, round);
         if (kind == kItemLong) return long_item(M, c, d.y);
         const int logg = (d.x >> 4) & 15;
         const int logk = (d.x >> 8) & 15;
         switch (logk) {
             case 0: return group_item<1>(M, c, d.y, d.z, logg);
             case 1: return group_item<2>(M, c, d.y, d.z, logg);
-            case 2: return group_item<4>(M, c, d.y, d.z, logg);
-            default: return group_item<8>(M, c, d.y, d.z, logg);
+            default: return group_item<4>(M, c, d.y, d.z, logg);
         }
     }
-    const int f = rows ? it - M.n_items : it;
+    const int f = it - M.n_items;
     if (f < M.n_fix_items) fixup_item(M, c, f);
     return false;
 }
@@ -314,7 +483,7 @@ __global__ __launch_bounds__(256) void round_kernel(DevMatrix M, DevState D, int
     const int waves = blockDim.x >> 6;
     const int it = blockIdx.x * waves + (threadIdx.x >> 6);
     bool moved = false;
-    if (it < M.n_items + M.n_fix_items) moved = process_item(M, c, it, true);
+    if (it < M.n_items + M.n_fix_items) moved = process_item(M, D, c, it, round);
     flag_change(c, moved);
 }
 
@@ -327,26 +496,31 @@ __global__ __launch_bounds__(256) void fixup_kernel(DevMatrix M, DevState D, int
 }
 
 // Device-wide barrier for the persistent kernel: arrival counter plus a
-// generation word; waiting is bounded by wall-clock time so that a fault
-// elsewhere ends in an error flag instead of a hang.
+// generation word, waiting bounded by wall-clock time so that a fault
+// elsewhere ends in an error flag instead of a hang.  Everything the rounds
+// exchange is written with atomics or device-coherent stores, so no cache
+// write-back is needed; each workgroup invalidates its caches once after the
+// barrier so that the next round's gathers of the snapshot buffer cannot hit
+// lines cached while that buffer was last read.
 __device__ bool grid_barrier(const DevState &D, unsigned nblocks) {
     __shared__ int ok_sh;
+    mem_wait();
     __syncthreads();
     if (threadIdx.x == 0) {
         int ok = 1;
         unsigned *count = D.barrier;
         unsigned *gen = D.barrier + 1;
-        const unsigned g0 = __hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        const unsigned arrived = __hip_atomic_fetch_add(count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned g0 = coherent_load(gen);
+        const unsigned arrived = __hip_atomic_fetch_add(count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (arrived == nblocks - 1) {
-            __hip_atomic_store(count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_fetch_add(gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            coherent_store(count, 0u);
+            mem_wait();
+            __hip_atomic_fetch_add(gen, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         } else {
             const long long t0 = wall_clock64();
-            while (__hip_atomic_load(gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g0) {
+            while (coherent_load(gen) == g0) {
                 __builtin_amdgcn_s_sleep(1);
-                if (__hip_atomic_load(&D.status[kGlobalError], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                if (coherent_load(&D.status[kGlobalError])) {
                     ok = 0;
                     break;
                 }
@@ -364,9 +538,7 @@ __device__ bool grid_barrier(const DevState &D, unsigned nblocks) {
     return ok_sh != 0;
 }
 
-__device__ __forceinline__ int load_status(const int *p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
+__device__ __forceinline__ int load_status(const int *p) { return coherent_load(p); }
 
 // All rounds in one persistent launch (MEGAKERNEL).  Every workgroup walks
 // the items of a round with a grid stride, then all meet at a barrier and
@@ -382,7 +554,7 @@ __global__ __launch_bounds__(256) void mega_kernel(DevMatrix M, DevState D, int 
     for (int round = 1; round <= max_rounds; round++) {
         const RoundCtx c = round_ctx(D, round);
         bool moved = false;
-        for (int it = wave; it < total; it += stride) moved |= process_item(M, c, it, true);
+        for (int it = wave; it < total; it += stride) moved |= process_item(M, D, c, it, round);
         flag_change(c, moved);
         if (!grid_barrier(D, nblocks)) { rounds = -1; break; }
         const int *st = D.status + kStatusWords * round;
@@ -431,6 +603,10 @@ __global__ __launch_bounds__(256) void init_kernel(DevMatrix M, DevState D, cons
         D.bm1[w] = 0u;
     }
     for (int k = tid; k < kStatusRows * kStatusWords; k += stride) D.status[k] = 0;
+    for (int k = tid; k < M.n_seg_rows; k += stride) {
+        D.seg_count[k] = 0u;
+        D.seg_flag[k] = 0;
+    }
     if (tid < 64) D.barrier[tid] = 0u;
 }
 

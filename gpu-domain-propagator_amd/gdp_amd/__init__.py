@@ -5,9 +5,10 @@ include/interface.h and include/gdp_ext.h); this package binds it with ctypes
 and provides the synthetic workloads and an MPS reader.
 """
 from .instances import Instance  # noqa: F401
+from . import interface  # noqa: F401
 from .interface import (  # noqa: F401
     CPU_LOOP, GPU_LOOP, MEGAKERNEL, GDP_OKAY, GDP_ERROR, GDP_NOTIMPLEMENTED, GDP_BINARY, GDP_INTEGER,
     GDP_CONTINUOUS, MAX_NUM_ROUNDS, GDP_INF, GDP_EPS, LIB_PATH, LEGACY_SYMBOLS, EXT_SYMBOLS, Problem, Outcome,
-    load_library, gpu_propagate, host_propagate, propagateGPUAtomic, propagateGPUReduction,
+    load_library, device_count, device_name, DeviceArray, gpu_propagate, host_propagate, propagateGPUAtomic, propagateGPUReduction,
     propagateSequential, propagateFullOMP, propagateSequentialDisjoint,
 )

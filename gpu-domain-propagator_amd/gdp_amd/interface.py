@@ -58,6 +58,13 @@ EXT_SYMBOLS = (
     "gdpPropagateDevice",
     "gdpSequentialPropagate",
     "gdpOMPPropagate",
+    "gdpDeviceCount",
+    "gdpDeviceMalloc",
+    "gdpDeviceFree",
+    "gdpMemcpyHtoD",
+    "gdpMemcpyDtoH",
+    "gdpMemcpyDtoD",
+    "gdpDeviceName",
     "gdpLastError",
     "gdpVersion",
 )
@@ -72,6 +79,7 @@ class Tuning(ctypes.Structure):
         ("item_nnz", c_int),
         ("mega_blocks_per_cu", c_int),
         ("graph_rounds", c_int),
+        ("max_split_segs", c_int),
         ("group_log", c_int * 10),
     ]
 
@@ -128,6 +136,12 @@ def load_library(path: Optional[str] = None) -> ctypes.CDLL:
     lib.gdpOMPPropagate.argtypes = [c_int, c_int, c_int, vp, vp, vp, vp, vp, vp, vp, vp, c_int, c_int,
                                     POINTER(RunInfo)]
     lib.gdpOMPPropagate.restype = c_int
+    lib.gdpDeviceCount.argtypes = [POINTER(c_int)]
+    lib.gdpDeviceMalloc.argtypes = [POINTER(vp), ctypes.c_size_t]
+    lib.gdpDeviceFree.argtypes = [vp]
+    for f in ("gdpMemcpyHtoD", "gdpMemcpyDtoH", "gdpMemcpyDtoD"):
+        getattr(lib, f).argtypes = [vp, vp, ctypes.c_size_t]
+    lib.gdpDeviceName.argtypes = [ctypes.c_char_p, c_int]
     lib.gdpLastError.argtypes = []
     lib.gdpLastError.restype = ctypes.c_char_p
     lib.gdpVersion.argtypes = []
@@ -139,6 +153,64 @@ def load_library(path: Optional[str] = None) -> ctypes.CDLL:
 
 def last_error() -> str:
     return load_library().gdpLastError().decode()
+
+
+def device_count() -> int:
+    c = c_int(0)
+    if load_library().gdpDeviceCount(ctypes.byref(c)) != GDP_OKAY:
+        return 0
+    return c.value
+
+
+def device_name() -> str:
+    buf = ctypes.create_string_buffer(256)
+    if load_library().gdpDeviceName(buf, 256) != GDP_OKAY:
+        raise RuntimeError(last_error())
+    return buf.value.decode()
+
+
+class DeviceArray:
+    """A float64 array in device memory, allocated by the library's own HIP
+    runtime (see gdpDeviceMalloc)."""
+
+    def __init__(self, host: np.ndarray):
+        self.lib = load_library()
+        host = np.ascontiguousarray(host, np.float64)
+        self.size = host.size
+        self.nbytes = host.nbytes
+        p = c_void_p()
+        if self.lib.gdpDeviceMalloc(ctypes.byref(p), self.nbytes) != GDP_OKAY:
+            raise RuntimeError(last_error())
+        self.ptr = p.value
+        self.upload(host)
+
+    def upload(self, host: np.ndarray):
+        host = np.ascontiguousarray(host, np.float64)
+        assert host.nbytes == self.nbytes
+        if self.lib.gdpMemcpyHtoD(c_void_p(self.ptr), c_void_p(host.ctypes.data), self.nbytes) != GDP_OKAY:
+            raise RuntimeError(last_error())
+
+    def copy_from(self, other: "DeviceArray"):
+        assert other.nbytes == self.nbytes
+        if self.lib.gdpMemcpyDtoD(c_void_p(self.ptr), c_void_p(other.ptr), self.nbytes) != GDP_OKAY:
+            raise RuntimeError(last_error())
+
+    def numpy(self) -> np.ndarray:
+        out = np.empty(self.size, np.float64)
+        if self.lib.gdpMemcpyDtoH(c_void_p(out.ctypes.data), c_void_p(self.ptr), self.nbytes) != GDP_OKAY:
+            raise RuntimeError(last_error())
+        return out
+
+    def free(self):
+        if self.ptr:
+            self.lib.gdpDeviceFree(c_void_p(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 # --------------------------------------------------------------------------

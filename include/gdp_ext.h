@@ -12,6 +12,8 @@
 #ifndef GDP_AMD_EXT_H
 #define GDP_AMD_EXT_H
 
+#include <stddef.h>
+
 #include "gdp_def.h"
 
 #ifdef __cplusplus
@@ -31,6 +33,8 @@ typedef struct {
     int item_nnz;            /* target nonzeros per work item (0: default 1024) */
     int mega_blocks_per_cu;  /* persistent grid: workgroups per CU (0: occupancy) */
     int graph_rounds;        /* rounds per captured graph in GPU_LOOP (0: 8) */
+    int max_split_segs;      /* rows with up to this many 512-entry segments are split
+                                across waves (-1: default 256; 0: never split) */
     int group_log[10];       /* lanes per row (log2) for rows of width 2^b; -1: default */
 } gdp_tuning;
 
@@ -60,7 +64,10 @@ GDP_RETCODE gdpProblemGetStats(gdp_problem p, gdp_problem_stats *out);
 GDP_RETCODE gdpPropagate(gdp_problem p, double *lbs, double *ubs, const int sync_type,
                          const int max_rounds, gdp_run_info *info);
 
-/* Same with device pointers (hipMalloc'd, length n_vars each); no host copy. */
+/* Same with device pointers (hipMalloc'd, length n_vars each); no host copy.
+ * The library works on its own stream: work that produced d_lbs / d_ubs on
+ * another stream must be complete before the call.  Returns after the device
+ * has finished. */
 GDP_RETCODE gdpPropagateDevice(gdp_problem p, double *d_lbs, double *d_ubs, const int sync_type,
                                const int max_rounds, gdp_run_info *info);
 
@@ -75,6 +82,17 @@ GDP_RETCODE gdpOMPPropagate(const int n_cons, const int n_vars, const int nnz,
                             const double *lhss, const double *rhss, double *lbs, double *ubs,
                             const int *vartypes, const int max_rounds, const int threads,
                             gdp_run_info *info);
+
+/* Device memory plumbing, so that callers need no second HIP runtime (a
+ * framework bundling its own ROCm runtime cannot share a process with this
+ * library's).  Synchronous; device 0 unless the caller selected another. */
+GDP_RETCODE gdpDeviceCount(int *count);
+GDP_RETCODE gdpDeviceMalloc(void **ptr, const size_t bytes);
+GDP_RETCODE gdpDeviceFree(void *ptr);
+GDP_RETCODE gdpMemcpyHtoD(void *dst, const void *src, const size_t bytes);
+GDP_RETCODE gdpMemcpyDtoH(void *dst, const void *src, const size_t bytes);
+GDP_RETCODE gdpMemcpyDtoD(void *dst, const void *src, const size_t bytes);
+GDP_RETCODE gdpDeviceName(char *buf, const int len);
 
 const char *gdpLastError(void);
 

@@ -10,7 +10,7 @@ import numpy as np
 
 REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB = os.path.join(REPO, "oracle", "lib", "libgdp_oracle.so")
-SEG_LEN = 512  # canonical segment length shared with the kernels
+SEG_LEN = 256  # canonical segment length shared with the kernels (kSegLen)
 
 
 class Result(ctypes.Structure):
