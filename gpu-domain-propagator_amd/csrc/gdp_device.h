@@ -1,15 +1,29 @@
 // gdp_device.h - device-side layout of a propagation problem.
 //
-// HBM layout (all arrays 256-byte aligned, rows grouped by length):
+// Columns are relabelled internally ("internal ids") and grouped by how their
+// bounds are stored:
+//   ids [0, n16)      integer columns whose bounds stay integral within
+//                     [-32767, 32767]: snapshot record 2 x int16 = 4 bytes
+//   ids [n16, n_int)  integer columns whose bounds stay integral within the
+//                     int32 range: snapshot record 2 x int32 = 8 bytes
+//   ids [n_int, n)    all other columns: snapshot record 2 x double = 16 bytes
+// The entry order inside every row is unchanged, so the relabelling does not
+// alter any arithmetic.  Small records keep the randomly gathered working set
+// of a round inside the per-XCD L2.
+//
+// HBM layout (rows grouped by length):
 //   row_ptr[m'+1]  int32   CSR row pointers of the m' nonempty rows
-//   sides[m']      double2 (lhs, rhs) per row, one 16-byte load
-//   col[nnz]       int32   column index; bit 31 set when the column is an
-//                          integer variable, so the rounding rule needs no
-//                          extra gather
+//   sides[m']      double2 (lhs, rhs) per row
+//   col[nnz]       int32   internal column id; bit 31 set for an integer column
 //   val[nnz]       double  coefficients
-//   items[]        int4    work items of a round (see gdp_kernels.hip)
-//   bnd0/bnd1[n]   double2 (lb, ub) per column, snapshot / accumulator
-//   bm0/bm1[nw]    uint32  bitmaps of the columns changed in a round
+//   items[]        int4    work items of the sweep
+//   snapshot       S16[n16] uint32, S32[n32] int2, SF[nf] double2: the bounds
+//                  every row of a round reads
+//   accumulator    AI[n_int] int2 (integer atomics), AF[nf] double2 (FP64
+//                  atomics), AX[n_int] double2 (exact values of integer
+//                  candidates beyond the int32 range; they only arise when
+//                  bounds cross)
+//   bm[nw]         uint32  columns changed in the current round
 //   status[]       int32   per-round flags and the outcome of a run
 #pragma once
 
@@ -17,7 +31,8 @@
 
 namespace gdp {
 
-constexpr int kColMask = 0x7fffffff;
+constexpr int kIdMask = 0x3fffffff;
+constexpr int kIntFlag = static_cast<int>(0x80000000u);
 constexpr int kItemGroup = 0;
 constexpr int kItemLong = 1;   // one row, one wave, all segments (very long rows)
 constexpr int kItemSeg = 2;    // one segment of a split row
@@ -31,17 +46,18 @@ constexpr int kMaxSplitSegs = 512;
 // status words: row 0 holds the outcome, row r >= 1 the flags of round r
 constexpr int kStatusWords = 8;
 constexpr int kOutRounds = 0;
-constexpr int kOutResultBuf = 1;
 constexpr int kOutInfeasible = 2;
 constexpr int kOutCapped = 3;
 constexpr int kGlobalError = 4;
 constexpr int kGlobalDone = 5;
+constexpr int kGlobalEnvelope = 6;  // input bounds outside the class envelope
+constexpr int kGlobalCrossed = 7;   // input bounds already crossed
 constexpr int kStatusChanged = 0;
-constexpr int kStatusInfeasPrev = 1;
-constexpr int kStatusRows = 128;  // rounds 1..101 fit
+constexpr int kStatusInfeasible = 1;
+constexpr int kStatusRows = 128;  // rounds 1..100 fit
 
-// wall_clock64 runs at 100 MHz on CDNA; give up a barrier wait after 1 s
-constexpr long long kBarrierTimeoutTicks = 100000000LL;
+// wall_clock64 runs at 100 MHz on CDNA; give up a wait after 1 s
+constexpr long long kWaitTimeoutTicks = 100000000LL;
 
 // Partial activity of one segment of a split row.
 struct SegPart {
@@ -59,9 +75,10 @@ struct DevMatrix {
     int m;            // nonempty rows
     int n;            // columns
     int nnz;
+    int n16, n_int;   // class boundaries of the internal ids
     int nwords;       // bitmap words, ceil(n / 32)
-    int n_items;      // row items
-    int n_fix_items;  // fixup items, ceil(nwords / 64)
+    int n_items;      // sweep items
+    int n_pub_items;  // publish items, ceil(nwords / 64)
     const int *row_ptr;
     const double2 *sides;
     const int *col;
@@ -70,13 +87,17 @@ struct DevMatrix {
     const int4 *seg_rows;  // per split row: {row, first partial, segments, 0}
     int n_seg_rows;
     int n_seg_parts;
+    const int *perm;       // internal id -> original column
 };
 
 struct DevState {
-    double2 *bnd0;
-    double2 *bnd1;
-    unsigned *bm0;
-    unsigned *bm1;
+    unsigned *S16;
+    int2 *S32;
+    double2 *SF;
+    int2 *AI;
+    double2 *AX;
+    double2 *AF;
+    unsigned *bm;
     int *status;        // kStatusRows * kStatusWords
     unsigned *barrier;  // [count, generation]
     SegPart *seg_parts;    // [n_seg_parts]

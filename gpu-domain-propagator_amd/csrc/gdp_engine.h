@@ -1,10 +1,11 @@
 // gdp_engine.h - host-side API of the MI355X propagation engine (C++).
 //
 // A Problem owns a device-resident copy of one constraint matrix laid out for
-// the round kernels (rows grouped by length, column indices carrying the
-// integrality flag) plus the per-run device state (two bound buffers, change
-// bitmaps, per-round status words).  It is built once and can propagate any
-// number of bound vectors.
+// the round kernels (rows grouped by length, columns relabelled by bound
+// storage class, integrality folded into the column index) plus the per-run
+// device state (bound snapshot and accumulator, change bitmap, per-round
+// status words).  It is built once and can propagate any number of bound
+// vectors that lie inside its bound envelope.
 #pragma once
 
 #include <cstdint>
@@ -25,7 +26,8 @@ struct Tuning {
     int block_threads = 256;  // threads per workgroup (multiple of 64)
     int mega_blocks_per_cu = 0;  // 0: occupancy limit
     int graph_rounds = 8;     // rounds per captured graph for GPU_LOOP
-    int max_split_segs = 256; // rows with more segments run on one wave (0: never split)
+    int max_split_segs = 512; // rows with more segments run on one wave (0: never split)
+    int compact_bounds = 1;   // 0: keep every column in 16-byte FP64 records
     // per tree-width bin (width 2^b, b = 0..9): lanes per row as log2
     // default: K = min(width, 4) consecutive entries per lane
     int group_log[10] = {0, 0, 0, 1, 2, 3, 4, 5, 6, 6};
@@ -45,14 +47,20 @@ struct ProblemStats {
     int n_long_rows = 0;
     int64_t bin_rows[11] = {0};
     int64_t bin_nnz[11] = {0};
+    int n16 = 0, n32 = 0, nf = 0;  // columns per bound storage class
 };
 
 struct Problem;
 
 // Throws std::runtime_error on invalid input or device failure.
+// env_lb / env_ub (may be null) bound every vector that will be propagated
+// with this problem, e.g. the root bounds of a branch-and-bound tree: integer
+// columns whose envelope is finite and integral get compact bound records.
+// Without an envelope every column uses FP64 records.
 Problem *problem_create(int m, int n, int nnz, const int *col, const int *row_ptr,
                         const double *val, const double *lhs, const double *rhs,
-                        const int *vartype, const Tuning *tuning);
+                        const int *vartype, const double *env_lb, const double *env_ub,
+                        const Tuning *tuning);
 void problem_destroy(Problem *p);
 ProblemStats problem_stats(const Problem *p);
 

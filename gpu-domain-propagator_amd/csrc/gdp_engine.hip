@@ -4,9 +4,10 @@
 // (src/propagators/GPU_propagator.cuh:98-268) and the row blocking of
 // fill_row_blocks (src/kernels/util_kernels.cu:12-62).  Differences by
 // design: the matrix is laid out once per Problem (rows grouped by tree
-// width, integrality folded into the column index), rounds are
-// round-synchronous through two bound buffers, and GPU_LOOP is a captured HIP
-// graph of round launches that stop themselves on the device.
+// width, columns relabelled by bound storage class, integrality folded into
+// the column index), rounds are round-synchronous (sweep, then publish), and
+// GPU_LOOP is a captured HIP graph of round launches that stop themselves on
+// the device.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -23,20 +24,19 @@
 
 namespace gdp {
 
-__global__ void round_kernel(DevMatrix M, DevState D, int round, int guard);
-__global__ void fixup_kernel(DevMatrix M, DevState D, int round);
+__global__ void sweep_kernel(DevMatrix M, DevState D, int round, int guard);
+__global__ void publish_kernel(DevMatrix M, DevState D, int round, int guard);
 __global__ void mega_kernel(DevMatrix M, DevState D, int max_rounds);
 __global__ void init_kernel(DevMatrix M, DevState D, const double *lbs, const double *ubs);
-__global__ void unpack_kernel(DevMatrix M, const double2 *bnd, double *lbs, double *ubs);
+__global__ void clear_envelope_kernel(DevState D);
+__global__ void unpack_kernel(DevMatrix M, DevState D, double *lbs, double *ubs);
 
 namespace {
 
 thread_local std::string g_last_error;
 
 void check(hipError_t e, const char *what) {
-    if (e != hipSuccess) {
-        throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
-    }
+    if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
 }
 
 int next_pow2_log(int len) {
@@ -52,6 +52,18 @@ T *dev_alloc(size_t count, std::vector<void *> &owned) {
     check(hipMalloc(&p, count * sizeof(T)), "hipMalloc");
     owned.push_back(p);
     return static_cast<T *>(p);
+}
+
+// Storage class of a column from its bound envelope: 0 = 2 x int16,
+// 1 = 2 x int32, 2 = 2 x double.
+int column_class(int vartype, const double *env_lb, const double *env_ub, int j) {
+    if (vartype == kVarContinuous || !env_lb || !env_ub) return 2;
+    const double l = env_lb[j], u = env_ub[j];
+    if (!(std::isfinite(l) && std::isfinite(u)) || std::floor(l) != l || std::floor(u) != u) return 2;
+    if (is_neg_inf(l) || is_pos_inf(u)) return 2;
+    if (l >= -32767.0 && u <= 32767.0 && l <= 32767.0 && u >= -32767.0) return 0;
+    if (l >= -2147483647.0 && u <= 2147483647.0 && l <= 2147483647.0 && u >= -2147483647.0) return 1;
+    return 2;
 }
 
 }  // namespace
@@ -71,7 +83,7 @@ struct Problem {
     int device = 0;
     int cu_count = 0;
     int mega_grid = 0;
-    // captured GPU_LOOP graphs, keyed by the first round they cover
+    // captured GPU_LOOP graphs, one per chunk of rounds
     std::vector<hipGraphExec_t> graphs;
     std::vector<int> graph_last;
     std::mutex run_mutex;
@@ -86,12 +98,12 @@ struct Problem {
     }
 };
 
-Problem *problem_create(int m, int n, int nnz, const int *col, const int *row_ptr,
-                        const double *val, const double *lhs, const double *rhs,
-                        const int *vartype, const Tuning *tuning) {
+Problem *problem_create(int m, int n, int nnz, const int *col, const int *row_ptr, const double *val,
+                        const double *lhs, const double *rhs, const int *vartype, const double *env_lb,
+                        const double *env_ub, const Tuning *tuning) {
     if (m <= 0 || n <= 0 || nnz <= 0) throw std::runtime_error("empty problem");
-    if (!col || !row_ptr || !val || !lhs || !rhs || !vartype)
-        throw std::runtime_error("null input array");
+    if (n > kIdMask) throw std::runtime_error("too many columns");
+    if (!col || !row_ptr || !val || !lhs || !rhs || !vartype) throw std::runtime_error("null input array");
     if (row_ptr[0] != 0 || row_ptr[m] != nnz) throw std::runtime_error("row_ptrs do not span nnz");
     for (int i = 0; i < m; i++)
         if (row_ptr[i + 1] < row_ptr[i]) throw std::runtime_error("row_ptrs not monotone");
@@ -105,28 +117,43 @@ Problem *problem_create(int m, int n, int nnz, const int *col, const int *row_pt
         P->nnz = nnz;
         if (tuning) P->tuning = *tuning;
         const Tuning &T = P->tuning;
-        if (T.block_threads % 64 != 0 || T.block_threads < 64 || T.block_threads > 256)
-            throw std::runtime_error("block_threads must be 64, 128, 192 or 256");
 
-        // ---- group rows by tree width: long rows first, then widest bins ----
+        // ---- columns: internal ids grouped by bound storage class ----
+        std::vector<int> cls(n);
+        int ncls[3] = {0, 0, 0};
+        for (int j = 0; j < n; j++) {
+            cls[j] = T.compact_bounds ? column_class(vartype[j], env_lb, env_ub, j) : 2;
+            ncls[cls[j]]++;
+        }
+        std::vector<int> perm(n), internal(n);
+        {
+            int next[3] = {0, ncls[0], ncls[0] + ncls[1]};
+            for (int j = 0; j < n; j++) {
+                const int id = next[cls[j]]++;
+                perm[id] = j;
+                internal[j] = id;
+            }
+        }
+        P->stats.n16 = ncls[0];
+        P->stats.n32 = ncls[1];
+        P->stats.nf = ncls[2];
+
+        // ---- rows grouped by tree width: long rows first, then wide to narrow ----
         // bin b < kLongBin holds rows with next_pow2(len) == 2^b; bin kLongBin
-        // rows longer than kSegLen.  Long rows lead so that their waves start
-        // first.
+        // rows longer than kSegLen.
         constexpr int kLongBin = 10;
-        std::vector<int> bin_of(m);
         std::vector<std::vector<int>> rows_in_bin(11);
         for (int i = 0; i < m; i++) {
             const int len = row_ptr[i + 1] - row_ptr[i];
-            if (len == 0) { bin_of[i] = -1; continue; }
+            if (len == 0) continue;
             const int b = len > kSegLen ? kLongBin : next_pow2_log(len);
-            bin_of[i] = b;
             rows_in_bin[b].push_back(i);
             P->stats.bin_rows[b]++;
             P->stats.bin_nnz[b] += len;
         }
         std::vector<int> order;
         order.reserve(m);
-        for (int b = 10; b >= 0; b--)
+        for (int b = kLongBin; b >= 0; b--)
             for (int i : rows_in_bin[b]) order.push_back(i);
         const int mp = static_cast<int>(order.size());
 
@@ -141,14 +168,14 @@ Problem *problem_create(int m, int n, int nnz, const int *col, const int *row_pt
             h_sides[r] = make_double2(lhs[i], rhs[i]);
             for (int k = row_ptr[i]; k < row_ptr[i + 1]; k++) {
                 const int j = col[k];
-                h_col[pos] = j | (vartype[j] == kVarContinuous ? 0 : static_cast<int>(0x80000000u));
+                h_col[pos] = internal[j] | (vartype[j] == kVarContinuous ? 0 : kIntFlag);
                 h_val[pos] = val[k];
                 pos++;
             }
         }
         h_rp[mp] = pos;
 
-        // ---- work items ----
+        // ---- sweep items ----
         // Order: segments of split rows, whole very long rows, then the
         // tree-width bins from wide to narrow, so that long work starts first.
         std::vector<int4> h_items;
@@ -156,7 +183,7 @@ Problem *problem_create(int m, int n, int nnz, const int *col, const int *row_pt
         int n_parts = 0;
         int r = 0;
         {
-            const int cnt = static_cast<int>(rows_in_bin[10].size());
+            const int cnt = static_cast<int>(rows_in_bin[kLongBin].size());
             std::vector<int> giant;
             for (int q = 0; q < cnt; q++) {
                 const int len = h_rp[r + q + 1] - h_rp[r + q];
@@ -174,7 +201,7 @@ Problem *problem_create(int m, int n, int nnz, const int *col, const int *row_pt
             P->stats.n_long_rows = cnt;
             r += cnt;
         }
-        for (int b = 9; b >= 0; b--) {
+        for (int b = kLongBin - 1; b >= 0; b--) {
             const int cnt = static_cast<int>(rows_in_bin[b].size());
             if (cnt > 0) {
                 const int width = 1 << b;
@@ -207,44 +234,55 @@ Problem *problem_create(int m, int n, int nnz, const int *col, const int *row_pt
         check(hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking), "hipStreamCreate");
         check(hipEventCreate(&P->ev0), "hipEventCreate");
         check(hipEventCreate(&P->ev1), "hipEventCreate");
+        hipStream_t s = P->stream;
 
         const int nwords = (n + 31) / 32;
         DevMatrix &M = P->M;
         M.m = mp;
         M.n = n;
         M.nnz = nnz;
+        M.n16 = ncls[0];
+        M.n_int = ncls[0] + ncls[1];
         M.nwords = nwords;
         M.n_items = static_cast<int>(h_items.size());
-        M.n_fix_items = (nwords + 63) / 64;
+        M.n_pub_items = (nwords + 63) / 64;
+        M.n_seg_rows = static_cast<int>(h_seg_rows.size());
+        M.n_seg_parts = n_parts;
         int *d_rp = dev_alloc<int>(mp + 1, P->owned);
         double2 *d_sides = dev_alloc<double2>(mp, P->owned);
         int *d_col = dev_alloc<int>(nnz, P->owned);
         double *d_val = dev_alloc<double>(nnz, P->owned);
         int4 *d_items = dev_alloc<int4>(h_items.size(), P->owned);
-        check(hipMemcpyAsync(d_rp, h_rp.data(), (mp + 1) * sizeof(int), hipMemcpyHostToDevice, P->stream), "H2D");
-        check(hipMemcpyAsync(d_sides, h_sides.data(), mp * sizeof(double2), hipMemcpyHostToDevice, P->stream), "H2D");
-        check(hipMemcpyAsync(d_col, h_col.data(), nnz * sizeof(int), hipMemcpyHostToDevice, P->stream), "H2D");
-        check(hipMemcpyAsync(d_val, h_val.data(), nnz * sizeof(double), hipMemcpyHostToDevice, P->stream), "H2D");
+        int4 *d_seg_rows = dev_alloc<int4>(h_seg_rows.size(), P->owned);
+        int *d_perm = dev_alloc<int>(n, P->owned);
+        check(hipMemcpyAsync(d_rp, h_rp.data(), (mp + 1) * sizeof(int), hipMemcpyHostToDevice, s), "H2D");
+        check(hipMemcpyAsync(d_sides, h_sides.data(), mp * sizeof(double2), hipMemcpyHostToDevice, s), "H2D");
+        check(hipMemcpyAsync(d_col, h_col.data(), nnz * sizeof(int), hipMemcpyHostToDevice, s), "H2D");
+        check(hipMemcpyAsync(d_val, h_val.data(), nnz * sizeof(double), hipMemcpyHostToDevice, s), "H2D");
         if (!h_items.empty())
-            check(hipMemcpyAsync(d_items, h_items.data(), h_items.size() * sizeof(int4), hipMemcpyHostToDevice, P->stream), "H2D");
+            check(hipMemcpyAsync(d_items, h_items.data(), h_items.size() * sizeof(int4), hipMemcpyHostToDevice, s),
+                  "H2D");
+        if (!h_seg_rows.empty())
+            check(hipMemcpyAsync(d_seg_rows, h_seg_rows.data(), h_seg_rows.size() * sizeof(int4),
+                                 hipMemcpyHostToDevice, s),
+                  "H2D");
+        check(hipMemcpyAsync(d_perm, perm.data(), n * sizeof(int), hipMemcpyHostToDevice, s), "H2D");
         M.row_ptr = d_rp;
         M.sides = d_sides;
         M.col = d_col;
         M.val = d_val;
         M.items = d_items;
-        M.n_seg_rows = static_cast<int>(h_seg_rows.size());
-        M.n_seg_parts = n_parts;
-        int4 *d_seg_rows = dev_alloc<int4>(h_seg_rows.size(), P->owned);
-        if (!h_seg_rows.empty())
-            check(hipMemcpyAsync(d_seg_rows, h_seg_rows.data(), h_seg_rows.size() * sizeof(int4),
-                                 hipMemcpyHostToDevice, P->stream), "H2D");
         M.seg_rows = d_seg_rows;
+        M.perm = d_perm;
 
         DevState &D = P->D;
-        D.bnd0 = dev_alloc<double2>(n, P->owned);
-        D.bnd1 = dev_alloc<double2>(n, P->owned);
-        D.bm0 = dev_alloc<unsigned>(nwords, P->owned);
-        D.bm1 = dev_alloc<unsigned>(nwords, P->owned);
+        D.S16 = dev_alloc<unsigned>(ncls[0], P->owned);
+        D.S32 = dev_alloc<int2>(ncls[1], P->owned);
+        D.SF = dev_alloc<double2>(ncls[2], P->owned);
+        D.AI = dev_alloc<int2>(M.n_int, P->owned);
+        D.AX = dev_alloc<double2>(M.n_int, P->owned);
+        D.AF = dev_alloc<double2>(ncls[2], P->owned);
+        D.bm = dev_alloc<unsigned>(nwords, P->owned);
         D.status = dev_alloc<int>(kStatusRows * kStatusWords, P->owned);
         D.barrier = dev_alloc<unsigned>(64, P->owned);
         D.seg_parts = dev_alloc<SegPart>(n_parts, P->owned);
@@ -253,16 +291,15 @@ Problem *problem_create(int m, int n, int nnz, const int *col, const int *row_pt
         D.seg_flag = dev_alloc<int>(h_seg_rows.size(), P->owned);
 
         int per_cu = 0;
-        check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(mega_kernel),
-                                                           256, 0),
+        check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(mega_kernel), 256,
+                                                           0),
               "occupancy");
         if (T.mega_blocks_per_cu > 0) per_cu = std::min(per_cu, T.mega_blocks_per_cu);
         P->mega_grid = std::max(1, per_cu) * P->cu_count;
-        // split rows wait for their sibling segments: all of them must be able
-        // to be resident at once, with room to spare
-        if (P->mega_grid * 4 < 4 * kMaxSplitSegs)
-            throw std::runtime_error("device too small for split-row items");
-        check(hipStreamSynchronize(P->stream), "upload");
+        // segment waves of a split row wait for each other: all of a row's
+        // segments must fit on the device at once, with room to spare
+        if (P->mega_grid * 4 < 4 * kMaxSplitSegs) throw std::runtime_error("device too small for split-row items");
+        check(hipStreamSynchronize(s), "upload");
     } catch (...) {
         delete P;
         throw;
@@ -276,80 +313,76 @@ ProblemStats problem_stats(const Problem *p) { return p->stats; }
 
 namespace {
 
-int round_grid(const Problem &P, int items) {
-    const int waves = 256 / 64;
-    return std::max(1, (items + waves - 1) / waves);
-}
+int grid_for(int items) { return std::max(1, (items + 3) / 4); }
 
-// Host-side outcome logic shared by CPU_LOOP and GPU_LOOP: walks the status
-// rows exactly as the persistent kernel does.
-void outcome_from_status(const std::vector<int> &st, int max_rounds, RunResult &res, int &result_buf,
-                         bool &need_final_check) {
+// Walks the per-round status rows exactly as the persistent kernel does.
+void outcome_from_status(const std::vector<int> &st, int max_rounds, RunResult &res) {
     res.rounds = 0;
     res.infeasible = 0;
     res.capped = 0;
-    result_buf = 0;
-    need_final_check = false;
     for (int round = 1; round <= max_rounds; round++) {
         const int *row = st.data() + kStatusWords * round;
-        if (row[kStatusInfeasPrev]) {
-            res.rounds = round - 1;
-            result_buf = (round - 1) & 1;
-            res.infeasible = 1;
-            return;
-        }
         res.rounds = round;
-        result_buf = round & 1;
+        if (row[kStatusInfeasible]) { res.infeasible = 1; return; }
         if (!row[kStatusChanged]) return;
-        if (round == max_rounds) need_final_check = true;
+        if (round == max_rounds) res.capped = 1;
     }
 }
 
 }  // namespace
 
-RunResult problem_propagate_device(Problem *P, double *d_lbs, double *d_ubs, int sync_type,
-                                   int max_rounds) {
+RunResult problem_propagate_device(Problem *P, double *d_lbs, double *d_ubs, int sync_type, int max_rounds) {
     if (sync_type < CPU_LOOP_ID || sync_type > MEGAKERNEL_ID) throw std::runtime_error("Unknown sync type");
     if (max_rounds < 1 || max_rounds > kMaxRounds) throw std::runtime_error("max_rounds out of range");
     std::lock_guard<std::mutex> lock(P->run_mutex);
     const DevMatrix &M = P->M;
     const DevState &D = P->D;
     hipStream_t s = P->stream;
-    const int n = P->n;
     RunResult res;
 
-    // initial state: both bound buffers hold the input, bitmaps and flags clear
-    const int fill_grid = std::min(4096, std::max(1, (n + 255) / 256));
+    const int fill_grid = std::min(4096, std::max(1, (P->n + 255) / 256));
+    hipLaunchKernelGGL(clear_envelope_kernel, dim3(1), dim3(1), 0, s, D);
     hipLaunchKernelGGL(init_kernel, dim3(fill_grid), dim3(256), 0, s, M, D, d_lbs, d_ubs);
     check(hipGetLastError(), "init_kernel launch");
 
-    const int total_items = M.n_items + M.n_fix_items;
-    const int grid = round_grid(*P, total_items);
+    const int sweep_grid = grid_for(M.n_items);
+    const int pub_grid = grid_for(M.n_pub_items);
     std::vector<int> st(kStatusRows * kStatusWords, 0);
-    int result_buf = 0;
-    bool need_final = false;
+    int head[kStatusWords];
+    check(hipMemcpyAsync(head, D.status, sizeof(head), hipMemcpyDeviceToHost, s), "flags D2H");
+    check(hipStreamSynchronize(s), "init sync");
+    if (head[kGlobalEnvelope]) throw std::runtime_error("bounds outside the problem's bound envelope");
+    if (head[kGlobalCrossed]) {
+        // infeasible before any round: bounds are returned unchanged
+        res.infeasible = 1;
+        return res;
+    }
 
     check(hipEventRecord(P->ev0, s), "event");
     if (sync_type == CPU_LOOP_ID) {
-        // one launch per round; the host reads the round's flags back
+        // two launches per round; the host reads the round's flags back
         for (int round = 1; round <= max_rounds; round++) {
-            hipLaunchKernelGGL(round_kernel, dim3(grid), dim3(256), 0, s, M, D, round, 0);
-            check(hipGetLastError(), "round_kernel launch");
-            int flags[kStatusWords], head[kStatusWords];
+            hipLaunchKernelGGL(sweep_kernel, dim3(sweep_grid), dim3(256), 0, s, M, D, round, 0);
+            hipLaunchKernelGGL(publish_kernel, dim3(pub_grid), dim3(256), 0, s, M, D, round, 0);
+            check(hipGetLastError(), "round launch");
+            int flags[kStatusWords];
             check(hipMemcpyAsync(flags, D.status + kStatusWords * round, sizeof(flags), hipMemcpyDeviceToHost, s),
                   "flags D2H");
             check(hipMemcpyAsync(head, D.status, sizeof(head), hipMemcpyDeviceToHost, s), "flags D2H");
             check(hipStreamSynchronize(s), "round sync");
             if (head[kGlobalError]) throw std::runtime_error("device wait timed out");
             std::memcpy(st.data() + kStatusWords * round, flags, sizeof(flags));
-            if (flags[kStatusInfeasPrev] || !flags[kStatusChanged]) break;
+            if (flags[kStatusInfeasible] || !flags[kStatusChanged]) break;
         }
     } else if (sync_type == GPU_LOOP_ID) {
-        // captured chunks of round launches that stop themselves on the device
+        // captured chunks of rounds whose launches stop themselves on the device
         const int chunk = std::max(1, P->tuning.graph_rounds);
         for (int first = 1, ci = 0; first <= max_rounds; first += chunk, ci++) {
             const int last = std::min(max_rounds, first + chunk - 1);
-            if (static_cast<int>(P->graphs.size()) <= ci) P->graphs.resize(ci + 1, nullptr);
+            if (static_cast<int>(P->graphs.size()) <= ci) {
+                P->graphs.resize(ci + 1, nullptr);
+                P->graph_last.resize(ci + 1, 0);
+            }
             if (P->graphs[ci] && P->graph_last[ci] != last) {
                 (void)hipGraphExecDestroy(P->graphs[ci]);
                 P->graphs[ci] = nullptr;
@@ -357,14 +390,15 @@ RunResult problem_propagate_device(Problem *P, double *d_lbs, double *d_ubs, int
             if (!P->graphs[ci]) {
                 hipGraph_t graph;
                 check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "capture");
-                for (int round = first; round <= last; round++)
-                    hipLaunchKernelGGL(round_kernel, dim3(grid), dim3(256), 0, s, M, D, round, 1);
+                for (int round = first; round <= last; round++) {
+                    hipLaunchKernelGGL(sweep_kernel, dim3(sweep_grid), dim3(256), 0, s, M, D, round, 1);
+                    hipLaunchKernelGGL(publish_kernel, dim3(pub_grid), dim3(256), 0, s, M, D, round, 1);
+                }
                 check(hipStreamEndCapture(s, &graph), "end capture");
                 hipGraphExec_t exec;
                 check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "instantiate");
                 (void)hipGraphDestroy(graph);
                 P->graphs[ci] = exec;
-                if (static_cast<int>(P->graph_last.size()) <= ci) P->graph_last.resize(ci + 1, 0);
                 P->graph_last[ci] = last;
             }
             check(hipGraphLaunch(P->graphs[ci], s), "graph launch");
@@ -372,9 +406,8 @@ RunResult problem_propagate_device(Problem *P, double *d_lbs, double *d_ubs, int
             check(hipMemcpyAsync(flags, D.status + kStatusWords * last, sizeof(flags), hipMemcpyDeviceToHost, s),
                   "flags D2H");
             check(hipStreamSynchronize(s), "graph sync");
-            // the chunk ran on if and only if its last round has a status set
-            // or the run ended inside it; read all rows once at the end
-            if (flags[kStatusInfeasPrev] || !flags[kStatusChanged]) break;
+            // the next chunk is needed only if the chunk's last round still changed
+            if (flags[kStatusInfeasible] || !flags[kStatusChanged]) break;
         }
         check(hipMemcpyAsync(st.data(), D.status, st.size() * sizeof(int), hipMemcpyDeviceToHost, s), "status D2H");
         check(hipStreamSynchronize(s), "status sync");
@@ -388,35 +421,22 @@ RunResult problem_propagate_device(Problem *P, double *d_lbs, double *d_ubs, int
     check(hipEventRecord(P->ev1, s), "event");
 
     if (sync_type == MEGAKERNEL_ID) {
-        check(hipMemcpyAsync(st.data(), D.status, kStatusWords * sizeof(int), hipMemcpyDeviceToHost, s), "outcome D2H");
+        check(hipMemcpyAsync(st.data(), D.status, kStatusWords * sizeof(int), hipMemcpyDeviceToHost, s),
+              "outcome D2H");
         check(hipStreamSynchronize(s), "mega sync");
-        if (st[kGlobalError] || st[kOutRounds] < 0) throw std::runtime_error("device barrier timed out");
+        if (st[kGlobalError] || st[kOutRounds] < 0) throw std::runtime_error("device wait timed out");
         res.rounds = st[kOutRounds];
         res.infeasible = st[kOutInfeasible];
         res.capped = st[kOutCapped];
-        result_buf = st[kOutResultBuf];
     } else {
-        outcome_from_status(st, max_rounds, res, result_buf, need_final);
-        if (need_final) {
-            // the cap stopped a run that still changed: check its result
-            const int fgrid = round_grid(*P, M.n_fix_items);
-            hipLaunchKernelGGL(fixup_kernel, dim3(fgrid), dim3(256), 0, s, M, D, max_rounds + 1);
-            check(hipGetLastError(), "fixup launch");
-            int flags[kStatusWords];
-            check(hipMemcpyAsync(flags, D.status + kStatusWords * (max_rounds + 1), sizeof(flags),
-                                 hipMemcpyDeviceToHost, s), "flags D2H");
-            check(hipStreamSynchronize(s), "fixup sync");
-            res.infeasible = flags[kStatusInfeasPrev] != 0;
-            res.capped = 1;
-        }
-        check(hipStreamSynchronize(s), "sync");
+        outcome_from_status(st, max_rounds, res);
     }
     float ms = 0.f;
+    check(hipEventSynchronize(P->ev1), "event sync");
     check(hipEventElapsedTime(&ms, P->ev0, P->ev1), "elapsed");
     res.kernel_ms = ms;
 
-    const double2 *out = result_buf ? D.bnd1 : D.bnd0;
-    hipLaunchKernelGGL(unpack_kernel, dim3(fill_grid), dim3(256), 0, s, M, out, d_lbs, d_ubs);
+    hipLaunchKernelGGL(unpack_kernel, dim3(fill_grid), dim3(256), 0, s, M, D, d_lbs, d_ubs);
     check(hipGetLastError(), "unpack_kernel launch");
     check(hipStreamSynchronize(s), "unpack sync");
     return res;
@@ -424,13 +444,6 @@ RunResult problem_propagate_device(Problem *P, double *d_lbs, double *d_ubs, int
 
 RunResult problem_propagate(Problem *P, double *lbs, double *ubs, int sync_type, int max_rounds) {
     const int n = P->n;
-    // an input with crossed bounds is infeasible before any round
-    for (int j = 0; j < n; j++)
-        if (eps_gt(lbs[j], ubs[j])) {
-            RunResult r;
-            r.infeasible = 1;
-            return r;
-        }
     double *d_b = nullptr;
     check(hipMalloc(&d_b, 2 * sizeof(double) * static_cast<size_t>(n)), "hipMalloc bounds");
     RunResult res;

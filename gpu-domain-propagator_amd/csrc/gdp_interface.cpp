@@ -62,7 +62,7 @@ GDP_RETCODE gpu_double(int m, int n, int nnz, const int *col, const int *rp, con
     if (empty_problem(m, n, nnz)) return GDP_OKAY;
     gdp::Problem *p = nullptr;
     try {
-        p = gdp::problem_create(m, n, nnz, col, rp, val, lhs, rhs, vt, nullptr);
+        p = gdp::problem_create(m, n, nnz, col, rp, val, lhs, rhs, vt, lb, ub, nullptr);
         std::vector<double> l(lb, lb + n), u(ub, ub + n);
         const RunResult r = gdp::problem_propagate(p, l.data(), u.data(), sync_type, MAX_NUM_ROUNDS);
         std::memcpy(lb, l.data(), n * sizeof(double));
@@ -246,12 +246,14 @@ void gdpTuningDefaults(gdp_tuning *t) {
     t->mega_blocks_per_cu = d.mega_blocks_per_cu;
     t->graph_rounds = d.graph_rounds;
     t->max_split_segs = d.max_split_segs;
+    t->compact_bounds = d.compact_bounds;
     for (int b = 0; b < 10; b++) t->group_log[b] = d.group_log[b];
 }
 
 GDP_RETCODE gdpProblemCreate(const int n_cons, const int n_vars, const int nnz, const int *col_indices,
                              const int *row_ptrs, const double *vals, const double *lhss, const double *rhss,
-                             const int *vartypes, const gdp_tuning *tuning, gdp_problem *out) {
+                             const int *vartypes, const double *env_lbs, const double *env_ubs,
+                             const gdp_tuning *tuning, gdp_problem *out) {
     if (!out) return GDP_ERROR;
     *out = nullptr;
     try {
@@ -261,11 +263,12 @@ GDP_RETCODE gdpProblemCreate(const int n_cons, const int n_vars, const int nnz, 
             if (tuning->mega_blocks_per_cu > 0) T.mega_blocks_per_cu = tuning->mega_blocks_per_cu;
             if (tuning->graph_rounds > 0) T.graph_rounds = tuning->graph_rounds;
             if (tuning->max_split_segs >= 0) T.max_split_segs = tuning->max_split_segs;
+            if (tuning->compact_bounds >= 0) T.compact_bounds = tuning->compact_bounds;
             for (int b = 0; b < 10; b++)
                 if (tuning->group_log[b] >= 0) T.group_log[b] = tuning->group_log[b];
         }
         gdp::Problem *p = gdp::problem_create(n_cons, n_vars, nnz, col_indices, row_ptrs, vals, lhss, rhss,
-                                              vartypes, &T);
+                                              vartypes, env_lbs, env_ubs, &T);
         *out = new gdp_problem_s{p};
         return GDP_OKAY;
     } catch (const std::exception &e) {
@@ -293,6 +296,9 @@ GDP_RETCODE gdpProblemGetStats(gdp_problem p, gdp_problem_stats *out) {
         out->bin_rows[b] = s.bin_rows[b];
         out->bin_nnz[b] = s.bin_nnz[b];
     }
+    out->n16 = s.n16;
+    out->n32 = s.n32;
+    out->nf = s.nf;
     return GDP_OKAY;
 }
 

@@ -1,24 +1,28 @@
-// gdp_kernels.hip - CDNA4 (gfx950) kernels of one propagation round.
+// gdp_kernels.hip - CDNA4 (gfx950) kernels of the propagation rounds.
 //
-// One round reads the bounds at the end of the previous round (snapshot S)
-// and folds every accepted candidate into the accumulator A with native FP64
-// atomic max/min (global_atomic_{max,min}_f64).  S and A are two buffers that
-// swap roles every round.  A starts each round equal to the result of two
-// rounds ago; the variables changed in the previous round are marked in a
-// bitmap, and this round also folds S into A for exactly those variables
-// (max/min commute, so that can happen in any order with the candidates).
+// A round has two phases.
+//   sweep    every row reads the bounds of the round's snapshot, computes its
+//            activities and the candidate bounds of its columns, and folds
+//            accepted candidates into the accumulator with native atomics
+//            (int32 max/min for integer columns, global_atomic_{max,min}_f64
+//            for the others); a bitmap records the columns that moved.
+//   publish  the columns of the bitmap are copied from the accumulator into
+//            the snapshot and checked for crossed bounds (lb > ub + eps).
+// Rows therefore see exactly the bounds at the end of the previous round:
+// the round-synchronous semantics of the oracle.
 //
-// Work is split into 64-lane wave items:
-//   group item  rows of tree width w = next_pow2(len) <= 512, processed by
-//               groups of g = 2^logg lanes with K = w/g consecutive entries
-//               per lane; the activity of a row is the pairwise tree over its
+// Sweep work is split into 64-lane wave items:
+//   group item  rows of tree width w = next_pow2(len) <= kSegLen, processed by
+//               groups of g = 2^logg lanes with K = w / g consecutive entries
+//               per lane; a row's activity is the pairwise tree over its
 //               entries in natural order: K entries in registers, then an
 //               xor butterfly over the g lanes.
-//   long item   one row longer than kSegLen: segments of kSegLen entries are
-//               reduced like a 512-wide group row and summed left to right.
-//   fixup item  64 bitmap words of the previous round's changed variables.
+//   seg item    one kSegLen-entry segment of a longer row; segment sums are
+//               added left to right by the last segment wave to arrive.
+//   long item   a row with too many segments for split processing, reduced
+//               segment by segment by one wave.
 // The summation order is thus a function of the row length only, whatever
-// (g, K) a bin uses, which makes every result reproducible bit for bit.
+// (g, K) a bin uses, which makes results reproducible bit for bit.
 //
 // Reference counterpart: GPUAtomicDomainPropagation
 // (src/kernels/atomic_kernel.cuh:9-450) and the megakernel
@@ -30,14 +34,18 @@
 
 namespace gdp {
 
-__device__ __forceinline__ double shfl_xor_d(double v, int mask) {
-    return __shfl_xor(v, mask, 64);
-}
+__device__ __forceinline__ double shfl_xor_d(double v, int mask) { return __shfl_xor(v, mask, 64); }
 
 __device__ __forceinline__ void atomic_max_d(double *p, double v) {
     __hip_atomic_fetch_max(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ void atomic_min_d(double *p, double v) {
+    __hip_atomic_fetch_min(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void atomic_max_i(int *p, int v) {
+    __hip_atomic_fetch_max(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void atomic_min_i(int *p, int v) {
     __hip_atomic_fetch_min(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ void atomic_or_u(unsigned *p, unsigned v) {
@@ -71,6 +79,95 @@ __device__ __forceinline__ void coherent_store_d(double *p, double v) {
 // waits until every memory operation this wave issued has completed
 __device__ __forceinline__ void mem_wait() { __builtin_amdgcn_s_waitcnt(0); }
 
+// ---------------------------------------------------------------------------
+// bounds snapshot and accumulator
+// ---------------------------------------------------------------------------
+
+// Bounds of internal column `id` at the start of the round.
+__device__ __forceinline__ void load_bounds(const DevMatrix &M, const DevState &D, int id, double &l, double &u) {
+    if (id < M.n16) {
+        const unsigned w = D.S16[id];
+        l = static_cast<double>(static_cast<short>(w & 0xffffu));
+        u = static_cast<double>(static_cast<short>(w >> 16));
+    } else if (id < M.n_int) {
+        const int2 w = D.S32[id - M.n16];
+        l = static_cast<double>(w.x);
+        u = static_cast<double>(w.y);
+    } else {
+        const double2 w = D.SF[id - M.n_int];
+        l = w.x;
+        u = w.y;
+    }
+}
+
+__device__ __forceinline__ bool fits_int32(double v) { return v >= -2147483647.0 && v <= 2147483647.0; }
+
+// Folds an accepted lower / upper candidate into the accumulator.
+__device__ __forceinline__ void accumulate(const DevMatrix &M, const DevState &D, int id, int t, double nl,
+                                           double nu) {
+    if (id < M.n_int) {
+        if (t & 1) {
+            if (fits_int32(nl)) atomic_max_i(&D.AI[id].x, static_cast<int>(nl));
+            else atomic_max_d(&D.AX[id].x, nl);
+        }
+        if (t & 2) {
+            if (fits_int32(nu)) atomic_min_i(&D.AI[id].y, static_cast<int>(nu));
+            else atomic_min_d(&D.AX[id].y, nu);
+        }
+    } else {
+        double2 *a = &D.AF[id - M.n_int];
+        if (t & 1) atomic_max_d(&a->x, nl);
+        if (t & 2) atomic_min_d(&a->y, nu);
+    }
+    atomic_or_u(&D.bm[id >> 5], 1u << (id & 31));
+}
+
+// Current accumulated bounds of internal column `id` (device-coherent).
+__device__ __forceinline__ void accumulated(const DevMatrix &M, const DevState &D, int id, double &l, double &u) {
+    if (id < M.n_int) {
+        const int *ai = reinterpret_cast<const int *>(&D.AI[id]);
+        l = static_cast<double>(coherent_load(ai));
+        u = static_cast<double>(coherent_load(ai + 1));
+        const double xl = coherent_load_d(&D.AX[id].x);
+        const double xu = coherent_load_d(&D.AX[id].y);
+        l = xl > l ? xl : l;
+        u = xu < u ? xu : u;
+    } else {
+        const double2 *a = &D.AF[id - M.n_int];
+        l = coherent_load_d(&a->x);
+        u = coherent_load_d(&a->y);
+    }
+}
+
+__device__ __forceinline__ short clamp16(double v) {
+    return static_cast<short>(v < -32767.0 ? -32767.0 : (v > 32767.0 ? 32767.0 : v));
+}
+
+__device__ __forceinline__ unsigned pack16(double l, double u) {
+    return static_cast<unsigned short>(clamp16(l)) | (static_cast<unsigned>(static_cast<unsigned short>(clamp16(u))) << 16);
+}
+
+// Writes bounds into the snapshot record of `id` (device-coherent).  Values
+// out of a record's range only occur once bounds have crossed, after which
+// the run stops and the snapshot is not read again.
+__device__ __forceinline__ void store_snapshot(const DevMatrix &M, const DevState &D, int id, double l, double u) {
+    if (id < M.n16) {
+        coherent_store(&D.S16[id], pack16(l, u));
+    } else if (id < M.n_int) {
+        int *s = reinterpret_cast<int *>(&D.S32[id - M.n16]);
+        coherent_store(s, fits_int32(l) ? static_cast<int>(l) : -2147483647);
+        coherent_store(s + 1, fits_int32(u) ? static_cast<int>(u) : 2147483647);
+    } else {
+        double2 *s = &D.SF[id - M.n_int];
+        coherent_store_d(&s->x, l);
+        coherent_store_d(&s->y, u);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// row gate
+// ---------------------------------------------------------------------------
+
 // Row gate with the sequential propagator's epsilon-maximum of |a|(u-l).
 // That maximum m depends on the order of the entries but always lies in
 // [lo, M] with M the exact maximum (computed in parallel, order-free) and
@@ -78,55 +175,116 @@ __device__ __forceinline__ void mem_wait() { __builtin_amdgcn_s_waitcnt(0); }
 // last argument, so when row_gate(lo) == row_gate(M) the answer is known.
 // Otherwise (entries within 2*eps of the decision threshold) the row is
 // rescanned in CSR order exactly as the sequential propagator does.
-__device__ __noinline__ double seq_eps_max(const DevMatrix &M, const double2 *S, int start, int len) {
+__device__ __forceinline__ double seq_eps_max(const DevMatrix &M, const DevState &D, int start, int len) {
     double run = 0.0;
     for (int e = 0; e < len; e++) {
-        const int jp = M.col[start + e];
-        const double2 b = S[jp & kColMask];
-        run = eps_max_step(run, fabs(M.val[start + e]) * (b.y - b.x));
+        double l, u;
+        load_bounds(M, D, M.col[start + e] & kIdMask, l, u);
+        run = eps_max_step(run, fabs(M.val[start + e]) * (u - l));
     }
     return run;
 }
 
-__device__ __forceinline__ bool decide_gate(const DevMatrix &M, const double2 *S, int start, int len,
-                                            double minact, double maxact, int mininf, int maxinf,
-                                            double lhs, double rhs, double dmax) {
+__device__ __forceinline__ bool decide_gate(const DevMatrix &M, const DevState &D, int start, int len,
+                                            double minact, double maxact, int mininf, int maxinf, double lhs,
+                                            double rhs, double dmax) {
     const bool hi = row_gate(minact, maxact, mininf, maxinf, lhs, rhs, dmax);
     double lo_d = dmax;
     if (isfinite(dmax)) lo_d = dmax - 4e-9 - fabs(dmax) * 0x1p-48;
     const bool lo = row_gate(minact, maxact, mininf, maxinf, lhs, rhs, lo_d);
     if (hi == lo) return hi;
-    const double m = seq_eps_max(M, S, start, len);
+    const double m = seq_eps_max(M, D, start, len);
     return row_gate(minact, maxact, mininf, maxinf, lhs, rhs, m);
 }
 
-struct RoundCtx {
-    const double2 *S;
-    double2 *A;
-    unsigned *bm_cur;
-    unsigned *bm_prev;
-    int *status;  // status words of this round
-};
-
 // Applies the candidates of one entry; returns true when a bound moved.
-__device__ __forceinline__ bool apply_entry(const RoundCtx &c, int jp, double a, double l, double u,
-                                            double lhs, double rhs, double minact, double maxact,
+__device__ __forceinline__ bool apply_entry(const DevMatrix &M, const DevState &D, int jp, double a, double l,
+                                            double u, double lhs, double rhs, double minact, double maxact,
                                             int mininf, int maxinf) {
-    const int j = jp & kColMask;
     const bool is_cont = jp >= 0;  // bit 31 set marks an integer column
     double nl = l, nu = u;
     const int t = tighten_variable(a, lhs, rhs, minact, maxact, mininf, maxinf, is_cont, l, u, nl, nu);
     if (t == 0) return false;
-    if (t & 1) atomic_max_d(&c.A[j].x, nl);
-    if (t & 2) atomic_min_d(&c.A[j].y, nu);
-    atomic_or_u(&c.bm_cur[j >> 5], 1u << (j & 31));
+    accumulate(M, D, jp & kIdMask, t, nl, nu);
     return true;
 }
+
+// ---------------------------------------------------------------------------
+// sweep items
+// ---------------------------------------------------------------------------
+
+// K consecutive entries of one lane: loads, contributions, pairwise tree of
+// the K contributions in natural order, and the cross-lane reduction.
+template <int K>
+struct LaneEntries {
+    int jp[K];
+    double a[K], l[K], u[K];
+    double tmin, tmax, dmax;
+    int mininf, maxinf;
+
+    __device__ __forceinline__ void load(const DevMatrix &M, const DevState &D, int base, int first, int len) {
+        double smin[K], smax[K];
+        mininf = 0;
+        maxinf = 0;
+        dmax = 0.0;
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const int e = first + k;
+            smin[k] = 0.0;
+            smax[k] = 0.0;
+            jp[k] = 0;
+            a[k] = 0.0;
+            l[k] = 0.0;
+            u[k] = 0.0;
+            if (e < len) {
+                jp[k] = M.col[base + e];
+                a[k] = M.val[base + e];
+                load_bounds(M, D, jp[k] & kIdMask, l[k], u[k]);
+                const Contribution q = contribution(a[k], l[k], u[k]);
+                smin[k] = q.min_part;
+                smax[k] = q.max_part;
+                mininf += q.min_inf;
+                maxinf += q.max_inf;
+                dmax = fmax(dmax, q.delta);
+            }
+        }
+#pragma unroll
+        for (int w = 1; w < K; w <<= 1) {
+#pragma unroll
+            for (int k = 0; k < K; k += 2 * w) {
+                smin[k] = smin[k] + smin[k + w];
+                smax[k] = smax[k] + smax[k + w];
+            }
+        }
+        tmin = smin[0];
+        tmax = smax[0];
+    }
+
+    // xor butterfly over groups of g lanes (natural pairwise order)
+    __device__ __forceinline__ void reduce(int g) {
+        for (int h = 1; h < g; h <<= 1) {
+            tmin = tmin + shfl_xor_d(tmin, h);
+            tmax = tmax + shfl_xor_d(tmax, h);
+            mininf += __shfl_xor(mininf, h, 64);
+            maxinf += __shfl_xor(maxinf, h, 64);
+            dmax = fmax(dmax, shfl_xor_d(dmax, h));
+        }
+    }
+
+    __device__ __forceinline__ bool apply(const DevMatrix &M, const DevState &D, int first, int len, double lhs,
+                                          double rhs, double minact, double maxact, int ni, int xi) {
+        bool moved = false;
+#pragma unroll
+        for (int k = 0; k < K; k++)
+            if (first + k < len) moved |= apply_entry(M, D, jp[k], a[k], l[k], u[k], lhs, rhs, minact, maxact, ni, xi);
+        return moved;
+    }
+};
 
 // Rows [first, first + nrows) of one tree-width bin: groups of 2^logg lanes,
 // K consecutive entries per lane.
 template <int K>
-__device__ bool group_item(const DevMatrix &M, const RoundCtx &c, int first, int nrows, int logg) {
+__device__ bool group_item(const DevMatrix &M, const DevState &D, int first, int nrows, int logg) {
     const int lane = threadIdx.x & 63;
     const int g = 1 << logg;
     const int gl = lane & (g - 1);
@@ -145,68 +303,21 @@ __device__ bool group_item(const DevMatrix &M, const RoundCtx &c, int first, int
             lhs = sd.x;
             rhs = sd.y;
         }
-        int jp[K];
-        double a[K], l[K], u[K], smin[K], smax[K];
-        int mininf = 0, maxinf = 0;
-        double dmax = 0.0;
-#pragma unroll
-        for (int k = 0; k < K; k++) {
-            const int e = gl * K + k;
-            smin[k] = 0.0;
-            smax[k] = 0.0;
-            jp[k] = 0;
-            a[k] = 0.0;
-            l[k] = 0.0;
-            u[k] = 0.0;
-            if (e < len) {
-                jp[k] = M.col[start + e];
-                a[k] = M.val[start + e];
-                const double2 b = c.S[jp[k] & kColMask];
-                l[k] = b.x;
-                u[k] = b.y;
-                const Contribution q = contribution(a[k], l[k], u[k]);
-                smin[k] = q.min_part;
-                smax[k] = q.max_part;
-                mininf += q.min_inf;
-                maxinf += q.max_inf;
-                dmax = fmax(dmax, q.delta);
-            }
-        }
-        // pairwise tree over the K entries of this lane
-#pragma unroll
-        for (int w = 1; w < K; w <<= 1) {
-#pragma unroll
-            for (int k = 0; k < K; k += 2 * w) {
-                smin[k] = smin[k] + smin[k + w];
-                smax[k] = smax[k] + smax[k + w];
-            }
-        }
-        double tmin = smin[0], tmax = smax[0];
-        for (int h = 1; h < g; h <<= 1) {
-            tmin = tmin + shfl_xor_d(tmin, h);
-            tmax = tmax + shfl_xor_d(tmax, h);
-            mininf += __shfl_xor(mininf, h, 64);
-            maxinf += __shfl_xor(maxinf, h, 64);
-            dmax = fmax(dmax, shfl_xor_d(dmax, h));
-        }
-        const double minact = 0.0 + tmin;
-        const double maxact = 0.0 + tmax;
+        LaneEntries<K> E;
+        E.load(M, D, start, gl * K, len);
+        E.reduce(g);
+        const double minact = 0.0 + E.tmin;
+        const double maxact = 0.0 + E.tmax;
         if (valid && len > 0 &&
-            decide_gate(M, c.S, start, len, minact, maxact, mininf, maxinf, lhs, rhs, dmax)) {
-#pragma unroll
-            for (int k = 0; k < K; k++) {
-                if (gl * K + k < len)
-                    moved |= apply_entry(c, jp[k], a[k], l[k], u[k], lhs, rhs, minact, maxact, mininf,
-                                         maxinf);
-            }
-        }
+            decide_gate(M, D, start, len, minact, maxact, E.mininf, E.maxinf, lhs, rhs, E.dmax))
+            moved |= E.apply(M, D, gl * K, len, lhs, rhs, minact, maxact, E.mininf, E.maxinf);
     }
     return moved;
 }
 
-// One row longer than kSegLen, processed by one wave: segment sums, then a
-// second pass for the candidates.
-__device__ bool long_item(const DevMatrix &M, const RoundCtx &c, int r) {
+// One row with too many segments to split: one wave, segment by segment,
+// then a second pass for the candidates.
+__device__ bool long_item(const DevMatrix &M, const DevState &D, int r) {
     constexpr int K = kSegLen / 64;
     const int lane = threadIdx.x & 63;
     const int start = M.row_ptr[r];
@@ -215,65 +326,33 @@ __device__ bool long_item(const DevMatrix &M, const RoundCtx &c, int r) {
     double acc_min = 0.0, acc_max = 0.0, dmax = 0.0;
     int mininf = 0, maxinf = 0;
     for (int s0 = 0; s0 < len; s0 += kSegLen) {
-        double smin[K], smax[K];
-#pragma unroll
-        for (int k = 0; k < K; k++) {
-            const int e = s0 + lane * K + k;
-            smin[k] = 0.0;
-            smax[k] = 0.0;
-            if (e < len) {
-                const int jp = M.col[start + e];
-                const double a = M.val[start + e];
-                const double2 b = c.S[jp & kColMask];
-                const Contribution q = contribution(a, b.x, b.y);
-                smin[k] = q.min_part;
-                smax[k] = q.max_part;
-                mininf += q.min_inf;
-                maxinf += q.max_inf;
-                dmax = fmax(dmax, q.delta);
-            }
-        }
-#pragma unroll
-        for (int w = 1; w < K; w <<= 1) {
-#pragma unroll
-            for (int k = 0; k < K; k += 2 * w) {
-                smin[k] = smin[k] + smin[k + w];
-                smax[k] = smax[k] + smax[k + w];
-            }
-        }
-        double tmin = smin[0], tmax = smax[0];
-#pragma unroll
-        for (int h = 1; h < 64; h <<= 1) {
-            tmin = tmin + shfl_xor_d(tmin, h);
-            tmax = tmax + shfl_xor_d(tmax, h);
-        }
-        acc_min = acc_min + tmin;
-        acc_max = acc_max + tmax;
-    }
-#pragma unroll
-    for (int h = 1; h < 64; h <<= 1) {
-        mininf += __shfl_xor(mininf, h, 64);
-        maxinf += __shfl_xor(maxinf, h, 64);
-        dmax = fmax(dmax, shfl_xor_d(dmax, h));
+        LaneEntries<K> E;
+        E.load(M, D, start + s0, lane * K, len - s0);
+        E.reduce(64);
+        acc_min = acc_min + E.tmin;
+        acc_max = acc_max + E.tmax;
+        mininf += E.mininf;
+        maxinf += E.maxinf;
+        dmax = fmax(dmax, E.dmax);
     }
     bool moved = false;
-    if (decide_gate(M, c.S, start, len, acc_min, acc_max, mininf, maxinf, sd.x, sd.y, dmax)) {
+    if (decide_gate(M, D, start, len, acc_min, acc_max, mininf, maxinf, sd.x, sd.y, dmax)) {
         for (int e = lane; e < len; e += 64) {
             const int jp = M.col[start + e];
-            const double a = M.val[start + e];
-            const double2 b = c.S[jp & kColMask];
-            moved |= apply_entry(c, jp, a, b.x, b.y, sd.x, sd.y, acc_min, acc_max, mininf, maxinf);
+            double l, u;
+            load_bounds(M, D, jp & kIdMask, l, u);
+            moved |= apply_entry(M, D, jp, M.val[start + e], l, u, sd.x, sd.y, acc_min, acc_max, mininf, maxinf);
         }
     }
     return moved;
 }
 
-// One segment of a split row.  The wave reduces its kSegLen entries exactly
-// like a 512-wide group row, stores the partial, and counts itself in.  The
-// last wave of the row to arrive adds the partials in segment order, decides
-// the row gate and publishes the row's state; the others wait for it (their
-// entries stay in registers) and then apply their candidates.
-__device__ bool seg_item(const DevMatrix &M, const DevState &D, const RoundCtx &c, int lr, int s, int round) {
+// One segment of a split row.  The wave reduces its kSegLen entries like a
+// kSegLen-wide group row, stores the partial and counts itself in.  The last
+// wave of the row to arrive adds the partials in segment order, decides the
+// row gate and publishes the row state; the others wait for it (their entries
+// stay in registers) and then apply their candidates.
+__device__ bool seg_item(const DevMatrix &M, const DevState &D, int lr, int s, int round) {
     constexpr int K = kSegLen / 64;
     const int lane = threadIdx.x & 63;
     const int4 info = M.seg_rows[lr];
@@ -281,59 +360,18 @@ __device__ bool seg_item(const DevMatrix &M, const DevState &D, const RoundCtx &
     const int start = M.row_ptr[r];
     const int len = M.row_ptr[r + 1] - start;
     const int s0 = s * kSegLen;
-    int jp[K];
-    double a[K], l[K], u[K], smin[K], smax[K];
-    int mininf = 0, maxinf = 0;
-    double dmax = 0.0;
-#pragma unroll
-    for (int k = 0; k < K; k++) {
-        const int e = s0 + lane * K + k;
-        smin[k] = 0.0;
-        smax[k] = 0.0;
-        jp[k] = 0;
-        a[k] = 0.0;
-        l[k] = 0.0;
-        u[k] = 0.0;
-        if (e < len) {
-            jp[k] = M.col[start + e];
-            a[k] = M.val[start + e];
-            const double2 b = c.S[jp[k] & kColMask];
-            l[k] = b.x;
-            u[k] = b.y;
-            const Contribution q = contribution(a[k], l[k], u[k]);
-            smin[k] = q.min_part;
-            smax[k] = q.max_part;
-            mininf += q.min_inf;
-            maxinf += q.max_inf;
-            dmax = fmax(dmax, q.delta);
-        }
-    }
-#pragma unroll
-    for (int w = 1; w < K; w <<= 1) {
-#pragma unroll
-        for (int k = 0; k < K; k += 2 * w) {
-            smin[k] = smin[k] + smin[k + w];
-            smax[k] = smax[k] + smax[k + w];
-        }
-    }
-    double tmin = smin[0], tmax = smax[0];
-#pragma unroll
-    for (int h = 1; h < 64; h <<= 1) {
-        tmin = tmin + shfl_xor_d(tmin, h);
-        tmax = tmax + shfl_xor_d(tmax, h);
-        mininf += __shfl_xor(mininf, h, 64);
-        maxinf += __shfl_xor(maxinf, h, 64);
-        dmax = fmax(dmax, shfl_xor_d(dmax, h));
-    }
+    LaneEntries<K> E;
+    E.load(M, D, start + s0, lane * K, len - s0);
+    E.reduce(64);
     int ok = 1;
     unsigned old = 0;
     SegPart *part = &D.seg_parts[info.y + s];
     if (lane == 0) {
-        coherent_store_d(&part->smin, tmin);
-        coherent_store_d(&part->smax, tmax);
-        coherent_store_d(&part->dmax, dmax);
-        coherent_store(&part->mininf, mininf);
-        coherent_store(&part->maxinf, maxinf);
+        coherent_store_d(&part->smin, E.tmin);
+        coherent_store_d(&part->smax, E.tmax);
+        coherent_store_d(&part->dmax, E.dmax);
+        coherent_store(&part->mininf, E.mininf);
+        coherent_store(&part->maxinf, E.maxinf);
         mem_wait();
         old = __hip_atomic_fetch_add(&D.seg_count[lr], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
@@ -344,8 +382,8 @@ __device__ bool seg_item(const DevMatrix &M, const DevState &D, const RoundCtx &
         // loads), which are then added in segment order
         double am = 0.0, ax = 0.0, dm = 0.0;
         int ni = 0, xi = 0;
-        for (int base = 0; base < info.z; base += 64) {
-            const int q = base + lane;
+        for (int b = 0; b < info.z; b += 64) {
+            const int q = b + lane;
             double pmin = 0.0, pmax = 0.0;
             if (q < info.z) {
                 const SegPart *pq = &D.seg_parts[info.y + q];
@@ -355,7 +393,7 @@ __device__ bool seg_item(const DevMatrix &M, const DevState &D, const RoundCtx &
                 ni += coherent_load(&pq->mininf);
                 xi += coherent_load(&pq->maxinf);
             }
-            const int cnt = min(64, info.z - base);
+            const int cnt = min(64, info.z - b);
             for (int t = 0; t < cnt; t++) {
                 am = am + __shfl(pmin, t, 64);
                 ax = ax + __shfl(pmax, t, 64);
@@ -369,7 +407,7 @@ __device__ bool seg_item(const DevMatrix &M, const DevState &D, const RoundCtx &
         }
         if (lane == 0) {
             const double2 sd = M.sides[r];
-            const int gate = decide_gate(M, c.S, start, len, am, ax, ni, xi, sd.x, sd.y, dm) ? 1 : 0;
+            const int gate = decide_gate(M, D, start, len, am, ax, ni, xi, sd.x, sd.y, dm) ? 1 : 0;
             coherent_store_d(&stp->minact, am);
             coherent_store_d(&stp->maxact, ax);
             coherent_store(&stp->mininf, ni);
@@ -383,7 +421,7 @@ __device__ bool seg_item(const DevMatrix &M, const DevState &D, const RoundCtx &
         const long long t0 = wall_clock64();
         while (coherent_load(&D.seg_flag[lr]) != round) {
             __builtin_amdgcn_s_sleep(2);
-            if (coherent_load(&D.status[kGlobalError]) || wall_clock64() - t0 > kBarrierTimeoutTicks) {
+            if (coherent_load(&D.status[kGlobalError]) || wall_clock64() - t0 > kWaitTimeoutTicks) {
                 atomic_or_i(&D.status[kGlobalError], 1);
                 ok = 0;
                 break;
@@ -392,116 +430,94 @@ __device__ bool seg_item(const DevMatrix &M, const DevState &D, const RoundCtx &
     }
     ok = __shfl(ok, 0, 64);
     if (!ok) return false;
-    SegState st;
-    st.minact = coherent_load_d(&stp->minact);
-    st.maxact = coherent_load_d(&stp->maxact);
-    st.mininf = coherent_load(&stp->mininf);
-    st.maxinf = coherent_load(&stp->maxinf);
-    st.gate = coherent_load(&stp->gate);
-    bool moved = false;
-    if (st.gate) {
-        const double2 sd = M.sides[r];
-#pragma unroll
-        for (int k = 0; k < K; k++) {
-            if (s0 + lane * K + k < len)
-                moved |= apply_entry(c, jp[k], a[k], l[k], u[k], sd.x, sd.y, st.minact, st.maxact, st.mininf,
-                                     st.maxinf);
-        }
-    }
-    return moved;
+    const int gate = coherent_load(&stp->gate);
+    if (!gate) return false;
+    const double minact = coherent_load_d(&stp->minact);
+    const double maxact = coherent_load_d(&stp->maxact);
+    const int ni = coherent_load(&stp->mininf);
+    const int xi = coherent_load(&stp->maxinf);
+    const double2 sd = M.sides[r];
+    return E.apply(M, D, lane * K, len - s0, sd.x, sd.y, minact, maxact, ni, xi);
 }
 
-// Folds the previous round's result into the accumulator for the variables
-// it changed, checks them for crossed bounds and clears the bitmap words.
-__device__ void fixup_item(const DevMatrix &M, const RoundCtx &c, int f) {
+__device__ bool sweep_item(const DevMatrix &M, const DevState &D, int it, int round) {
+    const int4 d = M.items[it];
+    const int kind = d.x & 15;
+    if (kind == kItemSeg) return seg_item(M, D, d.y, d.z, round);
+    if (kind == kItemLong) return long_item(M, D, d.y);
+    const int logg = (d.x >> 4) & 15;
+    const int logk = (d.x >> 8) & 15;
+    switch (logk) {
+        case 0: return group_item<1>(M, D, d.y, d.z, logg);
+        case 1: return group_item<2>(M, D, d.y, d.z, logg);
+        default: return group_item<4>(M, D, d.y, d.z, logg);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// publish items
+// ---------------------------------------------------------------------------
+
+// Copies the accumulated bounds of the columns that moved this round into the
+// snapshot, checks them for crossed bounds and clears the bitmap words.
+__device__ void publish_item(const DevMatrix &M, const DevState &D, int f, int *round_status) {
     const int w = f * 64 + (threadIdx.x & 63);
-    if (w >= M.nwords) return;
-    unsigned bits = coherent_load(&c.bm_prev[w]);
-    if (bits == 0) return;
-    coherent_store(&c.bm_prev[w], 0u);
     bool crossed = false;
-    while (bits) {
-        const int b = __builtin_ctz(bits);
-        bits &= bits - 1;
-        const int j = w * 32 + b;
-        double2 s;
-        s.x = coherent_load_d(&c.S[j].x);
-        s.y = coherent_load_d(&c.S[j].y);
-        crossed |= eps_gt(s.x, s.y);
-        atomic_max_d(&c.A[j].x, s.x);
-        atomic_min_d(&c.A[j].y, s.y);
-    }
-    if (crossed) atomic_or_i(&c.status[kStatusInfeasPrev], 1);
-}
-
-__device__ __forceinline__ RoundCtx round_ctx(const DevState &D, int round) {
-    RoundCtx c;
-    const bool odd = round & 1;
-    c.S = odd ? D.bnd0 : D.bnd1;
-    c.A = odd ? D.bnd1 : D.bnd0;
-    c.bm_cur = odd ? D.bm1 : D.bm0;
-    c.bm_prev = odd ? D.bm0 : D.bm1;
-    c.status = D.status + kStatusWords * round;
-    return c;
-}
-
-// Processes work item `it` of a round (row items, then fixup items).
-__device__ bool process_item(const DevMatrix &M, const DevState &D, const RoundCtx &c, int it, int round) {
-    if (it < M.n_items) {
-        const int4 d = M.items[it];
-        const int kind = d.x & 15;
-        if (kind == kItemSeg) return seg_item(M, D, c, d.y, d.z, round);
-        if (kind == kItemLong) return long_item(M, c, d.y);
-        const int logg = (d.x >> 4) & 15;
-        const int logk = (d.x >> 8) & 15;
-        switch (logk) {
-            case 0: return group_item<1>(M, c, d.y, d.z, logg);
-            case 1: return group_item<2>(M, c, d.y, d.z, logg);
-            default: return group_item<4>(M, c, d.y, d.z, logg);
+    if (w < M.nwords) {
+        unsigned bits = coherent_load(&D.bm[w]);
+        if (bits) {
+            coherent_store(&D.bm[w], 0u);
+            while (bits) {
+                const int b = __builtin_ctz(bits);
+                bits &= bits - 1;
+                const int id = w * 32 + b;
+                double l, u;
+                accumulated(M, D, id, l, u);
+                crossed |= eps_gt(l, u);
+                store_snapshot(M, D, id, l, u);
+            }
         }
     }
-    const int f = it - M.n_items;
-    if (f < M.n_fix_items) fixup_item(M, c, f);
-    return false;
+    if (__any(crossed) && (threadIdx.x & 63) == 0) atomic_or_i(&round_status[kStatusInfeasible], 1);
 }
 
-__device__ __forceinline__ void flag_change(const RoundCtx &c, bool moved) {
-    if (__any(moved) && (threadIdx.x & 63) == 0) atomic_or_i(&c.status[kStatusChanged], 1);
+__device__ __forceinline__ void flag_change(int *round_status, bool moved) {
+    if (__any(moved) && (threadIdx.x & 63) == 0) atomic_or_i(&round_status[kStatusChanged], 1);
 }
 
-// One round (CPU_LOOP and GPU_LOOP).  With `guard`, the kernel first checks
-// on the device whether the previous round ended the run and then does
-// nothing; a captured sequence of such launches stops by itself.
-__global__ __launch_bounds__(256) void round_kernel(DevMatrix M, DevState D, int round, int guard) {
-    if (guard && round > 1) {
-        const int *prev = D.status + kStatusWords * (round - 1);
-        if (D.status[kGlobalDone] != 0 || D.status[kGlobalError] != 0 ||
-            prev[kStatusInfeasPrev] != 0 || prev[kStatusChanged] == 0)
-            return;
-    }
-    const RoundCtx c = round_ctx(D, round);
-    const int waves = blockDim.x >> 6;
-    const int it = blockIdx.x * waves + (threadIdx.x >> 6);
+// With `guard` (captured graphs), a launch first checks on the device whether
+// the run already ended before this round and then does nothing.
+__device__ __forceinline__ bool round_skipped(const DevState &D, int round) {
+    if (D.status[kGlobalError] != 0 || D.status[kGlobalDone] != 0 || D.status[kGlobalEnvelope] != 0) return true;
+    if (round <= 1) return false;
+    const int *prev = D.status + kStatusWords * (round - 1);
+    return prev[kStatusInfeasible] != 0 || prev[kStatusChanged] == 0;
+}
+
+__global__ __launch_bounds__(256) void sweep_kernel(DevMatrix M, DevState D, int round, int guard) {
+    if (guard && round_skipped(D, round)) return;
+    const int it = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     bool moved = false;
-    if (it < M.n_items + M.n_fix_items) moved = process_item(M, D, c, it, round);
-    flag_change(c, moved);
+    if (it < M.n_items) moved = sweep_item(M, D, it, round);
+    flag_change(D.status + kStatusWords * round, moved);
 }
 
-// Fixup pass only: checks the result of the last capped round.
-__global__ __launch_bounds__(256) void fixup_kernel(DevMatrix M, DevState D, int round) {
-    const RoundCtx c = round_ctx(D, round);
-    const int waves = blockDim.x >> 6;
-    const int it = blockIdx.x * waves + (threadIdx.x >> 6);
-    if (it < M.n_fix_items) fixup_item(M, c, it);
+__global__ __launch_bounds__(256) void publish_kernel(DevMatrix M, DevState D, int round, int guard) {
+    if (guard && round_skipped(D, round)) return;
+    const int f = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (f < M.n_pub_items) publish_item(M, D, f, D.status + kStatusWords * round);
 }
 
-// Device-wide barrier for the persistent kernel: arrival counter plus a
-// generation word, waiting bounded by wall-clock time so that a fault
-// elsewhere ends in an error flag instead of a hang.  Everything the rounds
-// exchange is written with atomics or device-coherent stores, so no cache
-// write-back is needed; each workgroup invalidates its caches once after the
-// barrier so that the next round's gathers of the snapshot buffer cannot hit
-// lines cached while that buffer was last read.
+// ---------------------------------------------------------------------------
+// persistent kernel
+// ---------------------------------------------------------------------------
+
+// Device-wide barrier: arrival counter plus a generation word; waiting is
+// bounded by wall-clock time so that a fault elsewhere ends in an error flag
+// instead of a hang.  Everything the phases exchange is written with atomics
+// or device-coherent stores, so no cache write-back is needed; each
+// workgroup invalidates its caches once after the barrier so that its next
+// gathers of the snapshot cannot hit lines cached before the publish.
 __device__ bool grid_barrier(const DevState &D, unsigned nblocks) {
     __shared__ int ok_sh;
     mem_wait();
@@ -524,7 +540,7 @@ __device__ bool grid_barrier(const DevState &D, unsigned nblocks) {
                     ok = 0;
                     break;
                 }
-                if (wall_clock64() - t0 > kBarrierTimeoutTicks) {
+                if (wall_clock64() - t0 > kWaitTimeoutTicks) {
                     atomic_or_i(&D.status[kGlobalError], 1);
                     ok = 0;
                     break;
@@ -538,71 +554,70 @@ __device__ bool grid_barrier(const DevState &D, unsigned nblocks) {
     return ok_sh != 0;
 }
 
-__device__ __forceinline__ int load_status(const int *p) { return coherent_load(p); }
-
 // All rounds in one persistent launch (MEGAKERNEL).  Every workgroup walks
-// the items of a round with a grid stride, then all meet at a barrier and
-// read the same status words, so all take the same decision to go on.
-// Outcome words (status row 0): rounds, result buffer, infeasible, capped.
+// the items of a phase with a grid stride (consecutive items on consecutive
+// waves), all meet at a barrier, and all read the same status words, so all
+// take the same decision to go on.  Outcome words in status row 0.
 __global__ __launch_bounds__(256) void mega_kernel(DevMatrix M, DevState D, int max_rounds) {
     const unsigned nblocks = gridDim.x;
     const int waves = blockDim.x >> 6;
     const int wave = blockIdx.x * waves + (threadIdx.x >> 6);
     const int stride = gridDim.x * waves;
-    const int total = M.n_items + M.n_fix_items;
-    int rounds = 0, result_buf = 0, infeasible = 0, capped = 0;
-    for (int round = 1; round <= max_rounds; round++) {
-        const RoundCtx c = round_ctx(D, round);
-        bool moved = false;
-        for (int it = wave; it < total; it += stride) moved |= process_item(M, D, c, it, round);
-        flag_change(c, moved);
-        if (!grid_barrier(D, nblocks)) { rounds = -1; break; }
-        const int *st = D.status + kStatusWords * round;
-        if (load_status(st + kStatusInfeasPrev)) {
-            rounds = round - 1; result_buf = (round - 1) & 1; infeasible = 1;
-            break;
-        }
-        rounds = round;
-        result_buf = round & 1;
-        if (!load_status(st + kStatusChanged)) break;
-        if (round == max_rounds) {
-            // the cap stopped a run that still changed: check its result
-            const RoundCtx cf = round_ctx(D, round + 1);
-            for (int it = wave; it < M.n_fix_items; it += stride) fixup_item(M, cf, it);
+    int rounds = 0, infeasible = 0, capped = 0;
+    {
+        for (int round = 1; round <= max_rounds; round++) {
+            int *st = D.status + kStatusWords * round;
+            bool moved = false;
+            for (int it = wave; it < M.n_items; it += stride) moved |= sweep_item(M, D, it, round);
+            flag_change(st, moved);
             if (!grid_barrier(D, nblocks)) { rounds = -1; break; }
-            infeasible = load_status(D.status + kStatusWords * (round + 1) + kStatusInfeasPrev) != 0;
-            capped = 1;
+            for (int f = wave; f < M.n_pub_items; f += stride) publish_item(M, D, f, st);
+            if (!grid_barrier(D, nblocks)) { rounds = -1; break; }
+            rounds = round;
+            if (coherent_load(&st[kStatusInfeasible])) { infeasible = 1; break; }
+            if (!coherent_load(&st[kStatusChanged])) break;
+            if (round == max_rounds) capped = 1;
         }
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         D.status[kOutRounds] = rounds;
-        D.status[kOutResultBuf] = result_buf;
         D.status[kOutInfeasible] = infeasible;
         D.status[kOutCapped] = capped;
     }
 }
 
-}  // namespace gdp
+// ---------------------------------------------------------------------------
+// start and end of a run
+// ---------------------------------------------------------------------------
 
-namespace gdp {
-
-// Start of a run: both bound buffers get the input (lb, ub) pairs; bitmaps,
-// status words and the barrier are cleared.  One fill kernel instead of
-// memsets keeps the sequence safe to capture.
-__global__ __launch_bounds__(256) void init_kernel(DevMatrix M, DevState D, const double *lbs,
-                                                   const double *ubs) {
+// Loads the input bounds (original column order) into accumulator and
+// snapshot; clears bitmap, status words, barrier and split-row counters.
+// Sets the envelope flag when input bounds do not fit a column's storage
+// class (that word is cleared by the preceding clear_envelope_kernel).
+__global__ __launch_bounds__(256) void init_kernel(DevMatrix M, DevState D, const double *lbs, const double *ubs) {
     const int tid = blockIdx.x * blockDim.x + threadIdx.x;
     const int stride = gridDim.x * blockDim.x;
-    for (int j = tid; j < M.n; j += stride) {
-        const double2 b = make_double2(lbs[j], ubs[j]);
-        D.bnd0[j] = b;
-        D.bnd1[j] = b;
+    for (int k = tid; k < kStatusRows * kStatusWords; k += stride)
+        if (k != kGlobalEnvelope && k != kGlobalCrossed) D.status[k] = 0;
+    for (int id = tid; id < M.n; id += stride) {
+        const int j = M.perm[id];
+        const double l = lbs[j], u = ubs[j];
+        if (eps_gt(l, u)) atomic_or_i(&D.status[kGlobalCrossed], 1);
+        if (id < M.n_int) {
+            const double lim = id < M.n16 ? 32767.0 : 2147483647.0;
+            const bool fits = l >= -lim && l <= lim && u >= -lim && u <= lim && floor(l) == l && floor(u) == u;
+            if (!fits) atomic_or_i(&D.status[kGlobalEnvelope], 1);
+            const int il = fits ? static_cast<int>(l) : 0, iu = fits ? static_cast<int>(u) : 0;
+            D.AI[id] = make_int2(il, iu);
+            D.AX[id] = make_double2(-INFINITY, INFINITY);
+            if (id < M.n16) D.S16[id] = pack16(il, iu);
+            else D.S32[id - M.n16] = make_int2(il, iu);
+        } else {
+            D.AF[id - M.n_int] = make_double2(l, u);
+            D.SF[id - M.n_int] = make_double2(l, u);
+        }
     }
-    for (int w = tid; w < M.nwords; w += stride) {
-        D.bm0[w] = 0u;
-        D.bm1[w] = 0u;
-    }
-    for (int k = tid; k < kStatusRows * kStatusWords; k += stride) D.status[k] = 0;
+    for (int w = tid; w < M.nwords; w += stride) D.bm[w] = 0u;
     for (int k = tid; k < M.n_seg_rows; k += stride) {
         D.seg_count[k] = 0u;
         D.seg_flag[k] = 0;
@@ -610,14 +625,33 @@ __global__ __launch_bounds__(256) void init_kernel(DevMatrix M, DevState D, cons
     if (tid < 64) D.barrier[tid] = 0u;
 }
 
-// End of a run: copies the result buffer out to separate lb / ub arrays.
-__global__ __launch_bounds__(256) void unpack_kernel(DevMatrix M, const double2 *bnd, double *lbs, double *ubs) {
+// Clears the two words init_kernel sets from many threads (separate launch).
+__global__ void clear_envelope_kernel(DevState D) {
+    D.status[kGlobalEnvelope] = 0;
+    D.status[kGlobalCrossed] = 0;
+}
+
+// End of a run: accumulated bounds back to original column order.
+__global__ __launch_bounds__(256) void unpack_kernel(DevMatrix M, DevState D, double *lbs, double *ubs) {
     const int tid = blockIdx.x * blockDim.x + threadIdx.x;
     const int stride = gridDim.x * blockDim.x;
-    for (int j = tid; j < M.n; j += stride) {
-        const double2 b = bnd[j];
-        lbs[j] = b.x;
-        ubs[j] = b.y;
+    for (int id = tid; id < M.n; id += stride) {
+        double l, u;
+        if (id < M.n_int) {
+            const int2 a = D.AI[id];
+            const double2 x = D.AX[id];
+            l = static_cast<double>(a.x);
+            u = static_cast<double>(a.y);
+            l = x.x > l ? x.x : l;
+            u = x.y < u ? x.y : u;
+        } else {
+            const double2 a = D.AF[id - M.n_int];
+            l = a.x;
+            u = a.y;
+        }
+        const int j = M.perm[id];
+        lbs[j] = l;
+        ubs[j] = u;
     }
 }
 

@@ -80,6 +80,7 @@ class Tuning(ctypes.Structure):
         ("mega_blocks_per_cu", c_int),
         ("graph_rounds", c_int),
         ("max_split_segs", c_int),
+        ("compact_bounds", c_int),
         ("group_log", c_int * 10),
     ]
 
@@ -94,6 +95,9 @@ class ProblemStats(ctypes.Structure):
         ("n_long_rows", c_int),
         ("bin_rows", c_longlong * 11),
         ("bin_nnz", c_longlong * 11),
+        ("n16", c_int),
+        ("n32", c_int),
+        ("nf", c_int),
     ]
 
 
@@ -120,7 +124,8 @@ def load_library(path: Optional[str] = None) -> ctypes.CDLL:
         getattr(lib, name).restype = c_int
     lib.gdpTuningDefaults.argtypes = [POINTER(Tuning)]
     lib.gdpTuningDefaults.restype = None
-    lib.gdpProblemCreate.argtypes = [c_int, c_int, c_int, vp, vp, vp, vp, vp, vp, POINTER(Tuning), POINTER(vp)]
+    lib.gdpProblemCreate.argtypes = [c_int, c_int, c_int, vp, vp, vp, vp, vp, vp, vp, vp, POINTER(Tuning),
+                                     POINTER(vp)]
     lib.gdpProblemCreate.restype = c_int
     lib.gdpProblemDestroy.argtypes = [vp]
     lib.gdpProblemDestroy.restype = c_int
@@ -311,7 +316,9 @@ def _ptr(a: np.ndarray):
 class Problem:
     """A constraint matrix resident in device memory (gdpProblemCreate)."""
 
-    def __init__(self, inst, tuning: Optional[dict] = None):
+    def __init__(self, inst, tuning: Optional[dict] = None, envelope: bool = True):
+        """envelope: use the instance's bounds as the bound envelope (every
+        vector propagated later must lie inside them)."""
         self.lib = load_library()
         self.m, self.n = int(inst.m), int(inst.n)
         self.col = _arr(inst.col, np.int32)
@@ -329,9 +336,12 @@ class Problem:
             else:
                 setattr(t, k, int(v))
         h = c_void_p()
+        self.env_lb = _arr(inst.lb, np.float64) if envelope else None
+        self.env_ub = _arr(inst.ub, np.float64) if envelope else None
         rc = self.lib.gdpProblemCreate(self.m, self.n, len(self.col), _ptr(self.col), _ptr(self.row_ptr),
                                        _ptr(self.val), _ptr(self.lhs), _ptr(self.rhs), _ptr(self.vartype),
-                                       ctypes.byref(t), ctypes.byref(h))
+                                       _ptr(self.env_lb) if envelope else None,
+                                       _ptr(self.env_ub) if envelope else None, ctypes.byref(t), ctypes.byref(h))
         if rc != GDP_OKAY:
             raise RuntimeError(f"gdpProblemCreate failed: {last_error()}")
         self.handle = h

@@ -33,8 +33,10 @@ typedef struct {
     int item_nnz;            /* target nonzeros per work item (0: default 1024) */
     int mega_blocks_per_cu;  /* persistent grid: workgroups per CU (0: occupancy) */
     int graph_rounds;        /* rounds per captured graph in GPU_LOOP (0: 8) */
-    int max_split_segs;      /* rows with up to this many 512-entry segments are split
-                                across waves (-1: default 256; 0: never split) */
+    int max_split_segs;      /* rows with up to this many 256-entry segments are split
+                                across waves (-1: default 512; 0: never split) */
+    int compact_bounds;      /* 1 (default): integer columns with a finite integral
+                                envelope use 4- or 8-byte bound records; 0: FP64 only */
     int group_log[10];       /* lanes per row (log2) for rows of width 2^b; -1: default */
 } gdp_tuning;
 
@@ -43,18 +45,25 @@ typedef struct {
     int n_cons_nonempty;
     int n_items;
     int n_long_rows;
-    long long bin_rows[11];  /* rows per tree-width bin 2^0..2^9, then > 512 */
+    long long bin_rows[11];  /* rows per tree-width bin 2^0..2^9, then rows > 256 */
     long long bin_nnz[11];
+    int n16, n32, nf;        /* columns with 2x int16, 2x int32, 2x double records */
 } gdp_problem_stats;
 
 /* Fills *t with the defaults. */
 void gdpTuningDefaults(gdp_tuning *t);
 
 /* Uploads the matrix (same argument meaning as the functions of
- * interface.h).  tuning may be NULL. */
+ * interface.h).  env_lbs / env_ubs (may be NULL) are a bound envelope: every
+ * bound vector later propagated with this problem must lie inside it (e.g.
+ * the root bounds of a branch-and-bound tree); integer columns whose envelope
+ * is finite and integral then get compact bound records, which makes the
+ * rounds faster.  A propagation whose bounds leave the envelope returns
+ * GDP_ERROR.  tuning may be NULL. */
 GDP_RETCODE gdpProblemCreate(const int n_cons, const int n_vars, const int nnz,
                              const int *col_indices, const int *row_ptrs, const double *vals,
                              const double *lhss, const double *rhss, const int *vartypes,
+                             const double *env_lbs, const double *env_ubs,
                              const gdp_tuning *tuning, gdp_problem *out);
 GDP_RETCODE gdpProblemDestroy(gdp_problem p);
 GDP_RETCODE gdpProblemGetStats(gdp_problem p, gdp_problem_stats *out);
