@@ -34,14 +34,8 @@ namespace gdp {
 constexpr int kIdMask = 0x3fffffff;
 constexpr int kIntFlag = static_cast<int>(0x80000000u);
 constexpr int kItemGroup = 0;
-constexpr int kItemLong = 1;   // one row, one wave, all segments (very long rows)
+constexpr int kItemLong = 1;   // one row, one wave, all segments (rows not split)
 constexpr int kItemSeg = 2;    // one segment of a split row
-
-// Rows longer than kSegLen are split into one item per segment when they
-// have at most this many segments; longer rows go to single-wave items.
-// Segment waves of one row wait for each other, which is deadlock-free as
-// long as this bound is below the number of co-resident waves.
-constexpr int kMaxSplitSegs = 512;
 
 // status words: row 0 holds the outcome, row r >= 1 the flags of round r
 constexpr int kStatusWords = 8;
@@ -65,10 +59,11 @@ struct SegPart {
     int mininf, maxinf;
 };
 
-// Activity and gate of a split row, published by the last segment wave.
+// Activity of a split row, written by the last segment wave of a round
+// when the row gate passes.
 struct SegState {
     double minact, maxact;
-    int mininf, maxinf, gate, pad;
+    int mininf, maxinf;
 };
 
 struct DevMatrix {
@@ -85,8 +80,10 @@ struct DevMatrix {
     const double *val;
     const int4 *items;
     const int4 *seg_rows;  // per split row: {row, first partial, segments, 0}
+    const int *part_row;   // per segment of a split row: its split-row index
     int n_seg_rows;
     int n_seg_parts;
+    int n_apply_items;     // ceil(n_seg_parts / 64)
     const int *perm;       // internal id -> original column
 };
 
@@ -103,7 +100,7 @@ struct DevState {
     SegPart *seg_parts;    // [n_seg_parts]
     SegState *seg_state;   // [n_seg_rows]
     unsigned *seg_count;   // [n_seg_rows] arrivals in the current round
-    int *seg_flag;         // [n_seg_rows] round whose state is published
+    int *seg_flag;         // [n_seg_rows] last round in which the row gate passed
 };
 
 }  // namespace gdp

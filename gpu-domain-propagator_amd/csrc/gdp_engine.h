@@ -26,7 +26,7 @@ struct Tuning {
     int block_threads = 256;  // threads per workgroup (multiple of 64)
     int mega_blocks_per_cu = 0;  // 0: occupancy limit
     int graph_rounds = 8;     // rounds per captured graph for GPU_LOOP
-    int max_split_segs = 512; // rows with more segments run on one wave (0: never split)
+    int max_split_segs = -1;  // rows with more segments run on one wave (-1: split all, 0: none)
     int compact_bounds = 1;   // 0: keep every column in 16-byte FP64 records
     // per tree-width bin (width 2^b, b = 0..9): lanes per row as log2
     // default: K = min(width, 4) consecutive entries per lane

@@ -5,7 +5,7 @@
 // fill_row_blocks (src/kernels/util_kernels.cu:12-62).  Differences by
 // design: the matrix is laid out once per Problem (rows grouped by tree
 // width, columns relabelled by bound storage class, integrality folded into
-// the column index), rounds are round-synchronous (sweep, then publish), and
+// the column index), rounds are round-synchronous (sweep, apply of split rows, publish), and
 // GPU_LOOP is a captured HIP graph of round launches that stop themselves on
 // the device.
 #include <hip/hip_runtime.h>
@@ -25,6 +25,7 @@
 namespace gdp {
 
 __global__ void sweep_kernel(DevMatrix M, DevState D, int round, int guard);
+__global__ void apply_split_kernel(DevMatrix M, DevState D, int round, int guard);
 __global__ void publish_kernel(DevMatrix M, DevState D, int round, int guard);
 __global__ void mega_kernel(DevMatrix M, DevState D, int max_rounds);
 __global__ void init_kernel(DevMatrix M, DevState D, const double *lbs, const double *ubs);
@@ -180,6 +181,7 @@ Problem *problem_create(int m, int n, int nnz, const int *col, const int *row_pt
         // tree-width bins from wide to narrow, so that long work starts first.
         std::vector<int4> h_items;
         std::vector<int4> h_seg_rows;
+        std::vector<int> h_part_row;
         int n_parts = 0;
         int r = 0;
         {
@@ -188,10 +190,13 @@ Problem *problem_create(int m, int n, int nnz, const int *col, const int *row_pt
             for (int q = 0; q < cnt; q++) {
                 const int len = h_rp[r + q + 1] - h_rp[r + q];
                 const int nseg = (len + kSegLen - 1) / kSegLen;
-                if (nseg <= std::min(kMaxSplitSegs, T.max_split_segs)) {
+                if (T.max_split_segs < 0 || nseg <= T.max_split_segs) {
                     const int lr = static_cast<int>(h_seg_rows.size());
                     h_seg_rows.push_back(make_int4(r + q, n_parts, nseg, 0));
-                    for (int sg = 0; sg < nseg; sg++) h_items.push_back(make_int4(kItemSeg, lr, sg, 0));
+                    for (int sg = 0; sg < nseg; sg++) {
+                        h_items.push_back(make_int4(kItemSeg, lr, sg, 0));
+                        h_part_row.push_back(lr);
+                    }
                     n_parts += nseg;
                 } else {
                     giant.push_back(r + q);
@@ -248,12 +253,14 @@ Problem *problem_create(int m, int n, int nnz, const int *col, const int *row_pt
         M.n_pub_items = (nwords + 63) / 64;
         M.n_seg_rows = static_cast<int>(h_seg_rows.size());
         M.n_seg_parts = n_parts;
+        M.n_apply_items = (n_parts + 63) / 64;
         int *d_rp = dev_alloc<int>(mp + 1, P->owned);
         double2 *d_sides = dev_alloc<double2>(mp, P->owned);
         int *d_col = dev_alloc<int>(nnz, P->owned);
         double *d_val = dev_alloc<double>(nnz, P->owned);
         int4 *d_items = dev_alloc<int4>(h_items.size(), P->owned);
         int4 *d_seg_rows = dev_alloc<int4>(h_seg_rows.size(), P->owned);
+        int *d_part_row = dev_alloc<int>(h_part_row.size(), P->owned);
         int *d_perm = dev_alloc<int>(n, P->owned);
         check(hipMemcpyAsync(d_rp, h_rp.data(), (mp + 1) * sizeof(int), hipMemcpyHostToDevice, s), "H2D");
         check(hipMemcpyAsync(d_sides, h_sides.data(), mp * sizeof(double2), hipMemcpyHostToDevice, s), "H2D");
@@ -266,6 +273,10 @@ Problem *problem_create(int m, int n, int nnz, const int *col, const int *row_pt
             check(hipMemcpyAsync(d_seg_rows, h_seg_rows.data(), h_seg_rows.size() * sizeof(int4),
                                  hipMemcpyHostToDevice, s),
                   "H2D");
+        if (!h_part_row.empty())
+            check(hipMemcpyAsync(d_part_row, h_part_row.data(), h_part_row.size() * sizeof(int),
+                                 hipMemcpyHostToDevice, s),
+                  "H2D");
         check(hipMemcpyAsync(d_perm, perm.data(), n * sizeof(int), hipMemcpyHostToDevice, s), "H2D");
         M.row_ptr = d_rp;
         M.sides = d_sides;
@@ -273,6 +284,7 @@ Problem *problem_create(int m, int n, int nnz, const int *col, const int *row_pt
         M.val = d_val;
         M.items = d_items;
         M.seg_rows = d_seg_rows;
+        M.part_row = d_part_row;
         M.perm = d_perm;
 
         DevState &D = P->D;
@@ -296,9 +308,6 @@ Problem *problem_create(int m, int n, int nnz, const int *col, const int *row_pt
               "occupancy");
         if (T.mega_blocks_per_cu > 0) per_cu = std::min(per_cu, T.mega_blocks_per_cu);
         P->mega_grid = std::max(1, per_cu) * P->cu_count;
-        // segment waves of a split row wait for each other: all of a row's
-        // segments must fit on the device at once, with room to spare
-        if (P->mega_grid * 4 < 4 * kMaxSplitSegs) throw std::runtime_error("device too small for split-row items");
         check(hipStreamSynchronize(s), "upload");
     } catch (...) {
         delete P;
@@ -347,6 +356,7 @@ RunResult problem_propagate_device(Problem *P, double *d_lbs, double *d_ubs, int
 
     const int sweep_grid = grid_for(M.n_items);
     const int pub_grid = grid_for(M.n_pub_items);
+    const int apply_grid = grid_for(M.n_apply_items);
     std::vector<int> st(kStatusRows * kStatusWords, 0);
     int head[kStatusWords];
     check(hipMemcpyAsync(head, D.status, sizeof(head), hipMemcpyDeviceToHost, s), "flags D2H");
@@ -363,6 +373,8 @@ RunResult problem_propagate_device(Problem *P, double *d_lbs, double *d_ubs, int
         // two launches per round; the host reads the round's flags back
         for (int round = 1; round <= max_rounds; round++) {
             hipLaunchKernelGGL(sweep_kernel, dim3(sweep_grid), dim3(256), 0, s, M, D, round, 0);
+            if (M.n_apply_items > 0)
+                hipLaunchKernelGGL(apply_split_kernel, dim3(apply_grid), dim3(256), 0, s, M, D, round, 0);
             hipLaunchKernelGGL(publish_kernel, dim3(pub_grid), dim3(256), 0, s, M, D, round, 0);
             check(hipGetLastError(), "round launch");
             int flags[kStatusWords];
@@ -392,6 +404,8 @@ RunResult problem_propagate_device(Problem *P, double *d_lbs, double *d_ubs, int
                 check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "capture");
                 for (int round = first; round <= last; round++) {
                     hipLaunchKernelGGL(sweep_kernel, dim3(sweep_grid), dim3(256), 0, s, M, D, round, 1);
+                    if (M.n_apply_items > 0)
+                        hipLaunchKernelGGL(apply_split_kernel, dim3(apply_grid), dim3(256), 0, s, M, D, round, 1);
                     hipLaunchKernelGGL(publish_kernel, dim3(pub_grid), dim3(256), 0, s, M, D, round, 1);
                 }
                 check(hipStreamEndCapture(s, &graph), "end capture");

@@ -1,11 +1,14 @@
 // gdp_kernels.hip - CDNA4 (gfx950) kernels of the propagation rounds.
 //
-// A round has two phases.
+// A round has three phases.
 //   sweep    every row reads the bounds of the round's snapshot, computes its
 //            activities and the candidate bounds of its columns, and folds
 //            accepted candidates into the accumulator with native atomics
 //            (int32 max/min for integer columns, global_atomic_{max,min}_f64
 //            for the others); a bitmap records the columns that moved.
+//   apply    split rows (longer than kSegLen) whose gate passed apply the
+//            candidates of their segments (long rows rarely pass the gate,
+//            so this phase is mostly empty).
 //   publish  the columns of the bitmap are copied from the accumulator into
 //            the snapshot and checked for crossed bounds (lb > ub + eps).
 // Rows therefore see exactly the bounds at the end of the previous round:
@@ -19,8 +22,8 @@
 //               xor butterfly over the g lanes.
 //   seg item    one kSegLen-entry segment of a longer row; segment sums are
 //               added left to right by the last segment wave to arrive.
-//   long item   a row with too many segments for split processing, reduced
-//               segment by segment by one wave.
+//   long item   a row that is not split (a tuning option), reduced segment
+//               by segment by one wave.
 // The summation order is thus a function of the row length only, whatever
 // (g, K) a bin uses, which makes results reproducible bit for bit.
 //
@@ -169,12 +172,30 @@ __device__ __forceinline__ void store_snapshot(const DevMatrix &M, const DevStat
 // ---------------------------------------------------------------------------
 
 // Row gate with the sequential propagator's epsilon-maximum of |a|(u-l).
-// That maximum m depends on the order of the entries but always lies in
-// [lo, M] with M the exact maximum (computed in parallel, order-free) and
-// lo = M minus a little more than the tolerance; row_gate is monotone in its
-// last argument, so when row_gate(lo) == row_gate(M) the answer is known.
-// Otherwise (entries within 2*eps of the decision threshold) the row is
-// rescanned in CSR order exactly as the sequential propagator does.
+// That maximum depends on the order of the entries (a later term replaces
+// the running value only when larger by more than the tolerance) but always
+// lies in [lo, M] with M the exact maximum (computed in parallel, order-free)
+// and lo = M minus a little more than the tolerance.  row_gate is monotone in
+// its last argument, so when row_gate(lo) == row_gate(M) the answer is known;
+// otherwise (entries within 2*eps of the decision threshold) the
+// order-dependent maximum is computed exactly by a sequential scan.
+struct GateBounds {
+    bool hi, lo;
+    __device__ __forceinline__ bool decided() const { return hi == lo; }
+};
+
+__device__ __forceinline__ GateBounds gate_bounds(double minact, double maxact, int mininf, int maxinf, double lhs,
+                                                  double rhs, double dmax) {
+    GateBounds g;
+    g.hi = row_gate(minact, maxact, mininf, maxinf, lhs, rhs, dmax);
+    double lo_d = dmax;
+    if (isfinite(dmax)) lo_d = dmax - 4e-9 - fabs(dmax) * 0x1p-48;
+    g.lo = row_gate(minact, maxact, mininf, maxinf, lhs, rhs, lo_d);
+    return g;
+}
+
+// Sequential epsilon-maximum of a whole row read from memory, in CSR order
+// (rows that are not split, whose entries are not held in registers).
 __device__ __forceinline__ double seq_eps_max(const DevMatrix &M, const DevState &D, int start, int len) {
     double run = 0.0;
     for (int e = 0; e < len; e++) {
@@ -188,11 +209,8 @@ __device__ __forceinline__ double seq_eps_max(const DevMatrix &M, const DevState
 __device__ __forceinline__ bool decide_gate(const DevMatrix &M, const DevState &D, int start, int len,
                                             double minact, double maxact, int mininf, int maxinf, double lhs,
                                             double rhs, double dmax) {
-    const bool hi = row_gate(minact, maxact, mininf, maxinf, lhs, rhs, dmax);
-    double lo_d = dmax;
-    if (isfinite(dmax)) lo_d = dmax - 4e-9 - fabs(dmax) * 0x1p-48;
-    const bool lo = row_gate(minact, maxact, mininf, maxinf, lhs, rhs, lo_d);
-    if (hi == lo) return hi;
+    const GateBounds g = gate_bounds(minact, maxact, mininf, maxinf, lhs, rhs, dmax);
+    if (g.decided()) return g.hi;
     const double m = seq_eps_max(M, D, start, len);
     return row_gate(minact, maxact, mininf, maxinf, lhs, rhs, m);
 }
@@ -271,6 +289,22 @@ struct LaneEntries {
         }
     }
 
+    // Sequential epsilon-maximum over the entries of each group of g lanes in
+    // natural order (lane by lane, K entries each), starting from `run`;
+    // all lanes of a group return the same value.  Entries past the end of
+    // the row hold zeros, which never replace the running value.
+    __device__ __forceinline__ double eps_scan(int g, double run) const {
+        const int gbase = (threadIdx.x & 63) & ~(g - 1);
+        double d[K];
+#pragma unroll
+        for (int k = 0; k < K; k++) d[k] = fabs(a[k]) * (u[k] - l[k]);
+        for (int j = 0; j < g; j++) {
+#pragma unroll
+            for (int k = 0; k < K; k++) run = eps_max_step(run, __shfl(d[k], gbase + j, 64));
+        }
+        return run;
+    }
+
     __device__ __forceinline__ bool apply(const DevMatrix &M, const DevState &D, int first, int len, double lhs,
                                           double rhs, double minact, double maxact, int ni, int xi) {
         bool moved = false;
@@ -308,9 +342,15 @@ __device__ bool group_item(const DevMatrix &M, const DevState &D, int first, int
         E.reduce(g);
         const double minact = 0.0 + E.tmin;
         const double maxact = 0.0 + E.tmax;
-        if (valid && len > 0 &&
-            decide_gate(M, D, start, len, minact, maxact, E.mininf, E.maxinf, lhs, rhs, E.dmax))
-            moved |= E.apply(M, D, gl * K, len, lhs, rhs, minact, maxact, E.mininf, E.maxinf);
+        const bool active = valid && len > 0;
+        const GateBounds gb = gate_bounds(minact, maxact, E.mininf, E.maxinf, lhs, rhs, E.dmax);
+        bool gate = gb.hi;
+        const bool undecided = active && !gb.decided();
+        if (__any(undecided)) {
+            const double dm = E.eps_scan(g, 0.0);
+            if (undecided) gate = row_gate(minact, maxact, E.mininf, E.maxinf, lhs, rhs, dm);
+        }
+        if (active && gate) moved |= E.apply(M, D, gl * K, len, lhs, rhs, minact, maxact, E.mininf, E.maxinf);
     }
     return moved;
 }
@@ -347,12 +387,37 @@ __device__ bool long_item(const DevMatrix &M, const DevState &D, int r) {
     return moved;
 }
 
+// Sequential epsilon-maximum of a split row, computed by its last segment
+// wave.  A segment whose exact maximum Ms satisfies !(Ms - run > eps) cannot
+// replace the running value (floating-point subtraction is monotone), so it
+// is skipped; the others are scanned entry by entry in order.
+__device__ double split_eps_scan(const DevMatrix &M, const DevState &D, const int4 info, int start, int len) {
+    constexpr int K = kSegLen / 64;
+    const int lane = threadIdx.x & 63;
+    double run = 0.0;
+    for (int b = 0; b < info.z; b += 64) {
+        const int q = b + lane;
+        const double ms = q < info.z ? coherent_load_d(&D.seg_parts[info.y + q].dmax) : 0.0;
+        const int cnt = min(64, info.z - b);
+        for (int t = 0; t < cnt; t++) {
+            const double mt = __shfl(ms, t, 64);
+            if (!eps_gt(mt, run)) continue;
+            const int s0 = (b + t) * kSegLen;
+            LaneEntries<K> E;
+            E.load(M, D, start + s0, lane * K, len - s0);
+            run = E.eps_scan(64, run);
+        }
+    }
+    return run;
+}
+
 // One segment of a split row.  The wave reduces its kSegLen entries like a
-// kSegLen-wide group row, stores the partial and counts itself in.  The last
-// wave of the row to arrive adds the partials in segment order, decides the
-// row gate and publishes the row state; the others wait for it (their entries
-// stay in registers) and then apply their candidates.
-__device__ bool seg_item(const DevMatrix &M, const DevState &D, int lr, int s, int round) {
+// kSegLen-wide group row, stores the partial and counts itself in; nobody
+// waits.  The last wave of the row to arrive adds the partials in segment
+// order and decides the row gate; when it passes, the row's activity is
+// stored and the row is stamped with the round, and the apply phase that
+// follows the sweep applies the candidates of all its segments.
+__device__ void seg_item(const DevMatrix &M, const DevState &D, int lr, int s, int round) {
     constexpr int K = kSegLen / 64;
     const int lane = threadIdx.x & 63;
     const int4 info = M.seg_rows[lr];
@@ -363,7 +428,6 @@ __device__ bool seg_item(const DevMatrix &M, const DevState &D, int lr, int s, i
     LaneEntries<K> E;
     E.load(M, D, start + s0, lane * K, len - s0);
     E.reduce(64);
-    int ok = 1;
     unsigned old = 0;
     SegPart *part = &D.seg_parts[info.y + s];
     if (lane == 0) {
@@ -376,74 +440,102 @@ __device__ bool seg_item(const DevMatrix &M, const DevState &D, int lr, int s, i
         old = __hip_atomic_fetch_add(&D.seg_count[lr], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     old = __shfl(old, 0, 64);
-    SegState *stp = &D.seg_state[lr];
-    if (old == static_cast<unsigned>(info.z - 1)) {
-        // last arrival: the lanes load 64 partials at a time (independent
-        // loads), which are then added in segment order
-        double am = 0.0, ax = 0.0, dm = 0.0;
-        int ni = 0, xi = 0;
-        for (int b = 0; b < info.z; b += 64) {
-            const int q = b + lane;
-            double pmin = 0.0, pmax = 0.0;
-            if (q < info.z) {
-                const SegPart *pq = &D.seg_parts[info.y + q];
-                pmin = coherent_load_d(&pq->smin);
-                pmax = coherent_load_d(&pq->smax);
-                dm = fmax(dm, coherent_load_d(&pq->dmax));
-                ni += coherent_load(&pq->mininf);
-                xi += coherent_load(&pq->maxinf);
-            }
-            const int cnt = min(64, info.z - b);
-            for (int t = 0; t < cnt; t++) {
-                am = am + __shfl(pmin, t, 64);
-                ax = ax + __shfl(pmax, t, 64);
-            }
+    if (old != static_cast<unsigned>(info.z - 1)) return;
+
+    // last arrival: the lanes load 64 partials at a time (independent
+    // loads), which are then added in segment order
+    double am = 0.0, ax = 0.0, dm = 0.0;
+    int ni = 0, xi = 0;
+    for (int b = 0; b < info.z; b += 64) {
+        const int q = b + lane;
+        double pmin = 0.0, pmax = 0.0;
+        if (q < info.z) {
+            const SegPart *pq = &D.seg_parts[info.y + q];
+            pmin = coherent_load_d(&pq->smin);
+            pmax = coherent_load_d(&pq->smax);
+            dm = fmax(dm, coherent_load_d(&pq->dmax));
+            ni += coherent_load(&pq->mininf);
+            xi += coherent_load(&pq->maxinf);
         }
+        const int cnt = min(64, info.z - b);
+        for (int t = 0; t < cnt; t++) {
+            am = am + __shfl(pmin, t, 64);
+            ax = ax + __shfl(pmax, t, 64);
+        }
+    }
 #pragma unroll
-        for (int h = 1; h < 64; h <<= 1) {
-            ni += __shfl_xor(ni, h, 64);
-            xi += __shfl_xor(xi, h, 64);
-            dm = fmax(dm, shfl_xor_d(dm, h));
-        }
-        if (lane == 0) {
-            const double2 sd = M.sides[r];
-            const int gate = decide_gate(M, D, start, len, am, ax, ni, xi, sd.x, sd.y, dm) ? 1 : 0;
+    for (int h = 1; h < 64; h <<= 1) {
+        ni += __shfl_xor(ni, h, 64);
+        xi += __shfl_xor(xi, h, 64);
+        dm = fmax(dm, shfl_xor_d(dm, h));
+    }
+    const double2 sd = M.sides[r];
+    const GateBounds gb = gate_bounds(am, ax, ni, xi, sd.x, sd.y, dm);
+    bool gate = gb.hi;
+    if (!gb.decided()) gate = row_gate(am, ax, ni, xi, sd.x, sd.y, split_eps_scan(M, D, info, start, len));
+    if (lane == 0) {
+        coherent_store(&D.seg_count[lr], 0u);
+        if (gate) {
+            SegState *stp = &D.seg_state[lr];
             coherent_store_d(&stp->minact, am);
             coherent_store_d(&stp->maxact, ax);
             coherent_store(&stp->mininf, ni);
             coherent_store(&stp->maxinf, xi);
-            coherent_store(&stp->gate, gate);
-            coherent_store(&D.seg_count[lr], 0u);
             mem_wait();
             coherent_store(&D.seg_flag[lr], round);
         }
-    } else if (lane == 0) {
-        const long long t0 = wall_clock64();
-        while (coherent_load(&D.seg_flag[lr]) != round) {
-            __builtin_amdgcn_s_sleep(2);
-            if (coherent_load(&D.status[kGlobalError]) || wall_clock64() - t0 > kWaitTimeoutTicks) {
-                atomic_or_i(&D.status[kGlobalError], 1);
-                ok = 0;
-                break;
+    }
+}
+
+// Applies the candidates of the segments p in [64 f, 64 f + 64) whose split
+// row passed its gate in this round (one lane checks one segment's row).
+__device__ bool apply_split_item(const DevMatrix &M, const DevState &D, int f, int round) {
+    constexpr int K = kSegLen / 64;
+    const int lane = threadIdx.x & 63;
+    const int p = f * 64 + lane;
+    int lr = -1;
+    if (p < M.n_seg_parts) {
+        lr = M.part_row[p];
+        if (coherent_load(&D.seg_flag[lr]) != round) lr = -1;
+    }
+    unsigned long long pending = __ballot(lr >= 0);
+    bool moved = false;
+    while (pending) {
+        const int t = __ffsll(static_cast<long long>(pending)) - 1;
+        pending &= pending - 1;
+        const int row_l = __shfl(lr, t, 64);
+        const int4 info = M.seg_rows[row_l];
+        const int r = info.x;
+        const int start = M.row_ptr[r];
+        const int len = M.row_ptr[r + 1] - start;
+        const int s0 = (f * 64 + t - info.y) * kSegLen;
+        const SegState *stp = &D.seg_state[row_l];
+        const double minact = coherent_load_d(&stp->minact);
+        const double maxact = coherent_load_d(&stp->maxact);
+        const int ni = coherent_load(&stp->mininf);
+        const int xi = coherent_load(&stp->maxinf);
+        const double2 sd = M.sides[r];
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const int e = s0 + lane * K + k;
+            if (e < len) {
+                const int jp = M.col[start + e];
+                double l, u;
+                load_bounds(M, D, jp & kIdMask, l, u);
+                moved |= apply_entry(M, D, jp, M.val[start + e], l, u, sd.x, sd.y, minact, maxact, ni, xi);
             }
         }
     }
-    ok = __shfl(ok, 0, 64);
-    if (!ok) return false;
-    const int gate = coherent_load(&stp->gate);
-    if (!gate) return false;
-    const double minact = coherent_load_d(&stp->minact);
-    const double maxact = coherent_load_d(&stp->maxact);
-    const int ni = coherent_load(&stp->mininf);
-    const int xi = coherent_load(&stp->maxinf);
-    const double2 sd = M.sides[r];
-    return E.apply(M, D, lane * K, len - s0, sd.x, sd.y, minact, maxact, ni, xi);
+    return moved;
 }
 
 __device__ bool sweep_item(const DevMatrix &M, const DevState &D, int it, int round) {
     const int4 d = M.items[it];
     const int kind = d.x & 15;
-    if (kind == kItemSeg) return seg_item(M, D, d.y, d.z, round);
+    if (kind == kItemSeg) {
+        seg_item(M, D, d.y, d.z, round);
+        return false;
+    }
     if (kind == kItemLong) return long_item(M, D, d.y);
     const int logg = (d.x >> 4) & 15;
     const int logk = (d.x >> 8) & 15;
@@ -502,6 +594,14 @@ __global__ __launch_bounds__(256) void sweep_kernel(DevMatrix M, DevState D, int
     flag_change(D.status + kStatusWords * round, moved);
 }
 
+__global__ __launch_bounds__(256) void apply_split_kernel(DevMatrix M, DevState D, int round, int guard) {
+    if (guard && round_skipped(D, round)) return;
+    const int f = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    bool moved = false;
+    if (f < M.n_apply_items) moved = apply_split_item(M, D, f, round);
+    flag_change(D.status + kStatusWords * round, moved);
+}
+
 __global__ __launch_bounds__(256) void publish_kernel(DevMatrix M, DevState D, int round, int guard) {
     if (guard && round_skipped(D, round)) return;
     const int f = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
@@ -557,7 +657,8 @@ __device__ bool grid_barrier(const DevState &D, unsigned nblocks) {
 // All rounds in one persistent launch (MEGAKERNEL).  Every workgroup walks
 // the items of a phase with a grid stride (consecutive items on consecutive
 // waves), all meet at a barrier, and all read the same status words, so all
-// take the same decision to go on.  Outcome words in status row 0.
+// take the same decision to go on.  Phases of a round: sweep, apply of the
+// split rows (when there are any), publish.  Outcome words in status row 0.
 __global__ __launch_bounds__(256) void mega_kernel(DevMatrix M, DevState D, int max_rounds) {
     const unsigned nblocks = gridDim.x;
     const int waves = blockDim.x >> 6;
@@ -569,6 +670,10 @@ __global__ __launch_bounds__(256) void mega_kernel(DevMatrix M, DevState D, int 
             int *st = D.status + kStatusWords * round;
             bool moved = false;
             for (int it = wave; it < M.n_items; it += stride) moved |= sweep_item(M, D, it, round);
+            if (M.n_apply_items > 0) {
+                if (!grid_barrier(D, nblocks)) { rounds = -1; break; }
+                for (int f = wave; f < M.n_apply_items; f += stride) moved |= apply_split_item(M, D, f, round);
+            }
             flag_change(st, moved);
             if (!grid_barrier(D, nblocks)) { rounds = -1; break; }
             for (int f = wave; f < M.n_pub_items; f += stride) publish_item(M, D, f, st);

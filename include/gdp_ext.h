@@ -33,8 +33,9 @@ typedef struct {
     int item_nnz;            /* target nonzeros per work item (0: default 1024) */
     int mega_blocks_per_cu;  /* persistent grid: workgroups per CU (0: occupancy) */
     int graph_rounds;        /* rounds per captured graph in GPU_LOOP (0: 8) */
-    int max_split_segs;      /* rows with up to this many 256-entry segments are split
-                                across waves (-1: default 512; 0: never split) */
+    int max_split_segs;      /* rows longer than 256 entries with up to this many
+                                256-entry segments are split into one wave per
+                                segment (-1: default, all; 0: never split) */
     int compact_bounds;      /* 1 (default): integer columns with a finite integral
                                 envelope use 4- or 8-byte bound records; 0: FP64 only */
     int group_log[10];       /* lanes per row (log2) for rows of width 2^b; -1: default */

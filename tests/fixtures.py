@@ -122,3 +122,54 @@ def infeasible_instance() -> Instance:
     """x + y <= -1 with x, y in [0, 5]: bounds cross in the first round."""
     return from_rows(2, 3, [[0, 1], [1, 2]], [[1.0, 1.0], [1.0, -1.0]], [-INF, -INF], [-1.0, 2.0],
                      [0, 0, 0], [5, 5, 5], [1, 1, 3], "infeasible")
+
+
+# Row patterns of gate_ties: offsets (in units of 1e-9) of the tie entries
+# above R, in row order, and whether the sequential epsilon-maximum passes the
+# row gate.  The running maximum starts at 0 and takes a value only when it
+# exceeds the running one by more than 1e-9; the gate passes when that
+# maximum exceeds R by more than 1e-9.
+GATE_TIE_PATTERNS = (
+    ((0.8, 1.5), False),        # 1.5 is within eps of 0.8: maximum stays at R+0.8e-9
+    ((1.5, 0.8), True),
+    ((0.3, 1.2), False),
+    ((-2.0, -0.5, 0.9, 1.6), False),  # chain of replacements ends at R+0.9e-9
+    ((-2.0, -0.5, 1.6, 0.9), True),
+    ((1.2,), True),
+    ((0.9,), False),
+)
+
+
+def gate_ties(lengths=(2, 7, 64, 200, 256, 300, 1000, 5000), R: float = 10.0) -> Instance:
+    """Rows x_1 + ... + x_L <= R over continuous x in [0, u] whose row gate
+    depends on the order of the entries: filler columns have u = 1 and the
+    tie columns u = R + offset * 1e-9 (GATE_TIE_PATTERNS), spread over the
+    row (for rows longer than 256 entries: over different 256-entry
+    segments).  A row whose gate passes tightens its tie columns with
+    u > R + 1e-9 to u = R; the expected bounds are in meta."""
+    rows, vals, lhs, rhs = [], [], [], []
+    ub = []
+    ub_exp = []
+    for L in lengths:
+        for offsets, passes in GATE_TIE_PATTERNS:
+            if len(offsets) > L:
+                continue
+            start = len(ub)
+            pos = [int(round((t + 0.5) * L / len(offsets))) for t in range(len(offsets))]
+            pos = [min(p, L - 1) for p in pos]
+            u_row = [1.0] * L
+            for p, off in zip(pos, offsets):
+                u_row[p] = R + off * 1e-9
+            rows.append(list(range(start, start + L)))
+            vals.append([1.0] * L)
+            lhs.append(-INF)
+            rhs.append(R)
+            ub.extend(u_row)
+            for u in u_row:
+                ub_exp.append(R if passes and u - R > 1e-9 else u)
+    n = len(ub)
+    inst = from_rows(len(rows), n, rows, vals, lhs, rhs, np.zeros(n), np.array(ub), np.full(n, 3, np.int32),
+                     "gate_ties")
+    inst.meta["ub_expected"] = np.array(ub_exp)
+    inst.meta["lb_expected"] = np.zeros(n)
+    return inst

@@ -33,6 +33,8 @@ def _case(name):
         return fixtures.varied(int(arg))
     if kind == "gcfp":
         return fixtures.generalized_cfp(int(arg))
+    if kind == "gate_ties":
+        return fixtures.gate_ties()
     if kind == "infeasible":
         return fixtures.infeasible_instance()
     if kind == "tiny":
@@ -44,7 +46,7 @@ def _case(name):
 
 CASES = ([f"analytic:{n}" for n in fixtures.ANALYTIC] + [f"small:{s}" for s in range(3)] +
          ["small_int:7", "varied:0", "varied:1", "varied:2", "gcfp:30", "infeasible", "tiny:3", "tiny:11",
-          "powerlaw:5"])
+          "powerlaw:5", "gate_ties"])
 
 
 def assert_same(got, exp, label=""):
@@ -151,3 +153,27 @@ def test_input_with_crossed_bounds(gpu):
     lb[3] = 2.0  # above ub = 1
     got = I.Problem(inst).propagate(lb, inst.ub, I.MEGAKERNEL)
     assert got.infeasible and got.rounds == 0
+
+
+TUNINGS = {
+    "no_split": dict(max_split_segs=0),          # rows longer than 256 on one wave
+    "split_upto_4": dict(max_split_segs=4),
+    "fp64_bounds": dict(compact_bounds=0),       # no compact bound records
+    "one_entry_per_lane": dict(group_log=[0, 1, 2, 3, 4, 5, 6, 6, 6, 6]),
+    "small_items": dict(item_nnz=128),
+}
+
+
+@pytest.mark.parametrize("tuning", sorted(TUNINGS))
+@pytest.mark.parametrize("case", ["powerlaw:5", "gate_ties", "varied:2", "small_int:7"])
+def test_tunings_bit_exact(gpu, oracle, case, tuning):
+    """Every layout option yields the same bits: the summation order is a
+    function of the row length only."""
+    inst = _case(case)
+    exp = oracle.jacobi(inst)
+    p = I.Problem(inst, TUNINGS[tuning])
+    try:
+        for sync in (I.CPU_LOOP, I.MEGAKERNEL):
+            assert_same(p.propagate(inst.lb, inst.ub, sync), exp, f"{case} {tuning} sync={sync}")
+    finally:
+        p.close()

@@ -288,3 +288,22 @@ def test_canonical_activities_match_sequential_on_integers(oracle):
         b, ib = oracle.row_activities(inst, i, canonical=True)
         np.testing.assert_array_equal(a, b)
         np.testing.assert_array_equal(ia, ib)
+
+
+@pytest.mark.parametrize("driver", ["seq", "jacobi"])
+def test_gate_ties_known_answer(oracle, driver):
+    """Rows whose gate is decided by the order-dependent epsilon-maximum of
+    |a|(u-l): expected bounds derived by hand from the update rule."""
+    inst = fixtures.gate_ties()
+    lb, ub, rounds, infeasible, capped = getattr(oracle, driver)(inst)
+    np.testing.assert_array_equal(ub, inst.meta["ub_expected"])
+    np.testing.assert_array_equal(lb, inst.meta["lb_expected"])
+    assert (rounds, infeasible, capped) == (2, False, False)
+
+
+def test_gate_ties_discriminate_exact_maximum():
+    """The fixture separates the sequential epsilon-maximum from the exact
+    maximum: some rows fail the gate although their largest |a|(u-l)
+    exceeds R by more than the tolerance."""
+    differ = [p for p, passes in fixtures.GATE_TIE_PATTERNS if (max(p) > 1.0) != passes]
+    assert len(differ) >= 3
