@@ -35,7 +35,8 @@ constexpr int kIdMask = 0x3fffffff;
 constexpr int kIntFlag = static_cast<int>(0x80000000u);
 constexpr int kItemGroup = 0;
 constexpr int kItemLong = 1;   // one row, one wave, all segments (rows not split)
-constexpr int kItemSeg = 2;    // one segment of a split row
+constexpr int kItemSeg = 2;    // up to kSegsPerItem consecutive segments of a split row
+constexpr int kSegsPerItem = 4;
 
 // status words: row 0 holds the outcome, row r >= 1 the flags of round r
 constexpr int kStatusWords = 8;
@@ -48,7 +49,14 @@ constexpr int kGlobalEnvelope = 6;  // input bounds outside the class envelope
 constexpr int kGlobalCrossed = 7;   // input bounds already crossed
 constexpr int kStatusChanged = 0;
 constexpr int kStatusInfeasible = 1;
+constexpr int kStatusApply = 2;   // segments queued for the apply phase
 constexpr int kStatusRows = 128;  // rounds 1..100 fit
+
+// grid barrier words, each counter on its own 128-byte line
+constexpr int kBarrierTop = 0;
+constexpr int kBarrierGen = 32;
+constexpr int kBarrierGroup = 64;  // + 32 * group, 8 groups
+constexpr int kBarrierWords = 64 + 32 * 8;
 
 // wall_clock64 runs at 100 MHz on CDNA; give up a wait after 1 s
 constexpr long long kWaitTimeoutTicks = 100000000LL;
@@ -60,7 +68,7 @@ struct SegPart {
 };
 
 // Activity of a split row, written by the last segment wave of a round
-// when the row gate passes.
+// when the row gate passes (read by the apply phase of that round).
 struct SegState {
     double minact, maxact;
     int mininf, maxinf;
@@ -73,7 +81,6 @@ struct DevMatrix {
     int n16, n_int;   // class boundaries of the internal ids
     int nwords;       // bitmap words, ceil(n / 32)
     int n_items;      // sweep items
-    int n_pub_items;  // publish items, ceil(nwords / 64)
     const int *row_ptr;
     const double2 *sides;
     const int *col;
@@ -83,7 +90,6 @@ struct DevMatrix {
     const int *part_row;   // per segment of a split row: its split-row index
     int n_seg_rows;
     int n_seg_parts;
-    int n_apply_items;     // ceil(n_seg_parts / 64)
     const int *perm;       // internal id -> original column
 };
 
@@ -96,11 +102,11 @@ struct DevState {
     double2 *AF;
     unsigned *bm;
     int *status;        // kStatusRows * kStatusWords
-    unsigned *barrier;  // [count, generation]
+    unsigned *barrier;  // kBarrierWords: top counter, generation, 8 group counters
     SegPart *seg_parts;    // [n_seg_parts]
     SegState *seg_state;   // [n_seg_rows]
-    unsigned *seg_count;   // [n_seg_rows] arrivals in the current round
-    int *seg_flag;         // [n_seg_rows] last round in which the row gate passed
+    unsigned *seg_count;   // [n_seg_rows] segments reduced in the current round
+    int *apply_tasks;      // [n_seg_parts] segments whose row passed the gate
 };
 
 }  // namespace gdp

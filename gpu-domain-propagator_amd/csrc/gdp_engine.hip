@@ -193,10 +193,9 @@ Problem *problem_create(int m, int n, int nnz, const int *col, const int *row_pt
                 if (T.max_split_segs < 0 || nseg <= T.max_split_segs) {
                     const int lr = static_cast<int>(h_seg_rows.size());
                     h_seg_rows.push_back(make_int4(r + q, n_parts, nseg, 0));
-                    for (int sg = 0; sg < nseg; sg++) {
-                        h_items.push_back(make_int4(kItemSeg, lr, sg, 0));
-                        h_part_row.push_back(lr);
-                    }
+                    for (int sg = 0; sg < nseg; sg += kSegsPerItem)
+                        h_items.push_back(make_int4(kItemSeg, lr, sg, std::min(kSegsPerItem, nseg - sg)));
+                    for (int sg = 0; sg < nseg; sg++) h_part_row.push_back(lr);
                     n_parts += nseg;
                 } else {
                     giant.push_back(r + q);
@@ -250,10 +249,8 @@ Problem *problem_create(int m, int n, int nnz, const int *col, const int *row_pt
         M.n_int = ncls[0] + ncls[1];
         M.nwords = nwords;
         M.n_items = static_cast<int>(h_items.size());
-        M.n_pub_items = (nwords + 63) / 64;
         M.n_seg_rows = static_cast<int>(h_seg_rows.size());
         M.n_seg_parts = n_parts;
-        M.n_apply_items = (n_parts + 63) / 64;
         int *d_rp = dev_alloc<int>(mp + 1, P->owned);
         double2 *d_sides = dev_alloc<double2>(mp, P->owned);
         int *d_col = dev_alloc<int>(nnz, P->owned);
@@ -296,11 +293,11 @@ Problem *problem_create(int m, int n, int nnz, const int *col, const int *row_pt
         D.AF = dev_alloc<double2>(ncls[2], P->owned);
         D.bm = dev_alloc<unsigned>(nwords, P->owned);
         D.status = dev_alloc<int>(kStatusRows * kStatusWords, P->owned);
-        D.barrier = dev_alloc<unsigned>(64, P->owned);
+        D.barrier = dev_alloc<unsigned>(kBarrierWords, P->owned);
         D.seg_parts = dev_alloc<SegPart>(n_parts, P->owned);
         D.seg_state = dev_alloc<SegState>(h_seg_rows.size(), P->owned);
         D.seg_count = dev_alloc<unsigned>(h_seg_rows.size(), P->owned);
-        D.seg_flag = dev_alloc<int>(h_seg_rows.size(), P->owned);
+        D.apply_tasks = dev_alloc<int>(n_parts, P->owned);
 
         int per_cu = 0;
         check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(mega_kernel), 256,
@@ -355,8 +352,10 @@ RunResult problem_propagate_device(Problem *P, double *d_lbs, double *d_ubs, int
     check(hipGetLastError(), "init_kernel launch");
 
     const int sweep_grid = grid_for(M.n_items);
-    const int pub_grid = grid_for(M.n_pub_items);
-    const int apply_grid = grid_for(M.n_apply_items);
+    // publish: one thread per column; apply: a small grid that takes the
+    // queued segments in turn (an empty queue costs one short launch)
+    const int pub_grid = std::max(1, std::min(4096, (M.n + 255) / 256));
+    const int apply_grid = std::max(1, std::min(1024, grid_for(M.n_seg_parts)));
     std::vector<int> st(kStatusRows * kStatusWords, 0);
     int head[kStatusWords];
     check(hipMemcpyAsync(head, D.status, sizeof(head), hipMemcpyDeviceToHost, s), "flags D2H");
@@ -373,7 +372,7 @@ RunResult problem_propagate_device(Problem *P, double *d_lbs, double *d_ubs, int
         // two launches per round; the host reads the round's flags back
         for (int round = 1; round <= max_rounds; round++) {
             hipLaunchKernelGGL(sweep_kernel, dim3(sweep_grid), dim3(256), 0, s, M, D, round, 0);
-            if (M.n_apply_items > 0)
+            if (M.n_seg_parts > 0)
                 hipLaunchKernelGGL(apply_split_kernel, dim3(apply_grid), dim3(256), 0, s, M, D, round, 0);
             hipLaunchKernelGGL(publish_kernel, dim3(pub_grid), dim3(256), 0, s, M, D, round, 0);
             check(hipGetLastError(), "round launch");
@@ -404,7 +403,7 @@ RunResult problem_propagate_device(Problem *P, double *d_lbs, double *d_ubs, int
                 check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "capture");
                 for (int round = first; round <= last; round++) {
                     hipLaunchKernelGGL(sweep_kernel, dim3(sweep_grid), dim3(256), 0, s, M, D, round, 1);
-                    if (M.n_apply_items > 0)
+                    if (M.n_seg_parts > 0)
                         hipLaunchKernelGGL(apply_split_kernel, dim3(apply_grid), dim3(256), 0, s, M, D, round, 1);
                     hipLaunchKernelGGL(publish_kernel, dim3(pub_grid), dim3(256), 0, s, M, D, round, 1);
                 }

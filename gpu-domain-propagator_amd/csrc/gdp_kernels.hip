@@ -411,39 +411,47 @@ __device__ double split_eps_scan(const DevMatrix &M, const DevState &D, const in
     return run;
 }
 
-// One segment of a split row.  The wave reduces its kSegLen entries like a
-// kSegLen-wide group row, stores the partial and counts itself in; nobody
-// waits.  The last wave of the row to arrive adds the partials in segment
-// order and decides the row gate; when it passes, the row's activity is
-// stored and the row is stamped with the round, and the apply phase that
-// follows the sweep applies the candidates of all its segments.
-__device__ void seg_item(const DevMatrix &M, const DevState &D, int lr, int s, int round) {
+// Stores the activity partial of segment `part` (lane 0 of the wave).
+__device__ __forceinline__ void store_part(const DevState &D, int part, const LaneEntries<kSegLen / 64> &E) {
+    SegPart *p = &D.seg_parts[part];
+    coherent_store_d(&p->smin, E.tmin);
+    coherent_store_d(&p->smax, E.tmax);
+    coherent_store_d(&p->dmax, E.dmax);
+    coherent_store(&p->mininf, E.mininf);
+    coherent_store(&p->maxinf, E.maxinf);
+}
+
+// Segments [s, s + cnt) of a split row (cnt <= kSegsPerItem).  The wave
+// reduces each kSegLen-entry segment like a kSegLen-wide group row, stores
+// the partials and counts them in; nobody waits.  The wave that completes the row's count adds the
+// partials in segment order and decides the row gate; when it passes, the
+// row's activity is stored and its segments are queued for the apply phase
+// that follows the sweep.
+__device__ void seg_item(const DevMatrix &M, const DevState &D, int lr, int s, int cnt, int round) {
     constexpr int K = kSegLen / 64;
     const int lane = threadIdx.x & 63;
     const int4 info = M.seg_rows[lr];
     const int r = info.x;
     const int start = M.row_ptr[r];
     const int len = M.row_ptr[r + 1] - start;
-    const int s0 = s * kSegLen;
-    LaneEntries<K> E;
-    E.load(M, D, start + s0, lane * K, len - s0);
-    E.reduce(64);
+    for (int q = 0; q < cnt; q++) {
+        const int s0 = (s + q) * kSegLen;
+        LaneEntries<K> E;
+        E.load(M, D, start + s0, lane * K, len - s0);
+        E.reduce(64);
+        if (lane == 0) store_part(D, info.y + s + q, E);
+    }
     unsigned old = 0;
-    SegPart *part = &D.seg_parts[info.y + s];
     if (lane == 0) {
-        coherent_store_d(&part->smin, E.tmin);
-        coherent_store_d(&part->smax, E.tmax);
-        coherent_store_d(&part->dmax, E.dmax);
-        coherent_store(&part->mininf, E.mininf);
-        coherent_store(&part->maxinf, E.maxinf);
         mem_wait();
-        old = __hip_atomic_fetch_add(&D.seg_count[lr], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        old = __hip_atomic_fetch_add(&D.seg_count[lr], static_cast<unsigned>(cnt), __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_AGENT);
     }
     old = __shfl(old, 0, 64);
-    if (old != static_cast<unsigned>(info.z - 1)) return;
+    if (old + cnt != static_cast<unsigned>(info.z)) return;
 
-    // last arrival: the lanes load 64 partials at a time (independent
-    // loads), which are then added in segment order
+    // the row is complete: the lanes load 64 partials at a time
+    // (independent loads), which are then added in segment order
     double am = 0.0, ax = 0.0, dm = 0.0;
     int ni = 0, xi = 0;
     for (int b = 0; b < info.z; b += 64) {
@@ -457,8 +465,8 @@ __device__ void seg_item(const DevMatrix &M, const DevState &D, int lr, int s, i
             ni += coherent_load(&pq->mininf);
             xi += coherent_load(&pq->maxinf);
         }
-        const int cnt = min(64, info.z - b);
-        for (int t = 0; t < cnt; t++) {
+        const int n_here = min(64, info.z - b);
+        for (int t = 0; t < n_here; t++) {
             am = am + __shfl(pmin, t, 64);
             ax = ax + __shfl(pmax, t, 64);
         }
@@ -473,6 +481,7 @@ __device__ void seg_item(const DevMatrix &M, const DevState &D, int lr, int s, i
     const GateBounds gb = gate_bounds(am, ax, ni, xi, sd.x, sd.y, dm);
     bool gate = gb.hi;
     if (!gb.decided()) gate = row_gate(am, ax, ni, xi, sd.x, sd.y, split_eps_scan(M, D, info, start, len));
+    int pos = 0;
     if (lane == 0) {
         coherent_store(&D.seg_count[lr], 0u);
         if (gate) {
@@ -481,51 +490,51 @@ __device__ void seg_item(const DevMatrix &M, const DevState &D, int lr, int s, i
             coherent_store_d(&stp->maxact, ax);
             coherent_store(&stp->mininf, ni);
             coherent_store(&stp->maxinf, xi);
-            mem_wait();
-            coherent_store(&D.seg_flag[lr], round);
+            pos = __hip_atomic_fetch_add(&D.status[kStatusWords * round + kStatusApply], info.z, __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT);
         }
     }
+    if (!gate) return;
+    pos = __shfl(pos, 0, 64);
+    for (int q = lane; q < info.z; q += 64) coherent_store(&D.apply_tasks[pos + q], info.y + q);
 }
 
-// Applies the candidates of the segments p in [64 f, 64 f + 64) whose split
-// row passed its gate in this round (one lane checks one segment's row).
-__device__ bool apply_split_item(const DevMatrix &M, const DevState &D, int f, int round) {
+// Applies the candidates of segment `part` of a split row whose gate passed
+// (one wave; the row's activity is read back from its SegState).
+__device__ bool apply_split_task(const DevMatrix &M, const DevState &D, int part) {
     constexpr int K = kSegLen / 64;
     const int lane = threadIdx.x & 63;
-    const int p = f * 64 + lane;
-    int lr = -1;
-    if (p < M.n_seg_parts) {
-        lr = M.part_row[p];
-        if (coherent_load(&D.seg_flag[lr]) != round) lr = -1;
-    }
-    unsigned long long pending = __ballot(lr >= 0);
+    const int lr = M.part_row[part];
+    const int4 info = M.seg_rows[lr];
+    const int r = info.x;
+    const int start = M.row_ptr[r];
+    const int len = M.row_ptr[r + 1] - start;
+    const int s0 = (part - info.y) * kSegLen;
+    const SegState *stp = &D.seg_state[lr];
+    const double minact = coherent_load_d(&stp->minact);
+    const double maxact = coherent_load_d(&stp->maxact);
+    const int ni = coherent_load(&stp->mininf);
+    const int xi = coherent_load(&stp->maxinf);
+    const double2 sd = M.sides[r];
     bool moved = false;
-    while (pending) {
-        const int t = __ffsll(static_cast<long long>(pending)) - 1;
-        pending &= pending - 1;
-        const int row_l = __shfl(lr, t, 64);
-        const int4 info = M.seg_rows[row_l];
-        const int r = info.x;
-        const int start = M.row_ptr[r];
-        const int len = M.row_ptr[r + 1] - start;
-        const int s0 = (f * 64 + t - info.y) * kSegLen;
-        const SegState *stp = &D.seg_state[row_l];
-        const double minact = coherent_load_d(&stp->minact);
-        const double maxact = coherent_load_d(&stp->maxact);
-        const int ni = coherent_load(&stp->mininf);
-        const int xi = coherent_load(&stp->maxinf);
-        const double2 sd = M.sides[r];
 #pragma unroll
-        for (int k = 0; k < K; k++) {
-            const int e = s0 + lane * K + k;
-            if (e < len) {
-                const int jp = M.col[start + e];
-                double l, u;
-                load_bounds(M, D, jp & kIdMask, l, u);
-                moved |= apply_entry(M, D, jp, M.val[start + e], l, u, sd.x, sd.y, minact, maxact, ni, xi);
-            }
+    for (int k = 0; k < K; k++) {
+        const int e = s0 + lane * K + k;
+        if (e < len) {
+            const int jp = M.col[start + e];
+            double l, u;
+            load_bounds(M, D, jp & kIdMask, l, u);
+            moved |= apply_entry(M, D, jp, M.val[start + e], l, u, sd.x, sd.y, minact, maxact, ni, xi);
         }
     }
+    return moved;
+}
+
+// The apply phase: the waves of the grid take the queued segments in turn.
+__device__ bool apply_phase(const DevMatrix &M, const DevState &D, int round, int wave, int nwaves) {
+    const int count = coherent_load(&D.status[kStatusWords * round + kStatusApply]);
+    bool moved = false;
+    for (int t = wave; t < count; t += nwaves) moved |= apply_split_task(M, D, coherent_load(&D.apply_tasks[t]));
     return moved;
 }
 
@@ -533,7 +542,7 @@ __device__ bool sweep_item(const DevMatrix &M, const DevState &D, int it, int ro
     const int4 d = M.items[it];
     const int kind = d.x & 15;
     if (kind == kItemSeg) {
-        seg_item(M, D, d.y, d.z, round);
+        seg_item(M, D, d.y, d.z, d.w, round);
         return false;
     }
     if (kind == kItemLong) return long_item(M, D, d.y);
@@ -550,27 +559,33 @@ __device__ bool sweep_item(const DevMatrix &M, const DevState &D, int it, int ro
 // publish items
 // ---------------------------------------------------------------------------
 
-// Copies the accumulated bounds of the columns that moved this round into the
-// snapshot, checks them for crossed bounds and clears the bitmap words.
-__device__ void publish_item(const DevMatrix &M, const DevState &D, int f, int *round_status) {
-    const int w = f * 64 + (threadIdx.x & 63);
-    bool crossed = false;
-    if (w < M.nwords) {
-        unsigned bits = coherent_load(&D.bm[w]);
-        if (bits) {
-            coherent_store(&D.bm[w], 0u);
-            while (bits) {
-                const int b = __builtin_ctz(bits);
-                bits &= bits - 1;
-                const int id = w * 32 + b;
-                double l, u;
-                accumulated(M, D, id, l, u);
-                crossed |= eps_gt(l, u);
-                store_snapshot(M, D, id, l, u);
-            }
-        }
-    }
+// Copies the accumulated bounds of column `id` into the snapshot when it
+// moved this round (bitmap), checks them for crossed bounds and clears the
+// bitmap word (the lane of the word's first column, after the wave has read
+// it; the ids of a wave are 64 consecutive columns starting at a multiple of
+// 64).  Returns true on a crossing.
+__device__ __forceinline__ bool publish_column(const DevMatrix &M, const DevState &D, int id) {
+    if (id >= M.n) return false;
+    const unsigned bits = coherent_load(&D.bm[id >> 5]);
+    if (bits == 0u) return false;
+    if ((id & 31) == 0) coherent_store(&D.bm[id >> 5], 0u);
+    if (!((bits >> (id & 31)) & 1u)) return false;
+    double l, u;
+    accumulated(M, D, id, l, u);
+    store_snapshot(M, D, id, l, u);
+    return eps_gt(l, u);
+}
+
+__device__ __forceinline__ void flag_crossing(int *round_status, bool crossed) {
     if (__any(crossed) && (threadIdx.x & 63) == 0) atomic_or_i(&round_status[kStatusInfeasible], 1);
+}
+
+// Publish phase over the columns [first, n) with a stride that is a multiple
+// of 64 (every wave sees whole bitmap words).
+__device__ __forceinline__ bool publish_phase(const DevMatrix &M, const DevState &D, int first, int stride) {
+    bool crossed = false;
+    for (int id = first; id < M.n; id += stride) crossed |= publish_column(M, D, id);
+    return crossed;
 }
 
 __device__ __forceinline__ void flag_change(int *round_status, bool moved) {
@@ -596,43 +611,55 @@ __global__ __launch_bounds__(256) void sweep_kernel(DevMatrix M, DevState D, int
 
 __global__ __launch_bounds__(256) void apply_split_kernel(DevMatrix M, DevState D, int round, int guard) {
     if (guard && round_skipped(D, round)) return;
-    const int f = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    bool moved = false;
-    if (f < M.n_apply_items) moved = apply_split_item(M, D, f, round);
+    const int waves = blockDim.x >> 6;
+    const bool moved = apply_phase(M, D, round, blockIdx.x * waves + (threadIdx.x >> 6), gridDim.x * waves);
     flag_change(D.status + kStatusWords * round, moved);
 }
 
 __global__ __launch_bounds__(256) void publish_kernel(DevMatrix M, DevState D, int round, int guard) {
     if (guard && round_skipped(D, round)) return;
-    const int f = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    if (f < M.n_pub_items) publish_item(M, D, f, D.status + kStatusWords * round);
+    const bool crossed = publish_phase(M, D, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+    flag_crossing(D.status + kStatusWords * round, crossed);
 }
 
 // ---------------------------------------------------------------------------
 // persistent kernel
 // ---------------------------------------------------------------------------
 
-// Device-wide barrier: arrival counter plus a generation word; waiting is
-// bounded by wall-clock time so that a fault elsewhere ends in an error flag
-// instead of a hang.  Everything the phases exchange is written with atomics
-// or device-coherent stores, so no cache write-back is needed; each
-// workgroup invalidates its caches once after the barrier so that its next
+// Device-wide barrier.  Workgroups arrive at one of 8 group counters
+// (blockIdx mod 8, each on its own cache line); the last arrival of a group
+// arrives at the top counter, and the last of those advances the generation
+// word that all others wait for.  Waiting is bounded by wall-clock time so
+// that a fault elsewhere ends in an error flag instead of a hang.
+// Everything the phases exchange is written with atomics or device-coherent
+// stores, so no cache write-back is needed.  With `invalidate`, each
+// workgroup invalidates its caches after the barrier so that its next
 // gathers of the snapshot cannot hit lines cached before the publish.
-__device__ bool grid_barrier(const DevState &D, unsigned nblocks) {
+__device__ bool grid_barrier(const DevState &D, unsigned nblocks, bool invalidate) {
     __shared__ int ok_sh;
     mem_wait();
     __syncthreads();
     if (threadIdx.x == 0) {
         int ok = 1;
-        unsigned *count = D.barrier;
-        unsigned *gen = D.barrier + 1;
+        unsigned *top = D.barrier + kBarrierTop;
+        unsigned *gen = D.barrier + kBarrierGen;
+        const unsigned x = blockIdx.x & 7;
+        unsigned *grp = D.barrier + kBarrierGroup + 32 * x;
+        const unsigned per = nblocks / 8 + ((nblocks & 7) > x ? 1u : 0u);
+        const unsigned groups = nblocks < 8 ? nblocks : 8;
         const unsigned g0 = coherent_load(gen);
-        const unsigned arrived = __hip_atomic_fetch_add(count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (arrived == nblocks - 1) {
-            coherent_store(count, 0u);
+        bool released = false;
+        if (__hip_atomic_fetch_add(grp, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == per - 1) {
+            coherent_store(grp, 0u);
             mem_wait();
-            __hip_atomic_fetch_add(gen, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
+            if (__hip_atomic_fetch_add(top, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == groups - 1) {
+                coherent_store(top, 0u);
+                mem_wait();
+                __hip_atomic_fetch_add(gen, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                released = true;
+            }
+        }
+        if (!released) {
             const long long t0 = wall_clock64();
             while (coherent_load(gen) == g0) {
                 __builtin_amdgcn_s_sleep(1);
@@ -647,7 +674,7 @@ __device__ bool grid_barrier(const DevState &D, unsigned nblocks) {
                 }
             }
         }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        if (invalidate) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         ok_sh = ok;
     }
     __syncthreads();
@@ -657,32 +684,33 @@ __device__ bool grid_barrier(const DevState &D, unsigned nblocks) {
 // All rounds in one persistent launch (MEGAKERNEL).  Every workgroup walks
 // the items of a phase with a grid stride (consecutive items on consecutive
 // waves), all meet at a barrier, and all read the same status words, so all
-// take the same decision to go on.  Phases of a round: sweep, apply of the
-// split rows (when there are any), publish.  Outcome words in status row 0.
+// take the same decision to go on.  Phases of a round: sweep; apply of the
+// split rows when the sweep queued any; publish.  Outcome words in status
+// row 0.
 __global__ __launch_bounds__(256) void mega_kernel(DevMatrix M, DevState D, int max_rounds) {
     const unsigned nblocks = gridDim.x;
     const int waves = blockDim.x >> 6;
     const int wave = blockIdx.x * waves + (threadIdx.x >> 6);
-    const int stride = gridDim.x * waves;
+    const int nwaves = gridDim.x * waves;
     int rounds = 0, infeasible = 0, capped = 0;
-    {
-        for (int round = 1; round <= max_rounds; round++) {
-            int *st = D.status + kStatusWords * round;
-            bool moved = false;
-            for (int it = wave; it < M.n_items; it += stride) moved |= sweep_item(M, D, it, round);
-            if (M.n_apply_items > 0) {
-                if (!grid_barrier(D, nblocks)) { rounds = -1; break; }
-                for (int f = wave; f < M.n_apply_items; f += stride) moved |= apply_split_item(M, D, f, round);
-            }
-            flag_change(st, moved);
-            if (!grid_barrier(D, nblocks)) { rounds = -1; break; }
-            for (int f = wave; f < M.n_pub_items; f += stride) publish_item(M, D, f, st);
-            if (!grid_barrier(D, nblocks)) { rounds = -1; break; }
-            rounds = round;
-            if (coherent_load(&st[kStatusInfeasible])) { infeasible = 1; break; }
-            if (!coherent_load(&st[kStatusChanged])) break;
-            if (round == max_rounds) capped = 1;
+    for (int round = 1; round <= max_rounds; round++) {
+        int *st = D.status + kStatusWords * round;
+        bool moved = false;
+        for (int it = wave; it < M.n_items; it += nwaves) moved |= sweep_item(M, D, it, round);
+        if (!grid_barrier(D, nblocks, false)) { rounds = -1; break; }
+        if (coherent_load(&st[kStatusApply]) > 0) {
+            moved |= apply_phase(M, D, round, wave, nwaves);
+            if (!grid_barrier(D, nblocks, false)) { rounds = -1; break; }
         }
+        flag_change(st, moved);
+        const bool crossed =
+            publish_phase(M, D, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+        flag_crossing(st, crossed);
+        if (!grid_barrier(D, nblocks, true)) { rounds = -1; break; }
+        rounds = round;
+        if (coherent_load(&st[kStatusInfeasible])) { infeasible = 1; break; }
+        if (!coherent_load(&st[kStatusChanged])) break;
+        if (round == max_rounds) capped = 1;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         D.status[kOutRounds] = rounds;
@@ -723,11 +751,8 @@ __global__ __launch_bounds__(256) void init_kernel(DevMatrix M, DevState D, cons
         }
     }
     for (int w = tid; w < M.nwords; w += stride) D.bm[w] = 0u;
-    for (int k = tid; k < M.n_seg_rows; k += stride) {
-        D.seg_count[k] = 0u;
-        D.seg_flag[k] = 0;
-    }
-    if (tid < 64) D.barrier[tid] = 0u;
+    for (int k = tid; k < M.n_seg_rows; k += stride) D.seg_count[k] = 0u;
+    for (int k = tid; k < kBarrierWords; k += stride) D.barrier[k] = 0u;
 }
 
 // Clears the two words init_kernel sets from many threads (separate launch).
