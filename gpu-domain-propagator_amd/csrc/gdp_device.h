@@ -12,7 +12,8 @@
 // of a round inside the per-XCD L2.
 //
 // HBM layout (rows grouped by length):
-//   row_ptr[m'+1]  int32   CSR row pointers of the m' nonempty rows
+//   row_ptr[m'+1]  int32   start of each of the m' nonempty rows (multiple of
+//                          kRowAlign) | padding entries after the row
 //   sides[m']      double2 (lhs, rhs) per row
 //   col[nnz]       int32   internal column id; bit 31 set for an integer column
 //   val[nnz]       double  coefficients
@@ -37,6 +38,13 @@ constexpr int kItemGroup = 0;
 constexpr int kItemLong = 1;   // one row, one wave, all segments (rows not split)
 constexpr int kItemSeg = 2;    // up to kSegsPerItem consecutive segments of a split row
 constexpr int kSegsPerItem = 4;
+
+// Rows start at multiples of kRowAlign entries (padding entries hold column
+// 0 and coefficient 0 and are never used); the low bits of row_ptr[r] hold
+// the number of padding entries after row r.  Aligned starts let a lane load
+// its 4 consecutive entries with one 16-byte column load and two 16-byte
+// coefficient loads.
+constexpr int kRowAlign = 4;
 
 // status words: row 0 holds the outcome, row r >= 1 the flags of round r
 constexpr int kStatusWords = 8;

@@ -158,14 +158,25 @@ Problem *problem_create(int m, int n, int nnz, const int *col, const int *row_pt
             for (int i : rows_in_bin[b]) order.push_back(i);
         const int mp = static_cast<int>(order.size());
 
+        // rows padded to multiples of kRowAlign entries (padding: column 0,
+        // coefficient 0); the padding count sits in the low bits of h_rp
+        std::vector<int> h_len(mp);
+        long long padded = 0;
+        for (int r = 0; r < mp; r++) {
+            const int i = order[r];
+            h_len[r] = row_ptr[i + 1] - row_ptr[i];
+            padded += (h_len[r] + kRowAlign - 1) / kRowAlign * kRowAlign;
+        }
+        if (padded > 0x7fffff00LL) throw std::runtime_error("too many nonzeros");
         std::vector<int> h_rp(mp + 1);
         std::vector<double2> h_sides(mp);
-        std::vector<int> h_col(nnz);
-        std::vector<double> h_val(nnz);
+        std::vector<int> h_col(padded, 0);
+        std::vector<double> h_val(padded, 0.0);
         int pos = 0;
         for (int r = 0; r < mp; r++) {
             const int i = order[r];
-            h_rp[r] = pos;
+            const int pad = (kRowAlign - h_len[r] % kRowAlign) % kRowAlign;
+            h_rp[r] = pos | pad;
             h_sides[r] = make_double2(lhs[i], rhs[i]);
             for (int k = row_ptr[i]; k < row_ptr[i + 1]; k++) {
                 const int j = col[k];
@@ -173,8 +184,10 @@ Problem *problem_create(int m, int n, int nnz, const int *col, const int *row_pt
                 h_val[pos] = val[k];
                 pos++;
             }
+            pos += pad;
         }
         h_rp[mp] = pos;
+        const int n_padded = pos;
 
         // ---- sweep items ----
         // Order: segments of split rows, whole very long rows, then the
@@ -188,7 +201,7 @@ Problem *problem_create(int m, int n, int nnz, const int *col, const int *row_pt
             const int cnt = static_cast<int>(rows_in_bin[kLongBin].size());
             std::vector<int> giant;
             for (int q = 0; q < cnt; q++) {
-                const int len = h_rp[r + q + 1] - h_rp[r + q];
+                const int len = h_len[r + q];
                 const int nseg = (len + kSegLen - 1) / kSegLen;
                 if (T.max_split_segs < 0 || nseg <= T.max_split_segs) {
                     const int lr = static_cast<int>(h_seg_rows.size());
@@ -253,16 +266,16 @@ Problem *problem_create(int m, int n, int nnz, const int *col, const int *row_pt
         M.n_seg_parts = n_parts;
         int *d_rp = dev_alloc<int>(mp + 1, P->owned);
         double2 *d_sides = dev_alloc<double2>(mp, P->owned);
-        int *d_col = dev_alloc<int>(nnz, P->owned);
-        double *d_val = dev_alloc<double>(nnz, P->owned);
+        int *d_col = dev_alloc<int>(n_padded, P->owned);
+        double *d_val = dev_alloc<double>(n_padded, P->owned);
         int4 *d_items = dev_alloc<int4>(h_items.size(), P->owned);
         int4 *d_seg_rows = dev_alloc<int4>(h_seg_rows.size(), P->owned);
         int *d_part_row = dev_alloc<int>(h_part_row.size(), P->owned);
         int *d_perm = dev_alloc<int>(n, P->owned);
         check(hipMemcpyAsync(d_rp, h_rp.data(), (mp + 1) * sizeof(int), hipMemcpyHostToDevice, s), "H2D");
         check(hipMemcpyAsync(d_sides, h_sides.data(), mp * sizeof(double2), hipMemcpyHostToDevice, s), "H2D");
-        check(hipMemcpyAsync(d_col, h_col.data(), nnz * sizeof(int), hipMemcpyHostToDevice, s), "H2D");
-        check(hipMemcpyAsync(d_val, h_val.data(), nnz * sizeof(double), hipMemcpyHostToDevice, s), "H2D");
+        check(hipMemcpyAsync(d_col, h_col.data(), n_padded * sizeof(int), hipMemcpyHostToDevice, s), "H2D");
+        check(hipMemcpyAsync(d_val, h_val.data(), n_padded * sizeof(double), hipMemcpyHostToDevice, s), "H2D");
         if (!h_items.empty())
             check(hipMemcpyAsync(d_items, h_items.data(), h_items.size() * sizeof(int4), hipMemcpyHostToDevice, s),
                   "H2D");

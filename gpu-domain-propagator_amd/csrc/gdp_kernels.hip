@@ -231,6 +231,14 @@ __device__ __forceinline__ bool apply_entry(const DevMatrix &M, const DevState &
 // sweep items
 // ---------------------------------------------------------------------------
 
+// First entry and length of row r (starts are multiples of kRowAlign; the
+// low bits of row_ptr[r] count the padding entries after the row).
+__device__ __forceinline__ void row_extent(const DevMatrix &M, int r, int &start, int &len) {
+    const int a = M.row_ptr[r], b = M.row_ptr[r + 1];
+    start = a & ~(kRowAlign - 1);
+    len = (b & ~(kRowAlign - 1)) - start - (a & (kRowAlign - 1));
+}
+
 // K consecutive entries of one lane: loads, contributions, pairwise tree of
 // the K contributions in natural order, and the cross-lane reduction.
 template <int K>
@@ -247,16 +255,44 @@ struct LaneEntries {
         dmax = 0.0;
 #pragma unroll
         for (int k = 0; k < K; k++) {
+            jp[k] = 0;
+            a[k] = 0.0;
+        }
+        // the lane's K entries in one go: base + first is a multiple of K and
+        // the row is padded to a multiple of kRowAlign (a multiple of K)
+        if (first < len) {
+            const int *cp = M.col + base + first;
+            const double *vp = M.val + base + first;
+            if constexpr (K == 4) {
+                const int4 c = *reinterpret_cast<const int4 *>(cp);
+                const double2 v0 = *reinterpret_cast<const double2 *>(vp);
+                const double2 v1 = *reinterpret_cast<const double2 *>(vp + 2);
+                jp[0] = c.x; jp[1] = c.y; jp[2] = c.z; jp[3] = c.w;
+                a[0] = v0.x; a[1] = v0.y; a[2] = v1.x; a[3] = v1.y;
+            } else if constexpr (K == 2) {
+                const int2 c = *reinterpret_cast<const int2 *>(cp);
+                const double2 v0 = *reinterpret_cast<const double2 *>(vp);
+                jp[0] = c.x; jp[1] = c.y;
+                a[0] = v0.x; a[1] = v0.y;
+            } else {
+#pragma unroll
+                for (int k = 0; k < K; k++) {
+                    jp[k] = cp[k];
+                    a[k] = vp[k];
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < K; k++) {
             const int e = first + k;
             smin[k] = 0.0;
             smax[k] = 0.0;
-            jp[k] = 0;
-            a[k] = 0.0;
             l[k] = 0.0;
             u[k] = 0.0;
-            if (e < len) {
-                jp[k] = M.col[base + e];
-                a[k] = M.val[base + e];
+            if (e >= len) {
+                jp[k] = 0;
+                a[k] = 0.0;
+            } else {
                 load_bounds(M, D, jp[k] & kIdMask, l[k], u[k]);
                 const Contribution q = contribution(a[k], l[k], u[k]);
                 smin[k] = q.min_part;
@@ -331,8 +367,7 @@ __device__ bool group_item(const DevMatrix &M, const DevState &D, int first, int
         int start = 0, len = 0;
         double lhs = 0.0, rhs = 0.0;
         if (valid) {
-            start = M.row_ptr[r];
-            len = M.row_ptr[r + 1] - start;
+            row_extent(M, r, start, len);
             const double2 sd = M.sides[r];
             lhs = sd.x;
             rhs = sd.y;
@@ -360,8 +395,8 @@ __device__ bool group_item(const DevMatrix &M, const DevState &D, int first, int
 __device__ bool long_item(const DevMatrix &M, const DevState &D, int r) {
     constexpr int K = kSegLen / 64;
     const int lane = threadIdx.x & 63;
-    const int start = M.row_ptr[r];
-    const int len = M.row_ptr[r + 1] - start;
+    int start, len;
+    row_extent(M, r, start, len);
     const double2 sd = M.sides[r];
     double acc_min = 0.0, acc_max = 0.0, dmax = 0.0;
     int mininf = 0, maxinf = 0;
@@ -432,8 +467,8 @@ __device__ void seg_item(const DevMatrix &M, const DevState &D, int lr, int s, i
     const int lane = threadIdx.x & 63;
     const int4 info = M.seg_rows[lr];
     const int r = info.x;
-    const int start = M.row_ptr[r];
-    const int len = M.row_ptr[r + 1] - start;
+    int start, len;
+    row_extent(M, r, start, len);
     for (int q = 0; q < cnt; q++) {
         const int s0 = (s + q) * kSegLen;
         LaneEntries<K> E;
@@ -507,8 +542,8 @@ __device__ bool apply_split_task(const DevMatrix &M, const DevState &D, int part
     const int lr = M.part_row[part];
     const int4 info = M.seg_rows[lr];
     const int r = info.x;
-    const int start = M.row_ptr[r];
-    const int len = M.row_ptr[r + 1] - start;
+    int start, len;
+    row_extent(M, r, start, len);
     const int s0 = (part - info.y) * kSegLen;
     const SegState *stp = &D.seg_state[lr];
     const double minact = coherent_load_d(&stp->minact);
