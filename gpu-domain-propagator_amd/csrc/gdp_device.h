@@ -1,5 +1,19 @@
 // gdp_device.h - device-side layout of a propagation problem.
 //
+// A problem holds one matrix layout and the state of one or more propagation
+// units.  A unit is one bound vector being propagated with its own round
+// count, change and infeasibility flags.  Two unit modes:
+//   blocks    the matrix is a block-diagonal batch of independent instances;
+//             unit u owns the rows and columns of instance u, all units share
+//             the state arrays (disjoint parts of them)
+//   replicas  all units share one matrix; unit u has its own copy of every
+//             state array (offset u * per-unit size); a single propagation is
+//             the replicas mode with one unit
+// Every launch works on a step: unit u runs its round (step - start_u + 1)
+// in step `step`.  Units that have converged, crossed or reached their round
+// cap skip their work, so units of different ages share a step (continuous
+// batching) and finished units cost nothing.
+//
 // Columns are relabelled internally ("internal ids") and grouped by how their
 // bounds are stored:
 //   ids [0, n16)      integer columns whose bounds stay integral within
@@ -7,25 +21,29 @@
 //   ids [n16, n_int)  integer columns whose bounds stay integral within the
 //                     int32 range: snapshot record 2 x int32 = 8 bytes
 //   ids [n_int, n)    all other columns: snapshot record 2 x double = 16 bytes
-// The entry order inside every row is unchanged, so the relabelling does not
-// alter any arithmetic.  Small records keep the randomly gathered working set
-// of a round inside the per-XCD L2.
+// In blocks mode each (class, instance) range is padded to a multiple of 32
+// ids, so that every 32-column bitmap word belongs to one unit.  The entry
+// order inside every row is unchanged, so the relabelling does not alter any
+// arithmetic.
 //
 // HBM layout (rows grouped by length):
 //   row_ptr[m'+1]  int32   start of each of the m' nonempty rows (multiple of
 //                          kRowAlign) | padding entries after the row
 //   sides[m']      double2 (lhs, rhs) per row
-//   col[nnz]       int32   internal column id; bit 31 set for an integer column
-//   val[nnz]       double  coefficients
+//   col[nnz']      int32   internal column id; bit 31 set for an integer column
+//   val[nnz']      double  coefficients
 //   items[]        int4    work items of the sweep
+//   csc            col_ptr[n+1], col_rows[nnz]: rows of each column (marking)
+// per unit:
 //   snapshot       S16[n16] uint32, S32[n32] int2, SF[nf] double2: the bounds
 //                  every row of a round reads
 //   accumulator    AI[n_int] int2 (integer atomics), AF[nf] double2 (FP64
 //                  atomics), AX[n_int] double2 (exact values of integer
 //                  candidates beyond the int32 range; they only arise when
 //                  bounds cross)
-//   bm[nw]         uint32  columns changed in the current round
-//   status[]       int32   per-round flags and the outcome of a run
+//   bm[nwords]     uint32  columns moved in the current step (scanned by publish)
+//   marks[m']      int32   step in which the row must be processed
+//   split rows     seg_parts, seg_state, seg_count
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -38,6 +56,7 @@ constexpr int kItemGroup = 0;
 constexpr int kItemLong = 1;   // one row, one wave, all segments (rows not split)
 constexpr int kItemSeg = 2;    // up to kSegsPerItem consecutive segments of a split row
 constexpr int kSegsPerItem = 4;
+constexpr int kItemUnitShift = 12;  // items[].x >> 12: unit of the item (blocks mode)
 
 // Rows start at multiples of kRowAlign entries (padding entries hold column
 // 0 and coefficient 0 and are never used); the low bits of row_ptr[r] hold
@@ -46,21 +65,40 @@ constexpr int kSegsPerItem = 4;
 // coefficient loads.
 constexpr int kRowAlign = 4;
 
-// status words: row 0 holds the outcome, row r >= 1 the flags of round r
+// Per-step status words, in a ring of kStatusRows rows (row step % kStatusRows).
 constexpr int kStatusWords = 8;
-constexpr int kOutRounds = 0;
-constexpr int kOutInfeasible = 2;
-constexpr int kOutCapped = 3;
-constexpr int kGlobalError = 4;
-constexpr int kGlobalDone = 5;
-constexpr int kGlobalEnvelope = 6;  // input bounds outside the class envelope
-constexpr int kGlobalCrossed = 7;   // input bounds already crossed
-constexpr int kStatusChanged = 0;
-constexpr int kStatusInfeasible = 1;
-constexpr int kStatusApply = 2;   // segments queued for the apply phase
-constexpr int kStatusRows = 128;  // rounds 1..100 fit
+constexpr int kStatusRows = 128;
+constexpr int kStatusChanged = 0;  // some unit moved a bound in the step
+constexpr int kStatusApply = 2;    // split-row segments queued for the apply phase
+// Words of the problem-wide status block (separate from the ring).
+constexpr int kGlobalError = 0;    // a bounded device wait timed out
+constexpr int kGlobalSteps = 1;    // steps run by the persistent kernel
+constexpr int kGlobalWords = 8;
 
-// grid barrier words, each counter on its own 128-byte line
+// State of one unit (int32 words, read as 64-bit pairs).  A step's changes
+// go to slot step & 1 of `changed`, and step s asks slot (s - 1) & 1 whether
+// the unit moved in its previous round: writers and readers of one step
+// never share a slot.
+struct UnitState {
+    int start;        // step of the unit's round 1; 0: no bound vector loaded
+    int infeasible;   // step in which bounds crossed (0: none)
+    int markall;      // step in which every row of the unit is processed
+    int max_rounds;   // round cap
+    int crossed;      // input bounds already crossed
+    int envelope;     // input bounds outside the storage-class envelope
+    int changed[2];   // last step of each parity in which the unit moved a bound
+    int mark_limit;   // accepted candidates of a step from which all rows are processed
+    int pad[7];
+};
+
+// Accepted candidates counted per unit and step (ring of kMoveSlots steps,
+// kMoveLanes counters per slot to spread the atomics); a unit whose count
+// reaches its threshold processes all rows next step instead of pushing row
+// marks (the count bounds the number of moved columns from above).
+constexpr int kMoveSlots = 4;
+constexpr int kMoveLanes = 16;
+
+// Grid barrier words, each counter on its own 128-byte line.
 constexpr int kBarrierTop = 0;
 constexpr int kBarrierGen = 32;
 constexpr int kBarrierGroup = 64;  // + 32 * group, 8 groups
@@ -75,8 +113,8 @@ struct SegPart {
     int mininf, maxinf;
 };
 
-// Activity of a split row, written by the last segment wave of a round
-// when the row gate passes (read by the apply phase of that round).
+// Activity of a split row, written by the last segment wave of a step
+// when the row gate passes (read by the apply phase of that step).
 struct SegState {
     double minact, maxact;
     int mininf, maxinf;
@@ -84,21 +122,25 @@ struct SegState {
 
 struct DevMatrix {
     int m;            // nonempty rows
-    int n;            // columns
-    int nnz;
+    int n;            // internal column ids (including padding ids)
     int n16, n_int;   // class boundaries of the internal ids
-    int nwords;       // bitmap words, ceil(n / 32)
-    int n_items;      // sweep items
+    int nwords;       // bitmap words per unit, n / 32
+    int n_items;      // sweep items (of one unit in replicas mode)
+    int replicas;     // 1: replicas mode, 0: blocks mode
+    int mark_rows;    // 0: every step processes every row of an active unit
     const int *row_ptr;
     const double2 *sides;
     const int *col;
     const double *val;
     const int4 *items;
-    const int4 *seg_rows;  // per split row: {row, first partial, segments, 0}
+    const int4 *seg_rows;  // per split row: {row, first partial, segments, unit}
     const int *part_row;   // per segment of a split row: its split-row index
     int n_seg_rows;
     int n_seg_parts;
-    const int *perm;       // internal id -> original column
+    const int *col_ptr;    // csc of the internal ids (marking)
+    const int *col_rows;
+    const int *perm;       // internal id -> caller's column (-1: padding id)
+    const int *word_unit;  // blocks mode: unit of each bitmap word
 };
 
 struct DevState {
@@ -109,12 +151,38 @@ struct DevState {
     double2 *AX;
     double2 *AF;
     unsigned *bm;
-    int *status;        // kStatusRows * kStatusWords
-    unsigned *barrier;  // kBarrierWords: top counter, generation, 8 group counters
-    SegPart *seg_parts;    // [n_seg_parts]
-    SegState *seg_state;   // [n_seg_rows]
-    unsigned *seg_count;   // [n_seg_rows] segments reduced in the current round
-    int *apply_tasks;      // [n_seg_parts] segments whose row passed the gate
+    int *marks;
+    SegPart *seg_parts;
+    SegState *seg_state;
+    unsigned *seg_count;   // segments reduced in the current step
+    // shared by all units
+    UnitState *units;
+    int *moved;            // [units][kMoveSlots][kMoveLanes] accepted candidates
+    int *status;           // kStatusRows * kStatusWords ring
+    int *global;           // kGlobalWords
+    unsigned *barrier;     // kBarrierWords
+    int *apply_tasks;      // unit * n_seg_parts + segment, per step
+    const int *active;     // units of the launch (replicas mode)
+    int n_active;
 };
+
+// State arrays of unit u (replicas mode); the shared arrays are unchanged.
+__device__ __forceinline__ DevState unit_view(const DevMatrix &M, const DevState &D, int u) {
+    if (!M.replicas) return D;
+    DevState V = D;
+    const size_t k = static_cast<size_t>(u);
+    V.S16 += k * M.n16;
+    V.S32 += k * (M.n_int - M.n16);
+    V.SF += k * (M.n - M.n_int);
+    V.AI += k * M.n_int;
+    V.AX += k * M.n_int;
+    V.AF += k * (M.n - M.n_int);
+    V.bm += k * M.nwords;
+    V.marks += k * M.m;
+    V.seg_parts += k * M.n_seg_parts;
+    V.seg_state += k * M.n_seg_rows;
+    V.seg_count += k * M.n_seg_rows;
+    return V;
+}
 
 }  // namespace gdp

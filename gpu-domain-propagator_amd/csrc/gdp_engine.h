@@ -1,11 +1,16 @@
 // gdp_engine.h - host-side API of the MI355X propagation engine (C++).
 //
-// A Problem owns a device-resident copy of one constraint matrix laid out for
-// the round kernels (rows grouped by length, columns relabelled by bound
-// storage class, integrality folded into the column index) plus the per-run
-// device state (bound snapshot and accumulator, change bitmap, per-round
-// status words).  It is built once and can propagate any number of bound
-// vectors that lie inside its bound envelope.
+// A Problem owns a device-resident layout of a constraint matrix for the
+// step kernels (rows grouped by length, columns relabelled by bound storage
+// class, integrality folded into the column index, a column-to-row index for
+// row marking) plus the device state of its propagation units:
+//   * a single matrix with state for `capacity` bound vectors (replicas
+//     mode): one propagation, many node vectors at once, or a server that
+//     admits node requests into free slots between steps;
+//   * a block-diagonal batch of independent instances (blocks mode): all
+//     instances propagate in the same launches, each with its own rounds.
+// Every unit is round-synchronous: bit-identical to the oracle's
+// round-synchronous propagation of that unit alone.
 #pragma once
 
 #include <cstdint>
@@ -28,6 +33,7 @@ struct Tuning {
     int graph_rounds = 8;     // rounds per captured graph for GPU_LOOP
     int max_split_segs = -1;  // rows with more segments run on one wave (-1: split all, 0: none)
     int compact_bounds = 1;   // 0: keep every column in 16-byte FP64 records
+    int row_marking = 1;      // 0: every round processes every row of an active unit
     // per tree-width bin (width 2^b, b = 0..9): lanes per row as log2
     // default: K = min(width, 4) consecutive entries per lane
     int group_log[10] = {0, 0, 0, 1, 2, 3, 4, 5, 6, 6};
@@ -48,6 +54,17 @@ struct ProblemStats {
     int64_t bin_rows[11] = {0};
     int64_t bin_nnz[11] = {0};
     int n16 = 0, n32 = 0, nf = 0;  // columns per bound storage class
+    int n_units = 0;               // instances (batch) or unit capacity
+    int64_t unit_state_bytes = 0;  // device bytes of one unit's state (replicas)
+};
+
+// One instance of a batch (caller's arrays, CSR, zero-based).
+struct InstanceView {
+    int m = 0, n = 0, nnz = 0;
+    const int *col = nullptr, *row_ptr = nullptr;
+    const double *val = nullptr, *lhs = nullptr, *rhs = nullptr;
+    const int *vartype = nullptr;
+    const double *env_lb = nullptr, *env_ub = nullptr;
 };
 
 struct Problem;
@@ -61,8 +78,12 @@ Problem *problem_create(int m, int n, int nnz, const int *col, const int *row_pt
                         const double *val, const double *lhs, const double *rhs,
                         const int *vartype, const double *env_lb, const double *env_ub,
                         const Tuning *tuning);
+// A block-diagonal batch of independent instances.
+Problem *batch_create(const std::vector<InstanceView> &instances, const Tuning *tuning);
 void problem_destroy(Problem *p);
 ProblemStats problem_stats(const Problem *p);
+// Replicas mode: device state for `capacity` bound vectors.
+void problem_reserve(Problem *p, int capacity);
 
 // Propagates lbs/ubs (host arrays of length n, in/out).
 RunResult problem_propagate(Problem *p, double *lbs, double *ubs, int sync_type, int max_rounds);
@@ -71,6 +92,42 @@ RunResult problem_propagate(Problem *p, double *lbs, double *ubs, int sync_type,
 // runs on the problem's stream and synchronizes before returning.
 RunResult problem_propagate_device(Problem *p, double *d_lbs, double *d_ubs, int sync_type,
                                    int max_rounds);
+
+// Replicas mode: `count` device-resident bound vectors at once (vector k's
+// at d_lbs[k], d_ubs[k], host arrays of device pointers).  With hints (host
+// CSR: vector k's columns hints[hint_ptr[k] .. hint_ptr[k+1])), the vectors
+// differ from a propagated fixpoint only in those columns and round 1
+// processes only their rows.  Results per vector in out[k]; the kernel time
+// of the whole launch in each out[k].kernel_ms.
+void nodes_propagate_device(Problem *p, int count, double *const *d_lbs, double *const *d_ubs,
+                            const int *hint_ptr, const int *hints, int sync_type, int max_rounds,
+                            RunResult *out);
+
+// Blocks mode: every instance at once; d_lbs/d_ubs concatenate the
+// instances' bound vectors (device).  Results per instance in out[i].
+void batch_propagate_device(Problem *p, double *d_lbs, double *d_ubs, int sync_type, int max_rounds,
+                            RunResult *out);
+
+// Serving: node requests against a device-resident matrix (replicas mode).
+// Requests are queued by submit; each step admits queued requests into free
+// slots (up to the slot count the HBM budget allows), advances every active
+// slot by one round in one set of launches, and retires finished slots,
+// whose bounds are copied back.  Requests admitted in different steps share
+// the device: continuous batching.
+struct Server;
+Server *server_create(Problem *p, int64_t hbm_budget, int max_slots);
+void server_destroy(Server *s);
+int server_slots(const Server *s);
+// Copies the request; returns its ticket.
+int64_t server_submit(Server *s, const double *lbs, const double *ubs, const int *hints, int n_hints,
+                      int max_rounds);
+// One step; returns the number of requests completed by it.
+int server_step(Server *s);
+// Requests queued or running.
+int server_pending(const Server *s);
+// Completed request: copies its bounds and outcome and forgets it; false if
+// the ticket is not (yet) complete.
+bool server_result(Server *s, int64_t ticket, double *lbs, double *ubs, RunResult *r);
 
 std::string last_error();
 void set_last_error(const std::string &msg);

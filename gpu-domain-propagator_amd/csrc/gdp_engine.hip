@@ -5,17 +5,20 @@
 // fill_row_blocks (src/kernels/util_kernels.cu:12-62).  Differences by
 // design: the matrix is laid out once per Problem (rows grouped by tree
 // width, columns relabelled by bound storage class, integrality folded into
-// the column index), rounds are round-synchronous (sweep, apply of split rows, publish), and
-// GPU_LOOP is a captured HIP graph of round launches that stop themselves on
-// the device.
+// the column index, a column-to-row index for row marking); rounds are
+// round-synchronous steps (sweep, apply of split rows, publish); GPU_LOOP is
+// a captured HIP graph of step launches that stop themselves on the device;
+// several bound vectors or instances can share the launches (units).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <deque>
 #include <mutex>
 #include <stdexcept>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "gdp_arith.h"
@@ -24,13 +27,17 @@
 
 namespace gdp {
 
-__global__ void sweep_kernel(DevMatrix M, DevState D, int round, int guard);
-__global__ void apply_split_kernel(DevMatrix M, DevState D, int round, int guard);
-__global__ void publish_kernel(DevMatrix M, DevState D, int round, int guard);
-__global__ void mega_kernel(DevMatrix M, DevState D, int max_rounds);
-__global__ void init_kernel(DevMatrix M, DevState D, const double *lbs, const double *ubs);
-__global__ void clear_envelope_kernel(DevState D);
-__global__ void unpack_kernel(DevMatrix M, DevState D, double *lbs, double *ubs);
+__global__ void sweep_kernel(DevMatrix M, DevState D, int step, int guard, int first_step);
+__global__ void apply_split_kernel(DevMatrix M, DevState D, int step, int guard, int first_step);
+__global__ void publish_kernel(DevMatrix M, DevState D, int step, int guard, int first_step, int n_units);
+__global__ void mega_kernel(DevMatrix M, DevState D, int first_step, int max_steps, int n_units);
+__global__ void unit_reset_kernel(DevState D, int n_list, const int4 *params, int start_step, int clear_status);
+__global__ void load_kernel(DevMatrix M, DevState D, int n_list, const int *list, const double *const *lbp,
+                            const double *const *ubp);
+__global__ void hint_kernel(DevMatrix M, DevState D, int n_list, const int *list, const int *hint_ptr,
+                            const int *hints, const int *inv_perm, int start_step);
+__global__ void unpack_kernel(DevMatrix M, DevState D, int n_list, const int *list, double *const *lbp,
+                              double *const *ubp);
 
 namespace {
 
@@ -55,6 +62,11 @@ T *dev_alloc(size_t count, std::vector<void *> &owned) {
     return static_cast<T *>(p);
 }
 
+template <typename T>
+void upload(T *dst, const T *src, size_t count, hipStream_t s) {
+    if (count) check(hipMemcpyAsync(dst, src, count * sizeof(T), hipMemcpyHostToDevice, s), "H2D");
+}
+
 // Storage class of a column from its bound envelope: 0 = 2 x int16,
 // 1 = 2 x int32, 2 = 2 x double.
 int column_class(int vartype, const double *env_lb, const double *env_ub, int j) {
@@ -67,104 +79,256 @@ int column_class(int vartype, const double *env_lb, const double *env_ub, int j)
     return 2;
 }
 
+void validate(const InstanceView &v) {
+    if (v.m <= 0 || v.n <= 0 || v.nnz <= 0) throw std::runtime_error("empty problem");
+    if (!v.col || !v.row_ptr || !v.val || !v.lhs || !v.rhs || !v.vartype) throw std::runtime_error("null input array");
+    if (v.row_ptr[0] != 0 || v.row_ptr[v.m] != v.nnz) throw std::runtime_error("row_ptrs do not span nnz");
+    for (int i = 0; i < v.m; i++)
+        if (v.row_ptr[i + 1] < v.row_ptr[i]) throw std::runtime_error("row_ptrs not monotone");
+    for (int k = 0; k < v.nnz; k++)
+        if (v.col[k] < 0 || v.col[k] >= v.n) throw std::runtime_error("column index out of range");
+}
+
+// Outcome of one unit from its state words (the reference's round-count
+// convention: the final no-change round counts).
+RunResult outcome(const UnitState &s) {
+    RunResult r;
+    if (s.envelope) throw std::runtime_error("bounds outside the problem's bound envelope");
+    if (s.crossed) {
+        r.infeasible = 1;
+        return r;
+    }
+    if (s.infeasible) {
+        r.rounds = s.infeasible - s.start + 1;
+        r.infeasible = 1;
+        return r;
+    }
+    const int last_change = std::max(s.changed[0], s.changed[1]);
+    const int changed_rounds = last_change ? last_change - s.start + 1 : 0;
+    if (changed_rounds >= s.max_rounds) {
+        r.rounds = s.max_rounds;
+        r.capped = 1;
+    } else {
+        r.rounds = changed_rounds + 1;
+    }
+    return r;
+}
+
+// Whether a unit works in the step after `step` (host mirror of unit_step).
+bool active_after(const UnitState &s, int step) {
+    if (s.start <= 0 || s.crossed || s.envelope || s.infeasible) return false;
+    if (step + 1 - s.start >= s.max_rounds) return false;
+    return s.changed[step & 1] == step;
+}
+
+int grid_for(long long waves) { return static_cast<int>(std::max(1LL, (waves + 3) / 4)); }
+
 }  // namespace
 
 std::string last_error() { return g_last_error; }
 void set_last_error(const std::string &msg) { g_last_error = msg; }
 
 struct Problem {
-    int m = 0, n = 0, nnz = 0;
+    bool replicas = true;
+    int m = 0, n = 0, nnz = 0;       // caller's totals (all instances)
+    int n_inst = 1;
     Tuning tuning;
     ProblemStats stats;
     DevMatrix M{};
     DevState D{};
-    std::vector<void *> owned;
+    std::vector<void *> owned;       // matrix layout and fixed-size state
+    std::vector<void *> unit_owned;  // per-unit state (depends on the capacity)
+    int capacity = 0;                // units with state arrays (replicas)
+    int n_units = 0;                 // entries of D.units
+    std::vector<int> inst_ncols;     // blocks: columns per instance
+    std::vector<int> mark_limit;     // per instance (blocks) or for every unit (replicas)
+    int *d_list = nullptr;           // listed units of a launch
+    int4 *d_params = nullptr;        // unit_reset parameters
+    double **d_lbp = nullptr;        // bound-vector pointer tables
+    double **d_ubp = nullptr;
+    int *d_inv_perm = nullptr;
+    int *d_hint_ptr = nullptr, *d_hints = nullptr;
+    size_t hints_cap = 0;
+    std::vector<void *> hint_owned;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int device = 0;
     int cu_count = 0;
     int mega_grid = 0;
-    // captured GPU_LOOP graphs, one per chunk of rounds
+    // captured GPU_LOOP graphs, one per chunk of steps, for one launch shape
     std::vector<hipGraphExec_t> graphs;
     std::vector<int> graph_last;
+    long long graph_key = -1;
     std::mutex run_mutex;
 
-    ~Problem() {
+    void drop_graphs() {
         for (auto g : graphs)
             if (g) (void)hipGraphExecDestroy(g);
+        graphs.clear();
+        graph_last.clear();
+        graph_key = -1;
+    }
+
+    ~Problem() {
+        drop_graphs();
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         if (stream) (void)hipStreamDestroy(stream);
+        for (void *p : unit_owned) (void)hipFree(p);
+        for (void *p : hint_owned) (void)hipFree(p);
         for (void *p : owned) (void)hipFree(p);
     }
 };
 
-Problem *problem_create(int m, int n, int nnz, const int *col, const int *row_ptr, const double *val,
-                        const double *lhs, const double *rhs, const int *vartype, const double *env_lb,
-                        const double *env_ub, const Tuning *tuning) {
-    if (m <= 0 || n <= 0 || nnz <= 0) throw std::runtime_error("empty problem");
-    if (n > kIdMask) throw std::runtime_error("too many columns");
-    if (!col || !row_ptr || !val || !lhs || !rhs || !vartype) throw std::runtime_error("null input array");
-    if (row_ptr[0] != 0 || row_ptr[m] != nnz) throw std::runtime_error("row_ptrs do not span nnz");
-    for (int i = 0; i < m; i++)
-        if (row_ptr[i + 1] < row_ptr[i]) throw std::runtime_error("row_ptrs not monotone");
-    for (int k = 0; k < nnz; k++)
-        if (col[k] < 0 || col[k] >= n) throw std::runtime_error("column index out of range");
+namespace {
 
+// Device state of `cap` units (replicas) or of the batch (blocks).
+void alloc_units(Problem *P, int cap) {
+    check(hipStreamSynchronize(P->stream), "sync");
+    P->drop_graphs();
+    for (void *p : P->unit_owned) (void)hipFree(p);
+    P->unit_owned.clear();
+    DevMatrix &M = P->M;
+    DevState &D = P->D;
+    const size_t k = P->replicas ? static_cast<size_t>(cap) : 1;
+    const int n32 = M.n_int - M.n16, nf = M.n - M.n_int;
+    auto &o = P->unit_owned;
+    D.S16 = dev_alloc<unsigned>(k * M.n16, o);
+    D.S32 = dev_alloc<int2>(k * n32, o);
+    D.SF = dev_alloc<double2>(k * nf, o);
+    D.AI = dev_alloc<int2>(k * M.n_int, o);
+    D.AX = dev_alloc<double2>(k * M.n_int, o);
+    D.AF = dev_alloc<double2>(k * nf, o);
+    D.bm = dev_alloc<unsigned>(k * M.nwords, o);
+    D.marks = dev_alloc<int>(k * M.m, o);
+    D.seg_parts = dev_alloc<SegPart>(k * M.n_seg_parts, o);
+    D.seg_state = dev_alloc<SegState>(k * M.n_seg_rows, o);
+    D.seg_count = dev_alloc<unsigned>(k * M.n_seg_rows, o);
+    const int units = P->replicas ? cap : P->n_inst;
+    D.units = dev_alloc<UnitState>(units, o);
+    D.moved = dev_alloc<int>(static_cast<size_t>(units) * kMoveSlots * kMoveLanes, o);
+    D.apply_tasks = dev_alloc<int>(k * M.n_seg_parts, o);
+    P->d_list = dev_alloc<int>(units, o);
+    P->d_params = dev_alloc<int4>(units, o);
+    P->d_lbp = dev_alloc<double *>(units, o);
+    P->d_ubp = dev_alloc<double *>(units, o);
+    P->d_hint_ptr = dev_alloc<int>(static_cast<size_t>(units) + 1, o);
+    check(hipMemsetAsync(D.units, 0, sizeof(UnitState) * units, P->stream), "memset units");
+    if (M.n_seg_rows > 0)
+        check(hipMemsetAsync(D.seg_count, 0, sizeof(unsigned) * k * M.n_seg_rows, P->stream), "memset");
+    check(hipMemsetAsync(D.bm, 0, sizeof(unsigned) * k * M.nwords, P->stream), "memset");
+    check(hipStreamSynchronize(P->stream), "alloc sync");
+    P->capacity = cap;
+    P->n_units = units;
+    D.active = P->d_list;
+    const long long per_unit = static_cast<long long>(M.n16) * 4 + n32 * 8LL + nf * 16LL + M.n_int * 24LL +
+                               nf * 16LL + M.nwords * 8LL + M.m * 4LL + M.n_seg_parts * 36LL +
+                               M.n_seg_rows * 28LL + sizeof(UnitState) + kMoveSlots * kMoveLanes * 4;
+    P->stats.unit_state_bytes = per_unit;
+    P->stats.n_units = units;
+}
+
+Problem *build(const std::vector<InstanceView> &inst, bool replicas, const Tuning *tuning) {
+    if (inst.empty()) throw std::runtime_error("no instance");
+    for (const auto &v : inst) validate(v);
     auto *P = new Problem();
     try {
-        P->m = m;
-        P->n = n;
-        P->nnz = nnz;
+        P->replicas = replicas;
+        P->n_inst = static_cast<int>(inst.size());
         if (tuning) P->tuning = *tuning;
         const Tuning &T = P->tuning;
-
-        // ---- columns: internal ids grouped by bound storage class ----
-        std::vector<int> cls(n);
-        int ncls[3] = {0, 0, 0};
-        for (int j = 0; j < n; j++) {
-            cls[j] = T.compact_bounds ? column_class(vartype[j], env_lb, env_ub, j) : 2;
-            ncls[cls[j]]++;
+        const int NI = P->n_inst;
+        std::vector<int> col_off(NI + 1, 0), row_off(NI + 1, 0);
+        long long nnz_total = 0;
+        for (int i = 0; i < NI; i++) {
+            col_off[i + 1] = col_off[i] + inst[i].n;
+            row_off[i + 1] = row_off[i] + inst[i].m;
+            nnz_total += inst[i].nnz;
+            P->inst_ncols.push_back(inst[i].n);
         }
-        std::vector<int> perm(n), internal(n);
-        {
-            int next[3] = {0, ncls[0], ncls[0] + ncls[1]};
-            for (int j = 0; j < n; j++) {
-                const int id = next[cls[j]]++;
-                perm[id] = j;
-                internal[j] = id;
+        if (nnz_total > 0x7fffff00LL || col_off[NI] > kIdMask) throw std::runtime_error("problem too large");
+        // row marking pays while the rows it marks (about accepted candidates
+        // x mean column length) stay below a quarter of the rows
+        for (int i = 0; i < NI; i++) {
+            const long long lim = static_cast<long long>(inst[i].m) * inst[i].n / (4LL * std::max(1, inst[i].nnz));
+            P->mark_limit.push_back(static_cast<int>(std::max(64LL, std::min(lim, 1LL << 30))));
+        }
+        if (replicas) P->mark_limit.resize(1);
+        P->m = row_off[NI];
+        P->n = col_off[NI];
+        P->nnz = static_cast<int>(nnz_total);
+
+        // ---- columns: internal ids by storage class, instance-major within a
+        // class; blocks mode pads each (class, instance) range to 32 ids ----
+        std::vector<int> cls(P->n);
+        for (int i = 0; i < NI; i++)
+            for (int j = 0; j < inst[i].n; j++)
+                cls[col_off[i] + j] = T.compact_bounds ? column_class(inst[i].vartype[j], inst[i].env_lb,
+                                                                      inst[i].env_ub, j)
+                                                       : 2;
+        std::vector<int> perm, internal(P->n), word_unit;
+        int class_start[3];
+        for (int c = 0; c < 3; c++) {
+            class_start[c] = static_cast<int>(perm.size());
+            for (int i = 0; i < NI; i++) {
+                for (int j = 0; j < inst[i].n; j++) {
+                    const int g = col_off[i] + j;
+                    if (cls[g] != c) continue;
+                    internal[g] = static_cast<int>(perm.size());
+                    perm.push_back(g);
+                }
+                if (!replicas)
+                    while (perm.size() % 32) perm.push_back(-1);
             }
         }
-        P->stats.n16 = ncls[0];
-        P->stats.n32 = ncls[1];
-        P->stats.nf = ncls[2];
-
-        // ---- rows grouped by tree width: long rows first, then wide to narrow ----
-        // bin b < kLongBin holds rows with next_pow2(len) == 2^b; bin kLongBin
-        // rows longer than kSegLen.
-        constexpr int kLongBin = 10;
-        std::vector<std::vector<int>> rows_in_bin(11);
-        for (int i = 0; i < m; i++) {
-            const int len = row_ptr[i + 1] - row_ptr[i];
-            if (len == 0) continue;
-            const int b = len > kSegLen ? kLongBin : next_pow2_log(len);
-            rows_in_bin[b].push_back(i);
-            P->stats.bin_rows[b]++;
-            P->stats.bin_nnz[b] += len;
+        const int n_ids = static_cast<int>(perm.size());
+        if (n_ids > kIdMask) throw std::runtime_error("too many columns");
+        const int nwords = (n_ids + 31) / 32;
+        if (!replicas) {
+            word_unit.assign(nwords, 0);
+            std::vector<int> unit_of_col(P->n);
+            for (int i = 0; i < NI; i++)
+                for (int j = 0; j < inst[i].n; j++) unit_of_col[col_off[i] + j] = i;
+            for (int w = 0; w < nwords; w++) {
+                int u = -1;
+                for (int b = 0; b < 32 && w * 32 + b < n_ids; b++)
+                    if (perm[w * 32 + b] >= 0) {
+                        u = unit_of_col[perm[w * 32 + b]];
+                        break;
+                    }
+                word_unit[w] = u < 0 ? 0 : u;
+            }
         }
-        std::vector<int> order;
-        order.reserve(m);
+        P->stats.n16 = class_start[1] - class_start[0];
+        P->stats.n32 = class_start[2] - class_start[1];
+        P->stats.nf = n_ids - class_start[2];
+
+        // ---- rows grouped by tree width: long rows first, then wide to
+        // narrow; instance-major inside a bin so that items never mix units ----
+        constexpr int kLongBin = 10;
+        std::vector<std::vector<std::pair<int, int>>> rows_in_bin(11);  // (instance, row)
+        for (int i = 0; i < NI; i++)
+            for (int r = 0; r < inst[i].m; r++) {
+                const int len = inst[i].row_ptr[r + 1] - inst[i].row_ptr[r];
+                if (len == 0) continue;
+                const int b = len > kSegLen ? kLongBin : next_pow2_log(len);
+                rows_in_bin[b].push_back({i, r});
+                P->stats.bin_rows[b]++;
+                P->stats.bin_nnz[b] += len;
+            }
+        std::vector<std::pair<int, int>> order;
         for (int b = kLongBin; b >= 0; b--)
-            for (int i : rows_in_bin[b]) order.push_back(i);
+            for (auto &x : rows_in_bin[b]) order.push_back(x);
         const int mp = static_cast<int>(order.size());
 
         // rows padded to multiples of kRowAlign entries (padding: column 0,
         // coefficient 0); the padding count sits in the low bits of h_rp
-        std::vector<int> h_len(mp);
+        std::vector<int> h_len(mp), h_unit(mp);
         long long padded = 0;
         for (int r = 0; r < mp; r++) {
-            const int i = order[r];
-            h_len[r] = row_ptr[i + 1] - row_ptr[i];
+            const auto &v = inst[order[r].first];
+            h_len[r] = v.row_ptr[order[r].second + 1] - v.row_ptr[order[r].second];
+            h_unit[r] = order[r].first;
             padded += (h_len[r] + kRowAlign - 1) / kRowAlign * kRowAlign;
         }
         if (padded > 0x7fffff00LL) throw std::runtime_error("too many nonzeros");
@@ -172,22 +336,37 @@ Problem *problem_create(int m, int n, int nnz, const int *col, const int *row_pt
         std::vector<double2> h_sides(mp);
         std::vector<int> h_col(padded, 0);
         std::vector<double> h_val(padded, 0.0);
+        std::vector<int> col_cnt(n_ids + 1, 0);
         int pos = 0;
         for (int r = 0; r < mp; r++) {
-            const int i = order[r];
+            const int i = order[r].first, row = order[r].second;
+            const auto &v = inst[i];
             const int pad = (kRowAlign - h_len[r] % kRowAlign) % kRowAlign;
             h_rp[r] = pos | pad;
-            h_sides[r] = make_double2(lhs[i], rhs[i]);
-            for (int k = row_ptr[i]; k < row_ptr[i + 1]; k++) {
-                const int j = col[k];
-                h_col[pos] = internal[j] | (vartype[j] == kVarContinuous ? 0 : kIntFlag);
-                h_val[pos] = val[k];
+            h_sides[r] = make_double2(v.lhs[row], v.rhs[row]);
+            for (int k = v.row_ptr[row]; k < v.row_ptr[row + 1]; k++) {
+                const int id = internal[col_off[i] + v.col[k]];
+                h_col[pos] = id | (v.vartype[v.col[k]] == kVarContinuous ? 0 : kIntFlag);
+                h_val[pos] = v.val[k];
+                col_cnt[id + 1]++;
                 pos++;
             }
             pos += pad;
         }
         h_rp[mp] = pos;
         const int n_padded = pos;
+
+        // column-to-row index of the internal layout (row marking)
+        std::vector<int> h_cptr(n_ids + 1, 0);
+        for (int c = 0; c < n_ids; c++) h_cptr[c + 1] = h_cptr[c] + col_cnt[c + 1];
+        std::vector<int> h_crows(std::max<long long>(1, nnz_total));
+        {
+            std::vector<int> fill(h_cptr.begin(), h_cptr.end() - 1);
+            for (int r = 0; r < mp; r++) {
+                const int st = h_rp[r] & ~(kRowAlign - 1);
+                for (int e = 0; e < h_len[r]; e++) h_crows[fill[h_col[st + e] & kIdMask]++] = r;
+            }
+        }
 
         // ---- sweep items ----
         // Order: segments of split rows, whole very long rows, then the
@@ -203,18 +382,20 @@ Problem *problem_create(int m, int n, int nnz, const int *col, const int *row_pt
             for (int q = 0; q < cnt; q++) {
                 const int len = h_len[r + q];
                 const int nseg = (len + kSegLen - 1) / kSegLen;
+                const int unit_bits = h_unit[r + q] << kItemUnitShift;
                 if (T.max_split_segs < 0 || nseg <= T.max_split_segs) {
                     const int lr = static_cast<int>(h_seg_rows.size());
-                    h_seg_rows.push_back(make_int4(r + q, n_parts, nseg, 0));
+                    h_seg_rows.push_back(make_int4(r + q, n_parts, nseg, h_unit[r + q]));
                     for (int sg = 0; sg < nseg; sg += kSegsPerItem)
-                        h_items.push_back(make_int4(kItemSeg, lr, sg, std::min(kSegsPerItem, nseg - sg)));
+                        h_items.push_back(
+                            make_int4(kItemSeg | unit_bits, lr, sg, std::min(kSegsPerItem, nseg - sg)));
                     for (int sg = 0; sg < nseg; sg++) h_part_row.push_back(lr);
                     n_parts += nseg;
                 } else {
                     giant.push_back(r + q);
                 }
             }
-            for (int row : giant) h_items.push_back(make_int4(kItemLong, row, 1, 0));
+            for (int row : giant) h_items.push_back(make_int4(kItemLong | (h_unit[row] << kItemUnitShift), row, 1, 0));
             P->stats.n_long_rows = cnt;
             r += cnt;
         }
@@ -229,17 +410,27 @@ Problem *problem_create(int m, int n, int nnz, const int *col, const int *row_pt
                 const int per_pass = 64 >> logg;
                 int rows_per_item = std::max(1, T.item_nnz / width);
                 rows_per_item = ((rows_per_item + per_pass - 1) / per_pass) * per_pass;
-                for (int q = 0; q < cnt; q += rows_per_item) {
-                    const int nr = std::min(rows_per_item, cnt - q);
-                    h_items.push_back(make_int4(kItemGroup | (logg << 4) | (logk << 8), r + q, nr, 0));
+                int q = 0;
+                while (q < cnt) {
+                    const int u = h_unit[r + q];
+                    int nr = std::min(rows_per_item, cnt - q);
+                    for (int t = 1; t < nr; t++)
+                        if (h_unit[r + q + t] != u) {
+                            nr = t;
+                            break;
+                        }
+                    h_items.push_back(make_int4(kItemGroup | (logg << 4) | (logk << 8) | (u << kItemUnitShift),
+                                                r + q, nr, 0));
+                    q += nr;
                 }
             }
             r += cnt;
         }
+        if (NI >= (1 << (31 - kItemUnitShift))) throw std::runtime_error("too many instances in a batch");
 
-        P->stats.m = m;
-        P->stats.n = n;
-        P->stats.nnz = nnz;
+        P->stats.m = P->m;
+        P->stats.n = P->n;
+        P->stats.nnz = P->nnz;
         P->stats.m_nonempty = mp;
         P->stats.n_items = static_cast<int>(h_items.size());
 
@@ -253,41 +444,42 @@ Problem *problem_create(int m, int n, int nnz, const int *col, const int *row_pt
         check(hipEventCreate(&P->ev1), "hipEventCreate");
         hipStream_t s = P->stream;
 
-        const int nwords = (n + 31) / 32;
         DevMatrix &M = P->M;
         M.m = mp;
-        M.n = n;
-        M.nnz = nnz;
-        M.n16 = ncls[0];
-        M.n_int = ncls[0] + ncls[1];
+        M.n = n_ids;
+        M.n16 = class_start[1];
+        M.n_int = class_start[2];
         M.nwords = nwords;
         M.n_items = static_cast<int>(h_items.size());
+        M.replicas = replicas ? 1 : 0;
+        M.mark_rows = T.row_marking ? 1 : 0;
         M.n_seg_rows = static_cast<int>(h_seg_rows.size());
         M.n_seg_parts = n_parts;
-        int *d_rp = dev_alloc<int>(mp + 1, P->owned);
-        double2 *d_sides = dev_alloc<double2>(mp, P->owned);
-        int *d_col = dev_alloc<int>(n_padded, P->owned);
-        double *d_val = dev_alloc<double>(n_padded, P->owned);
-        int4 *d_items = dev_alloc<int4>(h_items.size(), P->owned);
-        int4 *d_seg_rows = dev_alloc<int4>(h_seg_rows.size(), P->owned);
-        int *d_part_row = dev_alloc<int>(h_part_row.size(), P->owned);
-        int *d_perm = dev_alloc<int>(n, P->owned);
-        check(hipMemcpyAsync(d_rp, h_rp.data(), (mp + 1) * sizeof(int), hipMemcpyHostToDevice, s), "H2D");
-        check(hipMemcpyAsync(d_sides, h_sides.data(), mp * sizeof(double2), hipMemcpyHostToDevice, s), "H2D");
-        check(hipMemcpyAsync(d_col, h_col.data(), n_padded * sizeof(int), hipMemcpyHostToDevice, s), "H2D");
-        check(hipMemcpyAsync(d_val, h_val.data(), n_padded * sizeof(double), hipMemcpyHostToDevice, s), "H2D");
-        if (!h_items.empty())
-            check(hipMemcpyAsync(d_items, h_items.data(), h_items.size() * sizeof(int4), hipMemcpyHostToDevice, s),
-                  "H2D");
-        if (!h_seg_rows.empty())
-            check(hipMemcpyAsync(d_seg_rows, h_seg_rows.data(), h_seg_rows.size() * sizeof(int4),
-                                 hipMemcpyHostToDevice, s),
-                  "H2D");
-        if (!h_part_row.empty())
-            check(hipMemcpyAsync(d_part_row, h_part_row.data(), h_part_row.size() * sizeof(int),
-                                 hipMemcpyHostToDevice, s),
-                  "H2D");
-        check(hipMemcpyAsync(d_perm, perm.data(), n * sizeof(int), hipMemcpyHostToDevice, s), "H2D");
+        auto &o = P->owned;
+        int *d_rp = dev_alloc<int>(mp + 1, o);
+        double2 *d_sides = dev_alloc<double2>(mp, o);
+        int *d_col = dev_alloc<int>(n_padded, o);
+        double *d_val = dev_alloc<double>(n_padded, o);
+        int4 *d_items = dev_alloc<int4>(h_items.size(), o);
+        int4 *d_seg_rows = dev_alloc<int4>(h_seg_rows.size(), o);
+        int *d_part_row = dev_alloc<int>(h_part_row.size(), o);
+        int *d_cptr = dev_alloc<int>(n_ids + 1, o);
+        int *d_crows = dev_alloc<int>(h_crows.size(), o);
+        int *d_perm = dev_alloc<int>(n_ids, o);
+        int *d_word_unit = dev_alloc<int>(word_unit.size(), o);
+        P->d_inv_perm = dev_alloc<int>(P->n, o);
+        upload(d_rp, h_rp.data(), mp + 1, s);
+        upload(d_sides, h_sides.data(), mp, s);
+        upload(d_col, h_col.data(), n_padded, s);
+        upload(d_val, h_val.data(), n_padded, s);
+        upload(d_items, h_items.data(), h_items.size(), s);
+        upload(d_seg_rows, h_seg_rows.data(), h_seg_rows.size(), s);
+        upload(d_part_row, h_part_row.data(), h_part_row.size(), s);
+        upload(d_cptr, h_cptr.data(), n_ids + 1, s);
+        upload(d_crows, h_crows.data(), h_crows.size(), s);
+        upload(d_perm, perm.data(), n_ids, s);
+        upload(d_word_unit, word_unit.data(), word_unit.size(), s);
+        upload(P->d_inv_perm, internal.data(), P->n, s);
         M.row_ptr = d_rp;
         M.sides = d_sides;
         M.col = d_col;
@@ -295,22 +487,17 @@ Problem *problem_create(int m, int n, int nnz, const int *col, const int *row_pt
         M.items = d_items;
         M.seg_rows = d_seg_rows;
         M.part_row = d_part_row;
+        M.col_ptr = d_cptr;
+        M.col_rows = d_crows;
         M.perm = d_perm;
+        M.word_unit = d_word_unit;
 
         DevState &D = P->D;
-        D.S16 = dev_alloc<unsigned>(ncls[0], P->owned);
-        D.S32 = dev_alloc<int2>(ncls[1], P->owned);
-        D.SF = dev_alloc<double2>(ncls[2], P->owned);
-        D.AI = dev_alloc<int2>(M.n_int, P->owned);
-        D.AX = dev_alloc<double2>(M.n_int, P->owned);
-        D.AF = dev_alloc<double2>(ncls[2], P->owned);
-        D.bm = dev_alloc<unsigned>(nwords, P->owned);
-        D.status = dev_alloc<int>(kStatusRows * kStatusWords, P->owned);
-        D.barrier = dev_alloc<unsigned>(kBarrierWords, P->owned);
-        D.seg_parts = dev_alloc<SegPart>(n_parts, P->owned);
-        D.seg_state = dev_alloc<SegState>(h_seg_rows.size(), P->owned);
-        D.seg_count = dev_alloc<unsigned>(h_seg_rows.size(), P->owned);
-        D.apply_tasks = dev_alloc<int>(n_parts, P->owned);
+        D.status = dev_alloc<int>(kStatusRows * kStatusWords, o);
+        D.global = dev_alloc<int>(kGlobalWords, o);
+        D.barrier = dev_alloc<unsigned>(kBarrierWords, o);
+        check(hipStreamSynchronize(s), "upload");
+        alloc_units(P, 1);
 
         int per_cu = 0;
         check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(mega_kernel), 256,
@@ -318,7 +505,6 @@ Problem *problem_create(int m, int n, int nnz, const int *col, const int *row_pt
               "occupancy");
         if (T.mega_blocks_per_cu > 0) per_cu = std::min(per_cu, T.mega_blocks_per_cu);
         P->mega_grid = std::max(1, per_cu) * P->cu_count;
-        check(hipStreamSynchronize(s), "upload");
     } catch (...) {
         delete P;
         throw;
@@ -326,100 +512,108 @@ Problem *problem_create(int m, int n, int nnz, const int *col, const int *row_pt
     return P;
 }
 
-void problem_destroy(Problem *p) { delete p; }
+// Hint buffers for `count` entries (device).
+void reserve_hints(Problem *P, size_t count) {
+    if (count <= P->hints_cap) return;
+    check(hipStreamSynchronize(P->stream), "sync");
+    for (void *p : P->hint_owned) (void)hipFree(p);
+    P->hint_owned.clear();
+    P->hints_cap = std::max<size_t>(count, 1024);
+    P->d_hints = dev_alloc<int>(P->hints_cap, P->hint_owned);
+}
 
-ProblemStats problem_stats(const Problem *p) { return p->stats; }
-
-namespace {
-
-int grid_for(int items) { return std::max(1, (items + 3) / 4); }
-
-// Walks the per-round status rows exactly as the persistent kernel does.
-void outcome_from_status(const std::vector<int> &st, int max_rounds, RunResult &res) {
-    res.rounds = 0;
-    res.infeasible = 0;
-    res.capped = 0;
-    for (int round = 1; round <= max_rounds; round++) {
-        const int *row = st.data() + kStatusWords * round;
-        res.rounds = round;
-        if (row[kStatusInfeasible]) { res.infeasible = 1; return; }
-        if (!row[kStatusChanged]) return;
-        if (round == max_rounds) res.capped = 1;
+// Loads the listed units' bound vectors for a fresh run (status ring
+// cleared, round 1 in step `start`) or for units joining a running server
+// (`fresh` false).
+void load_units(Problem *P, int n_list, const int *list, double *const *lbp, double *const *ubp,
+                const int *hint_ptr, const int *hints, const int *max_rounds, int start, bool fresh) {
+    hipStream_t s = P->stream;
+    DevMatrix &M = P->M;
+    DevState &D = P->D;
+    std::vector<int4> params(n_list);
+    for (int k = 0; k < n_list; k++) {
+        const int u = list[k];
+        const bool hinted = hint_ptr != nullptr && hint_ptr[k + 1] > hint_ptr[k] && M.mark_rows;
+        const int limit = P->mark_limit[P->replicas ? 0 : u];
+        params[k] = make_int4(u, limit, hinted ? 0 : 1, max_rounds[k]);
+    }
+    upload(P->d_list, list, n_list, s);
+    upload(P->d_params, params.data(), n_list, s);
+    const int n_tables = P->replicas ? n_list : 1;
+    upload(P->d_lbp, lbp, n_tables, s);
+    upload(P->d_ubp, ubp, n_tables, s);
+    hipLaunchKernelGGL(unit_reset_kernel, dim3(std::max(1, std::min(64, (n_list + 255) / 256))), dim3(256), 0, s, D,
+                       n_list, P->d_params, start, fresh ? 1 : 0);
+    const int gx = std::max(1, std::min(1024, (M.n + 255) / 256));
+    const int gy = P->replicas ? std::max(1, std::min(n_list, 1024)) : 1;
+    hipLaunchKernelGGL(load_kernel, dim3(gx, gy), dim3(256), 0, s, M, D, n_list, P->d_list, P->d_lbp, P->d_ubp);
+    check(hipGetLastError(), "load launch");
+    if (hint_ptr != nullptr && M.mark_rows && hint_ptr[n_list] > 0) {
+        reserve_hints(P, hint_ptr[n_list]);
+        upload(P->d_hint_ptr, hint_ptr, n_list + 1, s);
+        upload(P->d_hints, hints, hint_ptr[n_list], s);
+        hipLaunchKernelGGL(hint_kernel, dim3(16, std::max(1, std::min(n_list, 1024))), dim3(256), 0, s, M, D, n_list,
+                           P->d_list, P->d_hint_ptr, P->d_hints, P->d_inv_perm, start);
+        check(hipGetLastError(), "hint launch");
     }
 }
 
-}  // namespace
-
-RunResult problem_propagate_device(Problem *P, double *d_lbs, double *d_ubs, int sync_type, int max_rounds) {
-    if (sync_type < CPU_LOOP_ID || sync_type > MEGAKERNEL_ID) throw std::runtime_error("Unknown sync type");
-    if (max_rounds < 1 || max_rounds > kMaxRounds) throw std::runtime_error("max_rounds out of range");
-    std::lock_guard<std::mutex> lock(P->run_mutex);
+// One step of the listed units (already uploaded as the active list).
+void launch_step(Problem *P, int step, int guard, int first_step) {
     const DevMatrix &M = P->M;
     const DevState &D = P->D;
     hipStream_t s = P->stream;
-    RunResult res;
+    const long long n_work = P->replicas ? static_cast<long long>(M.n_items) * D.n_active : M.n_items;
+    const int apply_grid = std::max(1, std::min(1024, grid_for(static_cast<long long>(M.n_seg_parts) *
+                                                                 (P->replicas ? D.n_active : 1))));
+    hipLaunchKernelGGL(sweep_kernel, dim3(grid_for(n_work)), dim3(256), 0, s, M, D, step, guard, first_step);
+    if (M.n_seg_parts > 0)
+        hipLaunchKernelGGL(apply_split_kernel, dim3(apply_grid), dim3(256), 0, s, M, D, step, guard, first_step);
+    const long long pub_waves = static_cast<long long>(P->replicas ? D.n_active : 1) * ((M.nwords + 7) / 8);
+    hipLaunchKernelGGL(publish_kernel, dim3(std::max(1, std::min(8192, grid_for(pub_waves)))), dim3(256), 0, s, M, D,
+                       step, guard, first_step, P->n_units);
+}
 
-    const int fill_grid = std::min(4096, std::max(1, (P->n + 255) / 256));
-    hipLaunchKernelGGL(clear_envelope_kernel, dim3(1), dim3(1), 0, s, D);
-    hipLaunchKernelGGL(init_kernel, dim3(fill_grid), dim3(256), 0, s, M, D, d_lbs, d_ubs);
-    check(hipGetLastError(), "init_kernel launch");
-
-    const int sweep_grid = grid_for(M.n_items);
-    // publish: one thread per column; apply: a small grid that takes the
-    // queued segments in turn (an empty queue costs one short launch)
-    const int pub_grid = std::max(1, std::min(4096, (M.n + 255) / 256));
-    const int apply_grid = std::max(1, std::min(1024, grid_for(M.n_seg_parts)));
-    std::vector<int> st(kStatusRows * kStatusWords, 0);
-    int head[kStatusWords];
-    check(hipMemcpyAsync(head, D.status, sizeof(head), hipMemcpyDeviceToHost, s), "flags D2H");
-    check(hipStreamSynchronize(s), "init sync");
-    if (head[kGlobalEnvelope]) throw std::runtime_error("bounds outside the problem's bound envelope");
-    if (head[kGlobalCrossed]) {
-        // infeasible before any round: bounds are returned unchanged
-        res.infeasible = 1;
-        return res;
-    }
-
+// Runs steps 1.. for the listed units (loaded), at most max_rounds steps.
+// Returns the device time of the step loop.
+double run_steps(Problem *P, int sync_type, int max_rounds) {
+    const DevMatrix &M = P->M;
+    DevState &D = P->D;
+    hipStream_t s = P->stream;
     check(hipEventRecord(P->ev0, s), "event");
     if (sync_type == CPU_LOOP_ID) {
-        // two launches per round; the host reads the round's flags back
-        for (int round = 1; round <= max_rounds; round++) {
-            hipLaunchKernelGGL(sweep_kernel, dim3(sweep_grid), dim3(256), 0, s, M, D, round, 0);
-            if (M.n_seg_parts > 0)
-                hipLaunchKernelGGL(apply_split_kernel, dim3(apply_grid), dim3(256), 0, s, M, D, round, 0);
-            hipLaunchKernelGGL(publish_kernel, dim3(pub_grid), dim3(256), 0, s, M, D, round, 0);
-            check(hipGetLastError(), "round launch");
+        // one set of launches per step; the host reads the step's flags back
+        for (int step = 1; step <= max_rounds; step++) {
+            launch_step(P, step, 0, 1);
+            check(hipGetLastError(), "step launch");
             int flags[kStatusWords];
-            check(hipMemcpyAsync(flags, D.status + kStatusWords * round, sizeof(flags), hipMemcpyDeviceToHost, s),
+            int err = 0;
+            check(hipMemcpyAsync(flags, D.status + kStatusWords * (step % kStatusRows), sizeof(flags),
+                                 hipMemcpyDeviceToHost, s),
                   "flags D2H");
-            check(hipMemcpyAsync(head, D.status, sizeof(head), hipMemcpyDeviceToHost, s), "flags D2H");
-            check(hipStreamSynchronize(s), "round sync");
-            if (head[kGlobalError]) throw std::runtime_error("device wait timed out");
-            std::memcpy(st.data() + kStatusWords * round, flags, sizeof(flags));
-            if (flags[kStatusInfeasible] || !flags[kStatusChanged]) break;
+            check(hipMemcpyAsync(&err, D.global + kGlobalError, sizeof(int), hipMemcpyDeviceToHost, s), "D2H");
+            check(hipStreamSynchronize(s), "step sync");
+            if (err) throw std::runtime_error("device wait timed out");
+            if (!flags[kStatusChanged]) break;
         }
     } else if (sync_type == GPU_LOOP_ID) {
-        // captured chunks of rounds whose launches stop themselves on the device
+        // captured chunks of steps whose launches stop themselves on the device
         const int chunk = std::max(1, P->tuning.graph_rounds);
+        const long long key = static_cast<long long>(D.n_active) * 1000 + max_rounds;
+        if (key != P->graph_key) {
+            P->drop_graphs();
+            P->graph_key = key;
+        }
         for (int first = 1, ci = 0; first <= max_rounds; first += chunk, ci++) {
             const int last = std::min(max_rounds, first + chunk - 1);
             if (static_cast<int>(P->graphs.size()) <= ci) {
                 P->graphs.resize(ci + 1, nullptr);
                 P->graph_last.resize(ci + 1, 0);
             }
-            if (P->graphs[ci] && P->graph_last[ci] != last) {
-                (void)hipGraphExecDestroy(P->graphs[ci]);
-                P->graphs[ci] = nullptr;
-            }
             if (!P->graphs[ci]) {
                 hipGraph_t graph;
                 check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "capture");
-                for (int round = first; round <= last; round++) {
-                    hipLaunchKernelGGL(sweep_kernel, dim3(sweep_grid), dim3(256), 0, s, M, D, round, 1);
-                    if (M.n_seg_parts > 0)
-                        hipLaunchKernelGGL(apply_split_kernel, dim3(apply_grid), dim3(256), 0, s, M, D, round, 1);
-                    hipLaunchKernelGGL(publish_kernel, dim3(pub_grid), dim3(256), 0, s, M, D, round, 1);
-                }
+                for (int step = first; step <= last; step++) launch_step(P, step, 1, 1);
                 check(hipStreamEndCapture(s, &graph), "end capture");
                 hipGraphExec_t exec;
                 check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "instantiate");
@@ -429,43 +623,119 @@ RunResult problem_propagate_device(Problem *P, double *d_lbs, double *d_ubs, int
             }
             check(hipGraphLaunch(P->graphs[ci], s), "graph launch");
             int flags[kStatusWords];
-            check(hipMemcpyAsync(flags, D.status + kStatusWords * last, sizeof(flags), hipMemcpyDeviceToHost, s),
+            int err = 0;
+            check(hipMemcpyAsync(flags, D.status + kStatusWords * (last % kStatusRows), sizeof(flags),
+                                 hipMemcpyDeviceToHost, s),
                   "flags D2H");
+            check(hipMemcpyAsync(&err, D.global + kGlobalError, sizeof(int), hipMemcpyDeviceToHost, s), "D2H");
             check(hipStreamSynchronize(s), "graph sync");
-            // the next chunk is needed only if the chunk's last round still changed
-            if (flags[kStatusInfeasible] || !flags[kStatusChanged]) break;
+            if (err) throw std::runtime_error("device wait timed out");
+            // the next chunk is needed only if the chunk's last step still changed
+            if (!flags[kStatusChanged]) break;
         }
-        check(hipMemcpyAsync(st.data(), D.status, st.size() * sizeof(int), hipMemcpyDeviceToHost, s), "status D2H");
-        check(hipStreamSynchronize(s), "status sync");
-        if (st[kGlobalError]) throw std::runtime_error("device wait timed out");
     } else {
-        void *args[] = {const_cast<DevMatrix *>(&M), const_cast<DevState *>(&D), &max_rounds};
+        int first = 1, steps = max_rounds, n_units = P->n_units;
+        void *args[] = {const_cast<DevMatrix *>(&M), &D, &first, &steps, &n_units};
         check(hipLaunchCooperativeKernel(reinterpret_cast<const void *>(mega_kernel), dim3(P->mega_grid), dim3(256),
                                          args, 0, s),
               "mega_kernel launch");
+        int g[kGlobalWords];
+        check(hipMemcpyAsync(g, D.global, sizeof(g), hipMemcpyDeviceToHost, s), "D2H");
+        check(hipStreamSynchronize(s), "mega sync");
+        if (g[kGlobalError] || g[kGlobalSteps] < 0) throw std::runtime_error("device wait timed out");
     }
     check(hipEventRecord(P->ev1, s), "event");
-
-    if (sync_type == MEGAKERNEL_ID) {
-        check(hipMemcpyAsync(st.data(), D.status, kStatusWords * sizeof(int), hipMemcpyDeviceToHost, s),
-              "outcome D2H");
-        check(hipStreamSynchronize(s), "mega sync");
-        if (st[kGlobalError] || st[kOutRounds] < 0) throw std::runtime_error("device wait timed out");
-        res.rounds = st[kOutRounds];
-        res.infeasible = st[kOutInfeasible];
-        res.capped = st[kOutCapped];
-    } else {
-        outcome_from_status(st, max_rounds, res);
-    }
-    float ms = 0.f;
     check(hipEventSynchronize(P->ev1), "event sync");
+    float ms = 0.f;
     check(hipEventElapsedTime(&ms, P->ev0, P->ev1), "elapsed");
-    res.kernel_ms = ms;
+    return ms;
+}
 
-    hipLaunchKernelGGL(unpack_kernel, dim3(fill_grid), dim3(256), 0, s, M, D, d_lbs, d_ubs);
-    check(hipGetLastError(), "unpack_kernel launch");
-    check(hipStreamSynchronize(s), "unpack sync");
-    return res;
+std::vector<UnitState> read_units(Problem *P, int n_list, const int *list) {
+    std::vector<UnitState> all(P->n_units);
+    check(hipMemcpyAsync(all.data(), P->D.units, sizeof(UnitState) * P->n_units, hipMemcpyDeviceToHost, P->stream),
+          "units D2H");
+    check(hipStreamSynchronize(P->stream), "units sync");
+    std::vector<UnitState> out(n_list);
+    for (int k = 0; k < n_list; k++) out[k] = all[list[k]];
+    return out;
+}
+
+// A complete run of the listed units from round 1.
+void run_units(Problem *P, int n_list, const int *list, double *const *lbp, double *const *ubp, const int *hint_ptr,
+               const int *hints, int sync_type, int max_rounds, RunResult *out) {
+    if (sync_type < CPU_LOOP_ID || sync_type > MEGAKERNEL_ID) throw std::runtime_error("Unknown sync type");
+    if (max_rounds < 1 || max_rounds > kMaxRounds) throw std::runtime_error("max_rounds out of range");
+    std::vector<int> caps(n_list, max_rounds);
+    P->D.n_active = P->replicas ? n_list : 0;
+    load_units(P, n_list, list, lbp, ubp, hint_ptr, hints, caps.data(), 1, true);
+    // bounds outside the envelope are an error before any round
+    std::vector<UnitState> st = read_units(P, n_list, list);
+    bool any_active = false;
+    for (const auto &u : st) {
+        if (u.envelope) throw std::runtime_error("bounds outside the problem's bound envelope");
+        any_active |= !u.crossed;
+    }
+    double ms = 0.0;
+    if (any_active) {
+        ms = run_steps(P, sync_type, max_rounds);
+        st = read_units(P, n_list, list);
+    }
+    for (int k = 0; k < n_list; k++) {
+        out[k] = outcome(st[k]);
+        out[k].kernel_ms = ms;
+    }
+    const int gx = std::max(1, std::min(1024, (P->M.n + 255) / 256));
+    const int gy = P->replicas ? std::max(1, std::min(n_list, 1024)) : 1;
+    hipLaunchKernelGGL(unpack_kernel, dim3(gx, gy), dim3(256), 0, P->stream, P->M, P->D, n_list, P->d_list,
+                       P->d_lbp, P->d_ubp);
+    check(hipGetLastError(), "unpack launch");
+    check(hipStreamSynchronize(P->stream), "unpack sync");
+}
+
+}  // namespace
+
+Problem *problem_create(int m, int n, int nnz, const int *col, const int *row_ptr, const double *val,
+                        const double *lhs, const double *rhs, const int *vartype, const double *env_lb,
+                        const double *env_ub, const Tuning *tuning) {
+    if (n > kIdMask) throw std::runtime_error("too many columns");
+    InstanceView v;
+    v.m = m;
+    v.n = n;
+    v.nnz = nnz;
+    v.col = col;
+    v.row_ptr = row_ptr;
+    v.val = val;
+    v.lhs = lhs;
+    v.rhs = rhs;
+    v.vartype = vartype;
+    v.env_lb = env_lb;
+    v.env_ub = env_ub;
+    return build({v}, true, tuning);
+}
+
+Problem *batch_create(const std::vector<InstanceView> &instances, const Tuning *tuning) {
+    return build(instances, false, tuning);
+}
+
+void problem_destroy(Problem *p) { delete p; }
+
+ProblemStats problem_stats(const Problem *p) { return p->stats; }
+
+void problem_reserve(Problem *P, int capacity) {
+    if (!P->replicas) throw std::runtime_error("a batch of instances has no node capacity");
+    if (capacity < 1) throw std::runtime_error("capacity must be positive");
+    std::lock_guard<std::mutex> lock(P->run_mutex);
+    if (capacity != P->capacity) alloc_units(P, capacity);
+}
+
+RunResult problem_propagate_device(Problem *P, double *d_lbs, double *d_ubs, int sync_type, int max_rounds) {
+    std::lock_guard<std::mutex> lock(P->run_mutex);
+    const int list[1] = {0};
+    RunResult r;
+    if (!P->replicas) throw std::runtime_error("use the batch entry points for a batch of instances");
+    run_units(P, 1, list, &d_lbs, &d_ubs, nullptr, nullptr, sync_type, max_rounds, &r);
+    return r;
 }
 
 RunResult problem_propagate(Problem *P, double *lbs, double *ubs, int sync_type, int max_rounds) {
@@ -483,8 +753,214 @@ RunResult problem_propagate(Problem *P, double *lbs, double *ubs, int sync_type,
         (void)hipFree(d_b);
         throw;
     }
-    (void)hipFree(d_b);
+    check(hipFree(d_b), "hipFree");
     return res;
+}
+
+void nodes_propagate_device(Problem *P, int count, double *const *d_lbs, double *const *d_ubs, const int *hint_ptr,
+                            const int *hints, int sync_type, int max_rounds, RunResult *out) {
+    if (!P->replicas) throw std::runtime_error("use the batch entry points for a batch of instances");
+    if (count < 1) throw std::runtime_error("no bound vector");
+    if (hint_ptr) {
+        if (hint_ptr[0] != 0) throw std::runtime_error("hint_ptr[0] must be 0");
+        for (int k = 0; k < count; k++)
+            if (hint_ptr[k + 1] < hint_ptr[k]) throw std::runtime_error("hint_ptr not monotone");
+        for (int h = 0; h < hint_ptr[count]; h++)
+            if (!hints || hints[h] < 0 || hints[h] >= P->n) throw std::runtime_error("hint column out of range");
+    }
+    if (count > P->capacity) problem_reserve(P, count);
+    std::lock_guard<std::mutex> lock(P->run_mutex);
+    std::vector<int> list(count);
+    for (int k = 0; k < count; k++) list[k] = k;
+    run_units(P, count, list.data(), d_lbs, d_ubs, hint_ptr, hints, sync_type, max_rounds, out);
+}
+
+void batch_propagate_device(Problem *P, double *d_lbs, double *d_ubs, int sync_type, int max_rounds,
+                            RunResult *out) {
+    if (P->replicas) throw std::runtime_error("not a batch of instances");
+    std::lock_guard<std::mutex> lock(P->run_mutex);
+    std::vector<int> list(P->n_inst);
+    for (int k = 0; k < P->n_inst; k++) list[k] = k;
+    run_units(P, P->n_inst, list.data(), &d_lbs, &d_ubs, nullptr, nullptr, sync_type, max_rounds, out);
+}
+
+// ---------------------------------------------------------------------------
+// serving
+// ---------------------------------------------------------------------------
+
+struct Server {
+    struct Request {
+        int64_t ticket;
+        std::vector<double> lb, ub;
+        std::vector<int> hints;
+        int max_rounds;
+    };
+    struct Done {
+        std::vector<double> lb, ub;
+        RunResult r;
+    };
+    Problem *P = nullptr;
+    int slots = 0;
+    std::deque<Request> queue;
+    std::vector<int64_t> slot_ticket;  // -1: free
+    std::unordered_map<int64_t, Done> done;
+    int64_t next_ticket = 1;
+    int step = 1;
+    double *d_buf = nullptr;           // slots x (lb, ub) bound vectors
+    std::vector<double *> slot_lb, slot_ub;
+    ~Server() {
+        if (d_buf) (void)hipFree(d_buf);
+    }
+};
+
+Server *server_create(Problem *P, int64_t hbm_budget, int max_slots) {
+    if (!P->replicas) throw std::runtime_error("serving needs a single matrix");
+    if (max_slots < 1) throw std::runtime_error("max_slots must be positive");
+    const int64_t per_slot = P->stats.unit_state_bytes + 16LL * P->n;
+    int slots = static_cast<int>(std::min<int64_t>(max_slots, hbm_budget / std::max<int64_t>(1, per_slot)));
+    if (slots < 1) throw std::runtime_error("HBM budget below one node's state");
+    auto *S = new Server();
+    try {
+        S->P = P;
+        S->slots = slots;
+        problem_reserve(P, slots);
+        check(hipMalloc(&S->d_buf, sizeof(double) * 2 * static_cast<size_t>(P->n) * slots), "hipMalloc slots");
+        for (int k = 0; k < slots; k++) {
+            S->slot_lb.push_back(S->d_buf + static_cast<size_t>(2 * k) * P->n);
+            S->slot_ub.push_back(S->d_buf + static_cast<size_t>(2 * k + 1) * P->n);
+        }
+        S->slot_ticket.assign(slots, -1);
+        // fresh status ring for the server's steps
+        std::lock_guard<std::mutex> lock(P->run_mutex);
+        hipLaunchKernelGGL(unit_reset_kernel, dim3(1), dim3(256), 0, P->stream, P->D, 0, P->d_params, 0, 1);
+        check(hipStreamSynchronize(P->stream), "server init");
+    } catch (...) {
+        delete S;
+        throw;
+    }
+    return S;
+}
+
+void server_destroy(Server *s) { delete s; }
+
+int server_slots(const Server *s) { return s->slots; }
+
+int64_t server_submit(Server *S, const double *lbs, const double *ubs, const int *hints, int n_hints,
+                      int max_rounds) {
+    if (max_rounds < 1 || max_rounds > kMaxRounds) throw std::runtime_error("max_rounds out of range");
+    Server::Request q;
+    q.ticket = S->next_ticket++;
+    q.lb.assign(lbs, lbs + S->P->n);
+    q.ub.assign(ubs, ubs + S->P->n);
+    if (hints && n_hints > 0) {
+        for (int k = 0; k < n_hints; k++)
+            if (hints[k] < 0 || hints[k] >= S->P->n) throw std::runtime_error("hint column out of range");
+        q.hints.assign(hints, hints + n_hints);
+    }
+    q.max_rounds = max_rounds;
+    S->queue.push_back(std::move(q));
+    return S->queue.back().ticket;
+}
+
+int server_pending(const Server *S) {
+    int busy = 0;
+    for (auto t : S->slot_ticket) busy += t >= 0;
+    return busy + static_cast<int>(S->queue.size());
+}
+
+int server_step(Server *S) {
+    Problem *P = S->P;
+    std::lock_guard<std::mutex> lock(P->run_mutex);
+    hipStream_t s = P->stream;
+    const int n = P->n;
+    // admit queued requests into free slots; they start in this step
+    std::vector<int> adm;
+    std::vector<double *> adm_lb, adm_ub;
+    std::vector<int> hint_ptr(1, 0), hints, caps;
+    std::vector<int64_t> adm_ticket;
+    for (int k = 0; k < S->slots && !S->queue.empty(); k++) {
+        if (S->slot_ticket[k] >= 0) continue;
+        Server::Request &q = S->queue.front();
+        check(hipMemcpyAsync(S->slot_lb[k], q.lb.data(), n * sizeof(double), hipMemcpyHostToDevice, s), "H2D");
+        check(hipMemcpyAsync(S->slot_ub[k], q.ub.data(), n * sizeof(double), hipMemcpyHostToDevice, s), "H2D");
+        adm.push_back(k);
+        adm_lb.push_back(S->slot_lb[k]);
+        adm_ub.push_back(S->slot_ub[k]);
+        hints.insert(hints.end(), q.hints.begin(), q.hints.end());
+        hint_ptr.push_back(static_cast<int>(hints.size()));
+        caps.push_back(q.max_rounds);
+        adm_ticket.push_back(q.ticket);
+        // the request's host vectors must live until the copies finish
+        check(hipStreamSynchronize(s), "admit sync");
+        S->slot_ticket[k] = q.ticket;
+        S->queue.pop_front();
+    }
+    if (!adm.empty())
+        load_units(P, static_cast<int>(adm.size()), adm.data(), adm_lb.data(), adm_ub.data(), hint_ptr.data(),
+                   hints.data(), caps.data(), S->step, false);
+    // active slots advance one round
+    std::vector<int> active;
+    for (int k = 0; k < S->slots; k++)
+        if (S->slot_ticket[k] >= 0) active.push_back(k);
+    if (active.empty()) return 0;
+    upload(P->d_list, active.data(), active.size(), s);
+    P->D.n_active = static_cast<int>(active.size());
+    launch_step(P, S->step, 0, S->step);
+    check(hipGetLastError(), "step launch");
+    std::vector<UnitState> st = read_units(P, static_cast<int>(active.size()), active.data());
+    int err = 0;
+    check(hipMemcpy(&err, P->D.global + kGlobalError, sizeof(int), hipMemcpyDeviceToHost), "D2H");
+    if (err) throw std::runtime_error("device wait timed out");
+    // retire finished slots
+    std::vector<int> fin;
+    std::vector<RunResult> fin_r;
+    for (size_t k = 0; k < active.size(); k++) {
+        if (active_after(st[k], S->step)) continue;
+        RunResult r;
+        if (st[k].envelope) r.rounds = -1;
+        else r = outcome(st[k]);
+        fin.push_back(active[k]);
+        fin_r.push_back(r);
+    }
+    if (!fin.empty()) {
+        std::vector<double *> lbp, ubp;
+        for (int k : fin) {
+            lbp.push_back(S->slot_lb[k]);
+            ubp.push_back(S->slot_ub[k]);
+        }
+        upload(P->d_list, fin.data(), fin.size(), s);
+        upload(P->d_lbp, lbp.data(), fin.size(), s);
+        upload(P->d_ubp, ubp.data(), fin.size(), s);
+        const int gx = std::max(1, std::min(1024, (P->M.n + 255) / 256));
+        hipLaunchKernelGGL(unpack_kernel, dim3(gx, std::max(1, std::min(static_cast<int>(fin.size()), 1024))),
+                           dim3(256), 0, s, P->M, P->D, static_cast<int>(fin.size()), P->d_list,
+                           P->d_lbp, P->d_ubp);
+        check(hipGetLastError(), "unpack launch");
+        for (size_t q = 0; q < fin.size(); q++) {
+            const int k = fin[q];
+            Server::Done d;
+            d.lb.resize(n);
+            d.ub.resize(n);
+            d.r = fin_r[q];
+            check(hipMemcpyAsync(d.lb.data(), S->slot_lb[k], n * sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
+            check(hipMemcpyAsync(d.ub.data(), S->slot_ub[k], n * sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
+            check(hipStreamSynchronize(s), "retire sync");
+            S->done[S->slot_ticket[k]] = std::move(d);
+            S->slot_ticket[k] = -1;
+        }
+    }
+    S->step++;
+    return static_cast<int>(fin.size());
+}
+
+bool server_result(Server *S, int64_t ticket, double *lbs, double *ubs, RunResult *r) {
+    auto it = S->done.find(ticket);
+    if (it == S->done.end()) return false;
+    std::memcpy(lbs, it->second.lb.data(), sizeof(double) * S->P->n);
+    std::memcpy(ubs, it->second.ub.data(), sizeof(double) * S->P->n);
+    *r = it->second.r;
+    S->done.erase(it);
+    return true;
 }
 
 }  // namespace gdp

@@ -247,8 +247,28 @@ void gdpTuningDefaults(gdp_tuning *t) {
     t->graph_rounds = d.graph_rounds;
     t->max_split_segs = d.max_split_segs;
     t->compact_bounds = d.compact_bounds;
+    t->row_marking = d.row_marking;
     for (int b = 0; b < 10; b++) t->group_log[b] = d.group_log[b];
 }
+
+namespace {
+
+gdp::Tuning tuning_from(const gdp_tuning *tuning) {
+    gdp::Tuning T;
+    if (tuning) {
+        if (tuning->item_nnz > 0) T.item_nnz = tuning->item_nnz;
+        if (tuning->mega_blocks_per_cu > 0) T.mega_blocks_per_cu = tuning->mega_blocks_per_cu;
+        if (tuning->graph_rounds > 0) T.graph_rounds = tuning->graph_rounds;
+        if (tuning->max_split_segs >= 0) T.max_split_segs = tuning->max_split_segs;
+        if (tuning->compact_bounds >= 0) T.compact_bounds = tuning->compact_bounds;
+        if (tuning->row_marking >= 0) T.row_marking = tuning->row_marking;
+        for (int b = 0; b < 10; b++)
+            if (tuning->group_log[b] >= 0) T.group_log[b] = tuning->group_log[b];
+    }
+    return T;
+}
+
+}  // namespace
 
 GDP_RETCODE gdpProblemCreate(const int n_cons, const int n_vars, const int nnz, const int *col_indices,
                              const int *row_ptrs, const double *vals, const double *lhss, const double *rhss,
@@ -257,16 +277,7 @@ GDP_RETCODE gdpProblemCreate(const int n_cons, const int n_vars, const int nnz, 
     if (!out) return GDP_ERROR;
     *out = nullptr;
     try {
-        gdp::Tuning T;
-        if (tuning) {
-            if (tuning->item_nnz > 0) T.item_nnz = tuning->item_nnz;
-            if (tuning->mega_blocks_per_cu > 0) T.mega_blocks_per_cu = tuning->mega_blocks_per_cu;
-            if (tuning->graph_rounds > 0) T.graph_rounds = tuning->graph_rounds;
-            if (tuning->max_split_segs >= 0) T.max_split_segs = tuning->max_split_segs;
-            if (tuning->compact_bounds >= 0) T.compact_bounds = tuning->compact_bounds;
-            for (int b = 0; b < 10; b++)
-                if (tuning->group_log[b] >= 0) T.group_log[b] = tuning->group_log[b];
-        }
+        const gdp::Tuning T = tuning_from(tuning);
         gdp::Problem *p = gdp::problem_create(n_cons, n_vars, nnz, col_indices, row_ptrs, vals, lhss, rhss,
                                               vartypes, env_lbs, env_ubs, &T);
         *out = new gdp_problem_s{p};
@@ -299,6 +310,170 @@ GDP_RETCODE gdpProblemGetStats(gdp_problem p, gdp_problem_stats *out) {
     out->n16 = s.n16;
     out->n32 = s.n32;
     out->nf = s.nf;
+    out->n_units = s.n_units;
+    out->unit_state_bytes = s.unit_state_bytes;
+    return GDP_OKAY;
+}
+
+GDP_RETCODE gdpProblemReserve(gdp_problem p, const int capacity) {
+    if (!p) return GDP_ERROR;
+    try {
+        gdp::problem_reserve(p->p, capacity);
+        return GDP_OKAY;
+    } catch (const std::exception &e) {
+        return fail("gdpProblemReserve", e);
+    }
+}
+
+GDP_RETCODE gdpPropagateNodesDevice(gdp_problem p, const int count, double *const *d_lbs, double *const *d_ubs,
+                                    const int *hint_ptr, const int *hints, const int sync_type, const int max_rounds,
+                                    gdp_run_info *infos) {
+    if (!p || !d_lbs || !d_ubs || count < 1) return GDP_ERROR;
+    try {
+        std::vector<RunResult> r(count);
+        gdp::nodes_propagate_device(p->p, count, d_lbs, d_ubs, hint_ptr, hints, sync_type, max_rounds, r.data());
+        if (infos)
+            for (int k = 0; k < count; k++) fill_info(infos + k, r[k].rounds, r[k].infeasible, r[k].capped, r[k].kernel_ms);
+        return GDP_OKAY;
+    } catch (const std::exception &e) {
+        return fail("gdpPropagateNodesDevice", e);
+    }
+}
+
+GDP_RETCODE gdpBatchCreate(const int n_inst, const int *n_cons, const int *n_vars, const int *nnz,
+                           const int *const *col_indices, const int *const *row_ptrs, const double *const *vals,
+                           const double *const *lhss, const double *const *rhss, const int *const *vartypes,
+                           const double *const *env_lbs, const double *const *env_ubs, const gdp_tuning *tuning,
+                           gdp_problem *out) {
+    if (!out || n_inst < 1 || !n_cons || !n_vars || !nnz || !col_indices || !row_ptrs || !vals || !lhss || !rhss ||
+        !vartypes)
+        return GDP_ERROR;
+    *out = nullptr;
+    try {
+        std::vector<gdp::InstanceView> v(n_inst);
+        for (int i = 0; i < n_inst; i++) {
+            v[i].m = n_cons[i];
+            v[i].n = n_vars[i];
+            v[i].nnz = nnz[i];
+            v[i].col = col_indices[i];
+            v[i].row_ptr = row_ptrs[i];
+            v[i].val = vals[i];
+            v[i].lhs = lhss[i];
+            v[i].rhs = rhss[i];
+            v[i].vartype = vartypes[i];
+            v[i].env_lb = env_lbs ? env_lbs[i] : nullptr;
+            v[i].env_ub = env_ubs ? env_ubs[i] : nullptr;
+        }
+        const gdp::Tuning T = tuning_from(tuning);
+        *out = new gdp_problem_s{gdp::batch_create(v, &T)};
+        return GDP_OKAY;
+    } catch (const std::exception &e) {
+        return fail("gdpBatchCreate", e);
+    }
+}
+
+GDP_RETCODE gdpBatchPropagateDevice(gdp_problem p, double *d_lbs, double *d_ubs, const int sync_type,
+                                    const int max_rounds, gdp_run_info *infos) {
+    if (!p || !d_lbs || !d_ubs) return GDP_ERROR;
+    try {
+        const int ni = gdp::problem_stats(p->p).n_units;
+        std::vector<RunResult> r(ni);
+        gdp::batch_propagate_device(p->p, d_lbs, d_ubs, sync_type, max_rounds, r.data());
+        if (infos)
+            for (int k = 0; k < ni; k++) fill_info(infos + k, r[k].rounds, r[k].infeasible, r[k].capped, r[k].kernel_ms);
+        return GDP_OKAY;
+    } catch (const std::exception &e) {
+        return fail("gdpBatchPropagateDevice", e);
+    }
+}
+
+GDP_RETCODE gdpBatchPropagate(gdp_problem p, double *lbs, double *ubs, const int sync_type, const int max_rounds,
+                              gdp_run_info *infos) {
+    if (!p || !lbs || !ubs) return GDP_ERROR;
+    double *d = nullptr;
+    const size_t n = static_cast<size_t>(gdp::problem_stats(p->p).n);
+    if (hipMalloc(&d, 2 * n * sizeof(double)) != hipSuccess) {
+        gdp::set_last_error("gdpBatchPropagate: hipMalloc failed");
+        return GDP_ERROR;
+    }
+    GDP_RETCODE rc = GDP_ERROR;
+    if (hipMemcpy(d, lbs, n * sizeof(double), hipMemcpyHostToDevice) == hipSuccess &&
+        hipMemcpy(d + n, ubs, n * sizeof(double), hipMemcpyHostToDevice) == hipSuccess) {
+        rc = gdpBatchPropagateDevice(p, d, d + n, sync_type, max_rounds, infos);
+        if (rc == GDP_OKAY && (hipMemcpy(lbs, d, n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess ||
+                               hipMemcpy(ubs, d + n, n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)) {
+            gdp::set_last_error("gdpBatchPropagate: copy back failed");
+            rc = GDP_ERROR;
+        }
+    } else {
+        gdp::set_last_error("gdpBatchPropagate: upload failed");
+    }
+    (void)hipFree(d);
+    return rc;
+}
+
+struct gdp_server_s {
+    gdp::Server *s;
+};
+
+GDP_RETCODE gdpServerCreate(gdp_problem p, const long long hbm_budget, const int max_slots, gdp_server *out) {
+    if (!p || !out) return GDP_ERROR;
+    *out = nullptr;
+    try {
+        *out = new gdp_server_s{gdp::server_create(p->p, hbm_budget, max_slots)};
+        return GDP_OKAY;
+    } catch (const std::exception &e) {
+        return fail("gdpServerCreate", e);
+    }
+}
+
+GDP_RETCODE gdpServerDestroy(gdp_server s) {
+    if (!s) return GDP_OKAY;
+    gdp::server_destroy(s->s);
+    delete s;
+    return GDP_OKAY;
+}
+
+GDP_RETCODE gdpServerSlots(gdp_server s, int *slots) {
+    if (!s || !slots) return GDP_ERROR;
+    *slots = gdp::server_slots(s->s);
+    return GDP_OKAY;
+}
+
+GDP_RETCODE gdpServerSubmit(gdp_server s, const double *lbs, const double *ubs, const int *hints, const int n_hints,
+                            const int max_rounds, long long *ticket) {
+    if (!s || !lbs || !ubs || !ticket) return GDP_ERROR;
+    try {
+        *ticket = gdp::server_submit(s->s, lbs, ubs, hints, n_hints, max_rounds);
+        return GDP_OKAY;
+    } catch (const std::exception &e) {
+        return fail("gdpServerSubmit", e);
+    }
+}
+
+GDP_RETCODE gdpServerStep(gdp_server s, int *completed) {
+    if (!s) return GDP_ERROR;
+    try {
+        const int c = gdp::server_step(s->s);
+        if (completed) *completed = c;
+        return GDP_OKAY;
+    } catch (const std::exception &e) {
+        return fail("gdpServerStep", e);
+    }
+}
+
+GDP_RETCODE gdpServerPending(gdp_server s, int *pending) {
+    if (!s || !pending) return GDP_ERROR;
+    *pending = gdp::server_pending(s->s);
+    return GDP_OKAY;
+}
+
+GDP_RETCODE gdpServerResult(gdp_server s, const long long ticket, double *lbs, double *ubs, gdp_run_info *info,
+                            int *ready) {
+    if (!s || !lbs || !ubs || !ready) return GDP_ERROR;
+    RunResult r;
+    *ready = gdp::server_result(s->s, ticket, lbs, ubs, &r) ? 1 : 0;
+    if (*ready) fill_info(info, r.rounds, r.infeasible, r.capped, r.kernel_ms);
     return GDP_OKAY;
 }
 

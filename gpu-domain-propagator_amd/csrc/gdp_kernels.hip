@@ -1,18 +1,23 @@
-// gdp_kernels.hip - CDNA4 (gfx950) kernels of the propagation rounds.
+// gdp_kernels.hip - CDNA4 (gfx950) kernels of the propagation steps.
 //
-// A round has three phases.
-//   sweep    every row reads the bounds of the round's snapshot, computes its
-//            activities and the candidate bounds of its columns, and folds
-//            accepted candidates into the accumulator with native atomics
-//            (int32 max/min for integer columns, global_atomic_{max,min}_f64
-//            for the others); a bitmap records the columns that moved.
+// A step advances every active unit (bound vector) by one round, in three
+// phases:
+//   sweep    every marked row of every active unit reads the unit's bound
+//            snapshot, computes its activities and the candidate bounds of
+//            its columns, and folds accepted candidates into the unit's
+//            accumulator with native atomics (int32 max/min for compact
+//            integer columns, global_atomic_{max,min}_f64 for the others); the
+//            first candidate of a 32-column word queues the word for publish.
 //   apply    split rows (longer than kSegLen) whose gate passed apply the
 //            candidates of their segments (long rows rarely pass the gate,
 //            so this phase is mostly empty).
-//   publish  the columns of the bitmap are copied from the accumulator into
-//            the snapshot and checked for crossed bounds (lb > ub + eps).
-// Rows therefore see exactly the bounds at the end of the previous round:
-// the round-synchronous semantics of the oracle.
+//   publish  the queued words' moved columns are copied from the accumulator
+//            into the snapshot, checked for crossed bounds (lb > ub + eps),
+//            and the rows of the moved columns are marked for the next step.
+// Rows therefore see exactly the bounds at the end of the unit's previous
+// round: the round-synchronous semantics of the oracle.  A row none of whose
+// columns moved in the previous round is skipped; it would reproduce
+// candidates that were already applied, so skipping it changes no result.
 //
 // Sweep work is split into 64-lane wave items:
 //   group item  rows of tree width w = next_pow2(len) <= kSegLen, processed by
@@ -20,8 +25,9 @@
 //               per lane; a row's activity is the pairwise tree over its
 //               entries in natural order: K entries in registers, then an
 //               xor butterfly over the g lanes.
-//   seg item    one kSegLen-entry segment of a longer row; segment sums are
-//               added left to right by the last segment wave to arrive.
+//   seg item    up to kSegsPerItem kSegLen-entry segments of a longer row;
+//               segment sums are added left to right by the wave that
+//               completes the row.
 //   long item   a row that is not split (a tuning option), reduced segment
 //               by segment by one wave.
 // The summation order is thus a function of the row length only, whatever
@@ -51,8 +57,11 @@ __device__ __forceinline__ void atomic_max_i(int *p, int v) {
 __device__ __forceinline__ void atomic_min_i(int *p, int v) {
     __hip_atomic_fetch_min(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ void atomic_or_u(unsigned *p, unsigned v) {
-    __hip_atomic_fetch_or(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+__device__ __forceinline__ unsigned atomic_or_u(unsigned *p, unsigned v) {
+    return __hip_atomic_fetch_or(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int atomic_add_i(int *p, int v) {
+    return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ void atomic_or_i(int *p, int v) {
     __hip_atomic_fetch_or(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -105,9 +114,15 @@ __device__ __forceinline__ void load_bounds(const DevMatrix &M, const DevState &
 
 __device__ __forceinline__ bool fits_int32(double v) { return v >= -2147483647.0 && v <= 2147483647.0; }
 
-// Folds an accepted lower / upper candidate into the accumulator.
-__device__ __forceinline__ void accumulate(const DevMatrix &M, const DevState &D, int id, int t, double nl,
-                                           double nu) {
+// Where accepted candidates go: the unit's accumulator and bitmap.
+struct Sink {
+    DevState V;
+};
+
+// Folds an accepted lower / upper candidate into the accumulator and marks
+// the column in the unit's bitmap of moved columns.
+__device__ __forceinline__ void accumulate(const DevMatrix &M, const Sink &K, int id, int t, double nl, double nu) {
+    const DevState &D = K.V;
     if (id < M.n_int) {
         if (t & 1) {
             if (fits_int32(nl)) atomic_max_i(&D.AI[id].x, static_cast<int>(nl));
@@ -122,7 +137,7 @@ __device__ __forceinline__ void accumulate(const DevMatrix &M, const DevState &D
         if (t & 1) atomic_max_d(&a->x, nl);
         if (t & 2) atomic_min_d(&a->y, nu);
     }
-    atomic_or_u(&D.bm[id >> 5], 1u << (id & 31));
+    __hip_atomic_fetch_or(&D.bm[id >> 5], 1u << (id & 31), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Current accumulated bounds of internal column `id` (device-coherent).
@@ -216,19 +231,19 @@ __device__ __forceinline__ bool decide_gate(const DevMatrix &M, const DevState &
 }
 
 // Applies the candidates of one entry; returns true when a bound moved.
-__device__ __forceinline__ bool apply_entry(const DevMatrix &M, const DevState &D, int jp, double a, double l,
-                                            double u, double lhs, double rhs, double minact, double maxact,
-                                            int mininf, int maxinf) {
+__device__ __forceinline__ bool apply_entry(const DevMatrix &M, const Sink &K, int jp, double a, double l, double u,
+                                            double lhs, double rhs, double minact, double maxact, int mininf,
+                                            int maxinf) {
     const bool is_cont = jp >= 0;  // bit 31 set marks an integer column
     double nl = l, nu = u;
     const int t = tighten_variable(a, lhs, rhs, minact, maxact, mininf, maxinf, is_cont, l, u, nl, nu);
     if (t == 0) return false;
-    accumulate(M, D, jp & kIdMask, t, nl, nu);
+    accumulate(M, K, jp & kIdMask, t, nl, nu);
     return true;
 }
 
 // ---------------------------------------------------------------------------
-// sweep items
+// lane entries
 // ---------------------------------------------------------------------------
 
 // First entry and length of row r (starts are multiples of kRowAlign; the
@@ -341,43 +356,106 @@ struct LaneEntries {
         return run;
     }
 
-    __device__ __forceinline__ bool apply(const DevMatrix &M, const DevState &D, int first, int len, double lhs,
-                                          double rhs, double minact, double maxact, int ni, int xi) {
-        bool moved = false;
+    // number of accepted candidates
+    __device__ __forceinline__ int apply(const DevMatrix &M, const Sink &S, int first, int len, double lhs,
+                                         double rhs, double minact, double maxact, int ni, int xi) {
+        int moved = 0;
 #pragma unroll
         for (int k = 0; k < K; k++)
-            if (first + k < len) moved |= apply_entry(M, D, jp[k], a[k], l[k], u[k], lhs, rhs, minact, maxact, ni, xi);
+            if (first + k < len) moved += apply_entry(M, S, jp[k], a[k], l[k], u[k], lhs, rhs, minact, maxact, ni, xi);
         return moved;
     }
 };
 
+// ---------------------------------------------------------------------------
+// units and steps
+// ---------------------------------------------------------------------------
+
+// Value known to be the same on every lane of the wave (kept in a scalar
+// register).
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+__device__ __forceinline__ int *status_row(const DevState &D, int step) {
+    return D.status + kStatusWords * (step % kStatusRows);
+}
+
+// Whether unit u works in `step`, and whether it processes every row.
+struct UnitStep {
+    bool active;
+    bool all_rows;
+};
+
+__device__ __forceinline__ UnitStep unit_step(const DevMatrix &M, const DevState &D, int u, int step) {
+    const long long *p = reinterpret_cast<const long long *>(&D.units[u]);
+    const long long w0 = coherent_load(p), w1 = coherent_load(p + 1), w2 = coherent_load(p + 2),
+                    w3 = coherent_load(p + 3);
+    const int start = uniform(static_cast<int>(w0)), infeasible = uniform(static_cast<int>(w0 >> 32));
+    const int markall = uniform(static_cast<int>(w1)), max_rounds = uniform(static_cast<int>(w1 >> 32));
+    const int crossed = uniform(static_cast<int>(w2)), envelope = uniform(static_cast<int>(w2 >> 32));
+    const int prev_changed = uniform(static_cast<int>((step - 1) & 1 ? (w3 >> 32) : w3));
+    UnitStep r;
+    r.active = start > 0 && step >= start && !crossed && !envelope && step - start < max_rounds &&
+               (step == start || (prev_changed == step - 1 && infeasible == 0));
+    r.all_rows = !M.mark_rows || markall == step;
+    return r;
+}
+
+// Counts the wave's accepted candidates (`moved` per lane) of unit u in
+// `step` in one of the unit's kMoveLanes counters of the step (one
+// non-returning atomic per wave).  Whether the unit moved at all is recorded
+// by the publish phase, from its bitmap, so that nothing writes the unit's
+// state words while the items of a step read them.
+__device__ __forceinline__ void note_change(const DevState &D, int u, int step, int moved) {
+    if (!__any(moved != 0)) return;
+#pragma unroll
+    for (int h = 1; h < 64; h <<= 1) moved += __shfl_xor(moved, h, 64);
+    if ((threadIdx.x & 63) == 0) {
+        const int lane_slot = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) % kMoveLanes;
+        atomic_add_i(&D.moved[(u * kMoveSlots + step % kMoveSlots) * kMoveLanes + lane_slot], moved);
+    }
+}
+
+__device__ __forceinline__ Sink make_sink(const DevState &V) {
+    Sink k;
+    k.V = V;
+    return k;
+}
+
+// ---------------------------------------------------------------------------
+// sweep items
+// ---------------------------------------------------------------------------
+
 // Rows [first, first + nrows) of one tree-width bin: groups of 2^logg lanes,
-// K consecutive entries per lane.
+// K consecutive entries per lane.  Rows not marked for this step are
+// skipped (a pass with no marked row loads nothing).
 template <int K>
-__device__ bool group_item(const DevMatrix &M, const DevState &D, int first, int nrows, int logg) {
+__device__ __forceinline__ int group_item(const DevMatrix &M, const Sink &S, int first, int nrows, int logg,
+                                          int step, bool all_rows) {
+    const DevState &V = S.V;
     const int lane = threadIdx.x & 63;
     const int g = 1 << logg;
     const int gl = lane & (g - 1);
     const int per_pass = 64 >> logg;
     const int end = first + nrows;
-    bool moved = false;
+    int moved = 0;
     for (int base = first; base < end; base += per_pass) {
         const int r = base + (lane >> logg);
-        const bool valid = r < end;
+        const bool take = r < end && (all_rows || V.marks[r] == step);
+        if (!__any(take)) continue;
         int start = 0, len = 0;
         double lhs = 0.0, rhs = 0.0;
-        if (valid) {
+        if (take) {
             row_extent(M, r, start, len);
             const double2 sd = M.sides[r];
             lhs = sd.x;
             rhs = sd.y;
         }
         LaneEntries<K> E;
-        E.load(M, D, start, gl * K, len);
+        E.load(M, V, start, gl * K, len);
         E.reduce(g);
         const double minact = 0.0 + E.tmin;
         const double maxact = 0.0 + E.tmax;
-        const bool active = valid && len > 0;
+        const bool active = take && len > 0;
         const GateBounds gb = gate_bounds(minact, maxact, E.mininf, E.maxinf, lhs, rhs, E.dmax);
         bool gate = gb.hi;
         const bool undecided = active && !gb.decided();
@@ -385,15 +463,16 @@ __device__ bool group_item(const DevMatrix &M, const DevState &D, int first, int
             const double dm = E.eps_scan(g, 0.0);
             if (undecided) gate = row_gate(minact, maxact, E.mininf, E.maxinf, lhs, rhs, dm);
         }
-        if (active && gate) moved |= E.apply(M, D, gl * K, len, lhs, rhs, minact, maxact, E.mininf, E.maxinf);
+        if (active && gate) moved += E.apply(M, S, gl * K, len, lhs, rhs, minact, maxact, E.mininf, E.maxinf);
     }
     return moved;
 }
 
-// One row with too many segments to split: one wave, segment by segment,
-// then a second pass for the candidates.
-__device__ bool long_item(const DevMatrix &M, const DevState &D, int r) {
+// One row that is not split: one wave, segment by segment, then a second
+// pass for the candidates.
+__device__ __forceinline__ int long_item(const DevMatrix &M, const Sink &S, int r) {
     constexpr int K = kSegLen / 64;
+    const DevState &V = S.V;
     const int lane = threadIdx.x & 63;
     int start, len;
     row_extent(M, r, start, len);
@@ -402,7 +481,7 @@ __device__ bool long_item(const DevMatrix &M, const DevState &D, int r) {
     int mininf = 0, maxinf = 0;
     for (int s0 = 0; s0 < len; s0 += kSegLen) {
         LaneEntries<K> E;
-        E.load(M, D, start + s0, lane * K, len - s0);
+        E.load(M, V, start + s0, lane * K, len - s0);
         E.reduce(64);
         acc_min = acc_min + E.tmin;
         acc_max = acc_max + E.tmax;
@@ -410,13 +489,13 @@ __device__ bool long_item(const DevMatrix &M, const DevState &D, int r) {
         maxinf += E.maxinf;
         dmax = fmax(dmax, E.dmax);
     }
-    bool moved = false;
-    if (decide_gate(M, D, start, len, acc_min, acc_max, mininf, maxinf, sd.x, sd.y, dmax)) {
+    int moved = 0;
+    if (decide_gate(M, V, start, len, acc_min, acc_max, mininf, maxinf, sd.x, sd.y, dmax)) {
         for (int e = lane; e < len; e += 64) {
             const int jp = M.col[start + e];
             double l, u;
-            load_bounds(M, D, jp & kIdMask, l, u);
-            moved |= apply_entry(M, D, jp, M.val[start + e], l, u, sd.x, sd.y, acc_min, acc_max, mininf, maxinf);
+            load_bounds(M, V, jp & kIdMask, l, u);
+            moved += apply_entry(M, S, jp, M.val[start + e], l, u, sd.x, sd.y, acc_min, acc_max, mininf, maxinf);
         }
     }
     return moved;
@@ -426,20 +505,20 @@ __device__ bool long_item(const DevMatrix &M, const DevState &D, int r) {
 // wave.  A segment whose exact maximum Ms satisfies !(Ms - run > eps) cannot
 // replace the running value (floating-point subtraction is monotone), so it
 // is skipped; the others are scanned entry by entry in order.
-__device__ double split_eps_scan(const DevMatrix &M, const DevState &D, const int4 info, int start, int len) {
+__device__ __forceinline__ double split_eps_scan(const DevMatrix &M, const DevState &V, const int4 info, int start, int len) {
     constexpr int K = kSegLen / 64;
     const int lane = threadIdx.x & 63;
     double run = 0.0;
     for (int b = 0; b < info.z; b += 64) {
         const int q = b + lane;
-        const double ms = q < info.z ? coherent_load_d(&D.seg_parts[info.y + q].dmax) : 0.0;
+        const double ms = q < info.z ? coherent_load_d(&V.seg_parts[info.y + q].dmax) : 0.0;
         const int cnt = min(64, info.z - b);
         for (int t = 0; t < cnt; t++) {
             const double mt = __shfl(ms, t, 64);
             if (!eps_gt(mt, run)) continue;
             const int s0 = (b + t) * kSegLen;
             LaneEntries<K> E;
-            E.load(M, D, start + s0, lane * K, len - s0);
+            E.load(M, V, start + s0, lane * K, len - s0);
             run = E.eps_scan(64, run);
         }
     }
@@ -447,8 +526,8 @@ __device__ double split_eps_scan(const DevMatrix &M, const DevState &D, const in
 }
 
 // Stores the activity partial of segment `part` (lane 0 of the wave).
-__device__ __forceinline__ void store_part(const DevState &D, int part, const LaneEntries<kSegLen / 64> &E) {
-    SegPart *p = &D.seg_parts[part];
+__device__ __forceinline__ void store_part(const DevState &V, int part, const LaneEntries<kSegLen / 64> &E) {
+    SegPart *p = &V.seg_parts[part];
     coherent_store_d(&p->smin, E.tmin);
     coherent_store_d(&p->smax, E.tmax);
     coherent_store_d(&p->dmax, E.dmax);
@@ -456,13 +535,14 @@ __device__ __forceinline__ void store_part(const DevState &D, int part, const La
     coherent_store(&p->maxinf, E.maxinf);
 }
 
-// Segments [s, s + cnt) of a split row (cnt <= kSegsPerItem).  The wave
-// reduces each kSegLen-entry segment like a kSegLen-wide group row, stores
-// the partials and counts them in; nobody waits.  The wave that completes the row's count adds the
-// partials in segment order and decides the row gate; when it passes, the
-// row's activity is stored and its segments are queued for the apply phase
-// that follows the sweep.
-__device__ void seg_item(const DevMatrix &M, const DevState &D, int lr, int s, int cnt, int round) {
+// Segments [s, s + cnt) of split row lr (cnt <= kSegsPerItem) for unit u.
+// The wave reduces each kSegLen-entry segment like a kSegLen-wide group row,
+// stores the partials and counts them in; nobody waits.  The wave that
+// completes the row's count adds the partials in segment order and decides
+// the row gate; when it passes, the row's activity is stored and its
+// segments are queued for the apply phase that follows the sweep.
+__device__ __forceinline__ void seg_item(const DevMatrix &M, const DevState &D, const DevState &V, int u, int lr, int s, int cnt,
+                         int *st) {
     constexpr int K = kSegLen / 64;
     const int lane = threadIdx.x & 63;
     const int4 info = M.seg_rows[lr];
@@ -472,14 +552,14 @@ __device__ void seg_item(const DevMatrix &M, const DevState &D, int lr, int s, i
     for (int q = 0; q < cnt; q++) {
         const int s0 = (s + q) * kSegLen;
         LaneEntries<K> E;
-        E.load(M, D, start + s0, lane * K, len - s0);
+        E.load(M, V, start + s0, lane * K, len - s0);
         E.reduce(64);
-        if (lane == 0) store_part(D, info.y + s + q, E);
+        if (lane == 0) store_part(V, info.y + s + q, E);
     }
     unsigned old = 0;
     if (lane == 0) {
         mem_wait();
-        old = __hip_atomic_fetch_add(&D.seg_count[lr], static_cast<unsigned>(cnt), __ATOMIC_RELAXED,
+        old = __hip_atomic_fetch_add(&V.seg_count[lr], static_cast<unsigned>(cnt), __ATOMIC_RELAXED,
                                      __HIP_MEMORY_SCOPE_AGENT);
     }
     old = __shfl(old, 0, 64);
@@ -493,7 +573,7 @@ __device__ void seg_item(const DevMatrix &M, const DevState &D, int lr, int s, i
         const int q = b + lane;
         double pmin = 0.0, pmax = 0.0;
         if (q < info.z) {
-            const SegPart *pq = &D.seg_parts[info.y + q];
+            const SegPart *pq = &V.seg_parts[info.y + q];
             pmin = coherent_load_d(&pq->smin);
             pmax = coherent_load_d(&pq->smax);
             dm = fmax(dm, coherent_load_d(&pq->dmax));
@@ -515,146 +595,267 @@ __device__ void seg_item(const DevMatrix &M, const DevState &D, int lr, int s, i
     const double2 sd = M.sides[r];
     const GateBounds gb = gate_bounds(am, ax, ni, xi, sd.x, sd.y, dm);
     bool gate = gb.hi;
-    if (!gb.decided()) gate = row_gate(am, ax, ni, xi, sd.x, sd.y, split_eps_scan(M, D, info, start, len));
+    if (!gb.decided()) gate = row_gate(am, ax, ni, xi, sd.x, sd.y, split_eps_scan(M, V, info, start, len));
     int pos = 0;
     if (lane == 0) {
-        coherent_store(&D.seg_count[lr], 0u);
+        coherent_store(&V.seg_count[lr], 0u);
         if (gate) {
-            SegState *stp = &D.seg_state[lr];
+            SegState *stp = &V.seg_state[lr];
             coherent_store_d(&stp->minact, am);
             coherent_store_d(&stp->maxact, ax);
             coherent_store(&stp->mininf, ni);
             coherent_store(&stp->maxinf, xi);
-            pos = __hip_atomic_fetch_add(&D.status[kStatusWords * round + kStatusApply], info.z, __ATOMIC_RELAXED,
-                                         __HIP_MEMORY_SCOPE_AGENT);
+            pos = atomic_add_i(&st[kStatusApply], info.z);
         }
     }
     if (!gate) return;
     pos = __shfl(pos, 0, 64);
-    for (int q = lane; q < info.z; q += 64) coherent_store(&D.apply_tasks[pos + q], info.y + q);
+    const int base = u * M.n_seg_parts + info.y;
+    for (int q = lane; q < info.z; q += 64) coherent_store(&D.apply_tasks[pos + q], base + q);
 }
 
-// Applies the candidates of segment `part` of a split row whose gate passed
+// Sweep item `it` of `step`: decodes the unit, skips inactive units and
+// unmarked rows, runs the item and records the unit's change.
+__device__ __forceinline__ void sweep_item(const DevMatrix &M, const DevState &D, int it, int step, int *st) {
+    int u, b;
+    it = uniform(it);
+    if (M.replicas) {
+        u = uniform(D.active[it / M.n_items]);
+        b = it % M.n_items;
+    } else {
+        b = it;
+        u = uniform(M.items[b].x >> kItemUnitShift);
+    }
+    const UnitStep us = unit_step(M, D, u, step);
+    if (!us.active) return;
+    const DevState V = unit_view(M, D, u);
+    const Sink S = make_sink(V);
+    int4 d = M.items[b];
+    d.x = uniform(d.x);
+    d.y = uniform(d.y);
+    d.z = uniform(d.z);
+    d.w = uniform(d.w);
+    const int kind = d.x & 15;
+    int moved = 0;
+    if (kind == kItemSeg) {
+        const int r = M.seg_rows[d.y].x;
+        if (us.all_rows || V.marks[r] == step) seg_item(M, D, V, u, d.y, d.z, d.w, st);
+        return;
+    }
+    if (kind == kItemLong) {
+        if (us.all_rows || V.marks[d.y] == step) moved = long_item(M, S, d.y);
+    } else {
+        const int logg = (d.x >> 4) & 15;
+        const int logk = (d.x >> 8) & 15;
+        switch (logk) {
+            case 0: moved = group_item<1>(M, S, d.y, d.z, logg, step, us.all_rows); break;
+            case 1: moved = group_item<2>(M, S, d.y, d.z, logg, step, us.all_rows); break;
+            default: moved = group_item<4>(M, S, d.y, d.z, logg, step, us.all_rows); break;
+        }
+    }
+    note_change(D, u, step, moved);
+}
+
+// Applies the candidates of queued task `task` = unit * n_seg_parts + segment
 // (one wave; the row's activity is read back from its SegState).
-__device__ bool apply_split_task(const DevMatrix &M, const DevState &D, int part) {
+__device__ __forceinline__ void apply_split_task(const DevMatrix &M, const DevState &D, int task, int step, int *st) {
     constexpr int K = kSegLen / 64;
     const int lane = threadIdx.x & 63;
+    task = uniform(task);
+    const int u = task / M.n_seg_parts;
+    const int part = task - u * M.n_seg_parts;
+    const DevState V = unit_view(M, D, u);
+    const Sink S = make_sink(V);
     const int lr = M.part_row[part];
     const int4 info = M.seg_rows[lr];
     const int r = info.x;
     int start, len;
     row_extent(M, r, start, len);
     const int s0 = (part - info.y) * kSegLen;
-    const SegState *stp = &D.seg_state[lr];
+    const SegState *stp = &V.seg_state[lr];
     const double minact = coherent_load_d(&stp->minact);
     const double maxact = coherent_load_d(&stp->maxact);
     const int ni = coherent_load(&stp->mininf);
     const int xi = coherent_load(&stp->maxinf);
     const double2 sd = M.sides[r];
-    bool moved = false;
+    int moved = 0;
 #pragma unroll
     for (int k = 0; k < K; k++) {
         const int e = s0 + lane * K + k;
         if (e < len) {
             const int jp = M.col[start + e];
-            double l, u;
-            load_bounds(M, D, jp & kIdMask, l, u);
-            moved |= apply_entry(M, D, jp, M.val[start + e], l, u, sd.x, sd.y, minact, maxact, ni, xi);
+            double l, uu;
+            load_bounds(M, V, jp & kIdMask, l, uu);
+            moved += apply_entry(M, S, jp, M.val[start + e], l, uu, sd.x, sd.y, minact, maxact, ni, xi);
         }
     }
-    return moved;
+    note_change(D, u, step, moved);
 }
 
 // The apply phase: the waves of the grid take the queued segments in turn.
-__device__ bool apply_phase(const DevMatrix &M, const DevState &D, int round, int wave, int nwaves) {
-    const int count = coherent_load(&D.status[kStatusWords * round + kStatusApply]);
-    bool moved = false;
-    for (int t = wave; t < count; t += nwaves) moved |= apply_split_task(M, D, coherent_load(&D.apply_tasks[t]));
-    return moved;
+__device__ __forceinline__ void apply_phase(const DevMatrix &M, const DevState &D, int step, int wave, int nwaves) {
+    int *st = status_row(D, step);
+    const int count = coherent_load(&st[kStatusApply]);
+    for (int t = wave; t < count; t += nwaves) apply_split_task(M, D, coherent_load(&D.apply_tasks[t]), step, st);
 }
 
-__device__ bool sweep_item(const DevMatrix &M, const DevState &D, int it, int round) {
-    const int4 d = M.items[it];
-    const int kind = d.x & 15;
-    if (kind == kItemSeg) {
-        seg_item(M, D, d.y, d.z, d.w, round);
-        return false;
+// ---------------------------------------------------------------------------
+// publish phase
+// ---------------------------------------------------------------------------
+
+// Publishes bitmap word w of unit u (one half-wave of 32 lanes, lane b owns
+// column 32 w + b; `bits` is the word as read by the scan): copies the
+// accumulated bounds of the moved columns into the unit's snapshot, records
+// a crossing as the unit's infeasible step and marks the rows of the moved
+// columns for the next step.  A unit whose accepted candidates in this step
+// reach its threshold processes all rows next step instead (no marks are
+// pushed then).  Clears the word.
+// Whether unit u processes all rows in the step after `step` (its accepted
+// candidates of `step` reached its limit); sets the unit's all-rows step.
+// Lanes 0..15 of each half-wave read the unit's counters: call with every
+// lane of the wave, `valid` per half.
+__device__ __forceinline__ bool decide_all_rows(const DevMatrix &M, const DevState &D, int u, bool valid, int step) {
+    const int b = threadIdx.x & 31;
+    int cnt = 0;
+    if (valid && M.mark_rows && b < kMoveLanes)
+        cnt = coherent_load(&D.moved[(u * kMoveSlots + step % kMoveSlots) * kMoveLanes + b]);
+#pragma unroll
+    for (int h = 1; h < kMoveLanes; h <<= 1) cnt += __shfl_xor(cnt, h, 64);
+    cnt = __shfl(cnt, threadIdx.x & 32, 64);
+    const int limit = valid ? coherent_load(&D.units[u].mark_limit) : 0;
+    const bool all_rows = valid && M.mark_rows && cnt >= limit;
+    if (all_rows && b == 0 && coherent_load(&D.units[u].markall) != step + 1)
+        atomic_max_i(&D.units[u].markall, step + 1);
+    return all_rows;
+}
+
+__device__ __forceinline__ void publish_word(const DevMatrix &M, const DevState &D, int u, int w, unsigned bits,
+                                             bool valid, int step, int all_rows_known) {
+    const int lane = threadIdx.x & 63;
+    const int b = lane & 31;
+    if (valid && !M.replicas) u = M.word_unit[w];
+    const DevState V = unit_view(M, D, u);
+    if (valid && b == 0) {
+        coherent_store(&V.bm[w], 0u);
+        if (!M.replicas) atomic_max_i(&D.units[u].changed[step & 1], step);  // the unit moved a bound
     }
-    if (kind == kItemLong) return long_item(M, D, d.y);
-    const int logg = (d.x >> 4) & 15;
-    const int logk = (d.x >> 8) & 15;
-    switch (logk) {
-        case 0: return group_item<1>(M, D, d.y, d.z, logg);
-        case 1: return group_item<2>(M, D, d.y, d.z, logg);
-        default: return group_item<4>(M, D, d.y, d.z, logg);
+    const bool all_rows = all_rows_known >= 0 ? all_rows_known != 0 : decide_all_rows(M, D, u, valid, step);
+    const bool moved = valid && ((bits >> b) & 1u);
+    const int id = w * 32 + b;
+    if (moved) {
+        double l, uu;
+        accumulated(M, V, id, l, uu);
+        store_snapshot(M, V, id, l, uu);
+        if (eps_gt(l, uu)) atomic_max_i(&D.units[u].infeasible, step);
+    }
+    if (!M.mark_rows || !valid || all_rows) return;
+    // rows of the moved columns, flattened over the 32 lanes of the half:
+    // exclusive prefix of the columns' lengths, then lane i marks entries
+    // i, i + 32, ... found by a binary search over the prefix
+    const int c0 = moved ? M.col_ptr[id] : 0;
+    const int cn = moved ? M.col_ptr[id + 1] - c0 : 0;
+    int pre = cn;
+#pragma unroll
+    for (int d = 1; d < 32; d <<= 1) {
+        const int v = __shfl_up(pre, d, 32);
+        if (b >= d) pre += v;
+    }
+    const int total = __shfl(pre, 31, 32);
+    pre -= cn;  // exclusive
+    for (int base = 0; base < total; base += 32) {  // same trip count on every lane of the half
+        const int i = base + b;
+        int lo = 0;
+#pragma unroll
+        for (int st = 16; st > 0; st >>= 1) {
+            const int cand = lo + st;
+            if (cand < 32 && __shfl(pre, cand, 32) <= i) lo = cand;
+        }
+        const int k = __shfl(c0, lo, 32) + i - __shfl(pre, lo, 32);
+        if (i < total) coherent_store(&V.marks[M.col_rows[k]], step + 1);
+    }
+}
+
+// Publish phase of `step` over the waves of the grid: every wave scans 8
+// bitmap words of one unit (replicas: the launch's units; blocks: the one
+// shared bitmap) and publishes the nonzero ones two at a time; a nonzero word
+// records that its unit, and the step, changed.  Also clears the next step's
+// status row and the candidate counters two steps ahead.
+__device__ __forceinline__ void publish_phase(const DevMatrix &M, const DevState &D, int step, int wave,
+                                              int nwaves, int n_units) {
+    constexpr int kChunk = 8;  // bitmap words per wave
+    const int lane = threadIdx.x & 63;
+    const int half = lane >> 5;
+    const int chunks = (M.nwords + kChunk - 1) / kChunk;
+    const int n_chunks = (M.replicas ? D.n_active : 1) * chunks;
+    for (int c = wave; c < n_chunks; c += nwaves) {
+        const int k = c / chunks;
+        const int q = c - k * chunks;
+        const int u = M.replicas ? uniform(D.active[k]) : 0;
+        const DevState V = unit_view(M, D, u);
+        const int w = q * kChunk + lane;
+        const unsigned bits = lane < kChunk && w < M.nwords ? coherent_load(&V.bm[w]) : 0u;
+        unsigned long long pending = __ballot(bits != 0u);
+        if (!pending) continue;
+        if (lane == 0) {  // the chunk's unit (replicas) moved a bound in this step
+            if (M.replicas) atomic_max_i(&D.units[u].changed[step & 1], step);
+            atomic_or_i(&status_row(D, step)[kStatusChanged], 1);
+        }
+        // replicas: one decision for the chunk's unit
+        const int known = M.replicas ? (decide_all_rows(M, D, u, true, step) ? 1 : 0) : -1;
+        while (pending) {
+            const int t0 = __ffsll(static_cast<long long>(pending)) - 1;
+            pending &= pending - 1;
+            int t1 = -1;
+            if (pending) {
+                t1 = __ffsll(static_cast<long long>(pending)) - 1;
+                pending &= pending - 1;
+            }
+            const int t = half ? t1 : t0;
+            const unsigned wb = __shfl(bits, t < 0 ? 0 : t, 64);
+            publish_word(M, D, u, q * kChunk + (t < 0 ? 0 : t), wb, t >= 0, step, known);
+        }
+    }
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int stride = gridDim.x * blockDim.x;
+    if (tid < kStatusWords) coherent_store(&status_row(D, step + 1)[tid], 0);
+    for (int k = tid; k < n_units * kMoveLanes; k += stride) {
+        const int uu = k / kMoveLanes, j = k - uu * kMoveLanes;
+        coherent_store(&D.moved[(uu * kMoveSlots + (step + 2) % kMoveSlots) * kMoveLanes + j], 0);
     }
 }
 
 // ---------------------------------------------------------------------------
-// publish items
+// step kernels (CPU_LOOP, GPU_LOOP)
 // ---------------------------------------------------------------------------
-
-// Copies the accumulated bounds of column `id` into the snapshot when it
-// moved this round (bitmap), checks them for crossed bounds and clears the
-// bitmap word (the lane of the word's first column, after the wave has read
-// it; the ids of a wave are 64 consecutive columns starting at a multiple of
-// 64).  Returns true on a crossing.
-__device__ __forceinline__ bool publish_column(const DevMatrix &M, const DevState &D, int id) {
-    if (id >= M.n) return false;
-    const unsigned bits = coherent_load(&D.bm[id >> 5]);
-    if (bits == 0u) return false;
-    if ((id & 31) == 0) coherent_store(&D.bm[id >> 5], 0u);
-    if (!((bits >> (id & 31)) & 1u)) return false;
-    double l, u;
-    accumulated(M, D, id, l, u);
-    store_snapshot(M, D, id, l, u);
-    return eps_gt(l, u);
-}
-
-__device__ __forceinline__ void flag_crossing(int *round_status, bool crossed) {
-    if (__any(crossed) && (threadIdx.x & 63) == 0) atomic_or_i(&round_status[kStatusInfeasible], 1);
-}
-
-// Publish phase over the columns [first, n) with a stride that is a multiple
-// of 64 (every wave sees whole bitmap words).
-__device__ __forceinline__ bool publish_phase(const DevMatrix &M, const DevState &D, int first, int stride) {
-    bool crossed = false;
-    for (int id = first; id < M.n; id += stride) crossed |= publish_column(M, D, id);
-    return crossed;
-}
-
-__device__ __forceinline__ void flag_change(int *round_status, bool moved) {
-    if (__any(moved) && (threadIdx.x & 63) == 0) atomic_or_i(&round_status[kStatusChanged], 1);
-}
 
 // With `guard` (captured graphs), a launch first checks on the device whether
-// the run already ended before this round and then does nothing.
-__device__ __forceinline__ bool round_skipped(const DevState &D, int round) {
-    if (D.status[kGlobalError] != 0 || D.status[kGlobalDone] != 0 || D.status[kGlobalEnvelope] != 0) return true;
-    if (round <= 1) return false;
-    const int *prev = D.status + kStatusWords * (round - 1);
-    return prev[kStatusInfeasible] != 0 || prev[kStatusChanged] == 0;
+// the previous step changed anything; if not, the run is over and the launch
+// does nothing.
+__device__ __forceinline__ bool step_skipped(const DevState &D, int step, int first_step) {
+    if (coherent_load(&D.global[kGlobalError]) != 0) return true;
+    if (step <= first_step) return false;
+    return coherent_load(&status_row(D, step - 1)[kStatusChanged]) == 0;
 }
 
-__global__ __launch_bounds__(256) void sweep_kernel(DevMatrix M, DevState D, int round, int guard) {
-    if (guard && round_skipped(D, round)) return;
-    const int it = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    bool moved = false;
-    if (it < M.n_items) moved = sweep_item(M, D, it, round);
-    flag_change(D.status + kStatusWords * round, moved);
+__global__ __launch_bounds__(256) void sweep_kernel(DevMatrix M, DevState D, int step, int guard, int first_step) {
+    if (guard && step_skipped(D, step, first_step)) return;
+    const int it = uniform(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+    const int n_work = M.replicas ? M.n_items * D.n_active : M.n_items;
+    if (it < n_work) sweep_item(M, D, it, step, status_row(D, step));
 }
 
-__global__ __launch_bounds__(256) void apply_split_kernel(DevMatrix M, DevState D, int round, int guard) {
-    if (guard && round_skipped(D, round)) return;
+__global__ __launch_bounds__(256) void apply_split_kernel(DevMatrix M, DevState D, int step, int guard,
+                                                          int first_step) {
+    if (guard && step_skipped(D, step, first_step)) return;
     const int waves = blockDim.x >> 6;
-    const bool moved = apply_phase(M, D, round, blockIdx.x * waves + (threadIdx.x >> 6), gridDim.x * waves);
-    flag_change(D.status + kStatusWords * round, moved);
+    apply_phase(M, D, step, uniform(blockIdx.x * waves + (threadIdx.x >> 6)), gridDim.x * waves);
 }
 
-__global__ __launch_bounds__(256) void publish_kernel(DevMatrix M, DevState D, int round, int guard) {
-    if (guard && round_skipped(D, round)) return;
-    const bool crossed = publish_phase(M, D, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
-    flag_crossing(D.status + kStatusWords * round, crossed);
+__global__ __launch_bounds__(256) void publish_kernel(DevMatrix M, DevState D, int step, int guard, int first_step,
+                                                      int n_units) {
+    if (guard && step_skipped(D, step, first_step)) return;
+    const int waves = blockDim.x >> 6;
+    publish_phase(M, D, step, uniform(blockIdx.x * waves + (threadIdx.x >> 6)), gridDim.x * waves, n_units);
 }
 
 // ---------------------------------------------------------------------------
@@ -698,12 +899,12 @@ __device__ bool grid_barrier(const DevState &D, unsigned nblocks, bool invalidat
             const long long t0 = wall_clock64();
             while (coherent_load(gen) == g0) {
                 __builtin_amdgcn_s_sleep(1);
-                if (coherent_load(&D.status[kGlobalError])) {
+                if (coherent_load(&D.global[kGlobalError])) {
                     ok = 0;
                     break;
                 }
                 if (wall_clock64() - t0 > kWaitTimeoutTicks) {
-                    atomic_or_i(&D.status[kGlobalError], 1);
+                    atomic_or_i(&D.global[kGlobalError], 1);
                     ok = 0;
                     break;
                 }
@@ -716,107 +917,159 @@ __device__ bool grid_barrier(const DevState &D, unsigned nblocks, bool invalidat
     return ok_sh != 0;
 }
 
-// All rounds in one persistent launch (MEGAKERNEL).  Every workgroup walks
-// the items of a phase with a grid stride (consecutive items on consecutive
-// waves), all meet at a barrier, and all read the same status words, so all
-// take the same decision to go on.  Phases of a round: sweep; apply of the
-// split rows when the sweep queued any; publish.  Outcome words in status
-// row 0.
-__global__ __launch_bounds__(256) void mega_kernel(DevMatrix M, DevState D, int max_rounds) {
+// Steps [first_step, first_step + max_steps) in one persistent launch
+// (MEGAKERNEL).  Every workgroup walks the work of a phase with a grid
+// stride, all meet at a barrier, and all read the same status words, so all
+// take the same decision to go on: the run ends after a step in which no
+// unit moved a bound.  Phases of a step: sweep; apply of the split rows when
+// the sweep queued any; publish.  The number of steps run goes to the global
+// status block (-1 after a failed wait).
+__global__ __launch_bounds__(256) void mega_kernel(DevMatrix M, DevState D, int first_step, int max_steps,
+                                                   int n_units) {
     const unsigned nblocks = gridDim.x;
     const int waves = blockDim.x >> 6;
-    const int wave = blockIdx.x * waves + (threadIdx.x >> 6);
+    const int wave = uniform(blockIdx.x * waves + (threadIdx.x >> 6));
     const int nwaves = gridDim.x * waves;
-    int rounds = 0, infeasible = 0, capped = 0;
-    for (int round = 1; round <= max_rounds; round++) {
-        int *st = D.status + kStatusWords * round;
-        bool moved = false;
-        for (int it = wave; it < M.n_items; it += nwaves) moved |= sweep_item(M, D, it, round);
-        if (!grid_barrier(D, nblocks, false)) { rounds = -1; break; }
+    const int n_work = M.replicas ? M.n_items * D.n_active : M.n_items;
+    int steps = 0;
+    for (int step = first_step; step < first_step + max_steps; step++) {
+        int *st = status_row(D, step);
+        for (int it = wave; it < n_work; it += nwaves) sweep_item(M, D, it, step, st);
+        if (!grid_barrier(D, nblocks, false)) { steps = -1; break; }
         if (coherent_load(&st[kStatusApply]) > 0) {
-            moved |= apply_phase(M, D, round, wave, nwaves);
-            if (!grid_barrier(D, nblocks, false)) { rounds = -1; break; }
+            apply_phase(M, D, step, wave, nwaves);
+            if (!grid_barrier(D, nblocks, false)) { steps = -1; break; }
         }
-        flag_change(st, moved);
-        const bool crossed =
-            publish_phase(M, D, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
-        flag_crossing(st, crossed);
-        if (!grid_barrier(D, nblocks, true)) { rounds = -1; break; }
-        rounds = round;
-        if (coherent_load(&st[kStatusInfeasible])) { infeasible = 1; break; }
+        publish_phase(M, D, step, wave, nwaves, n_units);
+        if (!grid_barrier(D, nblocks, true)) { steps = -1; break; }
+        steps = step - first_step + 1;
         if (!coherent_load(&st[kStatusChanged])) break;
-        if (round == max_rounds) capped = 1;
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        D.status[kOutRounds] = rounds;
-        D.status[kOutInfeasible] = infeasible;
-        D.status[kOutCapped] = capped;
-    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) D.global[kGlobalSteps] = steps;
 }
 
 // ---------------------------------------------------------------------------
-// start and end of a run
+// loading and reading bound vectors
 // ---------------------------------------------------------------------------
 
-// Loads the input bounds (original column order) into accumulator and
-// snapshot; clears bitmap, status words, barrier and split-row counters.
-// Sets the envelope flag when input bounds do not fit a column's storage
-// class (that word is cleared by the preceding clear_envelope_kernel).
-__global__ __launch_bounds__(256) void init_kernel(DevMatrix M, DevState D, const double *lbs, const double *ubs) {
+// Resets the state words of the listed units, params[k] = {unit, mark
+// limit, all rows in round 1, round cap}: round 1 in `start_step`, every row
+// processed in that step unless marks come from changed-column hints.  With
+// `clear_status`, also clears the step-status ring, the global words and the
+// barrier.
+__global__ void unit_reset_kernel(DevState D, int n_list, const int4 *params, int start_step, int clear_status) {
     const int tid = blockIdx.x * blockDim.x + threadIdx.x;
     const int stride = gridDim.x * blockDim.x;
-    for (int k = tid; k < kStatusRows * kStatusWords; k += stride)
-        if (k != kGlobalEnvelope && k != kGlobalCrossed) D.status[k] = 0;
-    for (int id = tid; id < M.n; id += stride) {
-        const int j = M.perm[id];
-        const double l = lbs[j], u = ubs[j];
-        if (eps_gt(l, u)) atomic_or_i(&D.status[kGlobalCrossed], 1);
-        if (id < M.n_int) {
-            const double lim = id < M.n16 ? 32767.0 : 2147483647.0;
-            const bool fits = l >= -lim && l <= lim && u >= -lim && u <= lim && floor(l) == l && floor(u) == u;
-            if (!fits) atomic_or_i(&D.status[kGlobalEnvelope], 1);
-            const int il = fits ? static_cast<int>(l) : 0, iu = fits ? static_cast<int>(u) : 0;
-            D.AI[id] = make_int2(il, iu);
-            D.AX[id] = make_double2(-INFINITY, INFINITY);
-            if (id < M.n16) D.S16[id] = pack16(il, iu);
-            else D.S32[id - M.n16] = make_int2(il, iu);
-        } else {
-            D.AF[id - M.n_int] = make_double2(l, u);
-            D.SF[id - M.n_int] = make_double2(l, u);
+    for (int k = tid; k < n_list; k += stride) {
+        const int4 q = params[k];
+        UnitState s = {};
+        s.start = start_step;
+        s.markall = q.z ? start_step : 0;
+        s.max_rounds = q.w;
+        s.mark_limit = q.y;
+        D.units[q.x] = s;
+        for (int j = 0; j < kMoveSlots * kMoveLanes; j++) D.moved[q.x * kMoveSlots * kMoveLanes + j] = 0;
+    }
+    if (clear_status) {
+        for (int k = tid; k < kStatusRows * kStatusWords; k += stride) D.status[k] = 0;
+        for (int k = tid; k < kGlobalWords; k += stride) D.global[k] = 0;
+        for (int k = tid; k < kBarrierWords; k += stride) D.barrier[k] = 0u;
+    }
+}
+
+// Loads the bound vectors of the listed units (caller's column order: unit
+// k's bounds at lbp[k], ubp[k]; blocks mode: one concatenated vector) into
+// snapshot and accumulator, clears their bitmaps, marks and split-row
+// counters, and flags crossed input bounds and bounds outside a column's
+// storage class.  Grid: x over internal ids, y over listed units.
+__global__ __launch_bounds__(256) void load_kernel(DevMatrix M, DevState D, int n_list, const int *list,
+                                                   const double *const *lbp, const double *const *ubp) {
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int stride = gridDim.x * blockDim.x;
+    const int n_pass = M.replicas ? n_list : 1;  // blocks mode: one pass over all columns
+    for (int k = blockIdx.y; k < n_pass; k += gridDim.y) {
+        const int u = list[k];
+        const DevState V = unit_view(M, D, u);
+        const double *lbs = lbp[M.replicas ? k : 0];
+        const double *ubs = ubp[M.replicas ? k : 0];
+        for (int id = tid; id < M.n; id += stride) {
+            const int j = M.perm[id];
+            const int cu = M.replicas ? u : M.word_unit[id >> 5];
+            double l = 0.0, uu = 0.0;
+            if (j >= 0) {
+                l = lbs[j];
+                uu = ubs[j];
+                if (eps_gt(l, uu)) atomic_or_i(&D.units[cu].crossed, 1);
+            }
+            if (id < M.n_int) {
+                const double lim = id < M.n16 ? 32767.0 : 2147483647.0;
+                const bool fits =
+                    l >= -lim && l <= lim && uu >= -lim && uu <= lim && floor(l) == l && floor(uu) == uu;
+                if (!fits) atomic_or_i(&D.units[cu].envelope, 1);
+                const int il = fits ? static_cast<int>(l) : 0, iu = fits ? static_cast<int>(uu) : 0;
+                V.AI[id] = make_int2(il, iu);
+                V.AX[id] = make_double2(-INFINITY, INFINITY);
+                if (id < M.n16) V.S16[id] = pack16(il, iu);
+                else V.S32[id - M.n16] = make_int2(il, iu);
+            } else {
+                V.AF[id - M.n_int] = make_double2(l, uu);
+                V.SF[id - M.n_int] = make_double2(l, uu);
+            }
+        }
+        {
+            for (int w = tid; w < M.nwords; w += stride) V.bm[w] = 0u;
+            for (int r = tid; r < M.m; r += stride) V.marks[r] = 0;
+            for (int q = tid; q < M.n_seg_rows; q += stride) V.seg_count[q] = 0u;
         }
     }
-    for (int w = tid; w < M.nwords; w += stride) D.bm[w] = 0u;
-    for (int k = tid; k < M.n_seg_rows; k += stride) D.seg_count[k] = 0u;
-    for (int k = tid; k < kBarrierWords; k += stride) D.barrier[k] = 0u;
 }
 
-// Clears the two words init_kernel sets from many threads (separate launch).
-__global__ void clear_envelope_kernel(DevState D) {
-    D.status[kGlobalEnvelope] = 0;
-    D.status[kGlobalCrossed] = 0;
+// Marks the rows of hinted columns (caller's column indices; unit k's hints
+// at hints[hint_ptr[k] .. hint_ptr[k + 1])) for the listed units' first step:
+// the bound vector differs from a propagated fixpoint only in these columns.
+__global__ __launch_bounds__(256) void hint_kernel(DevMatrix M, DevState D, int n_list, const int *list,
+                                                   const int *hint_ptr, const int *hints, const int *inv_perm,
+                                                   int start_step) {
+    for (int k = blockIdx.y; k < n_list; k += gridDim.y) {
+        const DevState V = unit_view(M, D, list[k]);
+        for (int h = hint_ptr[k] + blockIdx.x * blockDim.x + threadIdx.x; h < hint_ptr[k + 1];
+             h += gridDim.x * blockDim.x) {
+            const int id = inv_perm[hints[h]];
+            for (int q = M.col_ptr[id]; q < M.col_ptr[id + 1]; q++) V.marks[M.col_rows[q]] = start_step;
+        }
+    }
 }
 
-// End of a run: accumulated bounds back to original column order.
-__global__ __launch_bounds__(256) void unpack_kernel(DevMatrix M, DevState D, double *lbs, double *ubs) {
+// Writes the accumulated bounds of the listed units back to the caller's
+// column order.
+__global__ __launch_bounds__(256) void unpack_kernel(DevMatrix M, DevState D, int n_list, const int *list,
+                                                     double *const *lbp, double *const *ubp) {
     const int tid = blockIdx.x * blockDim.x + threadIdx.x;
     const int stride = gridDim.x * blockDim.x;
-    for (int id = tid; id < M.n; id += stride) {
-        double l, u;
-        if (id < M.n_int) {
-            const int2 a = D.AI[id];
-            const double2 x = D.AX[id];
-            l = static_cast<double>(a.x);
-            u = static_cast<double>(a.y);
-            l = x.x > l ? x.x : l;
-            u = x.y < u ? x.y : u;
-        } else {
-            const double2 a = D.AF[id - M.n_int];
-            l = a.x;
-            u = a.y;
+    const int n_pass = M.replicas ? n_list : 1;  // blocks mode: one pass over all columns
+    for (int k = blockIdx.y; k < n_pass; k += gridDim.y) {
+        const DevState V = unit_view(M, D, list[k]);
+        double *lbs = lbp[M.replicas ? k : 0];
+        double *ubs = ubp[M.replicas ? k : 0];
+        for (int id = tid; id < M.n; id += stride) {
+            const int j = M.perm[id];
+            if (j < 0) continue;
+            double l, u;
+            if (id < M.n_int) {
+                const int2 a = V.AI[id];
+                const double2 x = V.AX[id];
+                l = static_cast<double>(a.x);
+                u = static_cast<double>(a.y);
+                l = x.x > l ? x.x : l;
+                u = x.y < u ? x.y : u;
+            } else {
+                const double2 a = V.AF[id - M.n_int];
+                l = a.x;
+                u = a.y;
+            }
+            lbs[j] = l;
+            ubs[j] = u;
         }
-        const int j = M.perm[id];
-        lbs[j] = l;
-        ubs[j] = u;
     }
 }
 

@@ -232,13 +232,16 @@ def batch_family(seed: int = 3, count: int = 256, m_lo: int = 1000, m_hi: int = 
     return specs
 
 
-def tighten_node(inst: Instance, rng, k_lo: int = 1, k_hi: int = 50):
-    """Bounds of a branch-and-bound node: k random integer columns of the
-    root bounds restricted to a random half of their domain."""
-    lb, ub = inst.lb.copy(), inst.ub.copy()
+def tighten_node(inst: Instance, rng, k_lo: int = 1, k_hi: int = 50, base_lb=None, base_ub=None):
+    """Bounds of a branch-and-bound node: k random integer columns of the base
+    bounds (default: the root bounds; typically a propagated parent's)
+    restricted to a random half of their domain.  Returns (lb, ub, columns)
+    with the columns whose bounds changed."""
+    lb = np.array(inst.lb if base_lb is None else base_lb, np.float64, copy=True)
+    ub = np.array(inst.ub if base_ub is None else base_ub, np.float64, copy=True)
     ints = np.flatnonzero((inst.vartype != CONT) & (ub - lb >= 1) & (ub < INF) & (lb > -INF))
     if ints.size == 0:
-        return lb, ub
+        return lb, ub, np.zeros(0, np.int32)
     k = int(rng.integers(k_lo, k_hi + 1))
     pick = rng.choice(ints, size=min(k, ints.size), replace=False)
     for j in pick:
@@ -247,4 +250,4 @@ def tighten_node(inst: Instance, rng, k_lo: int = 1, k_hi: int = 50):
             ub[j] = mid
         else:
             lb[j] = mid + 1 if mid + 1 <= ub[j] else mid
-    return lb, ub
+    return lb, ub, np.sort(pick).astype(np.int32)

@@ -56,6 +56,18 @@ EXT_SYMBOLS = (
     "gdpProblemGetStats",
     "gdpPropagate",
     "gdpPropagateDevice",
+    "gdpProblemReserve",
+    "gdpPropagateNodesDevice",
+    "gdpBatchCreate",
+    "gdpBatchPropagate",
+    "gdpBatchPropagateDevice",
+    "gdpServerCreate",
+    "gdpServerDestroy",
+    "gdpServerSlots",
+    "gdpServerSubmit",
+    "gdpServerStep",
+    "gdpServerPending",
+    "gdpServerResult",
     "gdpSequentialPropagate",
     "gdpOMPPropagate",
     "gdpDeviceCount",
@@ -81,6 +93,7 @@ class Tuning(ctypes.Structure):
         ("graph_rounds", c_int),
         ("max_split_segs", c_int),
         ("compact_bounds", c_int),
+        ("row_marking", c_int),
         ("group_log", c_int * 10),
     ]
 
@@ -98,6 +111,8 @@ class ProblemStats(ctypes.Structure):
         ("n16", c_int),
         ("n32", c_int),
         ("nf", c_int),
+        ("n_units", c_int),
+        ("unit_state_bytes", c_longlong),
     ]
 
 
@@ -135,6 +150,30 @@ def load_library(path: Optional[str] = None) -> ctypes.CDLL:
     lib.gdpPropagate.restype = c_int
     lib.gdpPropagateDevice.argtypes = [vp, vp, vp, c_int, c_int, POINTER(RunInfo)]
     lib.gdpPropagateDevice.restype = c_int
+    lib.gdpProblemReserve.argtypes = [vp, c_int]
+    lib.gdpProblemReserve.restype = c_int
+    lib.gdpPropagateNodesDevice.argtypes = [vp, c_int, vp, vp, vp, vp, c_int, c_int, POINTER(RunInfo)]
+    lib.gdpPropagateNodesDevice.restype = c_int
+    lib.gdpBatchCreate.argtypes = [c_int, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, POINTER(Tuning), POINTER(vp)]
+    lib.gdpBatchCreate.restype = c_int
+    lib.gdpBatchPropagate.argtypes = [vp, vp, vp, c_int, c_int, POINTER(RunInfo)]
+    lib.gdpBatchPropagate.restype = c_int
+    lib.gdpBatchPropagateDevice.argtypes = [vp, vp, vp, c_int, c_int, POINTER(RunInfo)]
+    lib.gdpBatchPropagateDevice.restype = c_int
+    lib.gdpServerCreate.argtypes = [vp, c_longlong, c_int, POINTER(vp)]
+    lib.gdpServerCreate.restype = c_int
+    lib.gdpServerDestroy.argtypes = [vp]
+    lib.gdpServerDestroy.restype = c_int
+    lib.gdpServerSlots.argtypes = [vp, POINTER(c_int)]
+    lib.gdpServerSlots.restype = c_int
+    lib.gdpServerSubmit.argtypes = [vp, vp, vp, vp, c_int, c_int, POINTER(c_longlong)]
+    lib.gdpServerSubmit.restype = c_int
+    lib.gdpServerStep.argtypes = [vp, POINTER(c_int)]
+    lib.gdpServerStep.restype = c_int
+    lib.gdpServerPending.argtypes = [vp, POINTER(c_int)]
+    lib.gdpServerPending.restype = c_int
+    lib.gdpServerResult.argtypes = [vp, c_longlong, vp, vp, POINTER(RunInfo), POINTER(c_int)]
+    lib.gdpServerResult.restype = c_int
     lib.gdpSequentialPropagate.argtypes = [c_int, c_int, c_int, vp, vp, vp, vp, vp, vp, vp, vp, c_int,
                                            POINTER(RunInfo)]
     lib.gdpSequentialPropagate.restype = c_int
@@ -313,6 +352,23 @@ def _ptr(a: np.ndarray):
     return c_void_p(a.ctypes.data)
 
 
+def make_tuning(tuning: Optional[dict]) -> Tuning:
+    """gdp_tuning with the library defaults, overridden by `tuning`."""
+    t = Tuning()
+    load_library().gdpTuningDefaults(ctypes.byref(t))
+    for k, v in (tuning or {}).items():
+        if k == "group_log":
+            for b, g in enumerate(v):
+                t.group_log[b] = int(g)
+        else:
+            setattr(t, k, int(v))
+    return t
+
+
+def _outcome(lb, ub, info: RunInfo) -> "Outcome":
+    return Outcome(lb, ub, info.rounds, bool(info.infeasible), bool(info.capped), info.kernel_ms)
+
+
 class Problem:
     """A constraint matrix resident in device memory (gdpProblemCreate)."""
 
@@ -327,14 +383,7 @@ class Problem:
         self.lhs = _arr(inst.lhs, np.float64)
         self.rhs = _arr(inst.rhs, np.float64)
         self.vartype = _arr(inst.vartype, np.int32)
-        t = Tuning()
-        self.lib.gdpTuningDefaults(ctypes.byref(t))
-        for k, v in (tuning or {}).items():
-            if k == "group_log":
-                for b, g in enumerate(v):
-                    t.group_log[b] = int(g)
-            else:
-                setattr(t, k, int(v))
+        t = make_tuning(tuning)
         h = c_void_p()
         self.env_lb = _arr(inst.lb, np.float64) if envelope else None
         self.env_ub = _arr(inst.ub, np.float64) if envelope else None
@@ -370,9 +419,183 @@ class Problem:
             raise RuntimeError(f"gdpPropagateDevice failed: {last_error()}")
         return info
 
+    def reserve(self, capacity: int):
+        """Device state for `capacity` bound vectors (gdpProblemReserve)."""
+        if self.lib.gdpProblemReserve(self.handle, capacity) != GDP_OKAY:
+            raise RuntimeError(f"gdpProblemReserve failed: {last_error()}")
+
+    def propagate_nodes_device(self, lb_ptrs: Sequence[int], ub_ptrs: Sequence[int],
+                               hints: Optional[Sequence[Sequence[int]]] = None, sync_type: int = CPU_LOOP,
+                               max_rounds: int = MAX_NUM_ROUNDS) -> List[RunInfo]:
+        """Device-resident bound vectors (device pointers) in shared launches."""
+        count = len(lb_ptrs)
+        lbp = (c_void_p * count)(*lb_ptrs)
+        ubp = (c_void_p * count)(*ub_ptrs)
+        hp = hs = None
+        if hints is not None:
+            ptr = np.zeros(count + 1, np.int32)
+            ptr[1:] = np.cumsum([len(h) for h in hints])
+            flat = np.ascontiguousarray(np.concatenate([np.asarray(h, np.int32) for h in hints]) if ptr[-1] else
+                                        np.zeros(1, np.int32), np.int32)
+            hp, hs = _ptr(ptr), _ptr(flat)
+            self._keep = (ptr, flat)
+        infos = (RunInfo * count)()
+        rc = self.lib.gdpPropagateNodesDevice(self.handle, count, lbp, ubp, hp, hs, sync_type, max_rounds, infos)
+        if rc != GDP_OKAY:
+            raise RuntimeError(f"gdpPropagateNodesDevice failed: {last_error()}")
+        return list(infos)
+
+    def propagate_nodes(self, lbs: Sequence[np.ndarray], ubs: Sequence[np.ndarray],
+                        hints: Optional[Sequence[Sequence[int]]] = None, sync_type: int = CPU_LOOP,
+                        max_rounds: int = MAX_NUM_ROUNDS) -> List["Outcome"]:
+        """Host bound vectors: uploaded, propagated together, copied back."""
+        dl = [DeviceArray(np.asarray(x, np.float64)) for x in lbs]
+        du = [DeviceArray(np.asarray(x, np.float64)) for x in ubs]
+        try:
+            infos = self.propagate_nodes_device([d.ptr for d in dl], [d.ptr for d in du], hints, sync_type,
+                                                max_rounds)
+            return [_outcome(a.numpy(), b.numpy(), i) for a, b, i in zip(dl, du, infos)]
+        finally:
+            for d in dl + du:
+                d.free()
+
     def close(self):
         if getattr(self, "handle", None):
             self.lib.gdpProblemDestroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Independent instances propagated in shared launches (gdpBatchCreate).
+    Each instance's bounds are its own; the envelope is each instance's
+    bounds when `envelope`."""
+
+    def __init__(self, instances, tuning: Optional[dict] = None, envelope: bool = True):
+        self.lib = load_library()
+        self.instances = list(instances)
+        k = len(self.instances)
+        self._keep = []
+
+        def arrs(attr, dt):
+            a = [_arr(getattr(x, attr), dt) for x in self.instances]
+            self._keep.append(a)
+            return (c_void_p * k)(*[x.ctypes.data for x in a])
+
+        m = np.array([x.m for x in self.instances], np.int32)
+        n = np.array([x.n for x in self.instances], np.int32)
+        nnz = np.array([len(x.col) for x in self.instances], np.int32)
+        self._keep += [m, n, nnz]
+        self.n_offsets = np.concatenate([[0], np.cumsum(n)]).astype(np.int64)
+        h = c_void_p()
+        rc = self.lib.gdpBatchCreate(k, _ptr(m), _ptr(n), _ptr(nnz), arrs("col", np.int32), arrs("row_ptr", np.int32),
+                                     arrs("val", np.float64), arrs("lhs", np.float64), arrs("rhs", np.float64),
+                                     arrs("vartype", np.int32), arrs("lb", np.float64) if envelope else None,
+                                     arrs("ub", np.float64) if envelope else None, ctypes.byref(make_tuning(tuning)),
+                                     ctypes.byref(h))
+        if rc != GDP_OKAY:
+            raise RuntimeError(f"gdpBatchCreate failed: {last_error()}")
+        self.handle = h
+
+    def stats(self) -> ProblemStats:
+        s = ProblemStats()
+        self.lib.gdpProblemGetStats(self.handle, ctypes.byref(s))
+        return s
+
+    def concat_bounds(self, lbs=None, ubs=None):
+        lbs = lbs if lbs is not None else [x.lb for x in self.instances]
+        ubs = ubs if ubs is not None else [x.ub for x in self.instances]
+        return (np.concatenate([np.asarray(x, np.float64) for x in lbs]),
+                np.concatenate([np.asarray(x, np.float64) for x in ubs]))
+
+    def propagate(self, lbs=None, ubs=None, sync_type: int = CPU_LOOP,
+                  max_rounds: int = MAX_NUM_ROUNDS) -> List["Outcome"]:
+        lb, ub = self.concat_bounds(lbs, ubs)
+        infos = (RunInfo * len(self.instances))()
+        rc = self.lib.gdpBatchPropagate(self.handle, _ptr(lb), _ptr(ub), sync_type, max_rounds, infos)
+        if rc != GDP_OKAY:
+            raise RuntimeError(f"gdpBatchPropagate failed: {last_error()}")
+        o = self.n_offsets
+        return [_outcome(lb[o[i]:o[i + 1]].copy(), ub[o[i]:o[i + 1]].copy(), infos[i])
+                for i in range(len(self.instances))]
+
+    def propagate_device(self, d_lb: int, d_ub: int, sync_type: int = CPU_LOOP,
+                         max_rounds: int = MAX_NUM_ROUNDS) -> List[RunInfo]:
+        infos = (RunInfo * len(self.instances))()
+        rc = self.lib.gdpBatchPropagateDevice(self.handle, c_void_p(d_lb), c_void_p(d_ub), sync_type, max_rounds,
+                                              infos)
+        if rc != GDP_OKAY:
+            raise RuntimeError(f"gdpBatchPropagateDevice failed: {last_error()}")
+        return list(infos)
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.gdpProblemDestroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Node requests against a device-resident matrix with continuous
+    batching (gdpServerCreate)."""
+
+    def __init__(self, problem: Problem, hbm_budget: int, max_slots: int = 1024):
+        self.lib = load_library()
+        self.problem = problem
+        self.n = problem.n
+        h = c_void_p()
+        if self.lib.gdpServerCreate(problem.handle, int(hbm_budget), int(max_slots), ctypes.byref(h)) != GDP_OKAY:
+            raise RuntimeError(f"gdpServerCreate failed: {last_error()}")
+        self.handle = h
+        c = c_int()
+        self.lib.gdpServerSlots(self.handle, ctypes.byref(c))
+        self.slots = c.value
+
+    def submit(self, lb, ub, hints=None, max_rounds: int = MAX_NUM_ROUNDS) -> int:
+        lb = _arr(lb, np.float64)
+        ub = _arr(ub, np.float64)
+        hs = _arr(hints, np.int32) if hints is not None and len(hints) else None
+        t = c_longlong()
+        rc = self.lib.gdpServerSubmit(self.handle, _ptr(lb), _ptr(ub), _ptr(hs) if hs is not None else None,
+                                      0 if hs is None else len(hs), max_rounds, ctypes.byref(t))
+        if rc != GDP_OKAY:
+            raise RuntimeError(f"gdpServerSubmit failed: {last_error()}")
+        return t.value
+
+    def step(self) -> int:
+        c = c_int()
+        if self.lib.gdpServerStep(self.handle, ctypes.byref(c)) != GDP_OKAY:
+            raise RuntimeError(f"gdpServerStep failed: {last_error()}")
+        return c.value
+
+    def pending(self) -> int:
+        c = c_int()
+        self.lib.gdpServerPending(self.handle, ctypes.byref(c))
+        return c.value
+
+    def result(self, ticket: int) -> Optional["Outcome"]:
+        lb = np.empty(self.n, np.float64)
+        ub = np.empty(self.n, np.float64)
+        info = RunInfo()
+        ready = c_int()
+        if self.lib.gdpServerResult(self.handle, ticket, _ptr(lb), _ptr(ub), ctypes.byref(info),
+                                    ctypes.byref(ready)) != GDP_OKAY:
+            raise RuntimeError(f"gdpServerResult failed: {last_error()}")
+        return _outcome(lb, ub, info) if ready.value else None
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.gdpServerDestroy(self.handle)
             self.handle = None
 
     def __del__(self):

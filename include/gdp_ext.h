@@ -38,6 +38,8 @@ typedef struct {
                                 segment (-1: default, all; 0: never split) */
     int compact_bounds;      /* 1 (default): integer columns with a finite integral
                                 envelope use 4- or 8-byte bound records; 0: FP64 only */
+    int row_marking;         /* 1 (default): a round processes only rows with a column
+                                that moved in the previous round; 0: every row */
     int group_log[10];       /* lanes per row (log2) for rows of width 2^b; -1: default */
 } gdp_tuning;
 
@@ -49,6 +51,8 @@ typedef struct {
     long long bin_rows[11];  /* rows per tree-width bin 2^0..2^9, then rows > 256 */
     long long bin_nnz[11];
     int n16, n32, nf;        /* columns with 2x int16, 2x int32, 2x double records */
+    int n_units;             /* instances of a batch, or bound-vector capacity */
+    long long unit_state_bytes; /* device bytes of one bound vector's state */
 } gdp_problem_stats;
 
 /* Fills *t with the defaults. */
@@ -80,6 +84,62 @@ GDP_RETCODE gdpPropagate(gdp_problem p, double *lbs, double *ubs, const int sync
  * has finished. */
 GDP_RETCODE gdpPropagateDevice(gdp_problem p, double *d_lbs, double *d_ubs, const int sync_type,
                                const int max_rounds, gdp_run_info *info);
+
+/* ---- many bound vectors against one matrix ----
+ * The problem keeps device state for `capacity` bound vectors (grown on
+ * demand by gdpPropagateNodesDevice). */
+GDP_RETCODE gdpProblemReserve(gdp_problem p, const int capacity);
+
+/* Propagates `count` device-resident bound vectors at once (vector k at
+ * d_lbs[k], d_ubs[k]: host arrays of device pointers) in shared launches;
+ * each vector gets its own rounds and outcome in infos[k] and is
+ * bit-identical to propagating it alone.  hint_ptr / hints (may be NULL):
+ * vector k differs from a propagated fixpoint only in the columns
+ * hints[hint_ptr[k] .. hint_ptr[k+1]), so its first round processes only their
+ * rows (a branch-and-bound child of a propagated parent). */
+GDP_RETCODE gdpPropagateNodesDevice(gdp_problem p, const int count, double *const *d_lbs, double *const *d_ubs,
+                                    const int *hint_ptr, const int *hints, const int sync_type,
+                                    const int max_rounds, gdp_run_info *infos);
+
+/* ---- a batch of independent instances in shared launches ----
+ * Instance i is given by the i-th entries of the arrays (same meaning as in
+ * gdpProblemCreate; env_lbs / env_ubs may be NULL, or hold NULL entries). */
+GDP_RETCODE gdpBatchCreate(const int n_inst, const int *n_cons, const int *n_vars, const int *nnz,
+                           const int *const *col_indices, const int *const *row_ptrs,
+                           const double *const *vals, const double *const *lhss, const double *const *rhss,
+                           const int *const *vartypes, const double *const *env_lbs,
+                           const double *const *env_ubs, const gdp_tuning *tuning, gdp_problem *out);
+/* Propagates every instance; lbs / ubs concatenate the instances' bound
+ * vectors in instance order (host arrays, in/out; the Device variant takes
+ * device pointers).  infos[i] (n_inst entries) gets instance i's outcome;
+ * kernel_ms is the device time of the whole batch. */
+GDP_RETCODE gdpBatchPropagate(gdp_problem p, double *lbs, double *ubs, const int sync_type,
+                              const int max_rounds, gdp_run_info *infos);
+GDP_RETCODE gdpBatchPropagateDevice(gdp_problem p, double *d_lbs, double *d_ubs, const int sync_type,
+                                    const int max_rounds, gdp_run_info *infos);
+
+/* ---- serving node requests with continuous batching ----
+ * A server admits submitted requests into free slots of the problem's
+ * device state (as many slots as hbm_budget bytes allow, at most max_slots),
+ * and each gdpServerStep advances every running request by one round in
+ * shared launches: requests that arrive while others run join at the next
+ * step.  A request's result is bit-identical to propagating it alone. */
+typedef struct gdp_server_s *gdp_server;
+GDP_RETCODE gdpServerCreate(gdp_problem p, const long long hbm_budget, const int max_slots, gdp_server *out);
+GDP_RETCODE gdpServerDestroy(gdp_server s);
+GDP_RETCODE gdpServerSlots(gdp_server s, int *slots);
+/* Copies the request (host bound vectors of length n_vars; hints as in
+ * gdpPropagateNodesDevice, may be NULL) and returns its ticket. */
+GDP_RETCODE gdpServerSubmit(gdp_server s, const double *lbs, const double *ubs, const int *hints,
+                            const int n_hints, const int max_rounds, long long *ticket);
+/* One step: admit, one round of every running request, retire. */
+GDP_RETCODE gdpServerStep(gdp_server s, int *completed);
+GDP_RETCODE gdpServerPending(gdp_server s, int *pending);
+/* *ready = 1 and the result copied out (and forgotten) if the request has
+ * completed; *ready = 0 otherwise.  info->rounds is -1 for a request whose
+ * bounds left the problem's bound envelope. */
+GDP_RETCODE gdpServerResult(gdp_server s, const long long ticket, double *lbs, double *ubs,
+                            gdp_run_info *info, int *ready);
 
 /* Host propagators with run information: cpu_seq (Gauss-Seidel), cpu_seq_dis
  * and cpu_omp (threads <= 0: GDP_OMP_THREADS or 8). */

@@ -1,0 +1,184 @@
+"""GPU parity of the shared-launch paths: a batch of independent instances,
+many bound vectors against one matrix (with and without changed-column
+hints), and the continuous-batching server.  Every unit must be
+bit-identical to the oracle's round-synchronous propagation of that unit
+alone (bounds, round count, infeasibility, cap)."""
+import numpy as np
+import pytest
+
+import fixtures
+from gdp_amd import interface as I
+from gdp_amd.instances import powerlaw_mip, small_mip, tighten_node
+
+pytestmark = pytest.mark.gpu
+
+SYNCS = [I.CPU_LOOP, I.GPU_LOOP, I.MEGAKERNEL]
+
+
+def assert_same(got, exp, label=""):
+    lb, ub, rounds, infeasible, capped = exp
+    np.testing.assert_array_equal(got.lbs, lb, err_msg=f"{label} lb")
+    np.testing.assert_array_equal(got.ubs, ub, err_msg=f"{label} ub")
+    assert (got.rounds, bool(got.infeasible), bool(got.capped)) == (rounds, bool(infeasible), bool(capped)), label
+
+
+def batch_instances():
+    insts = [fixtures.analytic(n) for n in fixtures.ANALYTIC]
+    insts += [small_mip(seed=s, m=200 + 150 * s, n=100 + 70 * s) for s in range(5)]
+    insts += [small_mip(seed=7, int_frac=1.0), fixtures.generalized_cfp(12), fixtures.gate_ties(),
+              fixtures.infeasible_instance(), fixtures.varied(1),
+              powerlaw_mip(seed=5, m=6000, n=3000, len_hi=3000, s=1.8)]
+    return insts
+
+
+@pytest.mark.parametrize("sync", SYNCS)
+def test_batch_each_instance_matches_oracle(gpu, oracle, sync):
+    insts = batch_instances()
+    b = I.Batch(insts)
+    try:
+        outs = b.propagate(sync_type=sync)
+    finally:
+        b.close()
+    for inst, got in zip(insts, outs):
+        assert_same(got, oracle.jacobi(inst), f"{inst.name} sync={sync}")
+
+
+def test_batch_matches_cpu_seq_fixpoint(gpu, oracle):
+    """Each instance's fixpoint against its own cpu_seq: integer bounds
+    exact, continuous within 1e-9 relative or absolute, same infeasibility."""
+    insts = batch_instances()
+    b = I.Batch(insts)
+    try:
+        outs = b.propagate(sync_type=I.MEGAKERNEL)
+    finally:
+        b.close()
+    for inst, got in zip(insts, outs):
+        seq = oracle.seq(inst)
+        assert bool(got.infeasible) == bool(seq[3]), inst.name
+        if seq[3] or seq[4] or got.capped:
+            continue
+        is_int = inst.vartype != 3
+        np.testing.assert_array_equal(got.lbs[is_int], seq[0][is_int], err_msg=inst.name)
+        np.testing.assert_array_equal(got.ubs[is_int], seq[1][is_int], err_msg=inst.name)
+        for g, s in ((got.lbs, seq[0]), (got.ubs, seq[1])):
+            d = np.abs(g - s)[~is_int]
+            tol = np.maximum(1e-9, 1e-9 * np.abs(s[~is_int]))
+            assert np.all(d <= tol), inst.name
+
+
+def test_batch_round_prefixes(gpu, oracle):
+    """A round cap applies to every instance of the batch."""
+    insts = batch_instances()
+    b = I.Batch(insts)
+    try:
+        for cap in (1, 2, 3):
+            outs = b.propagate(sync_type=I.CPU_LOOP, max_rounds=cap)
+            for inst, got in zip(insts, outs):
+                assert_same(got, oracle.jacobi(inst, max_rounds=cap), f"{inst.name} cap={cap}")
+    finally:
+        b.close()
+
+
+def node_vectors(inst, oracle, count, seed, from_fixpoint=True):
+    rng = np.random.default_rng(seed)
+    base = oracle.jacobi(inst) if from_fixpoint else (inst.lb, inst.ub)
+    nodes = [tighten_node(inst, rng, 1, 20, base[0], base[1]) for _ in range(count)]
+    return nodes
+
+
+@pytest.mark.parametrize("sync", SYNCS)
+@pytest.mark.parametrize("case", ["small", "powerlaw"])
+def test_nodes_match_oracle(gpu, oracle, case, sync):
+    inst = small_mip(seed=3, int_frac=0.8) if case == "small" else powerlaw_mip(seed=5, m=20000, n=10000,
+                                                                               len_hi=5000, s=1.8)
+    nodes = node_vectors(inst, oracle, 24, seed=11)
+    p = I.Problem(inst)
+    try:
+        outs = p.propagate_nodes([x[0] for x in nodes], [x[1] for x in nodes], sync_type=sync)
+        hinted = p.propagate_nodes([x[0] for x in nodes], [x[1] for x in nodes], hints=[x[2] for x in nodes],
+                                   sync_type=sync)
+    finally:
+        p.close()
+    for k, (lb, ub, _) in enumerate(nodes):
+        exp = oracle.jacobi(inst, lb=lb, ub=ub)
+        assert_same(outs[k], exp, f"node {k}")
+        assert_same(hinted[k], exp, f"hinted node {k}")
+
+
+def test_nodes_from_root_and_capacity_growth(gpu, oracle):
+    """Vectors that are not near a fixpoint (no hints), more vectors than the
+    reserved capacity, then fewer."""
+    inst = small_mip(seed=4)
+    nodes = node_vectors(inst, oracle, 9, seed=2, from_fixpoint=False)
+    p = I.Problem(inst)
+    try:
+        p.reserve(2)
+        for count in (9, 3):
+            outs = p.propagate_nodes([x[0] for x in nodes[:count]], [x[1] for x in nodes[:count]],
+                                     sync_type=I.MEGAKERNEL)
+            for k in range(count):
+                assert_same(outs[k], oracle.jacobi(inst, lb=nodes[k][0], ub=nodes[k][1]), f"node {k}")
+    finally:
+        p.close()
+
+
+@pytest.mark.parametrize("case", ["small_int:7", "powerlaw:5", "gate_ties", "varied:2"])
+def test_row_marking_off_same_bits(gpu, oracle, case):
+    from test_gpu_parity import _case
+    inst = _case(case)
+    exp = oracle.jacobi(inst)
+    for tuning in ({"row_marking": 0}, {"row_marking": 1}):
+        p = I.Problem(inst, tuning)
+        try:
+            for sync in SYNCS:
+                assert_same(p.propagate(inst.lb, inst.ub, sync), exp, f"{case} {tuning} sync={sync}")
+        finally:
+            p.close()
+
+
+def test_server_results_alone_and_in_the_mix(gpu, oracle):
+    """Requests submitted while others run join at the next step; every
+    result equals the request propagated alone (oracle and the plain entry
+    point)."""
+    inst = powerlaw_mip(seed=6, m=20000, n=10000, len_hi=5000, s=1.8)
+    nodes = node_vectors(inst, oracle, 40, seed=5)
+    p = I.Problem(inst)
+    try:
+        alone = [p.propagate(lb, ub, I.CPU_LOOP) for lb, ub, _ in nodes]
+        srv = I.Server(p, hbm_budget=64 << 20, max_slots=16)
+        assert 1 <= srv.slots <= 16
+        tickets = {}
+        k = 0
+        while k < len(nodes) or srv.pending():
+            for _ in range(3):  # a few arrivals per step
+                if k < len(nodes):
+                    lb, ub, cols = nodes[k]
+                    tickets[k] = srv.submit(lb, ub, hints=cols if k % 2 else None)
+                    k += 1
+            srv.step()
+        for k, t in tickets.items():
+            got = srv.result(t)
+            assert got is not None, k
+            exp = oracle.jacobi(inst, lb=nodes[k][0], ub=nodes[k][1])
+            assert_same(got, exp, f"request {k}")
+            np.testing.assert_array_equal(got.lbs, alone[k].lbs)
+            np.testing.assert_array_equal(got.ubs, alone[k].ubs)
+            assert got.rounds == alone[k].rounds
+        assert srv.result(tickets[0]) is None  # results are handed out once
+        srv.close()
+    finally:
+        p.close()
+
+
+def test_server_budget_limits_slots(gpu):
+    inst = small_mip(seed=1)
+    p = I.Problem(inst)
+    try:
+        per = p.stats().unit_state_bytes + 16 * inst.n
+        srv = I.Server(p, hbm_budget=3 * per + 1, max_slots=100)
+        assert srv.slots == 3
+        srv.close()
+        with pytest.raises(RuntimeError):
+            I.Server(p, hbm_budget=per // 2, max_slots=4)
+    finally:
+        p.close()
