@@ -20,6 +20,15 @@ Prints one JSON line:
                 summary when it was taken on this workload
   cpu_baseline  the oracle's sequential propagator (cpu_seq restated in C) on
                 the same workload on this host, one thread
+  batched       256 independent MIPs of uneven size (m log-uniform in
+                [1e3, 2e5], n = m/2, mean row length log-uniform in [3, 30]) in
+                shared launches vs back-to-back plain launches, instances/s;
+                every instance checked against its own cpu_seq
+  serving       node requests against a device-resident 2M-nonzero matrix:
+                1024 nodes in one shared launch, and a continuous-batching
+                server at Poisson arrival rates 1k..50k/s (full bound vectors
+                and changes to the propagated root) with p50/p99 latency, vs
+                serial plain launches
 
 Other modes:
   --calibrate-copy MB   device-to-device copies of MB MiB (profiler
@@ -90,6 +99,150 @@ def load_traffic(inst, rounds, kernel):
                                                                                      kernel):
         return None
     return p.get("traffic_bytes_per_launch")
+
+
+# ---------------------------------------------------------------------------
+# batched and served propagation
+# ---------------------------------------------------------------------------
+
+def batched_leg(sync, check_cpu):
+    """256 uneven MIPs: one shared run vs back-to-back plain launches."""
+    import numpy as np
+    from gdp_amd import interface as I
+    from gdp_amd.instances import batch_family, small_mip
+    insts = [small_mip(**sp) for sp in batch_family(count=256)]
+    nnz = sum(len(x.col) for x in insts)
+    print(f"bench: batched workload, {len(insts)} instances, {nnz} nonzeros", file=sys.stderr, flush=True)
+    b = I.Batch(insts)
+    lb, ub = b.concat_bounds()
+    root_l, root_u = I.DeviceArray(lb), I.DeviceArray(ub)
+    dl, du = I.DeviceArray(lb), I.DeviceArray(ub)
+    times = []
+    for rep in range(4):
+        dl.copy_from(root_l)
+        du.copy_from(root_u)
+        infos = b.propagate_device(dl.ptr, du.ptr, sync)
+        if rep:
+            times.append(infos[0].kernel_ms)
+    batched_ms = statistics.median(times)
+    out_l, out_u = dl.numpy(), du.numpy()
+    # back-to-back plain launches: one device-resident problem per instance
+    probs = [I.Problem(x) for x in insts]
+    arrs = [(I.DeviceArray(x.lb), I.DeviceArray(x.ub)) for x in insts]
+    plain_ms = 0.0
+    for p, (a, c), x in zip(probs, arrs, insts):
+        info = p.propagate_device(a.ptr, c.ptr, sync)
+        plain_ms += info.kernel_ms
+    res = {"workload": f"256 small_mip instances, m log-uniform [1e3, 2e5], n = m/2, mean row length log-uniform "
+                       f"[3, 30], {nnz} nonzeros in total, root bounds to the fixpoint",
+           "sync": SYNC_NAMES[sync], "instances": len(insts), "nnz_total": nnz,
+           "batched_ms": batched_ms, "nodes_per_s": len(insts) / (batched_ms * 1e-3),
+           "plain_ms": plain_ms, "plain_nodes_per_s": len(insts) / (plain_ms * 1e-3),
+           "speedup": plain_ms / batched_ms, "max_rounds": max(i.rounds for i in infos)}
+    for p in probs:
+        p.close()
+    if check_cpu:
+        # every instance against the round-synchronous oracle (bit for bit:
+        # bounds, rounds, flags) and, where neither run hit the round cap,
+        # against its own cpu_seq fixpoint (integer bounds exact, continuous
+        # within 1e-9 relative or absolute)
+        import oracle_binding
+        o = b.n_offsets
+        exact, bad_int, worst, feas, compared, capped = 0, 0, 0.0, 0, 0, 0
+        for i, x in enumerate(insts):
+            gl, gu = out_l[o[i]:o[i + 1]], out_u[o[i]:o[i + 1]]
+            jac = oracle_binding.jacobi(x)
+            exact += bool(np.array_equal(gl, jac[0]) and np.array_equal(gu, jac[1]) and jac[2] == infos[i].rounds and
+                          jac[3] == bool(infos[i].infeasible) and jac[4] == bool(infos[i].capped))
+            seq = oracle_binding.seq(x)
+            feas += bool(infos[i].infeasible) == bool(seq[3])
+            if infos[i].capped or seq[4] or seq[3]:
+                capped += bool(infos[i].capped or seq[4])
+                continue
+            compared += 1
+            is_int = x.vartype != 3
+            bad_int += int(np.sum(gl[is_int] != seq[0][is_int]) + np.sum(gu[is_int] != seq[1][is_int]))
+            for g, sv in ((gl, seq[0]), (gu, seq[1])):
+                d = np.abs(g - sv)[~is_int] / np.maximum(1.0, np.abs(sv[~is_int]))
+                worst = max(worst, float(d.max()) if d.size else 0.0)
+        res["parity"] = {"bitwise_equal_to_oracle": exact, "of": len(insts),
+                         "vs_cpu_seq": {"compared": compared, "capped_at_100_rounds": capped,
+                                        "integer_bound_mismatches": bad_int,
+                                        "max_continuous_rel_or_abs_dev": worst,
+                                        "infeasibility_agrees": feas == len(insts)}}
+    b.close()
+    return res
+
+
+def serving_leg(sync):
+    """Node requests against one device-resident matrix."""
+    import numpy as np
+    from gdp_amd import interface as I
+    from gdp_amd.instances import powerlaw_mip, tighten_node
+    inst = powerlaw_mip(seed=4, m=200_000, n=100_000, s=2.08, len_hi=5000)
+    print(f"bench: serving workload, nnz={inst.nnz}", file=sys.stderr, flush=True)
+    p = I.Problem(inst)
+    root = p.propagate(inst.lb, inst.ub, sync)
+    rng = np.random.default_rng(7)
+    pool = [tighten_node(inst, rng, 1, 50, root.lbs, root.ubs) for _ in range(400)]
+    t = time.perf_counter()
+    rounds = [p.propagate(lb, ub, sync).rounds for lb, ub, _ in pool[:200]]
+    plain = 200 / (time.perf_counter() - t)
+    B = 1024
+    p.reserve(B)
+    dls = [I.DeviceArray(pool[k % len(pool)][0]) for k in range(B)]
+    dus = [I.DeviceArray(pool[k % len(pool)][1]) for k in range(B)]
+    hints = [pool[k % len(pool)][2] for k in range(B)]
+    for rep in range(2):
+        for k in range(B):
+            dls[k].upload(pool[k % len(pool)][0])
+            dus[k].upload(pool[k % len(pool)][1])
+        infos = p.propagate_nodes_device([d.ptr for d in dls], [d.ptr for d in dus], hints, sync)
+    shared = B / (infos[0].kernel_ms * 1e-3)
+    for d in dls + dus:
+        d.free()
+    srv = I.Server(p, hbm_budget=16 << 30, max_slots=1024)
+    srv.set_base(root.lbs, root.ubs)
+    rates = []
+    for mode in ("full", "delta"):
+        for rate in (1000, 2000, 5000, 10000, 20000, 50000):
+            N = int(min(4000, max(600, rate)))
+            arr = np.cumsum(rng.exponential(1.0 / rate, size=N))
+            idx, done, got, k = {}, np.zeros(N), 0, 0
+            start = time.perf_counter()
+            while got < N:
+                now = time.perf_counter() - start
+                while k < N and arr[k] <= now:
+                    lb, ub, cols = pool[k % len(pool)]
+                    t_ = srv.submit(lb, ub, hints=cols) if mode == "full" else srv.submit_delta(cols, lb[cols],
+                                                                                                ub[cols])
+                    idx[t_] = k
+                    k += 1
+                if srv.pending():
+                    srv.step()
+                    tnow = time.perf_counter() - start
+                    for tk in srv.completed():
+                        done[idx[tk]] = tnow
+                        srv.result(tk)
+                        got += 1
+                elif k < N:
+                    while time.perf_counter() - start < arr[k]:
+                        pass
+            lat = (done - arr) * 1e3
+            rates.append({"requests": mode, "offered_per_s": rate, "n": N,
+                          "achieved_per_s": N / (done.max() - arr[0]),
+                          "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99))})
+            print(f"bench: serving {mode} {rate}/s -> {rates[-1]['achieved_per_s']:.0f}/s "
+                  f"p50 {rates[-1]['p50_ms']:.2f} ms p99 {rates[-1]['p99_ms']:.2f} ms", file=sys.stderr, flush=True)
+    srv.close()
+    p.close()
+    best = max(r["achieved_per_s"] for r in rates)
+    return {"workload": f"powerlaw_mip m={inst.m} n={inst.n} nnz={inst.nnz} (zipf 2.08, rows <= 5000); nodes = "
+                        f"propagated root with 1..50 integer columns restricted to half their domain",
+            "sync": SYNC_NAMES[sync], "slots": srv.slots, "mean_node_rounds": float(np.mean(rounds)),
+            "plain_nodes_per_s": plain, "shared_launch_1024_nodes_per_s": shared,
+            "shared_speedup": shared / plain, "served_best_per_s": best, "served_speedup": best / plain,
+            "rates": rates}
 
 
 # ---------------------------------------------------------------------------
@@ -197,6 +350,8 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cpu-runs", type=int, default=7)
     ap.add_argument("--no-cpu", action="store_true", help="skip the host baselines and the parity check")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-instances leg")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB")
     ap.add_argument("--summarize", default=None, metavar="DIR")
     ap.add_argument("--meta-out", default=None, help="write workload facts for --summarize")
@@ -303,6 +458,11 @@ def main():
         result["cpu_baseline"] = cpu_baseline(inst, args.cpu_runs)
         result["cpu_omp"] = omp_baseline(inst, min(16, os.cpu_count() or 8))
     prob.close()
+    del prob, lb, ub, root_lb, root_ub
+    if not args.no_batch:
+        result["batched"] = batched_leg(args.sync, not args.no_cpu)
+    if not args.no_serve:
+        result["serving"] = serving_leg(args.sync)
     print(json.dumps(result), flush=True)
 
 

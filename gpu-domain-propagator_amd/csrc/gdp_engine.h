@@ -118,13 +118,24 @@ struct Server;
 Server *server_create(Problem *p, int64_t hbm_budget, int max_slots);
 void server_destroy(Server *s);
 int server_slots(const Server *s);
-// Copies the request; returns its ticket.
+// Copies the request (full bound vectors; hints: columns that differ from a
+// propagated fixpoint, may be empty); returns its ticket.
 int64_t server_submit(Server *s, const double *lbs, const double *ubs, const int *hints, int n_hints,
                       int max_rounds);
+// Base vector of delta requests (kept in device memory), e.g. the
+// propagated root of a branch-and-bound tree.
+void server_set_base(Server *s, const double *lbs, const double *ubs);
+// A request given as the base vector with `n_changes` columns set to new
+// bounds; no bound vector crosses PCIe.  The base must be a propagated
+// fixpoint: round 1 processes only the rows of the changed columns.
+int64_t server_submit_delta(Server *s, int n_changes, const int *cols, const double *lbs, const double *ubs,
+                            int max_rounds);
 // One step; returns the number of requests completed by it.
 int server_step(Server *s);
 // Requests queued or running.
 int server_pending(const Server *s);
+// Tickets completed by the last step.
+const std::vector<int64_t> &server_completed(const Server *s);
 // Completed request: copies its bounds and outcome and forgets it; false if
 // the ticket is not (yet) complete.
 bool server_result(Server *s, int64_t ticket, double *lbs, double *ubs, RunResult *r);

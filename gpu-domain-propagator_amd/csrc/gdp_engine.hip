@@ -38,6 +38,8 @@ __global__ void hint_kernel(DevMatrix M, DevState D, int n_list, const int *list
                             const int *hints, const int *inv_perm, int start_step);
 __global__ void unpack_kernel(DevMatrix M, DevState D, int n_list, const int *list, double *const *lbp,
                               double *const *ubp);
+__global__ void delta_kernel(int n_list, double *const *lbp, double *const *ubp, const int *ptr, const int *cols,
+                             const double *lv, const double *uv);
 
 namespace {
 
@@ -790,27 +792,52 @@ void batch_propagate_device(Problem *P, double *d_lbs, double *d_ubs, int sync_t
 
 struct Server {
     struct Request {
-        int64_t ticket;
-        std::vector<double> lb, ub;
-        std::vector<int> hints;
-        int max_rounds;
+        int64_t ticket = 0;
+        int inbox = -1;                  // pinned inbox entry holding the vectors, or -1
+        std::vector<double> lb, ub;      // the vectors when no inbox entry was free
+        bool delta = false;              // base vector + changed columns
+        std::vector<int> cols;           // changed columns (delta) / hints (full)
+        std::vector<double> dlb, dub;    // new bounds of the changed columns (delta)
+        int max_rounds = kMaxRounds;
     };
     struct Done {
+        int outbox = -1;                 // pinned outbox entry holding the result, or -1
         std::vector<double> lb, ub;
         RunResult r;
     };
     Problem *P = nullptr;
     int slots = 0;
+    size_t n = 0;
     std::deque<Request> queue;
-    std::vector<int64_t> slot_ticket;  // -1: free
+    std::vector<int64_t> slot_ticket;    // -1: free
     std::unordered_map<int64_t, Done> done;
+    std::vector<int64_t> last_completed;
     int64_t next_ticket = 1;
     int step = 1;
-    double *d_buf = nullptr;           // slots x (lb, ub) bound vectors
+    double *d_buf = nullptr;             // slots x (lb, ub) bound vectors
+    double *d_base = nullptr;            // base vector of delta requests
+    bool has_base = false;
     std::vector<double *> slot_lb, slot_ub;
+    double *inbox = nullptr, *outbox = nullptr;  // pinned host memory
+    std::vector<int> inbox_free, outbox_free;
+    // device buffers for the changes of delta requests
+    int *d_dptr = nullptr, *d_dcols = nullptr;
+    double *d_dlb = nullptr, *d_dub = nullptr;
+    double **d_dl = nullptr, **d_du = nullptr;
+    size_t dcap = 0;
     ~Server() {
         if (d_buf) (void)hipFree(d_buf);
+        if (d_base) (void)hipFree(d_base);
+        if (inbox) (void)hipHostFree(inbox);
+        if (outbox) (void)hipHostFree(outbox);
+        for (void *q : {static_cast<void *>(d_dptr), static_cast<void *>(d_dcols), static_cast<void *>(d_dlb),
+                        static_cast<void *>(d_dub), static_cast<void *>(d_dl), static_cast<void *>(d_du)})
+            if (q) (void)hipFree(q);
     }
+    double *in_lb(int e) { return inbox + 2 * n * e; }
+    double *in_ub(int e) { return inbox + 2 * n * e + n; }
+    double *out_lb(int e) { return outbox + 2 * n * e; }
+    double *out_ub(int e) { return outbox + 2 * n * e + n; }
 };
 
 Server *server_create(Problem *P, int64_t hbm_budget, int max_slots) {
@@ -823,13 +850,26 @@ Server *server_create(Problem *P, int64_t hbm_budget, int max_slots) {
     try {
         S->P = P;
         S->slots = slots;
+        S->n = static_cast<size_t>(P->n);
         problem_reserve(P, slots);
-        check(hipMalloc(&S->d_buf, sizeof(double) * 2 * static_cast<size_t>(P->n) * slots), "hipMalloc slots");
+        check(hipMalloc(&S->d_buf, sizeof(double) * 2 * S->n * slots), "hipMalloc slots");
+        check(hipMalloc(&S->d_base, sizeof(double) * 2 * S->n), "hipMalloc base");
         for (int k = 0; k < slots; k++) {
-            S->slot_lb.push_back(S->d_buf + static_cast<size_t>(2 * k) * P->n);
-            S->slot_ub.push_back(S->d_buf + static_cast<size_t>(2 * k + 1) * P->n);
+            S->slot_lb.push_back(S->d_buf + 2 * k * S->n);
+            S->slot_ub.push_back(S->d_buf + (2 * k + 1) * S->n);
         }
         S->slot_ticket.assign(slots, -1);
+        // pinned staging for request and result vectors (up to 256 MiB each)
+        const int entries = static_cast<int>(
+            std::max<int64_t>(1, std::min<int64_t>(slots, (256LL << 20) / static_cast<int64_t>(16 * S->n))));
+        check(hipHostMalloc(reinterpret_cast<void **>(&S->inbox), sizeof(double) * 2 * S->n * entries, 0),
+              "hipHostMalloc inbox");
+        check(hipHostMalloc(reinterpret_cast<void **>(&S->outbox), sizeof(double) * 2 * S->n * entries, 0),
+              "hipHostMalloc outbox");
+        for (int e = entries - 1; e >= 0; e--) {
+            S->inbox_free.push_back(e);
+            S->outbox_free.push_back(e);
+        }
         // fresh status ring for the server's steps
         std::lock_guard<std::mutex> lock(P->run_mutex);
         hipLaunchKernelGGL(unit_reset_kernel, dim3(1), dim3(256), 0, P->stream, P->D, 0, P->d_params, 0, 1);
@@ -845,18 +885,49 @@ void server_destroy(Server *s) { delete s; }
 
 int server_slots(const Server *s) { return s->slots; }
 
+void server_set_base(Server *S, const double *lbs, const double *ubs) {
+    check(hipMemcpy(S->d_base, lbs, sizeof(double) * S->n, hipMemcpyHostToDevice), "H2D base");
+    check(hipMemcpy(S->d_base + S->n, ubs, sizeof(double) * S->n, hipMemcpyHostToDevice), "H2D base");
+    S->has_base = true;
+}
+
 int64_t server_submit(Server *S, const double *lbs, const double *ubs, const int *hints, int n_hints,
                       int max_rounds) {
     if (max_rounds < 1 || max_rounds > kMaxRounds) throw std::runtime_error("max_rounds out of range");
     Server::Request q;
     q.ticket = S->next_ticket++;
-    q.lb.assign(lbs, lbs + S->P->n);
-    q.ub.assign(ubs, ubs + S->P->n);
+    if (!S->inbox_free.empty()) {
+        q.inbox = S->inbox_free.back();
+        S->inbox_free.pop_back();
+        std::memcpy(S->in_lb(q.inbox), lbs, sizeof(double) * S->n);
+        std::memcpy(S->in_ub(q.inbox), ubs, sizeof(double) * S->n);
+    } else {
+        q.lb.assign(lbs, lbs + S->n);
+        q.ub.assign(ubs, ubs + S->n);
+    }
     if (hints && n_hints > 0) {
         for (int k = 0; k < n_hints; k++)
-            if (hints[k] < 0 || hints[k] >= S->P->n) throw std::runtime_error("hint column out of range");
-        q.hints.assign(hints, hints + n_hints);
+            if (hints[k] < 0 || hints[k] >= static_cast<int>(S->n)) throw std::runtime_error("hint column out of range");
+        q.cols.assign(hints, hints + n_hints);
     }
+    q.max_rounds = max_rounds;
+    S->queue.push_back(std::move(q));
+    return S->queue.back().ticket;
+}
+
+int64_t server_submit_delta(Server *S, int n_changes, const int *cols, const double *lbs, const double *ubs,
+                            int max_rounds) {
+    if (!S->has_base) throw std::runtime_error("no base vector set");
+    if (max_rounds < 1 || max_rounds > kMaxRounds) throw std::runtime_error("max_rounds out of range");
+    if (n_changes < 0 || (n_changes > 0 && (!cols || !lbs || !ubs))) throw std::runtime_error("bad changes");
+    Server::Request q;
+    q.ticket = S->next_ticket++;
+    q.delta = true;
+    for (int k = 0; k < n_changes; k++)
+        if (cols[k] < 0 || cols[k] >= static_cast<int>(S->n)) throw std::runtime_error("changed column out of range");
+    q.cols.assign(cols, cols + n_changes);
+    q.dlb.assign(lbs, lbs + n_changes);
+    q.dub.assign(ubs, ubs + n_changes);
     q.max_rounds = max_rounds;
     S->queue.push_back(std::move(q));
     return S->queue.back().ticket;
@@ -868,37 +939,91 @@ int server_pending(const Server *S) {
     return busy + static_cast<int>(S->queue.size());
 }
 
+const std::vector<int64_t> &server_completed(const Server *S) { return S->last_completed; }
+
+namespace {
+
+template <typename T>
+void grow(T *&p, size_t count) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    check(hipMalloc(&p, sizeof(T) * std::max<size_t>(1, count)), "hipMalloc");
+}
+
+}  // namespace
+
 int server_step(Server *S) {
     Problem *P = S->P;
     std::lock_guard<std::mutex> lock(P->run_mutex);
+    S->last_completed.clear();
     hipStream_t s = P->stream;
-    const int n = P->n;
-    // admit queued requests into free slots; they start in this step
-    std::vector<int> adm;
+    const size_t n = S->n;
+    // ---- admit queued requests into free slots; they start in this step ----
+    std::vector<int> adm, caps, hint_ptr(1, 0), hints, used_inbox;
     std::vector<double *> adm_lb, adm_ub;
-    std::vector<int> hint_ptr(1, 0), hints, caps;
-    std::vector<int64_t> adm_ticket;
+    std::vector<double *> del_lb, del_ub;
+    std::vector<int> dptr(1, 0), dcols;
+    std::vector<double> dlv, duv;
+    std::vector<std::vector<double>> keep;  // pageable request vectors in flight
     for (int k = 0; k < S->slots && !S->queue.empty(); k++) {
         if (S->slot_ticket[k] >= 0) continue;
         Server::Request &q = S->queue.front();
-        check(hipMemcpyAsync(S->slot_lb[k], q.lb.data(), n * sizeof(double), hipMemcpyHostToDevice, s), "H2D");
-        check(hipMemcpyAsync(S->slot_ub[k], q.ub.data(), n * sizeof(double), hipMemcpyHostToDevice, s), "H2D");
+        if (q.delta) {
+            check(hipMemcpyAsync(S->slot_lb[k], S->d_base, n * sizeof(double), hipMemcpyDeviceToDevice, s), "D2D");
+            check(hipMemcpyAsync(S->slot_ub[k], S->d_base + n, n * sizeof(double), hipMemcpyDeviceToDevice, s),
+                  "D2D");
+            del_lb.push_back(S->slot_lb[k]);
+            del_ub.push_back(S->slot_ub[k]);
+            dcols.insert(dcols.end(), q.cols.begin(), q.cols.end());
+            dlv.insert(dlv.end(), q.dlb.begin(), q.dlb.end());
+            duv.insert(duv.end(), q.dub.begin(), q.dub.end());
+            dptr.push_back(static_cast<int>(dcols.size()));
+        } else if (q.inbox >= 0) {
+            check(hipMemcpyAsync(S->slot_lb[k], S->in_lb(q.inbox), n * sizeof(double), hipMemcpyHostToDevice, s),
+                  "H2D");
+            check(hipMemcpyAsync(S->slot_ub[k], S->in_ub(q.inbox), n * sizeof(double), hipMemcpyHostToDevice, s),
+                  "H2D");
+            used_inbox.push_back(q.inbox);
+        } else {
+            check(hipMemcpyAsync(S->slot_lb[k], q.lb.data(), n * sizeof(double), hipMemcpyHostToDevice, s), "H2D");
+            check(hipMemcpyAsync(S->slot_ub[k], q.ub.data(), n * sizeof(double), hipMemcpyHostToDevice, s), "H2D");
+            keep.push_back(std::move(q.lb));
+            keep.push_back(std::move(q.ub));
+        }
         adm.push_back(k);
         adm_lb.push_back(S->slot_lb[k]);
         adm_ub.push_back(S->slot_ub[k]);
-        hints.insert(hints.end(), q.hints.begin(), q.hints.end());
+        hints.insert(hints.end(), q.cols.begin(), q.cols.end());
         hint_ptr.push_back(static_cast<int>(hints.size()));
         caps.push_back(q.max_rounds);
-        adm_ticket.push_back(q.ticket);
-        // the request's host vectors must live until the copies finish
-        check(hipStreamSynchronize(s), "admit sync");
         S->slot_ticket[k] = q.ticket;
         S->queue.pop_front();
+    }
+    if (!del_lb.empty()) {
+        const size_t nd = del_lb.size(), nc = dcols.size();
+        if (nc > S->dcap || !S->d_dl) {
+            S->dcap = std::max<size_t>(nc, 4096);
+            grow(S->d_dcols, S->dcap);
+            grow(S->d_dlb, S->dcap);
+            grow(S->d_dub, S->dcap);
+            grow(S->d_dptr, static_cast<size_t>(S->slots) + 1);
+            grow(S->d_dl, static_cast<size_t>(S->slots));
+            grow(S->d_du, static_cast<size_t>(S->slots));
+        }
+        upload(S->d_dptr, dptr.data(), nd + 1, s);
+        upload(S->d_dcols, dcols.data(), nc, s);
+        upload(S->d_dlb, dlv.data(), nc, s);
+        upload(S->d_dub, duv.data(), nc, s);
+        upload(S->d_dl, del_lb.data(), nd, s);
+        upload(S->d_du, del_ub.data(), nd, s);
+        hipLaunchKernelGGL(delta_kernel, dim3(1, std::max(1, std::min(static_cast<int>(nd), 1024))), dim3(256), 0, s,
+                           static_cast<int>(nd), S->d_dl, S->d_du, S->d_dptr, S->d_dcols, S->d_dlb, S->d_dub);
+        check(hipGetLastError(), "delta launch");
     }
     if (!adm.empty())
         load_units(P, static_cast<int>(adm.size()), adm.data(), adm_lb.data(), adm_ub.data(), hint_ptr.data(),
                    hints.data(), caps.data(), S->step, false);
-    // active slots advance one round
+    // ---- every active slot advances one round ----
     std::vector<int> active;
     for (int k = 0; k < S->slots; k++)
         if (S->slot_ticket[k] >= 0) active.push_back(k);
@@ -907,11 +1032,13 @@ int server_step(Server *S) {
     P->D.n_active = static_cast<int>(active.size());
     launch_step(P, S->step, 0, S->step);
     check(hipGetLastError(), "step launch");
-    std::vector<UnitState> st = read_units(P, static_cast<int>(active.size()), active.data());
+    std::vector<UnitState> st = read_units(P, static_cast<int>(active.size()), active.data());  // synchronizes
+    for (int e : used_inbox) S->inbox_free.push_back(e);
+    keep.clear();
     int err = 0;
     check(hipMemcpy(&err, P->D.global + kGlobalError, sizeof(int), hipMemcpyDeviceToHost), "D2H");
     if (err) throw std::runtime_error("device wait timed out");
-    // retire finished slots
+    // ---- retire finished slots: unpack and copy the results out ----
     std::vector<int> fin;
     std::vector<RunResult> fin_r;
     for (size_t k = 0; k < active.size(); k++) {
@@ -933,19 +1060,33 @@ int server_step(Server *S) {
         upload(P->d_ubp, ubp.data(), fin.size(), s);
         const int gx = std::max(1, std::min(1024, (P->M.n + 255) / 256));
         hipLaunchKernelGGL(unpack_kernel, dim3(gx, std::max(1, std::min(static_cast<int>(fin.size()), 1024))),
-                           dim3(256), 0, s, P->M, P->D, static_cast<int>(fin.size()), P->d_list,
-                           P->d_lbp, P->d_ubp);
+                           dim3(256), 0, s, P->M, P->D, static_cast<int>(fin.size()), P->d_list, P->d_lbp, P->d_ubp);
         check(hipGetLastError(), "unpack launch");
+        std::vector<Server::Done> out(fin.size());
         for (size_t q = 0; q < fin.size(); q++) {
             const int k = fin[q];
-            Server::Done d;
-            d.lb.resize(n);
-            d.ub.resize(n);
+            Server::Done &d = out[q];
             d.r = fin_r[q];
-            check(hipMemcpyAsync(d.lb.data(), S->slot_lb[k], n * sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
-            check(hipMemcpyAsync(d.ub.data(), S->slot_ub[k], n * sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
-            check(hipStreamSynchronize(s), "retire sync");
-            S->done[S->slot_ticket[k]] = std::move(d);
+            double *hl, *hu;
+            if (!S->outbox_free.empty()) {
+                d.outbox = S->outbox_free.back();
+                S->outbox_free.pop_back();
+                hl = S->out_lb(d.outbox);
+                hu = S->out_ub(d.outbox);
+            } else {
+                d.lb.resize(n);
+                d.ub.resize(n);
+                hl = d.lb.data();
+                hu = d.ub.data();
+            }
+            check(hipMemcpyAsync(hl, S->slot_lb[k], n * sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
+            check(hipMemcpyAsync(hu, S->slot_ub[k], n * sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
+        }
+        check(hipStreamSynchronize(s), "retire sync");
+        for (size_t q = 0; q < fin.size(); q++) {
+            const int k = fin[q];
+            S->done[S->slot_ticket[k]] = std::move(out[q]);
+            S->last_completed.push_back(S->slot_ticket[k]);
             S->slot_ticket[k] = -1;
         }
     }
@@ -956,9 +1097,16 @@ int server_step(Server *S) {
 bool server_result(Server *S, int64_t ticket, double *lbs, double *ubs, RunResult *r) {
     auto it = S->done.find(ticket);
     if (it == S->done.end()) return false;
-    std::memcpy(lbs, it->second.lb.data(), sizeof(double) * S->P->n);
-    std::memcpy(ubs, it->second.ub.data(), sizeof(double) * S->P->n);
-    *r = it->second.r;
+    Server::Done &d = it->second;
+    if (d.outbox >= 0) {
+        std::memcpy(lbs, S->out_lb(d.outbox), sizeof(double) * S->n);
+        std::memcpy(ubs, S->out_ub(d.outbox), sizeof(double) * S->n);
+        S->outbox_free.push_back(d.outbox);
+    } else {
+        std::memcpy(lbs, d.lb.data(), sizeof(double) * S->n);
+        std::memcpy(ubs, d.ub.data(), sizeof(double) * S->n);
+    }
+    *r = d.r;
     S->done.erase(it);
     return true;
 }

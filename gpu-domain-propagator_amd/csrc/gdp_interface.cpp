@@ -451,6 +451,27 @@ GDP_RETCODE gdpServerSubmit(gdp_server s, const double *lbs, const double *ubs, 
     }
 }
 
+GDP_RETCODE gdpServerSetBase(gdp_server s, const double *lbs, const double *ubs) {
+    if (!s || !lbs || !ubs) return GDP_ERROR;
+    try {
+        gdp::server_set_base(s->s, lbs, ubs);
+        return GDP_OKAY;
+    } catch (const std::exception &e) {
+        return fail("gdpServerSetBase", e);
+    }
+}
+
+GDP_RETCODE gdpServerSubmitDelta(gdp_server s, const int n_changes, const int *cols, const double *lbs,
+                                 const double *ubs, const int max_rounds, long long *ticket) {
+    if (!s || !ticket) return GDP_ERROR;
+    try {
+        *ticket = gdp::server_submit_delta(s->s, n_changes, cols, lbs, ubs, max_rounds);
+        return GDP_OKAY;
+    } catch (const std::exception &e) {
+        return fail("gdpServerSubmitDelta", e);
+    }
+}
+
 GDP_RETCODE gdpServerStep(gdp_server s, int *completed) {
     if (!s) return GDP_ERROR;
     try {
@@ -465,6 +486,15 @@ GDP_RETCODE gdpServerStep(gdp_server s, int *completed) {
 GDP_RETCODE gdpServerPending(gdp_server s, int *pending) {
     if (!s || !pending) return GDP_ERROR;
     *pending = gdp::server_pending(s->s);
+    return GDP_OKAY;
+}
+
+GDP_RETCODE gdpServerCompleted(gdp_server s, long long *tickets, const int cap, int *count) {
+    if (!s || !count || (cap > 0 && !tickets)) return GDP_ERROR;
+    const std::vector<int64_t> &c = gdp::server_completed(s->s);
+    const int n = static_cast<int>(c.size());
+    for (int k = 0; k < n && k < cap; k++) tickets[k] = c[k];
+    *count = n;
     return GDP_OKAY;
 }
 

@@ -1073,4 +1073,17 @@ __global__ __launch_bounds__(256) void unpack_kernel(DevMatrix M, DevState D, in
     }
 }
 
+// Writes the changed columns of delta requests into their bound vectors:
+// vector k gets lbs[cols[h]] = lv[h], ubs[cols[h]] = uv[h] for h in
+// [ptr[k], ptr[k + 1]) (caller's column order).
+__global__ __launch_bounds__(256) void delta_kernel(int n_list, double *const *lbp, double *const *ubp,
+                                                    const int *ptr, const int *cols, const double *lv,
+                                                    const double *uv) {
+    for (int k = blockIdx.y; k < n_list; k += gridDim.y)
+        for (int h = ptr[k] + blockIdx.x * blockDim.x + threadIdx.x; h < ptr[k + 1]; h += gridDim.x * blockDim.x) {
+            lbp[k][cols[h]] = lv[h];
+            ubp[k][cols[h]] = uv[h];
+        }
+}
+
 }  // namespace gdp

@@ -65,8 +65,11 @@ EXT_SYMBOLS = (
     "gdpServerDestroy",
     "gdpServerSlots",
     "gdpServerSubmit",
+    "gdpServerSetBase",
+    "gdpServerSubmitDelta",
     "gdpServerStep",
     "gdpServerPending",
+    "gdpServerCompleted",
     "gdpServerResult",
     "gdpSequentialPropagate",
     "gdpOMPPropagate",
@@ -168,10 +171,16 @@ def load_library(path: Optional[str] = None) -> ctypes.CDLL:
     lib.gdpServerSlots.restype = c_int
     lib.gdpServerSubmit.argtypes = [vp, vp, vp, vp, c_int, c_int, POINTER(c_longlong)]
     lib.gdpServerSubmit.restype = c_int
+    lib.gdpServerSetBase.argtypes = [vp, vp, vp]
+    lib.gdpServerSetBase.restype = c_int
+    lib.gdpServerSubmitDelta.argtypes = [vp, c_int, vp, vp, vp, c_int, POINTER(c_longlong)]
+    lib.gdpServerSubmitDelta.restype = c_int
     lib.gdpServerStep.argtypes = [vp, POINTER(c_int)]
     lib.gdpServerStep.restype = c_int
     lib.gdpServerPending.argtypes = [vp, POINTER(c_int)]
     lib.gdpServerPending.restype = c_int
+    lib.gdpServerCompleted.argtypes = [vp, POINTER(c_longlong), c_int, POINTER(c_int)]
+    lib.gdpServerCompleted.restype = c_int
     lib.gdpServerResult.argtypes = [vp, c_longlong, vp, vp, POINTER(RunInfo), POINTER(c_int)]
     lib.gdpServerResult.restype = c_int
     lib.gdpSequentialPropagate.argtypes = [c_int, c_int, c_int, vp, vp, vp, vp, vp, vp, vp, vp, c_int,
@@ -572,6 +581,26 @@ class Server:
             raise RuntimeError(f"gdpServerSubmit failed: {last_error()}")
         return t.value
 
+    def set_base(self, lb, ub):
+        """Base vector of delta requests (a propagated fixpoint)."""
+        lb = _arr(lb, np.float64)
+        ub = _arr(ub, np.float64)
+        if self.lib.gdpServerSetBase(self.handle, _ptr(lb), _ptr(ub)) != GDP_OKAY:
+            raise RuntimeError(f"gdpServerSetBase failed: {last_error()}")
+        self.base = (lb.copy(), ub.copy())
+
+    def submit_delta(self, cols, lb_vals, ub_vals, max_rounds: int = MAX_NUM_ROUNDS) -> int:
+        """The base vector with columns `cols` set to (lb_vals, ub_vals)."""
+        c = _arr(cols, np.int32)
+        lv = _arr(lb_vals, np.float64)
+        uv = _arr(ub_vals, np.float64)
+        t = c_longlong()
+        rc = self.lib.gdpServerSubmitDelta(self.handle, len(c), _ptr(c), _ptr(lv), _ptr(uv), max_rounds,
+                                           ctypes.byref(t))
+        if rc != GDP_OKAY:
+            raise RuntimeError(f"gdpServerSubmitDelta failed: {last_error()}")
+        return t.value
+
     def step(self) -> int:
         c = c_int()
         if self.lib.gdpServerStep(self.handle, ctypes.byref(c)) != GDP_OKAY:
@@ -582,6 +611,15 @@ class Server:
         c = c_int()
         self.lib.gdpServerPending(self.handle, ctypes.byref(c))
         return c.value
+
+    def completed(self) -> List[int]:
+        """Tickets completed by the last step."""
+        cap = max(1, self.slots)
+        buf = (c_longlong * cap)()
+        cnt = c_int()
+        if self.lib.gdpServerCompleted(self.handle, buf, cap, ctypes.byref(cnt)) != GDP_OKAY:
+            raise RuntimeError(f"gdpServerCompleted failed: {last_error()}")
+        return [buf[i] for i in range(min(cnt.value, cap))]
 
     def result(self, ticket: int) -> Optional["Outcome"]:
         lb = np.empty(self.n, np.float64)

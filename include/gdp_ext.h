@@ -132,9 +132,20 @@ GDP_RETCODE gdpServerSlots(gdp_server s, int *slots);
  * gdpPropagateNodesDevice, may be NULL) and returns its ticket. */
 GDP_RETCODE gdpServerSubmit(gdp_server s, const double *lbs, const double *ubs, const int *hints,
                             const int n_hints, const int max_rounds, long long *ticket);
+/* Base vector of delta requests, kept in device memory: a propagated
+ * fixpoint, e.g. the root of a branch-and-bound tree. */
+GDP_RETCODE gdpServerSetBase(gdp_server s, const double *lbs, const double *ubs);
+/* A request given as the base vector with n_changes columns (cols) set to new
+ * bounds (lbs, ubs); no bound vector crosses PCIe, and round 1 processes only
+ * the rows of the changed columns. */
+GDP_RETCODE gdpServerSubmitDelta(gdp_server s, const int n_changes, const int *cols, const double *lbs,
+                                 const double *ubs, const int max_rounds, long long *ticket);
 /* One step: admit, one round of every running request, retire. */
 GDP_RETCODE gdpServerStep(gdp_server s, int *completed);
 GDP_RETCODE gdpServerPending(gdp_server s, int *pending);
+/* Tickets of the requests completed by the last gdpServerStep: up to cap of
+ * them into tickets, their number into *count. */
+GDP_RETCODE gdpServerCompleted(gdp_server s, long long *tickets, const int cap, int *count);
 /* *ready = 1 and the result copied out (and forgotten) if the request has
  * completed; *ready = 0 otherwise.  info->rounds is -1 for a request whose
  * bounds left the problem's bound envelope. */

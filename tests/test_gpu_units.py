@@ -182,3 +182,36 @@ def test_server_budget_limits_slots(gpu):
             I.Server(p, hbm_budget=per // 2, max_slots=4)
     finally:
         p.close()
+
+
+def test_server_delta_requests(gpu, oracle):
+    """Requests given as changes to a device-resident base (the propagated
+    root) give the same bits as the full vectors and the oracle; the
+    completed-ticket list names every request once."""
+    inst = small_mip(seed=9, int_frac=0.8)
+    base = oracle.jacobi(inst)
+    nodes = node_vectors(inst, oracle, 30, seed=3)
+    p = I.Problem(inst)
+    try:
+        srv = I.Server(p, hbm_budget=1 << 30, max_slots=8)
+        srv.set_base(base[0], base[1])
+        tickets = {}
+        seen = []
+        for k, (lb, ub, cols) in enumerate(nodes):
+            if k % 3 == 0:
+                tickets[k] = srv.submit(lb, ub, hints=cols)
+            else:
+                tickets[k] = srv.submit_delta(cols, lb[cols], ub[cols])
+            if k % 4 == 3:
+                srv.step()
+                seen += srv.completed()
+        while srv.pending():
+            srv.step()
+            seen += srv.completed()
+        assert sorted(seen) == sorted(tickets.values())
+        for k, t in tickets.items():
+            got = srv.result(t)
+            assert_same(got, oracle.jacobi(inst, lb=nodes[k][0], ub=nodes[k][1]), f"request {k}")
+        srv.close()
+    finally:
+        p.close()
