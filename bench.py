@@ -148,7 +148,7 @@ def batched_leg(sync, check_cpu):
         # within 1e-9 relative or absolute)
         import oracle_binding
         o = b.n_offsets
-        exact, bad_int, worst, feas, compared, capped = 0, 0, 0.0, 0, 0, 0
+        exact, bad_int, worst, feas, compared, capped, outside = 0, 0, 0.0, 0, 0, 0, 0
         for i, x in enumerate(insts):
             gl, gu = out_l[o[i]:o[i + 1]], out_u[o[i]:o[i + 1]]
             jac = oracle_binding.jacobi(x)
@@ -165,10 +165,12 @@ def batched_leg(sync, check_cpu):
             for g, sv in ((gl, seq[0]), (gu, seq[1])):
                 d = np.abs(g - sv)[~is_int] / np.maximum(1.0, np.abs(sv[~is_int]))
                 worst = max(worst, float(d.max()) if d.size else 0.0)
+                outside += int(np.sum(d > 1e-9))
         res["parity"] = {"bitwise_equal_to_oracle": exact, "of": len(insts),
                          "vs_cpu_seq": {"compared": compared, "capped_at_100_rounds": capped,
                                         "integer_bound_mismatches": bad_int,
                                         "max_continuous_rel_or_abs_dev": worst,
+                                        "continuous_bounds_beyond_1e-9": outside,
                                         "infeasibility_agrees": feas == len(insts)}}
     b.close()
     return res
@@ -223,7 +225,8 @@ def serving_leg(sync):
                     tnow = time.perf_counter() - start
                     for tk in srv.completed():
                         done[idx[tk]] = tnow
-                        srv.result(tk)
+                        # full vectors back for full requests, changed columns for delta requests
+                        srv.result(tk) if mode == "full" else srv.result_changes(tk)
                         got += 1
                 elif k < N:
                     while time.perf_counter() - start < arr[k]:

@@ -139,6 +139,11 @@ const std::vector<int64_t> &server_completed(const Server *s);
 // Completed request: copies its bounds and outcome and forgets it; false if
 // the ticket is not (yet) complete.
 bool server_result(Server *s, int64_t ticket, double *lbs, double *ubs, RunResult *r);
+// Completed request as the columns whose bounds differ from the base vector
+// (bit for bit) with their bounds; 1 if it was complete, 0 otherwise.
+// Delta requests are returned this way without copying full vectors.
+int server_result_changes(Server *s, int64_t ticket, std::vector<int> &cols, std::vector<double> &lbs,
+                          std::vector<double> &ubs, RunResult *r);
 
 std::string last_error();
 void set_last_error(const std::string &msg);

@@ -40,6 +40,8 @@ __global__ void unpack_kernel(DevMatrix M, DevState D, int n_list, const int *li
                               double *const *ubp);
 __global__ void delta_kernel(int n_list, double *const *lbp, double *const *ubp, const int *ptr, const int *cols,
                              const double *lv, const double *uv);
+__global__ void diff_kernel(int n, int n_list, double *const *lbp, double *const *ubp, const double *base_l,
+                            const double *base_u, int cap, int *cnt, int *cols, double *vl, double *vu);
 
 namespace {
 
@@ -802,9 +804,13 @@ struct Server {
     };
     struct Done {
         int outbox = -1;                 // pinned outbox entry holding the result, or -1
-        std::vector<double> lb, ub;
+        std::vector<double> lb, ub;      // the result (full vector), or empty
+        bool changes = false;            // the result as columns that differ from the base
+        std::vector<int> cols;
+        std::vector<double> clb, cub;
         RunResult r;
     };
+    static constexpr int kDiffCap = 4096;  // changed columns returned without the full vector
     Problem *P = nullptr;
     int slots = 0;
     size_t n = 0;
@@ -817,6 +823,14 @@ struct Server {
     double *d_buf = nullptr;             // slots x (lb, ub) bound vectors
     double *d_base = nullptr;            // base vector of delta requests
     bool has_base = false;
+    std::vector<double> base_lb, base_ub;  // host copy of the base
+    std::vector<char> slot_delta;        // slot holds a delta request
+    // changed columns of finished delta requests (device and pinned host)
+    int *d_ccnt = nullptr, *d_ccols = nullptr;
+    double *d_cl = nullptr, *d_cu = nullptr;
+    int *h_ccnt = nullptr, *h_ccols = nullptr;
+    double *h_cl = nullptr, *h_cu = nullptr;
+    int diff_slots = 0;
     std::vector<double *> slot_lb, slot_ub;
     double *inbox = nullptr, *outbox = nullptr;  // pinned host memory
     std::vector<int> inbox_free, outbox_free;
@@ -830,6 +844,12 @@ struct Server {
         if (d_base) (void)hipFree(d_base);
         if (inbox) (void)hipHostFree(inbox);
         if (outbox) (void)hipHostFree(outbox);
+        for (void *q : {static_cast<void *>(h_ccnt), static_cast<void *>(h_ccols), static_cast<void *>(h_cl),
+                        static_cast<void *>(h_cu)})
+            if (q) (void)hipHostFree(q);
+        for (void *q : {static_cast<void *>(d_ccnt), static_cast<void *>(d_ccols), static_cast<void *>(d_cl),
+                        static_cast<void *>(d_cu)})
+            if (q) (void)hipFree(q);
         for (void *q : {static_cast<void *>(d_dptr), static_cast<void *>(d_dcols), static_cast<void *>(d_dlb),
                         static_cast<void *>(d_dub), static_cast<void *>(d_dl), static_cast<void *>(d_du)})
             if (q) (void)hipFree(q);
@@ -859,6 +879,18 @@ Server *server_create(Problem *P, int64_t hbm_budget, int max_slots) {
             S->slot_ub.push_back(S->d_buf + (2 * k + 1) * S->n);
         }
         S->slot_ticket.assign(slots, -1);
+        S->slot_delta.assign(slots, 0);
+        // changed-column regions for finished delta requests (kDiffCap each)
+        S->diff_slots = slots;
+        const size_t dc = static_cast<size_t>(slots) * Server::kDiffCap;
+        check(hipMalloc(&S->d_ccnt, sizeof(int) * slots), "hipMalloc");
+        check(hipMalloc(&S->d_ccols, sizeof(int) * dc), "hipMalloc");
+        check(hipMalloc(&S->d_cl, sizeof(double) * dc), "hipMalloc");
+        check(hipMalloc(&S->d_cu, sizeof(double) * dc), "hipMalloc");
+        check(hipHostMalloc(reinterpret_cast<void **>(&S->h_ccnt), sizeof(int) * slots, 0), "hipHostMalloc");
+        check(hipHostMalloc(reinterpret_cast<void **>(&S->h_ccols), sizeof(int) * dc, 0), "hipHostMalloc");
+        check(hipHostMalloc(reinterpret_cast<void **>(&S->h_cl), sizeof(double) * dc, 0), "hipHostMalloc");
+        check(hipHostMalloc(reinterpret_cast<void **>(&S->h_cu), sizeof(double) * dc, 0), "hipHostMalloc");
         // pinned staging for request and result vectors (up to 256 MiB each)
         const int entries = static_cast<int>(
             std::max<int64_t>(1, std::min<int64_t>(slots, (256LL << 20) / static_cast<int64_t>(16 * S->n))));
@@ -888,6 +920,8 @@ int server_slots(const Server *s) { return s->slots; }
 void server_set_base(Server *S, const double *lbs, const double *ubs) {
     check(hipMemcpy(S->d_base, lbs, sizeof(double) * S->n, hipMemcpyHostToDevice), "H2D base");
     check(hipMemcpy(S->d_base + S->n, ubs, sizeof(double) * S->n, hipMemcpyHostToDevice), "H2D base");
+    S->base_lb.assign(lbs, lbs + S->n);
+    S->base_ub.assign(ubs, ubs + S->n);
     S->has_base = true;
 }
 
@@ -997,6 +1031,7 @@ int server_step(Server *S) {
         hint_ptr.push_back(static_cast<int>(hints.size()));
         caps.push_back(q.max_rounds);
         S->slot_ticket[k] = q.ticket;
+        S->slot_delta[k] = q.delta ? 1 : 0;
         S->queue.pop_front();
     }
     if (!del_lb.empty()) {
@@ -1063,10 +1098,46 @@ int server_step(Server *S) {
                            dim3(256), 0, s, P->M, P->D, static_cast<int>(fin.size()), P->d_list, P->d_lbp, P->d_ubp);
         check(hipGetLastError(), "unpack launch");
         std::vector<Server::Done> out(fin.size());
+        // delta requests: the columns that differ from the base, on the device
+        std::vector<int> dq;  // indices into fin of delta requests
+        std::vector<double *> dl, du;
+        for (size_t q = 0; q < fin.size(); q++)
+            if (S->slot_delta[fin[q]]) {
+                dq.push_back(static_cast<int>(q));
+                dl.push_back(S->slot_lb[fin[q]]);
+                du.push_back(S->slot_ub[fin[q]]);
+            }
+        if (!dq.empty()) {
+            const int nd = static_cast<int>(dq.size());
+            upload(P->d_lbp, dl.data(), nd, s);
+            upload(P->d_ubp, du.data(), nd, s);
+            check(hipMemsetAsync(S->d_ccnt, 0, sizeof(int) * nd, s), "memset");
+            hipLaunchKernelGGL(diff_kernel, dim3(gx, std::max(1, std::min(nd, 1024))), dim3(256), 0, s,
+                               static_cast<int>(n), nd, P->d_lbp, P->d_ubp, S->d_base, S->d_base + n,
+                               Server::kDiffCap, S->d_ccnt, S->d_ccols, S->d_cl, S->d_cu);
+            check(hipGetLastError(), "diff launch");
+            check(hipMemcpyAsync(S->h_ccnt, S->d_ccnt, sizeof(int) * nd, hipMemcpyDeviceToHost, s), "D2H");
+            const size_t dc = static_cast<size_t>(nd) * Server::kDiffCap;
+            check(hipMemcpyAsync(S->h_ccols, S->d_ccols, sizeof(int) * dc, hipMemcpyDeviceToHost, s), "D2H");
+            check(hipMemcpyAsync(S->h_cl, S->d_cl, sizeof(double) * dc, hipMemcpyDeviceToHost, s), "D2H");
+            check(hipMemcpyAsync(S->h_cu, S->d_cu, sizeof(double) * dc, hipMemcpyDeviceToHost, s), "D2H");
+            check(hipStreamSynchronize(s), "diff sync");
+            for (int j = 0; j < nd; j++) {
+                const int c = S->h_ccnt[j];
+                if (c > Server::kDiffCap) continue;  // too many: the full vector below
+                Server::Done &d = out[dq[j]];
+                d.changes = true;
+                const size_t o = static_cast<size_t>(j) * Server::kDiffCap;
+                d.cols.assign(S->h_ccols + o, S->h_ccols + o + c);
+                d.clb.assign(S->h_cl + o, S->h_cl + o + c);
+                d.cub.assign(S->h_cu + o, S->h_cu + o + c);
+            }
+        }
         for (size_t q = 0; q < fin.size(); q++) {
             const int k = fin[q];
             Server::Done &d = out[q];
             d.r = fin_r[q];
+            if (d.changes) continue;
             double *hl, *hu;
             if (!S->outbox_free.empty()) {
                 d.outbox = S->outbox_free.back();
@@ -1098,7 +1169,14 @@ bool server_result(Server *S, int64_t ticket, double *lbs, double *ubs, RunResul
     auto it = S->done.find(ticket);
     if (it == S->done.end()) return false;
     Server::Done &d = it->second;
-    if (d.outbox >= 0) {
+    if (d.changes) {
+        std::memcpy(lbs, S->base_lb.data(), sizeof(double) * S->n);
+        std::memcpy(ubs, S->base_ub.data(), sizeof(double) * S->n);
+        for (size_t q = 0; q < d.cols.size(); q++) {
+            lbs[d.cols[q]] = d.clb[q];
+            ubs[d.cols[q]] = d.cub[q];
+        }
+    } else if (d.outbox >= 0) {
         std::memcpy(lbs, S->out_lb(d.outbox), sizeof(double) * S->n);
         std::memcpy(ubs, S->out_ub(d.outbox), sizeof(double) * S->n);
         S->outbox_free.push_back(d.outbox);
@@ -1109,6 +1187,35 @@ bool server_result(Server *S, int64_t ticket, double *lbs, double *ubs, RunResul
     *r = d.r;
     S->done.erase(it);
     return true;
+}
+
+int server_result_changes(Server *S, int64_t ticket, std::vector<int> &cols, std::vector<double> &lbs,
+                          std::vector<double> &ubs, RunResult *r) {
+    auto it = S->done.find(ticket);
+    if (it == S->done.end()) return 0;
+    Server::Done &d = it->second;
+    if (!S->has_base) throw std::runtime_error("no base vector set");
+    cols.clear();
+    lbs.clear();
+    ubs.clear();
+    if (d.changes) {
+        cols = std::move(d.cols);
+        lbs = std::move(d.clb);
+        ubs = std::move(d.cub);
+    } else {
+        const double *fl = d.outbox >= 0 ? S->out_lb(d.outbox) : d.lb.data();
+        const double *fu = d.outbox >= 0 ? S->out_ub(d.outbox) : d.ub.data();
+        for (size_t j = 0; j < S->n; j++)
+            if (std::memcmp(&fl[j], &S->base_lb[j], sizeof(double)) || std::memcmp(&fu[j], &S->base_ub[j], sizeof(double))) {
+                cols.push_back(static_cast<int>(j));
+                lbs.push_back(fl[j]);
+                ubs.push_back(fu[j]);
+            }
+        if (d.outbox >= 0) S->outbox_free.push_back(d.outbox);
+    }
+    *r = d.r;
+    S->done.erase(it);
+    return 1;
 }
 
 }  // namespace gdp

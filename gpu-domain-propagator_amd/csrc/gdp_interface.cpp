@@ -4,6 +4,7 @@
 // Error behaviour follows src/interface.cu:7-16 (an exception becomes
 // GDP_ERROR and a message on stderr), except that device errors never
 // terminate the process (the reference exit()s, src/cuda_def.cuh:8-15).
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -496,6 +497,29 @@ GDP_RETCODE gdpServerCompleted(gdp_server s, long long *tickets, const int cap, 
     for (int k = 0; k < n && k < cap; k++) tickets[k] = c[k];
     *count = n;
     return GDP_OKAY;
+}
+
+GDP_RETCODE gdpServerResultChanges(gdp_server s, const long long ticket, int *cols, double *lbs, double *ubs,
+                                   const int cap, int *count, gdp_run_info *info, int *ready) {
+    if (!s || !count || !ready || cap < 0 || (cap > 0 && (!cols || !lbs || !ubs))) return GDP_ERROR;
+    try {
+        std::vector<int> c;
+        std::vector<double> l, u;
+        RunResult r;
+        *ready = gdp::server_result_changes(s->s, ticket, c, l, u, &r);
+        *count = 0;
+        if (*ready) {
+            *count = static_cast<int>(c.size());
+            const size_t k = std::min<size_t>(c.size(), static_cast<size_t>(cap));
+            std::copy(c.begin(), c.begin() + k, cols);
+            std::copy(l.begin(), l.begin() + k, lbs);
+            std::copy(u.begin(), u.begin() + k, ubs);
+            fill_info(info, r.rounds, r.infeasible, r.capped, r.kernel_ms);
+        }
+        return GDP_OKAY;
+    } catch (const std::exception &e) {
+        return fail("gdpServerResultChanges", e);
+    }
 }
 
 GDP_RETCODE gdpServerResult(gdp_server s, const long long ticket, double *lbs, double *ubs, gdp_run_info *info,

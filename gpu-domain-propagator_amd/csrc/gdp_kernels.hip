@@ -1086,4 +1086,30 @@ __global__ __launch_bounds__(256) void delta_kernel(int n_list, double *const *l
         }
 }
 
+// Columns in which vector k (lbp[k], ubp[k]) differs from the base vector,
+// bit for bit: their indices and bounds go to region k (capacity cap) of
+// cols / vl / vu, their number to cnt[k] (which may exceed cap).  cnt must be
+// zero on entry.
+__global__ __launch_bounds__(256) void diff_kernel(int n, int n_list, double *const *lbp, double *const *ubp,
+                                                   const double *base_l, const double *base_u, int cap, int *cnt,
+                                                   int *cols, double *vl, double *vu) {
+    for (int k = blockIdx.y; k < n_list; k += gridDim.y) {
+        const double *l = lbp[k];
+        const double *u = ubp[k];
+        for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+            const double a = l[j], b = u[j];
+            if (__double_as_longlong(a) == __double_as_longlong(base_l[j]) &&
+                __double_as_longlong(b) == __double_as_longlong(base_u[j]))
+                continue;
+            const int pos = atomic_add_i(&cnt[k], 1);
+            if (pos < cap) {
+                const size_t q = static_cast<size_t>(k) * cap + pos;
+                cols[q] = j;
+                vl[q] = a;
+                vu[q] = b;
+            }
+        }
+    }
+}
+
 }  // namespace gdp

@@ -70,6 +70,7 @@ EXT_SYMBOLS = (
     "gdpServerStep",
     "gdpServerPending",
     "gdpServerCompleted",
+    "gdpServerResultChanges",
     "gdpServerResult",
     "gdpSequentialPropagate",
     "gdpOMPPropagate",
@@ -179,6 +180,9 @@ def load_library(path: Optional[str] = None) -> ctypes.CDLL:
     lib.gdpServerStep.restype = c_int
     lib.gdpServerPending.argtypes = [vp, POINTER(c_int)]
     lib.gdpServerPending.restype = c_int
+    lib.gdpServerResultChanges.argtypes = [vp, c_longlong, vp, vp, vp, c_int, POINTER(c_int), POINTER(RunInfo),
+                                           POINTER(c_int)]
+    lib.gdpServerResultChanges.restype = c_int
     lib.gdpServerCompleted.argtypes = [vp, POINTER(c_longlong), c_int, POINTER(c_int)]
     lib.gdpServerCompleted.restype = c_int
     lib.gdpServerResult.argtypes = [vp, c_longlong, vp, vp, POINTER(RunInfo), POINTER(c_int)]
@@ -630,6 +634,23 @@ class Server:
                                     ctypes.byref(ready)) != GDP_OKAY:
             raise RuntimeError(f"gdpServerResult failed: {last_error()}")
         return _outcome(lb, ub, info) if ready.value else None
+
+    def result_changes(self, ticket: int):
+        """Completed request as (columns, lbs, ubs, RunInfo): the columns
+        whose bounds differ from the base vector; None if not complete."""
+        cols = np.empty(self.n, np.int32)
+        lb = np.empty(self.n, np.float64)
+        ub = np.empty(self.n, np.float64)
+        cnt = c_int()
+        info = RunInfo()
+        ready = c_int()
+        if self.lib.gdpServerResultChanges(self.handle, ticket, _ptr(cols), _ptr(lb), _ptr(ub), self.n,
+                                           ctypes.byref(cnt), ctypes.byref(info), ctypes.byref(ready)) != GDP_OKAY:
+            raise RuntimeError(f"gdpServerResultChanges failed: {last_error()}")
+        if not ready.value:
+            return None
+        k = cnt.value
+        return cols[:k].copy(), lb[:k].copy(), ub[:k].copy(), info
 
     def close(self):
         if getattr(self, "handle", None):

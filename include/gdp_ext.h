@@ -152,6 +152,14 @@ GDP_RETCODE gdpServerCompleted(gdp_server s, long long *tickets, const int cap, 
 GDP_RETCODE gdpServerResult(gdp_server s, const long long ticket, double *lbs, double *ubs,
                             gdp_run_info *info, int *ready);
 
+/* Completed request as the columns whose bounds differ (bit for bit) from the
+ * base vector of gdpServerSetBase, with their new bounds: *count of them, of
+ * which min(*count, cap) are copied (cap = n_vars always suffices).  Delta
+ * requests are returned this way without copying full vectors.  *ready as in
+ * gdpServerResult. */
+GDP_RETCODE gdpServerResultChanges(gdp_server s, const long long ticket, int *cols, double *lbs, double *ubs,
+                                   const int cap, int *count, gdp_run_info *info, int *ready);
+
 /* Host propagators with run information: cpu_seq (Gauss-Seidel), cpu_seq_dis
  * and cpu_omp (threads <= 0: GDP_OMP_THREADS or 8). */
 GDP_RETCODE gdpSequentialPropagate(const int n_cons, const int n_vars, const int nnz,

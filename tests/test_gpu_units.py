@@ -210,8 +210,19 @@ def test_server_delta_requests(gpu, oracle):
             seen += srv.completed()
         assert sorted(seen) == sorted(tickets.values())
         for k, t in tickets.items():
-            got = srv.result(t)
-            assert_same(got, oracle.jacobi(inst, lb=nodes[k][0], ub=nodes[k][1]), f"request {k}")
+            exp = oracle.jacobi(inst, lb=nodes[k][0], ub=nodes[k][1])
+            if k % 2:
+                got = srv.result(t)
+                assert_same(got, exp, f"request {k}")
+            else:  # as changes against the base vector
+                cols, cl, cu, info = srv.result_changes(t)
+                lb, ub = base[0].copy(), base[1].copy()
+                lb[cols] = cl
+                ub[cols] = cu
+                np.testing.assert_array_equal(lb, exp[0], err_msg=f"request {k}")
+                np.testing.assert_array_equal(ub, exp[1], err_msg=f"request {k}")
+                assert info.rounds == exp[2]
+            assert srv.result(t) is None
         srv.close()
     finally:
         p.close()
