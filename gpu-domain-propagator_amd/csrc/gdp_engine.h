@@ -108,6 +108,19 @@ void nodes_propagate_device(Problem *p, int count, double *const *d_lbs, double 
 void batch_propagate_device(Problem *p, double *d_lbs, double *d_ubs, int sync_type, int max_rounds,
                             RunResult *out);
 
+// Autotuning: builds the problem (one instance, or a batch of several) with
+// candidate tunings and times the propagation of lbs/ubs (host, the
+// instances' vectors concatenated) to the fixpoint; a coordinate search over
+// the work-item size, the lanes per row of each tree-width bin, the
+// persistent grid size (MEGAKERNEL) and row marking, starting from `start`
+// (may be null).  Every trial is bit-identical; only the time differs.
+struct AutotuneTrial {
+    Tuning tuning;
+    double kernel_ms = 0.0;
+};
+Tuning autotune(const std::vector<InstanceView> &instances, const double *lbs, const double *ubs, int sync_type,
+                const Tuning *start, std::vector<AutotuneTrial> *trials);
+
 // Serving: node requests against a device-resident matrix (replicas mode).
 // Requests are queued by submit; each step admits queued requests into free
 // slots (up to the slot count the HBM budget allows), advances every active

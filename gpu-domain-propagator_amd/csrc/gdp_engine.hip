@@ -789,6 +789,95 @@ void batch_propagate_device(Problem *P, double *d_lbs, double *d_ubs, int sync_t
 }
 
 // ---------------------------------------------------------------------------
+// autotuning
+// ---------------------------------------------------------------------------
+
+Tuning autotune(const std::vector<InstanceView> &instances, const double *lbs, const double *ubs, int sync_type,
+                const Tuning *start, std::vector<AutotuneTrial> *trials) {
+    if (instances.empty()) throw std::runtime_error("no instance");
+    const bool batch = instances.size() > 1;
+    long long n_total = 0;
+    for (const auto &v : instances) n_total += v.n;
+    // device copies of the bounds every trial starts from
+    double *d = nullptr;
+    check(hipMalloc(&d, sizeof(double) * 4 * n_total), "hipMalloc");
+    double *root_l = d, *root_u = d + n_total, *work_l = d + 2 * n_total, *work_u = d + 3 * n_total;
+    Tuning best = start ? *start : Tuning();
+    double best_ms = 1e300;
+    auto measure = [&](const Tuning &t) {
+        Problem *P = batch ? batch_create(instances, &t) : build({instances[0]}, true, &t);
+        double ms = 1e300;
+        try {
+            std::vector<RunResult> out(batch ? instances.size() : 1);
+            for (int rep = 0; rep < 3; rep++) {
+                check(hipMemcpy(work_l, root_l, sizeof(double) * n_total, hipMemcpyDeviceToDevice), "D2D");
+                check(hipMemcpy(work_u, root_u, sizeof(double) * n_total, hipMemcpyDeviceToDevice), "D2D");
+                check(hipDeviceSynchronize(), "sync");
+                if (batch) batch_propagate_device(P, work_l, work_u, sync_type, kMaxRounds, out.data());
+                else out[0] = problem_propagate_device(P, work_l, work_u, sync_type, kMaxRounds);
+                if (rep > 0) ms = std::min(ms, out[0].kernel_ms);  // the first run warms up
+            }
+        } catch (...) {
+            problem_destroy(P);
+            throw;
+        }
+        problem_destroy(P);
+        if (trials) {
+            AutotuneTrial tr;
+            tr.tuning = t;
+            tr.kernel_ms = ms;
+            trials->push_back(tr);
+        }
+        return ms;
+    };
+    try {
+        check(hipMemcpy(root_l, lbs, sizeof(double) * n_total, hipMemcpyHostToDevice), "H2D");
+        check(hipMemcpy(root_u, ubs, sizeof(double) * n_total, hipMemcpyHostToDevice), "H2D");
+        best_ms = measure(best);
+        // coordinate search: work-item size, lanes per row, persistent grid, row marking
+        const int items[] = {512, 1024, 2048, 4096};
+        const int lanes[3][10] = {{0, 0, 0, 1, 2, 3, 4, 5, 6, 6},   // 4 entries per lane
+                                  {0, 0, 1, 2, 3, 4, 5, 6, 6, 6},   // 2 entries per lane
+                                  {0, 1, 2, 3, 4, 5, 6, 6, 6, 6}};  // 1 entry per lane
+        auto try_one = [&](const Tuning &t) {
+            const double ms = measure(t);
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = t;
+            }
+        };
+        for (int v : items)
+            if (v != best.item_nnz) {
+                Tuning t = best;
+                t.item_nnz = v;
+                try_one(t);
+            }
+        for (const auto &g : lanes) {
+            Tuning t = best;
+            std::copy(g, g + 10, t.group_log);
+            if (!std::equal(g, g + 10, best.group_log)) try_one(t);
+        }
+        if (sync_type == MEGAKERNEL_ID)
+            for (int b : {2, 3, 4})
+                if (b != best.mega_blocks_per_cu) {
+                    Tuning t = best;
+                    t.mega_blocks_per_cu = b;
+                    try_one(t);
+                }
+        {
+            Tuning t = best;
+            t.row_marking = best.row_marking ? 0 : 1;
+            try_one(t);
+        }
+    } catch (...) {
+        (void)hipFree(d);
+        throw;
+    }
+    check(hipFree(d), "hipFree");
+    return best;
+}
+
+// ---------------------------------------------------------------------------
 // serving
 // ---------------------------------------------------------------------------
 

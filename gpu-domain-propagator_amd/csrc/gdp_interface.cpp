@@ -413,6 +413,82 @@ GDP_RETCODE gdpBatchPropagate(gdp_problem p, double *lbs, double *ubs, const int
     return rc;
 }
 
+namespace {
+
+void tuning_to(const gdp::Tuning &T, gdp_tuning *t) {
+    t->item_nnz = T.item_nnz;
+    t->mega_blocks_per_cu = T.mega_blocks_per_cu;
+    t->graph_rounds = T.graph_rounds;
+    t->max_split_segs = T.max_split_segs;
+    t->compact_bounds = T.compact_bounds;
+    t->row_marking = T.row_marking;
+    for (int b = 0; b < 10; b++) t->group_log[b] = T.group_log[b];
+}
+
+GDP_RETCODE autotune_views(const std::vector<gdp::InstanceView> &v, const double *lbs, const double *ubs,
+                           const int sync_type, gdp_tuning *tuning, double *best_ms, const char *who) {
+    try {
+        const gdp::Tuning start = tuning_from(tuning);
+        std::vector<gdp::AutotuneTrial> trials;
+        const gdp::Tuning best = gdp::autotune(v, lbs, ubs, sync_type, &start, &trials);
+        double ms = 0.0;
+        for (const auto &t : trials)
+            if (std::memcmp(&t.tuning, &best, sizeof(best)) == 0) ms = t.kernel_ms;
+        tuning_to(best, tuning);
+        if (best_ms) *best_ms = ms;
+        return GDP_OKAY;
+    } catch (const std::exception &e) {
+        return fail(who, e);
+    }
+}
+
+}  // namespace
+
+GDP_RETCODE gdpAutotune(const int n_cons, const int n_vars, const int nnz, const int *col_indices, const int *row_ptrs,
+                        const double *vals, const double *lhss, const double *rhss, const int *vartypes,
+                        const double *lbs, const double *ubs, const int sync_type, gdp_tuning *tuning,
+                        double *best_ms) {
+    if (!tuning || !lbs || !ubs) return GDP_ERROR;
+    gdp::InstanceView v;
+    v.m = n_cons;
+    v.n = n_vars;
+    v.nnz = nnz;
+    v.col = col_indices;
+    v.row_ptr = row_ptrs;
+    v.val = vals;
+    v.lhs = lhss;
+    v.rhs = rhss;
+    v.vartype = vartypes;
+    v.env_lb = lbs;
+    v.env_ub = ubs;
+    return autotune_views({v}, lbs, ubs, sync_type, tuning, best_ms, "gdpAutotune");
+}
+
+GDP_RETCODE gdpBatchAutotune(const int n_inst, const int *n_cons, const int *n_vars, const int *nnz,
+                             const int *const *col_indices, const int *const *row_ptrs, const double *const *vals,
+                             const double *const *lhss, const double *const *rhss, const int *const *vartypes,
+                             const double *lbs, const double *ubs, const int sync_type, gdp_tuning *tuning,
+                             double *best_ms) {
+    if (!tuning || !lbs || !ubs || n_inst < 1 || !n_cons || !n_vars || !nnz) return GDP_ERROR;
+    std::vector<gdp::InstanceView> v(n_inst);
+    size_t off = 0;
+    for (int i = 0; i < n_inst; i++) {
+        v[i].m = n_cons[i];
+        v[i].n = n_vars[i];
+        v[i].nnz = nnz[i];
+        v[i].col = col_indices[i];
+        v[i].row_ptr = row_ptrs[i];
+        v[i].val = vals[i];
+        v[i].lhs = lhss[i];
+        v[i].rhs = rhss[i];
+        v[i].vartype = vartypes[i];
+        v[i].env_lb = lbs + off;
+        v[i].env_ub = ubs + off;
+        off += static_cast<size_t>(n_vars[i]);
+    }
+    return autotune_views(v, lbs, ubs, sync_type, tuning, best_ms, "gdpBatchAutotune");
+}
+
 struct gdp_server_s {
     gdp::Server *s;
 };

@@ -6,6 +6,7 @@ and provides the synthetic workloads and an MPS reader.
 """
 from .instances import Instance  # noqa: F401
 from . import interface  # noqa: F401
+from .mps import parse_mps, read_mps  # noqa: F401
 from .interface import (  # noqa: F401
     CPU_LOOP, GPU_LOOP, MEGAKERNEL, GDP_OKAY, GDP_ERROR, GDP_NOTIMPLEMENTED, GDP_BINARY, GDP_INTEGER,
     GDP_CONTINUOUS, MAX_NUM_ROUNDS, GDP_INF, GDP_EPS, LIB_PATH, LEGACY_SYMBOLS, EXT_SYMBOLS, Problem, Outcome,

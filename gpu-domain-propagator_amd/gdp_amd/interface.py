@@ -58,6 +58,8 @@ EXT_SYMBOLS = (
     "gdpPropagateDevice",
     "gdpProblemReserve",
     "gdpPropagateNodesDevice",
+    "gdpAutotune",
+    "gdpBatchAutotune",
     "gdpBatchCreate",
     "gdpBatchPropagate",
     "gdpBatchPropagateDevice",
@@ -74,6 +76,8 @@ EXT_SYMBOLS = (
     "gdpServerResult",
     "gdpSequentialPropagate",
     "gdpOMPPropagate",
+    "gdpParseMPS",
+    "gdpMpsFree",
     "gdpDeviceCount",
     "gdpDeviceMalloc",
     "gdpDeviceFree",
@@ -100,6 +104,14 @@ class Tuning(ctypes.Structure):
         ("row_marking", c_int),
         ("group_log", c_int * 10),
     ]
+
+
+class MpsModel(ctypes.Structure):
+    _fields_ = [("n_cons", c_int), ("n_vars", c_int), ("nnz", c_int),
+                ("row_ptrs", POINTER(c_int)), ("col_indices", POINTER(c_int)),
+                ("vals", POINTER(c_double)), ("lhss", POINTER(c_double)), ("rhss", POINTER(c_double)),
+                ("lbs", POINTER(c_double)), ("ubs", POINTER(c_double)), ("vartypes", POINTER(c_int)),
+                ("name", ctypes.c_char * 128)]
 
 
 class ProblemStats(ctypes.Structure):
@@ -158,6 +170,12 @@ def load_library(path: Optional[str] = None) -> ctypes.CDLL:
     lib.gdpProblemReserve.restype = c_int
     lib.gdpPropagateNodesDevice.argtypes = [vp, c_int, vp, vp, vp, vp, c_int, c_int, POINTER(RunInfo)]
     lib.gdpPropagateNodesDevice.restype = c_int
+    lib.gdpAutotune.argtypes = [c_int, c_int, c_int, vp, vp, vp, vp, vp, vp, vp, vp, c_int, POINTER(Tuning),
+                                POINTER(c_double)]
+    lib.gdpAutotune.restype = c_int
+    lib.gdpBatchAutotune.argtypes = [c_int, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, c_int, POINTER(Tuning),
+                                     POINTER(c_double)]
+    lib.gdpBatchAutotune.restype = c_int
     lib.gdpBatchCreate.argtypes = [c_int, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, POINTER(Tuning), POINTER(vp)]
     lib.gdpBatchCreate.restype = c_int
     lib.gdpBatchPropagate.argtypes = [vp, vp, vp, c_int, c_int, POINTER(RunInfo)]
@@ -193,6 +211,10 @@ def load_library(path: Optional[str] = None) -> ctypes.CDLL:
     lib.gdpOMPPropagate.argtypes = [c_int, c_int, c_int, vp, vp, vp, vp, vp, vp, vp, vp, c_int, c_int,
                                     POINTER(RunInfo)]
     lib.gdpOMPPropagate.restype = c_int
+    lib.gdpParseMPS.argtypes = [ctypes.c_char_p, ctypes.c_size_t, POINTER(MpsModel)]
+    lib.gdpParseMPS.restype = c_int
+    lib.gdpMpsFree.argtypes = [POINTER(MpsModel)]
+    lib.gdpMpsFree.restype = None
     lib.gdpDeviceCount.argtypes = [POINTER(c_int)]
     lib.gdpDeviceMalloc.argtypes = [POINTER(vp), ctypes.c_size_t]
     lib.gdpDeviceFree.argtypes = [vp]
@@ -662,6 +684,55 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+def tuning_dict(t: Tuning) -> dict:
+    d = {f[0]: getattr(t, f[0]) for f in Tuning._fields_ if f[0] != "group_log"}
+    d["group_log"] = list(t.group_log)
+    return d
+
+
+def autotune(inst, sync_type: int = MEGAKERNEL, start: Optional[dict] = None):
+    """Fastest tuning for propagating the instance's bounds (gdpAutotune):
+    returns (tuning dict, device ms)."""
+    lib = load_library()
+    a = [_arr(inst.col, np.int32), _arr(inst.row_ptr, np.int32), _arr(inst.val, np.float64),
+         _arr(inst.lhs, np.float64), _arr(inst.rhs, np.float64), _arr(inst.vartype, np.int32),
+         _arr(inst.lb, np.float64), _arr(inst.ub, np.float64)]
+    t = make_tuning(start)
+    ms = c_double()
+    rc = lib.gdpAutotune(int(inst.m), int(inst.n), len(a[0]), *[_ptr(x) for x in a], sync_type, ctypes.byref(t),
+                         ctypes.byref(ms))
+    if rc != GDP_OKAY:
+        raise RuntimeError(f"gdpAutotune failed: {last_error()}")
+    return tuning_dict(t), ms.value
+
+
+def batch_autotune(instances, sync_type: int = MEGAKERNEL, start: Optional[dict] = None):
+    """Fastest tuning for a batch (gdpBatchAutotune): (tuning dict, device ms)."""
+    lib = load_library()
+    k = len(instances)
+    keep = []
+
+    def arrs(attr, dt):
+        x = [_arr(getattr(i, attr), dt) for i in instances]
+        keep.append(x)
+        return (c_void_p * k)(*[y.ctypes.data for y in x])
+
+    m = np.array([i.m for i in instances], np.int32)
+    n = np.array([i.n for i in instances], np.int32)
+    nnz = np.array([len(i.col) for i in instances], np.int32)
+    lb = np.concatenate([np.asarray(i.lb, np.float64) for i in instances])
+    ub = np.concatenate([np.asarray(i.ub, np.float64) for i in instances])
+    t = make_tuning(start)
+    ms = c_double()
+    rc = lib.gdpBatchAutotune(k, _ptr(m), _ptr(n), _ptr(nnz), arrs("col", np.int32), arrs("row_ptr", np.int32),
+                              arrs("val", np.float64), arrs("lhs", np.float64), arrs("rhs", np.float64),
+                              arrs("vartype", np.int32), _ptr(lb), _ptr(ub), sync_type, ctypes.byref(t),
+                              ctypes.byref(ms))
+    if rc != GDP_OKAY:
+        raise RuntimeError(f"gdpBatchAutotune failed: {last_error()}")
+    return tuning_dict(t), ms.value
 
 
 def gpu_propagate(inst, sync_type: int = CPU_LOOP, max_rounds: int = MAX_NUM_ROUNDS,

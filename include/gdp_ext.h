@@ -101,6 +101,26 @@ GDP_RETCODE gdpPropagateNodesDevice(gdp_problem p, const int count, double *cons
                                     const int *hint_ptr, const int *hints, const int sync_type,
                                     const int max_rounds, gdp_run_info *infos);
 
+/* ---- autotuning ----
+ * Builds the problem with candidate tunings and times the propagation of
+ * lbs/ubs to the fixpoint with sync_type: a coordinate search over the
+ * work-item size, the lanes per row of each row-length bin, the persistent
+ * grid size (MEGAKERNEL) and row marking, starting from *tuning (or the
+ * defaults when it holds the defaults).  *tuning receives the fastest
+ * candidate and *best_ms its device time.  The bounds double as the bound
+ * envelope.  Every candidate gives the same bits. */
+GDP_RETCODE gdpAutotune(const int n_cons, const int n_vars, const int nnz, const int *col_indices,
+                        const int *row_ptrs, const double *vals, const double *lhss, const double *rhss,
+                        const int *vartypes, const double *lbs, const double *ubs, const int sync_type,
+                        gdp_tuning *tuning, double *best_ms);
+/* The same for a batch (arrays as in gdpBatchCreate; lbs / ubs concatenate
+ * the instances' bound vectors). */
+GDP_RETCODE gdpBatchAutotune(const int n_inst, const int *n_cons, const int *n_vars, const int *nnz,
+                             const int *const *col_indices, const int *const *row_ptrs,
+                             const double *const *vals, const double *const *lhss, const double *const *rhss,
+                             const int *const *vartypes, const double *lbs, const double *ubs,
+                             const int sync_type, gdp_tuning *tuning, double *best_ms);
+
 /* ---- a batch of independent instances in shared launches ----
  * Instance i is given by the i-th entries of the arrays (same meaning as in
  * gdpProblemCreate; env_lbs / env_ubs may be NULL, or hold NULL entries). */
@@ -159,6 +179,24 @@ GDP_RETCODE gdpServerResult(gdp_server s, const long long ticket, double *lbs, d
  * gdpServerResult. */
 GDP_RETCODE gdpServerResultChanges(gdp_server s, const long long ticket, int *cols, double *lbs, double *ubs,
                                    const int cap, int *count, gdp_run_info *info, int *ready);
+
+/* ---- MPS input ----
+ * Parses an MPS model (fixed or free format text; decompress .gz first) into
+ * the arrays of the propagation entry points: N rows dropped, L/G/E rows as
+ * constraints with lhs/rhs (RANGES applied), variables in order of first
+ * appearance, MARKER integer sections, bounds UP/LO/FX/FR/MI/PL/BV/LI/UI,
+ * default bounds [0, +inf), infinite bounds as +-GDP_INF, types BINARY (an
+ * integer in [0, 1]), INTEGER, CONTINUOUS.  Free the arrays with
+ * gdpMpsFree. */
+typedef struct {
+    int n_cons, n_vars, nnz;
+    int *row_ptrs, *col_indices;
+    double *vals, *lhss, *rhss, *lbs, *ubs;
+    int *vartypes;
+    char name[128];
+} gdp_mps;
+GDP_RETCODE gdpParseMPS(const char *text, const size_t len, gdp_mps *out);
+void gdpMpsFree(gdp_mps *p);
 
 /* Host propagators with run information: cpu_seq (Gauss-Seidel), cpu_seq_dis
  * and cpu_omp (threads <= 0: GDP_OMP_THREADS or 8). */

@@ -226,3 +226,23 @@ def test_server_delta_requests(gpu, oracle):
         srv.close()
     finally:
         p.close()
+
+
+def test_autotune_returns_a_valid_tuning(gpu, oracle):
+    """The autotuner's pick gives the same bits (every candidate does)."""
+    inst = powerlaw_mip(seed=5, m=20000, n=10000, len_hi=5000, s=1.8)
+    tuning, ms = I.autotune(inst, I.MEGAKERNEL)
+    assert ms > 0
+    p = I.Problem(inst, tuning)
+    try:
+        assert_same(p.propagate(inst.lb, inst.ub, I.MEGAKERNEL), oracle.jacobi(inst), "autotuned")
+    finally:
+        p.close()
+    insts = [small_mip(seed=s, m=300, n=150) for s in range(4)]
+    bt, bms = I.batch_autotune(insts, I.CPU_LOOP)
+    b = I.Batch(insts, bt)
+    try:
+        for x, got in zip(insts, b.propagate(sync_type=I.CPU_LOOP)):
+            assert_same(got, oracle.jacobi(x), x.name)
+    finally:
+        b.close()
