@@ -12,9 +12,14 @@ Prints one JSON line:
   value         propagation throughput: nnz x rounds / second over the timed
                 steps (kernel time of the propagations, HIP events)
   roofline      HBM roofline of the persistent kernel: algorithmic bytes per
-                launch, rounds x (12 nnz + 20 m + 20 n), divided by the
-                average launch time (HIP events on the library's stream around
-                the launch), against 8 TB/s.  `traffic` is the measured HBM
+                launch, summed over the rounds of 12 nnz_r + 20 m_r + 20 n_r
+                for the rows a round processes (row marking skips rows none of
+                whose columns changed; n_r: distinct columns of those rows,
+                from an untimed diagnostic run of the same propagation),
+                divided by the average launch time (HIP events on the
+                library's stream around the launch), against 8 TB/s; the
+                dense-sweep equivalent, rounds x (12 nnz + 20 m + 20 n), is
+                reported beside it.  `traffic` is the measured HBM
                 traffic per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE scaled by a
                 copy kernel of known size), read from the committed profile
                 summary when it was taken on this workload
@@ -59,7 +64,15 @@ PROFILE_MD = os.path.join(REPO, "profiles", "mega_kernel_powerlaw_1m_rows.md")
 
 
 def algorithmic_bytes(inst, rounds):
+    """Bytes of `rounds` dense sweeps: per nonzero its column index and
+    coefficient, per row its extent and sides, per column its bounds and type."""
     return rounds * (12 * inst.nnz + 20 * inst.m + 20 * inst.n)
+
+
+def processed_bytes(rows, nnz, cols):
+    """Same per-unit bytes over the rows each round actually processes (and
+    the distinct columns those rows hold)."""
+    return int(12 * sum(nnz) + 20 * sum(rows) + 20 * sum(cols))
 
 
 def cpu_baseline(inst, runs):
@@ -76,6 +89,56 @@ def cpu_baseline(inst, runs):
                       f"workload, median of {runs} runs; {rounds} rounds to the fixpoint in {t:.3f} s "
                       f"(value = nnz x its rounds / time)",
             "time_to_fixpoint_ms": t * 1e3, "rounds": rounds}
+
+
+def batched_parity(out):
+    """Every batched instance against the round-synchronous oracle (bit for
+    bit: bounds, rounds, flags) and, where neither run hit the round cap,
+    against its own cpu_seq fixpoint (integer bounds exact, continuous within
+    1e-9 relative or absolute)."""
+    import numpy as np
+    import oracle_binding
+    insts, o, out_l, out_u, infos = out["insts"], out["offsets"], out["lb"], out["ub"], out["infos"]
+    exact, bad_int, worst, feas, compared, capped, outside = 0, 0, 0.0, 0, 0, 0, 0
+    for i, x in enumerate(insts):
+        gl, gu = out_l[o[i]:o[i + 1]], out_u[o[i]:o[i + 1]]
+        rounds, infeasible, is_capped = infos[i]
+        jac = oracle_binding.jacobi(x)
+        exact += bool(np.array_equal(gl, jac[0]) and np.array_equal(gu, jac[1]) and jac[2] == rounds and
+                      jac[3] == infeasible and jac[4] == is_capped)
+        seq = oracle_binding.seq(x)
+        feas += infeasible == bool(seq[3])
+        if is_capped or seq[4] or seq[3]:
+            capped += bool(is_capped or seq[4])
+            continue
+        compared += 1
+        is_int = x.vartype != 3
+        bad_int += int(np.sum(gl[is_int] != seq[0][is_int]) + np.sum(gu[is_int] != seq[1][is_int]))
+        for g, sv in ((gl, seq[0]), (gu, seq[1])):
+            d = np.abs(g - sv)[~is_int] / np.maximum(1.0, np.abs(sv[~is_int]))
+            worst = max(worst, float(d.max()) if d.size else 0.0)
+            outside += int(np.sum(d > 1e-9))
+    return {"bitwise_equal_to_oracle": exact, "of": len(insts),
+            "vs_cpu_seq": {"compared": compared, "capped_at_100_rounds": capped,
+                           "integer_bound_mismatches": bad_int,
+                           "max_continuous_rel_or_abs_dev": worst,
+                           "continuous_bounds_beyond_1e-9": outside,
+                           "infeasibility_agrees": feas == len(insts)}}
+
+
+def cpu_leg(inst, out_lb, out_ub, rounds, runs, batched_out):
+    """The host side: cpu_seq timed as the baseline, and the oracle as the
+    checker of the GPU results of the other legs."""
+    import numpy as np
+    import oracle_binding
+    print("bench: parity check against the oracle", file=sys.stderr, flush=True)
+    exp = oracle_binding.jacobi(inst, max_rounds=100)
+    parity = {"bitwise_equal_to_oracle": bool(np.array_equal(out_lb, exp[0]) and np.array_equal(out_ub, exp[1])),
+              "rounds_equal": exp[2] == rounds}
+    print("bench: timing cpu baselines", file=sys.stderr, flush=True)
+    base = cpu_baseline(inst, runs)
+    batched = batched_parity(batched_out) if batched_out is not None else None
+    return parity, base, batched
 
 
 def omp_baseline(inst, threads):
@@ -105,8 +168,9 @@ def load_traffic(inst, rounds, kernel):
 # batched and served propagation
 # ---------------------------------------------------------------------------
 
-def batched_leg(sync, check_cpu):
-    """256 uneven MIPs: one shared run vs back-to-back plain launches."""
+def batched_leg(sync):
+    """256 uneven MIPs: one shared run vs back-to-back plain launches.
+    Returns the result record and the outputs for the parity check."""
     import numpy as np
     from gdp_amd import interface as I
     from gdp_amd.instances import batch_family, small_mip
@@ -141,39 +205,10 @@ def batched_leg(sync, check_cpu):
            "speedup": plain_ms / batched_ms, "max_rounds": max(i.rounds for i in infos)}
     for p in probs:
         p.close()
-    if check_cpu:
-        # every instance against the round-synchronous oracle (bit for bit:
-        # bounds, rounds, flags) and, where neither run hit the round cap,
-        # against its own cpu_seq fixpoint (integer bounds exact, continuous
-        # within 1e-9 relative or absolute)
-        import oracle_binding
-        o = b.n_offsets
-        exact, bad_int, worst, feas, compared, capped, outside = 0, 0, 0.0, 0, 0, 0, 0
-        for i, x in enumerate(insts):
-            gl, gu = out_l[o[i]:o[i + 1]], out_u[o[i]:o[i + 1]]
-            jac = oracle_binding.jacobi(x)
-            exact += bool(np.array_equal(gl, jac[0]) and np.array_equal(gu, jac[1]) and jac[2] == infos[i].rounds and
-                          jac[3] == bool(infos[i].infeasible) and jac[4] == bool(infos[i].capped))
-            seq = oracle_binding.seq(x)
-            feas += bool(infos[i].infeasible) == bool(seq[3])
-            if infos[i].capped or seq[4] or seq[3]:
-                capped += bool(infos[i].capped or seq[4])
-                continue
-            compared += 1
-            is_int = x.vartype != 3
-            bad_int += int(np.sum(gl[is_int] != seq[0][is_int]) + np.sum(gu[is_int] != seq[1][is_int]))
-            for g, sv in ((gl, seq[0]), (gu, seq[1])):
-                d = np.abs(g - sv)[~is_int] / np.maximum(1.0, np.abs(sv[~is_int]))
-                worst = max(worst, float(d.max()) if d.size else 0.0)
-                outside += int(np.sum(d > 1e-9))
-        res["parity"] = {"bitwise_equal_to_oracle": exact, "of": len(insts),
-                         "vs_cpu_seq": {"compared": compared, "capped_at_100_rounds": capped,
-                                        "integer_bound_mismatches": bad_int,
-                                        "max_continuous_rel_or_abs_dev": worst,
-                                        "continuous_bounds_beyond_1e-9": outside,
-                                        "infeasibility_agrees": feas == len(insts)}}
+    outputs = {"insts": insts, "offsets": list(b.n_offsets), "lb": out_l, "ub": out_u,
+               "infos": [(i.rounds, bool(i.infeasible), bool(i.capped)) for i in infos]}
     b.close()
-    return res
+    return res, outputs
 
 
 def serving_leg(sync):
@@ -336,7 +371,7 @@ def summarize(d, kernel="mega_kernel"):
         if rdreq is not None:
             fh.write(f"| read bytes from 32/64/128-byte read request counts (cross-check of the FETCH factor) | "
                      f"{rdreq / 1e6:.1f} MB |\n")
-        fh.write(f"| algorithmic bytes per launch | {alg / 1e6:.1f} MB |\n")
+        fh.write(f"| algorithmic bytes per launch (rows each round processes) | {alg / 1e6:.1f} MB |\n")
         fh.write(f"| achieved, algorithmic bytes / duration | {alg / avg_ns:.1f} GB/s |\n")
         fh.write(f"| achieved, measured traffic / duration | {traffic / avg_ns:.1f} GB/s |\n")
     print(json.dumps(out))
@@ -415,18 +450,25 @@ def main():
     kernel_ms = statistics.mean(i.kernel_ms for i in infos)
     ms_per_step = kernel_ms
     value = inst.nnz * rounds / (kernel_ms * 1e-3)
-    alg = algorithmic_bytes(inst, rounds)
-    achieved = alg / (kernel_ms * 1e-3) / 1e9
     wall_ms = (t1 - t0) / args.steps * 1e3
     kernel = KERNEL_OF_SYNC[args.sync]
+    out_lb, out_ub = lb.numpy(), ub.numpy()
+    # bytes the launch has to move: each round's processed rows (row marking
+    # skips the others), untimed diagnostic run of the same propagation
+    print("bench: per-round work profile", file=sys.stderr, flush=True)
+    wp, w_rows, w_nnz, w_cols = prob.work_profile(inst.lb, inst.ub)
+    assert wp.rounds == rounds and len(w_rows) == rounds, "work profile differs from the timed propagation"
+    assert np.array_equal(wp.lbs, out_lb) and np.array_equal(wp.ubs, out_ub), "work profile result differs"
+    alg = processed_bytes(w_rows, w_nnz, w_cols)
+    dense = algorithmic_bytes(inst, rounds)
+    achieved = alg / (kernel_ms * 1e-3) / 1e9
     print(f"bench: {rounds} rounds, kernel {kernel_ms:.3f} ms/step, wall {wall_ms:.3f} ms/step incl. bound restore, "
-          f"{achieved:.1f} GB/s algorithmic", file=sys.stderr, flush=True)
+          f"{achieved:.1f} GB/s on processed rows ({alg / dense:.3f} of dense sweeps)", file=sys.stderr, flush=True)
     if args.meta_out:
         with open(args.meta_out, "w") as fh:
             json.dump({"workload": {"m": inst.m, "n": inst.n, "nnz": inst.nnz, "rounds": rounds},
-                       "algorithmic_bytes_per_launch": alg}, fh)
+                       "algorithmic_bytes_per_launch": alg, "dense_sweep_bytes_per_launch": dense}, fh)
 
-    out_lb, out_ub = lb.numpy(), ub.numpy()
     traffic = load_traffic(inst, rounds, kernel) if args.sync == 2 else None
     result = {
         "metric": METRIC,
@@ -447,25 +489,26 @@ def main():
                    "sync": SYNC_NAMES[args.sync], "rounds": rounds, "global_batch": 1},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": kernel, "launch_ms": kernel_ms, "algorithmic_bytes_per_launch": alg},
+                     "kernel": kernel, "launch_ms": kernel_ms, "algorithmic_bytes_per_launch": alg,
+                     "processed_nnz_per_launch": int(w_nnz.sum()), "processed_rows_per_launch": int(w_rows.sum()),
+                     "dense_sweep_bytes_per_launch": dense,
+                     "dense_equivalent_gbs": dense / (kernel_ms * 1e-3) / 1e9},
         "wall_ms_per_step": wall_ms,
     }
-    if not args.no_cpu:
-        import oracle_binding
-        print("bench: parity check against the oracle", file=sys.stderr, flush=True)
-        exp = oracle_binding.jacobi(inst, max_rounds=100)
-        result["parity"] = {"bitwise_equal_to_oracle": bool(np.array_equal(out_lb, exp[0]) and
-                                                            np.array_equal(out_ub, exp[1])),
-                            "rounds_equal": exp[2] == rounds}
-        print("bench: timing cpu baselines", file=sys.stderr, flush=True)
-        result["cpu_baseline"] = cpu_baseline(inst, args.cpu_runs)
-        result["cpu_omp"] = omp_baseline(inst, min(16, os.cpu_count() or 8))
     prob.close()
     del prob, lb, ub, root_lb, root_ub
+    batched_out = None
     if not args.no_batch:
-        result["batched"] = batched_leg(args.sync, not args.no_cpu)
+        result["batched"], batched_out = batched_leg(args.sync)
     if not args.no_serve:
         result["serving"] = serving_leg(args.sync)
+    if not args.no_cpu:
+        parity, base, bparity = cpu_leg(inst, out_lb, out_ub, rounds, args.cpu_runs, batched_out)
+        result["parity"] = parity
+        result["cpu_baseline"] = base
+        result["cpu_omp"] = omp_baseline(inst, min(16, os.cpu_count() or 8))
+        if bparity is not None:
+            result["batched"]["parity"] = bparity
     print(json.dumps(result), flush=True)
 
 

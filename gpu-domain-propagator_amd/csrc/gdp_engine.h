@@ -93,6 +93,16 @@ RunResult problem_propagate(Problem *p, double *lbs, double *ubs, int sync_type,
 RunResult problem_propagate_device(Problem *p, double *d_lbs, double *d_ubs, int sync_type,
                                    int max_rounds);
 
+// Work of each round of one propagation of host vectors lbs/ubs (in/out,
+// the CPU_LOOP launches, same result): the rows whose activities the round
+// computes (rows marked by the previous round, or every row), their
+// nonzeros and the distinct columns they hold.  Diagnostic: reads the row
+// marks back after every round.
+struct RoundWork {
+    int64_t rows = 0, nnz = 0, cols = 0;
+};
+RunResult problem_work_profile(Problem *p, double *lbs, double *ubs, int max_rounds, std::vector<RoundWork> &work);
+
 // Replicas mode: `count` device-resident bound vectors at once (vector k's
 // at d_lbs[k], d_ubs[k], host arrays of device pointers).  With hints (host
 // CSR: vector k's columns hints[hint_ptr[k] .. hint_ptr[k+1])), the vectors

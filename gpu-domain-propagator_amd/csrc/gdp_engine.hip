@@ -761,6 +761,78 @@ RunResult problem_propagate(Problem *P, double *lbs, double *ubs, int sync_type,
     return res;
 }
 
+RunResult problem_work_profile(Problem *P, double *lbs, double *ubs, int max_rounds, std::vector<RoundWork> &work) {
+    if (!P->replicas) throw std::runtime_error("work profiles are for a single instance");
+    if (max_rounds < 1 || max_rounds > kMaxRounds) throw std::runtime_error("max_rounds out of range");
+    std::lock_guard<std::mutex> lock(P->run_mutex);
+    const DevMatrix &M = P->M;
+    hipStream_t s = P->stream;
+    const int n = P->n;
+    // layout rows and columns, host side
+    std::vector<int> rp(M.m + 1);
+    check(hipMemcpy(rp.data(), M.row_ptr, sizeof(int) * (M.m + 1), hipMemcpyDeviceToHost), "D2H row_ptr");
+    const int n_padded = rp[M.m] & ~(kRowAlign - 1);
+    std::vector<int> cols(std::max(1, n_padded));
+    if (n_padded > 0)
+        check(hipMemcpy(cols.data(), M.col, sizeof(int) * n_padded, hipMemcpyDeviceToHost), "D2H col");
+    std::vector<int> marks(M.m), stamp(M.n, 0);
+    double *d_b = nullptr;
+    check(hipMalloc(&d_b, 2 * sizeof(double) * static_cast<size_t>(n)), "hipMalloc bounds");
+    RunResult res;
+    work.clear();
+    try {
+        check(hipMemcpy(d_b, lbs, n * sizeof(double), hipMemcpyHostToDevice), "H2D lbs");
+        check(hipMemcpy(d_b + n, ubs, n * sizeof(double), hipMemcpyHostToDevice), "H2D ubs");
+        const int list[1] = {0};
+        double *lbp = d_b, *ubp = d_b + n;
+        const int caps[1] = {max_rounds};
+        P->D.n_active = 1;
+        load_units(P, 1, list, &lbp, &ubp, nullptr, nullptr, caps, 1, true);
+        std::vector<UnitState> st = read_units(P, 1, list);
+        if (st[0].envelope) throw std::runtime_error("bounds outside the problem's bound envelope");
+        for (int step = 1; step <= max_rounds; step++) {
+            if (step == 1 ? st[0].crossed != 0 : !active_after(st[0], step - 1)) break;
+            RoundWork w;
+            const bool all_rows = !M.mark_rows || st[0].markall == step;
+            if (!all_rows)
+                check(hipMemcpy(marks.data(), P->D.marks, sizeof(int) * M.m, hipMemcpyDeviceToHost), "D2H marks");
+            for (int r = 0; r < M.m; r++) {
+                if (!all_rows && marks[r] != step) continue;
+                const int a = rp[r], b = rp[r + 1];
+                const int start = a & ~(kRowAlign - 1);
+                const int len = (b & ~(kRowAlign - 1)) - start - (a & (kRowAlign - 1));
+                w.rows++;
+                w.nnz += len;
+                for (int e = 0; e < len; e++) {
+                    const int id = cols[start + e] & kIdMask;
+                    if (stamp[id] != step) {
+                        stamp[id] = step;
+                        w.cols++;
+                    }
+                }
+            }
+            work.push_back(w);
+            launch_step(P, step, 0, 1);
+            check(hipGetLastError(), "step launch");
+            check(hipStreamSynchronize(s), "step sync");
+            st = read_units(P, 1, list);
+        }
+        res = outcome(st[0]);
+        const int gx = std::max(1, std::min(1024, (M.n + 255) / 256));
+        hipLaunchKernelGGL(unpack_kernel, dim3(gx, 1), dim3(256), 0, s, P->M, P->D, 1, P->d_list, P->d_lbp,
+                           P->d_ubp);
+        check(hipGetLastError(), "unpack launch");
+        check(hipStreamSynchronize(s), "unpack sync");
+        check(hipMemcpy(lbs, d_b, n * sizeof(double), hipMemcpyDeviceToHost), "D2H lbs");
+        check(hipMemcpy(ubs, d_b + n, n * sizeof(double), hipMemcpyDeviceToHost), "D2H ubs");
+    } catch (...) {
+        (void)hipFree(d_b);
+        throw;
+    }
+    check(hipFree(d_b), "hipFree");
+    return res;
+}
+
 void nodes_propagate_device(Problem *P, int count, double *const *d_lbs, double *const *d_ubs, const int *hint_ptr,
                             const int *hints, int sync_type, int max_rounds, RunResult *out) {
     if (!P->replicas) throw std::runtime_error("use the batch entry points for a batch of instances");

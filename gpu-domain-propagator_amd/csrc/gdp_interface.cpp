@@ -619,6 +619,25 @@ GDP_RETCODE gdpPropagate(gdp_problem p, double *lbs, double *ubs, const int sync
     }
 }
 
+GDP_RETCODE gdpProblemWorkProfile(gdp_problem p, double *lbs, double *ubs, const int max_rounds, long long *rows,
+                                  long long *nnz, long long *cols, int *n_rounds, gdp_run_info *info) {
+    if (!p || !lbs || !ubs || !rows || !nnz || !cols || !n_rounds) return GDP_ERROR;
+    try {
+        std::vector<gdp::RoundWork> work;
+        const RunResult r = gdp::problem_work_profile(p->p, lbs, ubs, max_rounds, work);
+        for (size_t k = 0; k < work.size(); k++) {
+            rows[k] = work[k].rows;
+            nnz[k] = work[k].nnz;
+            cols[k] = work[k].cols;
+        }
+        *n_rounds = static_cast<int>(work.size());
+        fill_info(info, r.rounds, r.infeasible, r.capped, r.kernel_ms);
+        return GDP_OKAY;
+    } catch (const std::exception &e) {
+        return fail("gdpProblemWorkProfile", e);
+    }
+}
+
 GDP_RETCODE gdpPropagateDevice(gdp_problem p, double *d_lbs, double *d_ubs, const int sync_type,
                                const int max_rounds, gdp_run_info *info) {
     if (!p || !d_lbs || !d_ubs) return GDP_ERROR;

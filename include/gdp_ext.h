@@ -78,6 +78,16 @@ GDP_RETCODE gdpProblemGetStats(gdp_problem p, gdp_problem_stats *out);
 GDP_RETCODE gdpPropagate(gdp_problem p, double *lbs, double *ubs, const int sync_type,
                          const int max_rounds, gdp_run_info *info);
 
+/* Work of each round of one propagation of host vectors lbs/ubs (in/out;
+ * the CPU_LOOP launches, same result as gdpPropagate): rows[k], nnz[k] and
+ * cols[k] (arrays of max_rounds entries) are the rows whose activities round
+ * k+1 computes, their nonzeros and the distinct columns they hold; the
+ * number of rounds run is stored in *n_rounds.  A diagnostic (it reads the
+ * row marks back after every round), e.g. for the bytes a round moves.
+ * No counterpart in the reference, whose rounds always process every row. */
+GDP_RETCODE gdpProblemWorkProfile(gdp_problem p, double *lbs, double *ubs, const int max_rounds, long long *rows,
+                                  long long *nnz, long long *cols, int *n_rounds, gdp_run_info *info);
+
 /* Same with device pointers (hipMalloc'd, length n_vars each); no host copy.
  * The library works on its own stream: work that produced d_lbs / d_ubs on
  * another stream must be complete before the call.  Returns after the device

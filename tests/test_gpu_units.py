@@ -136,6 +136,62 @@ def test_row_marking_off_same_bits(gpu, oracle, case):
             p.close()
 
 
+def _changed_row_sets(inst, oracle, rounds):
+    """Per round k >= 2: the rows holding a column that round k-1 changed
+    (round-synchronous oracle, one round at a time)."""
+    lens = np.diff(inst.row_ptr)
+    row_of = np.repeat(np.arange(inst.m), lens)
+    prev = (inst.lb, inst.ub)
+    sets = []
+    for k in range(1, rounds):
+        cur = oracle.jacobi(inst, max_rounds=k)
+        changed = (cur[0] != prev[0]) | (cur[1] != prev[1])
+        rows = np.zeros(inst.m, bool)
+        rows[row_of[changed[inst.col]]] = True
+        sets.append(rows)
+        prev = cur[:2]
+    return sets
+
+
+@pytest.mark.parametrize("case", ["small:1", "powerlaw:5", "varied:2"])
+def test_work_profile(gpu, oracle, case):
+    """Round 1 processes every nonempty row; a later round processes every
+    row (marking gave way to a full sweep) or exactly the rows holding a
+    column the previous round changed.  The run itself is the propagation."""
+    from test_gpu_parity import _case
+    inst = _case(case)
+    exp = oracle.jacobi(inst)
+    lens = np.diff(inst.row_ptr)
+    nonempty = lens > 0
+    p = I.Problem(inst)
+    try:
+        out, rows, nnz, cols = p.work_profile(inst.lb, inst.ub)
+    finally:
+        p.close()
+    assert_same(out, exp, case)
+    assert len(rows) == exp[2]
+    all_cols = np.unique(inst.col).size
+    assert (rows[0], nnz[0], cols[0]) == (nonempty.sum(), inst.nnz, all_cols)
+    n_marked = 0
+    for k, marked in enumerate(_changed_row_sets(inst, oracle, exp[2]), start=1):
+        if rows[k] == nonempty.sum() and nnz[k] == inst.nnz:
+            continue
+        n_marked += 1
+        sel = marked & nonempty
+        assert rows[k] == sel.sum(), f"round {k + 1}"
+        assert nnz[k] == lens[sel].sum(), f"round {k + 1}"
+        assert cols[k] == np.unique(np.concatenate([inst.col[inst.row_ptr[r]:inst.row_ptr[r + 1]]
+                                                    for r in np.flatnonzero(sel)] or [[]])).size
+    assert n_marked > 0, "no round used row marks"
+    p = I.Problem(inst, {"row_marking": 0})
+    try:
+        out, rows, nnz, cols = p.work_profile(inst.lb, inst.ub)
+    finally:
+        p.close()
+    assert_same(out, exp, case)
+    assert np.all(rows == nonempty.sum()) and np.all(nnz == inst.nnz)
+
+
 def test_server_results_alone_and_in_the_mix(gpu, oracle):
     """Requests submitted while others run join at the next step; every
     result equals the request propagated alone (oracle and the plain entry
