@@ -456,7 +456,7 @@ def main():
     # bytes the launch has to move: each round's processed rows (row marking
     # skips the others), untimed diagnostic run of the same propagation
     print("bench: per-round work profile", file=sys.stderr, flush=True)
-    wp, w_rows, w_nnz, w_cols = prob.work_profile(inst.lb, inst.ub)
+    wp, w_rows, w_nnz, w_cols, w_acc = prob.work_profile(inst.lb, inst.ub)
     assert wp.rounds == rounds and len(w_rows) == rounds, "work profile differs from the timed propagation"
     assert np.array_equal(wp.lbs, out_lb) and np.array_equal(wp.ubs, out_ub), "work profile result differs"
     alg = processed_bytes(w_rows, w_nnz, w_cols)
@@ -491,6 +491,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": kernel, "launch_ms": kernel_ms, "algorithmic_bytes_per_launch": alg,
                      "processed_nnz_per_launch": int(w_nnz.sum()), "processed_rows_per_launch": int(w_rows.sum()),
+                     "accepted_candidates_per_launch": int(w_acc.sum()),
                      "dense_sweep_bytes_per_launch": dense,
                      "dense_equivalent_gbs": dense / (kernel_ms * 1e-3) / 1e9},
         "wall_ms_per_step": wall_ms,

@@ -36,7 +36,11 @@
 //   csc            col_ptr[n+1], col_rows[nnz]: rows of each column (marking)
 // per unit:
 //   snapshot       S16[n16] uint32, S32[n32] int2, SF[nf] double2: the bounds
-//                  every row of a round reads
+//                  every row of a round reads, in one block of 16-byte chunks
+//                  (S32 from byte snap_off32, SF from byte snap_offF, both
+//                  multiples of 16; blocks of consecutive units snap_stride
+//                  bytes apart) so that any record is read with one aligned
+//                  16-byte load
 //   accumulator    AI[n_int] int2 (integer atomics), AF[nf] double2 (FP64
 //                  atomics), AX[n_int] double2 (exact values of integer
 //                  candidates beyond the int32 range; they only arise when
@@ -141,6 +145,9 @@ struct DevMatrix {
     const int *col_rows;
     const int *perm;       // internal id -> caller's column (-1: padding id)
     const int *word_unit;  // blocks mode: unit of each bitmap word
+    int snap_off32;        // byte offset of S32 in a unit's snapshot block
+    int snap_offF;         // byte offset of SF
+    long long snap_stride; // bytes between the snapshot blocks of two units
 };
 
 struct DevState {
@@ -171,9 +178,10 @@ __device__ __forceinline__ DevState unit_view(const DevMatrix &M, const DevState
     if (!M.replicas) return D;
     DevState V = D;
     const size_t k = static_cast<size_t>(u);
-    V.S16 += k * M.n16;
-    V.S32 += k * (M.n_int - M.n16);
-    V.SF += k * (M.n - M.n_int);
+    const size_t snap = k * static_cast<size_t>(M.snap_stride);
+    V.S16 = reinterpret_cast<unsigned *>(reinterpret_cast<char *>(D.S16) + snap);
+    V.S32 = reinterpret_cast<int2 *>(reinterpret_cast<char *>(D.S32) + snap);
+    V.SF = reinterpret_cast<double2 *>(reinterpret_cast<char *>(D.SF) + snap);
     V.AI += k * M.n_int;
     V.AX += k * M.n_int;
     V.AF += k * (M.n - M.n_int);

@@ -30,6 +30,7 @@ struct Tuning {
     int item_nnz = 1024;      // target nonzeros per work item
     int block_threads = 256;  // threads per workgroup (multiple of 64)
     int mega_blocks_per_cu = 0;  // 0: occupancy limit
+    int mega_threads = 512;   // workgroup size of the persistent kernel: 256, 512 or 1024
     int graph_rounds = 8;     // rounds per captured graph for GPU_LOOP
     int max_split_segs = -1;  // rows with more segments run on one wave (-1: split all, 0: none)
     int compact_bounds = 1;   // 0: keep every column in 16-byte FP64 records
@@ -56,6 +57,8 @@ struct ProblemStats {
     int n16 = 0, n32 = 0, nf = 0;  // columns per bound storage class
     int n_units = 0;               // instances (batch) or unit capacity
     int64_t unit_state_bytes = 0;  // device bytes of one unit's state (replicas)
+    int mega_threads = 0;          // persistent kernel: workgroup size
+    int mega_grid = 0;             // and its workgroups
 };
 
 // One instance of a batch (caller's arrays, CSR, zero-based).
@@ -96,10 +99,11 @@ RunResult problem_propagate_device(Problem *p, double *d_lbs, double *d_ubs, int
 // Work of each round of one propagation of host vectors lbs/ubs (in/out,
 // the CPU_LOOP launches, same result): the rows whose activities the round
 // computes (rows marked by the previous round, or every row), their
-// nonzeros and the distinct columns they hold.  Diagnostic: reads the row
+// nonzeros, the distinct columns they hold, and the candidates it accepted
+// (bound changes folded into the accumulator).  Diagnostic: reads the row
 // marks back after every round.
 struct RoundWork {
-    int64_t rows = 0, nnz = 0, cols = 0;
+    int64_t rows = 0, nnz = 0, cols = 0, accepted = 0;
 };
 RunResult problem_work_profile(Problem *p, double *lbs, double *ubs, int max_rounds, std::vector<RoundWork> &work);
 

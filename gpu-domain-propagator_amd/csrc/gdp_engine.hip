@@ -30,6 +30,7 @@ namespace gdp {
 __global__ void sweep_kernel(DevMatrix M, DevState D, int step, int guard, int first_step);
 __global__ void apply_split_kernel(DevMatrix M, DevState D, int step, int guard, int first_step);
 __global__ void publish_kernel(DevMatrix M, DevState D, int step, int guard, int first_step, int n_units);
+template <int kThreads>
 __global__ void mega_kernel(DevMatrix M, DevState D, int first_step, int max_steps, int n_units);
 __global__ void unit_reset_kernel(DevState D, int n_list, const int4 *params, int start_step, int clear_status);
 __global__ void load_kernel(DevMatrix M, DevState D, int n_list, const int *list, const double *const *lbp,
@@ -159,6 +160,8 @@ struct Problem {
     int device = 0;
     int cu_count = 0;
     int mega_grid = 0;
+    int mega_threads = 0;             // workgroup size of the persistent kernel
+    const void *mega_fn = nullptr;
     // captured GPU_LOOP graphs, one per chunk of steps, for one launch shape
     std::vector<hipGraphExec_t> graphs;
     std::vector<int> graph_last;
@@ -197,9 +200,14 @@ void alloc_units(Problem *P, int cap) {
     const size_t k = P->replicas ? static_cast<size_t>(cap) : 1;
     const int n32 = M.n_int - M.n16, nf = M.n - M.n_int;
     auto &o = P->unit_owned;
-    D.S16 = dev_alloc<unsigned>(k * M.n16, o);
-    D.S32 = dev_alloc<int2>(k * n32, o);
-    D.SF = dev_alloc<double2>(k * nf, o);
+    // snapshot: one block of 16-byte chunks per unit
+    M.snap_off32 = (M.n16 * 4 + 15) / 16 * 16;
+    M.snap_offF = M.snap_off32 + (n32 * 8 + 15) / 16 * 16;
+    M.snap_stride = M.snap_offF + 16LL * nf + 16;
+    char *snap = dev_alloc<char>(k * static_cast<size_t>(M.snap_stride), o);
+    D.S16 = reinterpret_cast<unsigned *>(snap);
+    D.S32 = reinterpret_cast<int2 *>(snap + M.snap_off32);
+    D.SF = reinterpret_cast<double2 *>(snap + M.snap_offF);
     D.AI = dev_alloc<int2>(k * M.n_int, o);
     D.AX = dev_alloc<double2>(k * M.n_int, o);
     D.AF = dev_alloc<double2>(k * nf, o);
@@ -503,12 +511,16 @@ Problem *build(const std::vector<InstanceView> &inst, bool replicas, const Tunin
         check(hipStreamSynchronize(s), "upload");
         alloc_units(P, 1);
 
+        P->mega_threads = T.mega_threads == 256 || T.mega_threads == 1024 ? T.mega_threads : 512;
+        P->mega_fn = P->mega_threads == 256    ? reinterpret_cast<const void *>(mega_kernel<256>)
+                     : P->mega_threads == 1024 ? reinterpret_cast<const void *>(mega_kernel<1024>)
+                                               : reinterpret_cast<const void *>(mega_kernel<512>);
         int per_cu = 0;
-        check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(mega_kernel), 256,
-                                                           0),
-              "occupancy");
+        check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, P->mega_fn, P->mega_threads, 0), "occupancy");
         if (T.mega_blocks_per_cu > 0) per_cu = std::min(per_cu, T.mega_blocks_per_cu);
         P->mega_grid = std::max(1, per_cu) * P->cu_count;
+        P->stats.mega_threads = P->mega_threads;
+        P->stats.mega_grid = P->mega_grid;
     } catch (...) {
         delete P;
         throw;
@@ -640,8 +652,7 @@ double run_steps(Problem *P, int sync_type, int max_rounds) {
     } else {
         int first = 1, steps = max_rounds, n_units = P->n_units;
         void *args[] = {const_cast<DevMatrix *>(&M), &D, &first, &steps, &n_units};
-        check(hipLaunchCooperativeKernel(reinterpret_cast<const void *>(mega_kernel), dim3(P->mega_grid), dim3(256),
-                                         args, 0, s),
+        check(hipLaunchCooperativeKernel(P->mega_fn, dim3(P->mega_grid), dim3(P->mega_threads), args, 0, s),
               "mega_kernel launch");
         int g[kGlobalWords];
         check(hipMemcpyAsync(g, D.global, sizeof(g), hipMemcpyDeviceToHost, s), "D2H");
@@ -811,10 +822,15 @@ RunResult problem_work_profile(Problem *P, double *lbs, double *ubs, int max_rou
                     }
                 }
             }
-            work.push_back(w);
             launch_step(P, step, 0, 1);
             check(hipGetLastError(), "step launch");
+            int cnt[kMoveLanes];
+            check(hipMemcpyAsync(cnt, P->D.moved + (step % kMoveSlots) * kMoveLanes, sizeof(cnt),
+                                 hipMemcpyDeviceToHost, s),
+                  "D2H moved");
             check(hipStreamSynchronize(s), "step sync");
+            for (int j = 0; j < kMoveLanes; j++) w.accepted += cnt[j];
+            work.push_back(w);
             st = read_units(P, 1, list);
         }
         res = outcome(st[0]);
@@ -930,10 +946,10 @@ Tuning autotune(const std::vector<InstanceView> &instances, const double *lbs, c
             if (!std::equal(g, g + 10, best.group_log)) try_one(t);
         }
         if (sync_type == MEGAKERNEL_ID)
-            for (int b : {2, 3, 4})
-                if (b != best.mega_blocks_per_cu) {
+            for (int w : {256, 512, 1024})
+                if (w != best.mega_threads) {
                     Tuning t = best;
-                    t.mega_blocks_per_cu = b;
+                    t.mega_threads = w;
                     try_one(t);
                 }
         {

@@ -245,6 +245,7 @@ void gdpTuningDefaults(gdp_tuning *t) {
     const gdp::Tuning d;
     t->item_nnz = d.item_nnz;
     t->mega_blocks_per_cu = d.mega_blocks_per_cu;
+    t->mega_threads = d.mega_threads;
     t->graph_rounds = d.graph_rounds;
     t->max_split_segs = d.max_split_segs;
     t->compact_bounds = d.compact_bounds;
@@ -259,6 +260,7 @@ gdp::Tuning tuning_from(const gdp_tuning *tuning) {
     if (tuning) {
         if (tuning->item_nnz > 0) T.item_nnz = tuning->item_nnz;
         if (tuning->mega_blocks_per_cu > 0) T.mega_blocks_per_cu = tuning->mega_blocks_per_cu;
+        if (tuning->mega_threads > 0) T.mega_threads = tuning->mega_threads;
         if (tuning->graph_rounds > 0) T.graph_rounds = tuning->graph_rounds;
         if (tuning->max_split_segs >= 0) T.max_split_segs = tuning->max_split_segs;
         if (tuning->compact_bounds >= 0) T.compact_bounds = tuning->compact_bounds;
@@ -313,6 +315,8 @@ GDP_RETCODE gdpProblemGetStats(gdp_problem p, gdp_problem_stats *out) {
     out->nf = s.nf;
     out->n_units = s.n_units;
     out->unit_state_bytes = s.unit_state_bytes;
+    out->mega_threads = s.mega_threads;
+    out->mega_grid = s.mega_grid;
     return GDP_OKAY;
 }
 
@@ -418,6 +422,7 @@ namespace {
 void tuning_to(const gdp::Tuning &T, gdp_tuning *t) {
     t->item_nnz = T.item_nnz;
     t->mega_blocks_per_cu = T.mega_blocks_per_cu;
+    t->mega_threads = T.mega_threads;
     t->graph_rounds = T.graph_rounds;
     t->max_split_segs = T.max_split_segs;
     t->compact_bounds = T.compact_bounds;
@@ -620,8 +625,9 @@ GDP_RETCODE gdpPropagate(gdp_problem p, double *lbs, double *ubs, const int sync
 }
 
 GDP_RETCODE gdpProblemWorkProfile(gdp_problem p, double *lbs, double *ubs, const int max_rounds, long long *rows,
-                                  long long *nnz, long long *cols, int *n_rounds, gdp_run_info *info) {
-    if (!p || !lbs || !ubs || !rows || !nnz || !cols || !n_rounds) return GDP_ERROR;
+                                  long long *nnz, long long *cols, long long *accepted, int *n_rounds,
+                                  gdp_run_info *info) {
+    if (!p || !lbs || !ubs || !rows || !nnz || !cols || !accepted || !n_rounds) return GDP_ERROR;
     try {
         std::vector<gdp::RoundWork> work;
         const RunResult r = gdp::problem_work_profile(p->p, lbs, ubs, max_rounds, work);
@@ -629,6 +635,7 @@ GDP_RETCODE gdpProblemWorkProfile(gdp_problem p, double *lbs, double *ubs, const
             rows[k] = work[k].rows;
             nnz[k] = work[k].nnz;
             cols[k] = work[k].cols;
+            accepted[k] = work[k].accepted;
         }
         *n_rounds = static_cast<int>(work.size());
         fill_info(info, r.rounds, r.infeasible, r.capped, r.kernel_ms);

@@ -95,21 +95,48 @@ __device__ __forceinline__ void mem_wait() { __builtin_amdgcn_s_waitcnt(0); }
 // bounds snapshot and accumulator
 // ---------------------------------------------------------------------------
 
+// Snapshot records are read with one aligned 16-byte load of the chunk that
+// holds them, decoded without branches, so that the gathers of a lane issue
+// back to back whatever the storage classes of their columns.
+// Byte offset of the record of internal column `id` in the unit's block.
+__device__ __forceinline__ unsigned record_offset(const DevMatrix &M, int id) {
+    const unsigned uid = static_cast<unsigned>(id);
+    const unsigned a16 = uid * 4u;
+    const unsigned a32 = static_cast<unsigned>(M.snap_off32) + (uid - static_cast<unsigned>(M.n16)) * 8u;
+    const unsigned af = static_cast<unsigned>(M.snap_offF) + (uid - static_cast<unsigned>(M.n_int)) * 16u;
+    const unsigned a = id < M.n16 ? a16 : a32;
+    return id < M.n_int ? a : af;
+}
+
+__device__ __forceinline__ uint4 load_chunk(const DevState &D, unsigned off) {
+    return *reinterpret_cast<const uint4 *>(reinterpret_cast<const char *>(D.S16) + (off & ~15u));
+}
+
+// Bounds of `id` from the chunk at its record offset `off`.
+__device__ __forceinline__ void decode_bounds(const DevMatrix &M, int id, unsigned off, const uint4 c, double &l,
+                                              double &u) {
+    const unsigned odd = (off >> 2) & 1u, upper = (off >> 3) & 1u;
+    // 4-byte record: dword (off >> 2) & 3; 8-byte record: that dword and the next
+    const unsigned x01 = odd ? c.y : c.x;
+    const unsigned x23 = odd ? c.w : c.z;
+    const unsigned lo = upper ? x23 : x01;
+    const unsigned hi = upper ? c.w : c.y;
+    const double l16 = static_cast<double>(static_cast<short>(lo & 0xffffu));
+    const double u16 = static_cast<double>(static_cast<short>(lo >> 16));
+    const double l32 = static_cast<double>(static_cast<int>(lo));
+    const double u32 = static_cast<double>(static_cast<int>(hi));
+    const double lf = __hiloint2double(static_cast<int>(c.y), static_cast<int>(c.x));
+    const double uf = __hiloint2double(static_cast<int>(c.w), static_cast<int>(c.z));
+    const bool c16 = id < M.n16, c32 = id < M.n_int;
+    const double li = c16 ? l16 : l32, ui = c16 ? u16 : u32;
+    l = c32 ? li : lf;
+    u = c32 ? ui : uf;
+}
+
 // Bounds of internal column `id` at the start of the round.
 __device__ __forceinline__ void load_bounds(const DevMatrix &M, const DevState &D, int id, double &l, double &u) {
-    if (id < M.n16) {
-        const unsigned w = D.S16[id];
-        l = static_cast<double>(static_cast<short>(w & 0xffffu));
-        u = static_cast<double>(static_cast<short>(w >> 16));
-    } else if (id < M.n_int) {
-        const int2 w = D.S32[id - M.n16];
-        l = static_cast<double>(w.x);
-        u = static_cast<double>(w.y);
-    } else {
-        const double2 w = D.SF[id - M.n_int];
-        l = w.x;
-        u = w.y;
-    }
+    const unsigned off = record_offset(M, id);
+    decode_bounds(M, id, off, load_chunk(D, off), l, u);
 }
 
 __device__ __forceinline__ bool fits_int32(double v) { return v >= -2147483647.0 && v <= 2147483647.0; }
@@ -297,18 +324,29 @@ struct LaneEntries {
                 }
             }
         }
+        // every gather first (entries past the end read column 0's record and
+        // are discarded), then the decoding and the contributions
+        unsigned off[K];
+        uint4 chunk[K];
 #pragma unroll
         for (int k = 0; k < K; k++) {
-            const int e = first + k;
-            smin[k] = 0.0;
-            smax[k] = 0.0;
-            l[k] = 0.0;
-            u[k] = 0.0;
-            if (e >= len) {
+            if (first + k >= len) {
                 jp[k] = 0;
                 a[k] = 0.0;
+            }
+            off[k] = record_offset(M, jp[k] & kIdMask);
+            chunk[k] = load_chunk(D, off[k]);
+        }
+#pragma unroll
+        for (int k = 0; k < K; k++) decode_bounds(M, jp[k] & kIdMask, off[k], chunk[k], l[k], u[k]);
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            smin[k] = 0.0;
+            smax[k] = 0.0;
+            if (first + k >= len) {
+                l[k] = 0.0;
+                u[k] = 0.0;
             } else {
-                load_bounds(M, D, jp[k] & kIdMask, l[k], u[k]);
                 const Contribution q = contribution(a[k], l[k], u[k]);
                 smin[k] = q.min_part;
                 smax[k] = q.max_part;
@@ -924,8 +962,9 @@ __device__ bool grid_barrier(const DevState &D, unsigned nblocks, bool invalidat
 // unit moved a bound.  Phases of a step: sweep; apply of the split rows when
 // the sweep queued any; publish.  The number of steps run goes to the global
 // status block (-1 after a failed wait).
-__global__ __launch_bounds__(256) void mega_kernel(DevMatrix M, DevState D, int first_step, int max_steps,
-                                                   int n_units) {
+template <int kThreads>
+__global__ __launch_bounds__(kThreads) void mega_kernel(DevMatrix M, DevState D, int first_step, int max_steps,
+                                                        int n_units) {
     const unsigned nblocks = gridDim.x;
     const int waves = blockDim.x >> 6;
     const int wave = uniform(blockIdx.x * waves + (threadIdx.x >> 6));
@@ -947,6 +986,10 @@ __global__ __launch_bounds__(256) void mega_kernel(DevMatrix M, DevState D, int 
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) D.global[kGlobalSteps] = steps;
 }
+
+template __global__ void mega_kernel<256>(DevMatrix, DevState, int, int, int);
+template __global__ void mega_kernel<512>(DevMatrix, DevState, int, int, int);
+template __global__ void mega_kernel<1024>(DevMatrix, DevState, int, int, int);
 
 // ---------------------------------------------------------------------------
 // loading and reading bound vectors

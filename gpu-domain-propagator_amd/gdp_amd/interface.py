@@ -99,6 +99,7 @@ class Tuning(ctypes.Structure):
     _fields_ = [
         ("item_nnz", c_int),
         ("mega_blocks_per_cu", c_int),
+        ("mega_threads", c_int),
         ("graph_rounds", c_int),
         ("max_split_segs", c_int),
         ("compact_bounds", c_int),
@@ -130,6 +131,8 @@ class ProblemStats(ctypes.Structure):
         ("nf", c_int),
         ("n_units", c_int),
         ("unit_state_bytes", c_longlong),
+        ("mega_threads", c_int),
+        ("mega_grid", c_int),
     ]
 
 
@@ -167,7 +170,7 @@ def load_library(path: Optional[str] = None) -> ctypes.CDLL:
     lib.gdpPropagate.restype = c_int
     lib.gdpPropagateDevice.argtypes = [vp, vp, vp, c_int, c_int, POINTER(RunInfo)]
     lib.gdpPropagateDevice.restype = c_int
-    lib.gdpProblemWorkProfile.argtypes = [vp, vp, vp, c_int, vp, vp, vp, POINTER(c_int), POINTER(RunInfo)]
+    lib.gdpProblemWorkProfile.argtypes = [vp, vp, vp, c_int, vp, vp, vp, vp, POINTER(c_int), POINTER(RunInfo)]
     lib.gdpProblemWorkProfile.restype = c_int
     lib.gdpProblemReserve.argtypes = [vp, c_int]
     lib.gdpProblemReserve.restype = c_int
@@ -449,22 +452,23 @@ class Problem:
 
     def work_profile(self, lbs, ubs, max_rounds: int = MAX_NUM_ROUNDS):
         """Per-round work of one propagation (gdpProblemWorkProfile): returns
-        (outcome, rows, nnz, cols), the last three int64 arrays with one entry
-        per round run."""
+        (outcome, rows, nnz, cols, accepted), the last four int64 arrays with
+        one entry per round run."""
         lb = np.array(lbs, dtype=np.float64, copy=True)
         ub = np.array(ubs, dtype=np.float64, copy=True)
         rows = np.zeros(max_rounds, np.int64)
         nnz = np.zeros(max_rounds, np.int64)
         cols = np.zeros(max_rounds, np.int64)
+        acc = np.zeros(max_rounds, np.int64)
         n_rounds = c_int(0)
         info = RunInfo()
         rc = self.lib.gdpProblemWorkProfile(self.handle, _ptr(lb), _ptr(ub), max_rounds, _ptr(rows), _ptr(nnz),
-                                            _ptr(cols), ctypes.byref(n_rounds), ctypes.byref(info))
+                                            _ptr(cols), _ptr(acc), ctypes.byref(n_rounds), ctypes.byref(info))
         if rc != GDP_OKAY:
             raise RuntimeError(f"gdpProblemWorkProfile failed: {last_error()}")
         k = n_rounds.value
         out = Outcome(lb, ub, info.rounds, bool(info.infeasible), bool(info.capped), info.kernel_ms)
-        return out, rows[:k], nnz[:k], cols[:k]
+        return out, rows[:k], nnz[:k], cols[:k], acc[:k]
 
     def propagate_device(self, d_lbs_ptr: int, d_ubs_ptr: int, sync_type: int = CPU_LOOP,
                          max_rounds: int = MAX_NUM_ROUNDS) -> RunInfo:

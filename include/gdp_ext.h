@@ -32,6 +32,7 @@ typedef struct {
 typedef struct {
     int item_nnz;            /* target nonzeros per work item (0: default 1024) */
     int mega_blocks_per_cu;  /* persistent grid: workgroups per CU (0: occupancy) */
+    int mega_threads;        /* persistent kernel workgroup size: 256, 512 (default) or 1024 */
     int graph_rounds;        /* rounds per captured graph in GPU_LOOP (0: 8) */
     int max_split_segs;      /* rows longer than 256 entries with up to this many
                                 256-entry segments are split into one wave per
@@ -53,6 +54,8 @@ typedef struct {
     int n16, n32, nf;        /* columns with 2x int16, 2x int32, 2x double records */
     int n_units;             /* instances of a batch, or bound-vector capacity */
     long long unit_state_bytes; /* device bytes of one bound vector's state */
+    int mega_threads;           /* persistent kernel: workgroup size */
+    int mega_grid;              /* persistent kernel: workgroups */
 } gdp_problem_stats;
 
 /* Fills *t with the defaults. */
@@ -84,9 +87,12 @@ GDP_RETCODE gdpPropagate(gdp_problem p, double *lbs, double *ubs, const int sync
  * k+1 computes, their nonzeros and the distinct columns they hold; the
  * number of rounds run is stored in *n_rounds.  A diagnostic (it reads the
  * row marks back after every round), e.g. for the bytes a round moves.
+ * accepted[k]: the bound candidates round k+1 accepted (each folded into the
+ * bounds by an atomic).
  * No counterpart in the reference, whose rounds always process every row. */
 GDP_RETCODE gdpProblemWorkProfile(gdp_problem p, double *lbs, double *ubs, const int max_rounds, long long *rows,
-                                  long long *nnz, long long *cols, int *n_rounds, gdp_run_info *info);
+                                  long long *nnz, long long *cols, long long *accepted, int *n_rounds,
+                                  gdp_run_info *info);
 
 /* Same with device pointers (hipMalloc'd, length n_vars each); no host copy.
  * The library works on its own stream: work that produced d_lbs / d_ubs on

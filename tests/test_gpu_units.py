@@ -165,7 +165,7 @@ def test_work_profile(gpu, oracle, case):
     nonempty = lens > 0
     p = I.Problem(inst)
     try:
-        out, rows, nnz, cols = p.work_profile(inst.lb, inst.ub)
+        out, rows, nnz, cols, acc = p.work_profile(inst.lb, inst.ub)
     finally:
         p.close()
     assert_same(out, exp, case)
@@ -185,7 +185,7 @@ def test_work_profile(gpu, oracle, case):
     assert n_marked > 0, "no round used row marks"
     p = I.Problem(inst, {"row_marking": 0})
     try:
-        out, rows, nnz, cols = p.work_profile(inst.lb, inst.ub)
+        out, rows, nnz, cols, acc = p.work_profile(inst.lb, inst.ub)
     finally:
         p.close()
     assert_same(out, exp, case)
