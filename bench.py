@@ -310,13 +310,18 @@ def _counter_per_dispatch(rows, name_part, counter):
 def summarize(d, kernel="mega_kernel"):
     stats = _csv_rows(f"{d}/trace/**/*kernel_stats.csv")
     line = next(r for r in stats if kernel in r["Name"])
-    avg_ns = float(line["AverageNs"])
+    with open(f"{d}/meta.json") as fh:
+        meta = json.load(fh)
+    # the timed launches: the kernel trace in dispatch order, less the warmup
+    trace = [r for r in _csv_rows(f"{d}/trace/**/*kernel_trace.csv") if kernel in r["Kernel_Name"]]
+    trace.sort(key=lambda r: int(r["Start_Timestamp"]))
+    durs = [int(r["End_Timestamp"]) - int(r["Start_Timestamp"]) for r in trace]
+    timed = durs[meta.get("warmup", 0):][:meta.get("steps", len(durs))]
+    avg_ns = statistics.mean(timed)
     fetch = _counter_per_dispatch(_csv_rows(f"{d}/fetch/**/*counter_collection.csv"), kernel, "FETCH_SIZE")
     write = _counter_per_dispatch(_csv_rows(f"{d}/write/**/*counter_collection.csv"), kernel, "WRITE_SIZE")
     cal_f = _counter_per_dispatch(_csv_rows(f"{d}/cal_fetch/**/*counter_collection.csv"), "copyBuffer", "FETCH_SIZE")
     cal_w = _counter_per_dispatch(_csv_rows(f"{d}/cal_write/**/*counter_collection.csv"), "copyBuffer", "WRITE_SIZE")
-    with open(f"{d}/meta.json") as fh:
-        meta = json.load(fh)
     cal_bytes = meta["calibration_copy_bytes"]
     # the calibration copies are the largest copyBuffer dispatches of their run
     kf = statistics.median(sorted(cal_f.values())[-3:]) * 1024.0 / cal_bytes
@@ -339,6 +344,9 @@ def summarize(d, kernel="mega_kernel"):
         "kernel": kernel,
         "workload": meta["workload"],
         "avg_launch_ns": avg_ns,
+        "timed_launches": len(timed),
+        "stats_average_ns_all_launches": float(line["AverageNs"]),
+        "bench_launch_ms_same_run": meta.get("bench_launch_ms"),
         "calls": int(line["Calls"]),
         "fetch_size_bytes_per_launch": fetch_b,
         "write_size_bytes_per_launch": write_b,
@@ -363,7 +371,12 @@ def summarize(d, kernel="mega_kernel"):
             fh.write(f"    {c}\n")
         fh.write("\n| quantity | value |\n|---|---|\n")
         fh.write(f"| {kernel} calls (kernel-trace run) | {int(line['Calls'])} |\n")
-        fh.write(f"| average launch duration (rocprofv3 --stats) | {avg_ns / 1e3:.1f} us |\n")
+        fh.write(f"| average duration of the {len(timed)} timed launches (kernel trace, warmup excluded) | "
+                 f"{avg_ns / 1e3:.1f} us |\n")
+        fh.write(f"| rocprofv3 --stats average over all {int(line['Calls'])} launches (incl. warmup) | "
+                 f"{float(line['AverageNs']) / 1e3:.1f} us |\n")
+        if meta.get("bench_launch_ms"):
+            fh.write(f"| bench.py's own HIP-event average of the same launches | {meta['bench_launch_ms'] * 1e3:.1f} us |\n")
         fh.write(f"| FETCH_SIZE per launch | {fetch_b / 1e6:.1f} MB |\n")
         fh.write(f"| WRITE_SIZE per launch | {write_b / 1e6:.1f} MB |\n")
         fh.write(f"| copy calibration ({cal_bytes / 2**20:.0f} MiB device copy) | FETCH x{kf:.3f}, WRITE x{kw:.3f} |\n")
@@ -467,7 +480,8 @@ def main():
     if args.meta_out:
         with open(args.meta_out, "w") as fh:
             json.dump({"workload": {"m": inst.m, "n": inst.n, "nnz": inst.nnz, "rounds": rounds},
-                       "algorithmic_bytes_per_launch": alg, "dense_sweep_bytes_per_launch": dense}, fh)
+                       "algorithmic_bytes_per_launch": alg, "dense_sweep_bytes_per_launch": dense,
+                       "warmup": args.warmup, "steps": args.steps, "bench_launch_ms": kernel_ms}, fh)
 
     traffic = load_traffic(inst, rounds, kernel) if args.sync == 2 else None
     result = {
