@@ -35,6 +35,8 @@ struct Tuning {
     int max_split_segs = -1;  // rows with more segments run on one wave (-1: split all, 0: none)
     int compact_bounds = 1;   // 0: keep every column in 16-byte FP64 records
     int row_marking = 1;      // 0: every round processes every row of an active unit
+    int mark_limit = 0;       // accepted candidates of a round from which the next round
+                              // processes every row (0: max(64, m n / (4 nnz)))
     // per tree-width bin (width 2^b, b = 0..9): lanes per row as log2
     // default: K = min(width, 4) consecutive entries per lane
     int group_log[10] = {0, 0, 0, 1, 2, 3, 4, 5, 6, 6};

@@ -262,7 +262,9 @@ Problem *build(const std::vector<InstanceView> &inst, bool replicas, const Tunin
         // row marking pays while the rows it marks (about accepted candidates
         // x mean column length) stay below a quarter of the rows
         for (int i = 0; i < NI; i++) {
-            const long long lim = static_cast<long long>(inst[i].m) * inst[i].n / (4LL * std::max(1, inst[i].nnz));
+            const long long lim = T.mark_limit > 0 ? T.mark_limit
+                                                    : static_cast<long long>(inst[i].m) * inst[i].n /
+                                                          (4LL * std::max(1, inst[i].nnz));
             P->mark_limit.push_back(static_cast<int>(std::max(64LL, std::min(lim, 1LL << 30))));
         }
         if (replicas) P->mark_limit.resize(1);
@@ -956,6 +958,17 @@ Tuning autotune(const std::vector<InstanceView> &instances, const double *lbs, c
             Tuning t = best;
             t.row_marking = best.row_marking ? 0 : 1;
             try_one(t);
+        }
+        if (best.row_marking) {
+            // the full-sweep threshold: the default and larger ones (marks
+            // pushed in more rounds)
+            const auto &v = instances[0];
+            const long long def = std::max(64LL, static_cast<long long>(v.m) * v.n / (4LL * std::max(1, v.nnz)));
+            for (long long f : {4LL, 16LL}) {
+                Tuning t = best;
+                t.mark_limit = static_cast<int>(std::min(def * f, 1LL << 30));
+                if (t.mark_limit != best.mark_limit) try_one(t);
+            }
         }
     } catch (...) {
         (void)hipFree(d);

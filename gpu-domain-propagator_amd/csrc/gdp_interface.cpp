@@ -250,6 +250,7 @@ void gdpTuningDefaults(gdp_tuning *t) {
     t->max_split_segs = d.max_split_segs;
     t->compact_bounds = d.compact_bounds;
     t->row_marking = d.row_marking;
+    t->mark_limit = d.mark_limit;
     for (int b = 0; b < 10; b++) t->group_log[b] = d.group_log[b];
 }
 
@@ -265,6 +266,7 @@ gdp::Tuning tuning_from(const gdp_tuning *tuning) {
         if (tuning->max_split_segs >= 0) T.max_split_segs = tuning->max_split_segs;
         if (tuning->compact_bounds >= 0) T.compact_bounds = tuning->compact_bounds;
         if (tuning->row_marking >= 0) T.row_marking = tuning->row_marking;
+        if (tuning->mark_limit >= 0) T.mark_limit = tuning->mark_limit;
         for (int b = 0; b < 10; b++)
             if (tuning->group_log[b] >= 0) T.group_log[b] = tuning->group_log[b];
     }
@@ -427,6 +429,7 @@ void tuning_to(const gdp::Tuning &T, gdp_tuning *t) {
     t->max_split_segs = T.max_split_segs;
     t->compact_bounds = T.compact_bounds;
     t->row_marking = T.row_marking;
+    t->mark_limit = T.mark_limit;
     for (int b = 0; b < 10; b++) t->group_log[b] = T.group_log[b];
 }
 

@@ -104,6 +104,7 @@ class Tuning(ctypes.Structure):
         ("max_split_segs", c_int),
         ("compact_bounds", c_int),
         ("row_marking", c_int),
+        ("mark_limit", c_int),
         ("group_log", c_int * 10),
     ]
 

@@ -41,6 +41,8 @@ typedef struct {
                                 envelope use 4- or 8-byte bound records; 0: FP64 only */
     int row_marking;         /* 1 (default): a round processes only rows with a column
                                 that moved in the previous round; 0: every row */
+    int mark_limit;          /* accepted candidates of a round from which the next round
+                                processes every row instead (0: default max(64, m n / (4 nnz))) */
     int group_log[10];       /* lanes per row (log2) for rows of width 2^b; -1: default */
 } gdp_tuning;
 
@@ -121,7 +123,7 @@ GDP_RETCODE gdpPropagateNodesDevice(gdp_problem p, const int count, double *cons
  * Builds the problem with candidate tunings and times the propagation of
  * lbs/ubs to the fixpoint with sync_type: a coordinate search over the
  * work-item size, the lanes per row of each row-length bin, the persistent
- * grid size (MEGAKERNEL) and row marking, starting from *tuning (or the
+ * kernel's workgroup size (MEGAKERNEL) and row marking, starting from *tuning (or the
  * defaults when it holds the defaults).  *tuning receives the fastest
  * candidate and *best_ms its device time.  The bounds double as the bound
  * envelope.  Every candidate gives the same bits. */

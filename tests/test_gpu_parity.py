@@ -161,6 +161,7 @@ TUNINGS = {
     "fp64_bounds": dict(compact_bounds=0),       # no compact bound records
     "one_entry_per_lane": dict(group_log=[0, 1, 2, 3, 4, 5, 6, 6, 6, 6]),
     "small_items": dict(item_nnz=128),
+    "marks_always": dict(mark_limit=1 << 30),     # row marks in every round, never a full sweep
     "mega_256": dict(mega_threads=256),          # persistent workgroups of 256 / 1024 threads
     "mega_1024": dict(mega_threads=1024),
 }
