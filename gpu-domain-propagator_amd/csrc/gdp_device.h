@@ -47,7 +47,10 @@
 //                  bounds cross)
 //   bm[nwords]     uint32  columns moved in the current step (scanned by publish)
 //   marks[m']      int32   step in which the row must be processed
-//   split rows     seg_parts, seg_state, seg_count
+//   split rows     seg_parts, seg_state, seg_count, seg_marks: a marked split
+//                  row recomputes only the segments holding a column that
+//                  moved in the previous step (and segments not yet computed
+//                  in the run); the others keep their stored partials
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -115,6 +118,8 @@ constexpr long long kWaitTimeoutTicks = 100000000LL;
 struct SegPart {
     double smin, smax, dmax;
     int mininf, maxinf;
+    int stamp;   // step in which the partial was computed (0: not in this run)
+    int pad;
 };
 
 // Activity of a split row, written by the last segment wave of a step
@@ -143,6 +148,7 @@ struct DevMatrix {
     int n_seg_parts;
     const int *col_ptr;    // csc of the internal ids (marking)
     const int *col_rows;
+    const int *col_seg;    // per csc entry: its segment of a split row (-1: other rows)
     const int *perm;       // internal id -> caller's column (-1: padding id)
     const int *word_unit;  // blocks mode: unit of each bitmap word
     int snap_off32;        // byte offset of S32 in a unit's snapshot block
@@ -160,6 +166,7 @@ struct DevState {
     unsigned *bm;
     int *marks;
     SegPart *seg_parts;
+    int *seg_marks;        // step in which a split row's segment must be recomputed
     SegState *seg_state;
     unsigned *seg_count;   // segments reduced in the current step
     // shared by all units
@@ -188,6 +195,7 @@ __device__ __forceinline__ DevState unit_view(const DevMatrix &M, const DevState
     V.bm += k * M.nwords;
     V.marks += k * M.m;
     V.seg_parts += k * M.n_seg_parts;
+    V.seg_marks += k * M.n_seg_parts;
     V.seg_state += k * M.n_seg_rows;
     V.seg_count += k * M.n_seg_rows;
     return V;

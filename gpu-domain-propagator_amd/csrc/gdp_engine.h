@@ -27,7 +27,8 @@ constexpr int GPU_LOOP_ID = 1;
 constexpr int MEGAKERNEL_ID = 2;
 
 struct Tuning {
-    int item_nnz = 1024;      // target nonzeros per work item
+    int item_nnz = 0;         // target nonzeros per work item (0: 2048 for instances of
+                              // at least 4M nonzeros, else 1024)
     int block_threads = 256;  // threads per workgroup (multiple of 64)
     int mega_blocks_per_cu = 0;  // 0: occupancy limit
     int mega_threads = 512;   // workgroup size of the persistent kernel: 256, 512 or 1024

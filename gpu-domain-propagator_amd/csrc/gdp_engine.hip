@@ -214,6 +214,7 @@ void alloc_units(Problem *P, int cap) {
     D.bm = dev_alloc<unsigned>(k * M.nwords, o);
     D.marks = dev_alloc<int>(k * M.m, o);
     D.seg_parts = dev_alloc<SegPart>(k * M.n_seg_parts, o);
+    D.seg_marks = dev_alloc<int>(k * M.n_seg_parts, o);
     D.seg_state = dev_alloc<SegState>(k * M.n_seg_rows, o);
     D.seg_count = dev_alloc<unsigned>(k * M.n_seg_rows, o);
     const int units = P->replicas ? cap : P->n_inst;
@@ -234,7 +235,7 @@ void alloc_units(Problem *P, int cap) {
     P->n_units = units;
     D.active = P->d_list;
     const long long per_unit = static_cast<long long>(M.n16) * 4 + n32 * 8LL + nf * 16LL + M.n_int * 24LL +
-                               nf * 16LL + M.nwords * 8LL + M.m * 4LL + M.n_seg_parts * 36LL +
+                               nf * 16LL + M.nwords * 8LL + M.m * 4LL + M.n_seg_parts * 44LL +
                                M.n_seg_rows * 28LL + sizeof(UnitState) + kMoveSlots * kMoveLanes * 4;
     P->stats.unit_state_bytes = per_unit;
     P->stats.n_units = units;
@@ -383,6 +384,10 @@ Problem *build(const std::vector<InstanceView> &inst, bool replicas, const Tunin
         }
 
         // ---- sweep items ----
+        // larger group items for large instances (fewer, longer waves)
+        int max_inst_nnz = 0;
+        for (const auto &v : inst) max_inst_nnz = std::max(max_inst_nnz, v.nnz);
+        const int item_nnz = T.item_nnz > 0 ? T.item_nnz : (max_inst_nnz >= 4000000 ? 2048 : 1024);
         // Order: segments of split rows, whole very long rows, then the
         // tree-width bins from wide to narrow, so that long work starts first.
         std::vector<int4> h_items;
@@ -422,7 +427,7 @@ Problem *build(const std::vector<InstanceView> &inst, bool replicas, const Tunin
                 logg = std::min(logg, 6);
                 const int logk = b - logg;
                 const int per_pass = 64 >> logg;
-                int rows_per_item = std::max(1, T.item_nnz / width);
+                int rows_per_item = std::max(1, item_nnz / width);
                 rows_per_item = ((rows_per_item + per_pass - 1) / per_pass) * per_pass;
                 int q = 0;
                 while (q < cnt) {
@@ -441,6 +446,20 @@ Problem *build(const std::vector<InstanceView> &inst, bool replicas, const Tunin
             r += cnt;
         }
         if (NI >= (1 << (31 - kItemUnitShift))) throw std::runtime_error("too many instances in a batch");
+
+        // segment of each csc entry that lies in a split row (same fill order
+        // as col_rows)
+        std::vector<int> h_cseg(h_crows.size(), -1);
+        {
+            std::vector<int> row_part(mp, -1);
+            for (const int4 &sr : h_seg_rows) row_part[sr.x] = sr.y;
+            std::vector<int> fill(h_cptr.begin(), h_cptr.end() - 1);
+            for (int rr = 0; rr < mp; rr++) {
+                const int st = h_rp[rr] & ~(kRowAlign - 1);
+                for (int e = 0; e < h_len[rr]; e++)
+                    h_cseg[fill[h_col[st + e] & kIdMask]++] = row_part[rr] >= 0 ? row_part[rr] + e / kSegLen : -1;
+            }
+        }
 
         P->stats.m = P->m;
         P->stats.n = P->n;
@@ -479,6 +498,7 @@ Problem *build(const std::vector<InstanceView> &inst, bool replicas, const Tunin
         int *d_part_row = dev_alloc<int>(h_part_row.size(), o);
         int *d_cptr = dev_alloc<int>(n_ids + 1, o);
         int *d_crows = dev_alloc<int>(h_crows.size(), o);
+        int *d_cseg = dev_alloc<int>(h_cseg.size(), o);
         int *d_perm = dev_alloc<int>(n_ids, o);
         int *d_word_unit = dev_alloc<int>(word_unit.size(), o);
         P->d_inv_perm = dev_alloc<int>(P->n, o);
@@ -491,6 +511,7 @@ Problem *build(const std::vector<InstanceView> &inst, bool replicas, const Tunin
         upload(d_part_row, h_part_row.data(), h_part_row.size(), s);
         upload(d_cptr, h_cptr.data(), n_ids + 1, s);
         upload(d_crows, h_crows.data(), h_crows.size(), s);
+        upload(d_cseg, h_cseg.data(), h_cseg.size(), s);
         upload(d_perm, perm.data(), n_ids, s);
         upload(d_word_unit, word_unit.data(), word_unit.size(), s);
         upload(P->d_inv_perm, internal.data(), P->n, s);
@@ -503,6 +524,7 @@ Problem *build(const std::vector<InstanceView> &inst, bool replicas, const Tunin
         M.part_row = d_part_row;
         M.col_ptr = d_cptr;
         M.col_rows = d_crows;
+        M.col_seg = d_cseg;
         M.perm = d_perm;
         M.word_unit = d_word_unit;
 
@@ -789,6 +811,16 @@ RunResult problem_work_profile(Problem *P, double *lbs, double *ubs, int max_rou
     if (n_padded > 0)
         check(hipMemcpy(cols.data(), M.col, sizeof(int) * n_padded, hipMemcpyDeviceToHost), "D2H col");
     std::vector<int> marks(M.m), stamp(M.n, 0);
+    // split rows: a marked row recomputes only some segments (the kernel's
+    // rule: every row processed in a full sweep; otherwise segments not yet
+    // computed in the run or marked for the step)
+    std::vector<int4> seg_rows(M.n_seg_rows);
+    if (M.n_seg_rows > 0)
+        check(hipMemcpy(seg_rows.data(), M.seg_rows, sizeof(int4) * M.n_seg_rows, hipMemcpyDeviceToHost),
+              "D2H seg_rows");
+    std::vector<int> split_of(M.m, -1);
+    for (int q = 0; q < M.n_seg_rows; q++) split_of[seg_rows[q].x] = q;
+    std::vector<int> seg_marks(std::max(1, M.n_seg_parts)), seg_stamp(std::max(1, M.n_seg_parts), 0);
     double *d_b = nullptr;
     check(hipMalloc(&d_b, 2 * sizeof(double) * static_cast<size_t>(n)), "hipMalloc bounds");
     RunResult res;
@@ -807,21 +839,39 @@ RunResult problem_work_profile(Problem *P, double *lbs, double *ubs, int max_rou
             if (step == 1 ? st[0].crossed != 0 : !active_after(st[0], step - 1)) break;
             RoundWork w;
             const bool all_rows = !M.mark_rows || st[0].markall == step;
-            if (!all_rows)
+            if (!all_rows) {
                 check(hipMemcpy(marks.data(), P->D.marks, sizeof(int) * M.m, hipMemcpyDeviceToHost), "D2H marks");
+                if (M.n_seg_parts > 0)
+                    check(hipMemcpy(seg_marks.data(), P->D.seg_marks, sizeof(int) * M.n_seg_parts,
+                                    hipMemcpyDeviceToHost),
+                          "D2H seg_marks");
+            }
+            auto count_entries = [&](int start, int from, int to) {
+                w.nnz += to - from;
+                for (int e = from; e < to; e++) {
+                    const int id = cols[start + e] & kIdMask;
+                    if (stamp[id] != step) {
+                        stamp[id] = step;
+                        w.cols++;
+                    }
+                }
+            };
             for (int r = 0; r < M.m; r++) {
                 if (!all_rows && marks[r] != step) continue;
                 const int a = rp[r], b = rp[r + 1];
                 const int start = a & ~(kRowAlign - 1);
                 const int len = (b & ~(kRowAlign - 1)) - start - (a & (kRowAlign - 1));
                 w.rows++;
-                w.nnz += len;
-                for (int e = 0; e < len; e++) {
-                    const int id = cols[start + e] & kIdMask;
-                    if (stamp[id] != step) {
-                        stamp[id] = step;
-                        w.cols++;
-                    }
+                const int q = split_of[r];
+                if (q < 0) {
+                    count_entries(start, 0, len);
+                    continue;
+                }
+                for (int g = 0; g < seg_rows[q].z; g++) {
+                    const int part = seg_rows[q].y + g;
+                    if (!all_rows && seg_stamp[part] >= 1 && seg_marks[part] != step) continue;
+                    seg_stamp[part] = step;
+                    count_entries(start, g * kSegLen, std::min(len, (g + 1) * kSegLen));
                 }
             }
             launch_step(P, step, 0, 1);

@@ -421,6 +421,7 @@ __device__ __forceinline__ int *status_row(const DevState &D, int step) {
 struct UnitStep {
     bool active;
     bool all_rows;
+    int start;  // step of the unit's round 1
 };
 
 __device__ __forceinline__ UnitStep unit_step(const DevMatrix &M, const DevState &D, int u, int step) {
@@ -435,6 +436,7 @@ __device__ __forceinline__ UnitStep unit_step(const DevMatrix &M, const DevState
     r.active = start > 0 && step >= start && !crossed && !envelope && step - start < max_rounds &&
                (step == start || (prev_changed == step - 1 && infeasible == 0));
     r.all_rows = !M.mark_rows || markall == step;
+    r.start = start;
     return r;
 }
 
@@ -563,14 +565,17 @@ __device__ __forceinline__ double split_eps_scan(const DevMatrix &M, const DevSt
     return run;
 }
 
-// Stores the activity partial of segment `part` (lane 0 of the wave).
-__device__ __forceinline__ void store_part(const DevState &V, int part, const LaneEntries<kSegLen / 64> &E) {
+// Stores the activity partial of segment `part` and the step that computed
+// it (lane 0 of the wave).
+__device__ __forceinline__ void store_part(const DevState &V, int part, const LaneEntries<kSegLen / 64> &E,
+                                           int step) {
     SegPart *p = &V.seg_parts[part];
     coherent_store_d(&p->smin, E.tmin);
     coherent_store_d(&p->smax, E.tmax);
     coherent_store_d(&p->dmax, E.dmax);
     coherent_store(&p->mininf, E.mininf);
     coherent_store(&p->maxinf, E.maxinf);
+    coherent_store(&p->stamp, step);
 }
 
 // Segments [s, s + cnt) of split row lr (cnt <= kSegsPerItem) for unit u.
@@ -580,7 +585,7 @@ __device__ __forceinline__ void store_part(const DevState &V, int part, const La
 // the row gate; when it passes, the row's activity is stored and its
 // segments are queued for the apply phase that follows the sweep.
 __device__ __forceinline__ void seg_item(const DevMatrix &M, const DevState &D, const DevState &V, int u, int lr, int s, int cnt,
-                         int *st) {
+                         int *st, int step, bool all_rows, int unit_start) {
     constexpr int K = kSegLen / 64;
     const int lane = threadIdx.x & 63;
     const int4 info = M.seg_rows[lr];
@@ -588,11 +593,20 @@ __device__ __forceinline__ void seg_item(const DevMatrix &M, const DevState &D, 
     int start, len;
     row_extent(M, r, start, len);
     for (int q = 0; q < cnt; q++) {
+        const int part = info.y + s + q;
+        // a segment computed earlier in this run none of whose columns moved
+        // in the previous step keeps its partial (its columns' bounds have
+        // not changed since it was computed: every change marks it)
+        if (!all_rows) {
+            const int stamp = uniform(coherent_load(&V.seg_parts[part].stamp));
+            const int mark = uniform(coherent_load(&V.seg_marks[part]));
+            if (stamp >= unit_start && mark != step) continue;
+        }
         const int s0 = (s + q) * kSegLen;
         LaneEntries<K> E;
         E.load(M, V, start + s0, lane * K, len - s0);
         E.reduce(64);
-        if (lane == 0) store_part(V, info.y + s + q, E);
+        if (lane == 0) store_part(V, part, E, step);
     }
     unsigned old = 0;
     if (lane == 0) {
@@ -677,7 +691,7 @@ __device__ __forceinline__ void sweep_item(const DevMatrix &M, const DevState &D
     int moved = 0;
     if (kind == kItemSeg) {
         const int r = M.seg_rows[d.y].x;
-        if (us.all_rows || V.marks[r] == step) seg_item(M, D, V, u, d.y, d.z, d.w, st);
+        if (us.all_rows || V.marks[r] == step) seg_item(M, D, V, u, d.y, d.z, d.w, st, step, us.all_rows, us.start);
         return;
     }
     if (kind == kItemLong) {
@@ -809,7 +823,11 @@ __device__ __forceinline__ void publish_word(const DevMatrix &M, const DevState 
             if (cand < 32 && __shfl(pre, cand, 32) <= i) lo = cand;
         }
         const int k = __shfl(c0, lo, 32) + i - __shfl(pre, lo, 32);
-        if (i < total) coherent_store(&V.marks[M.col_rows[k]], step + 1);
+        if (i < total) {
+            coherent_store(&V.marks[M.col_rows[k]], step + 1);
+            const int sp = M.col_seg[k];
+            if (sp >= 0) coherent_store(&V.seg_marks[sp], step + 1);
+        }
     }
 }
 
@@ -1063,6 +1081,7 @@ __global__ __launch_bounds__(256) void load_kernel(DevMatrix M, DevState D, int 
             for (int w = tid; w < M.nwords; w += stride) V.bm[w] = 0u;
             for (int r = tid; r < M.m; r += stride) V.marks[r] = 0;
             for (int q = tid; q < M.n_seg_rows; q += stride) V.seg_count[q] = 0u;
+            for (int q = tid; q < M.n_seg_parts; q += stride) V.seg_parts[q].stamp = 0;
         }
     }
 }

@@ -137,8 +137,8 @@ def test_row_marking_off_same_bits(gpu, oracle, case):
 
 
 def _changed_row_sets(inst, oracle, rounds):
-    """Per round k >= 2: the rows holding a column that round k-1 changed
-    (round-synchronous oracle, one round at a time)."""
+    """Per round k >= 2: the rows holding a column that round k-1 changed,
+    and those columns (round-synchronous oracle, one round at a time)."""
     lens = np.diff(inst.row_ptr)
     row_of = np.repeat(np.arange(inst.m), lens)
     prev = (inst.lb, inst.ub)
@@ -148,7 +148,7 @@ def _changed_row_sets(inst, oracle, rounds):
         changed = (cur[0] != prev[0]) | (cur[1] != prev[1])
         rows = np.zeros(inst.m, bool)
         rows[row_of[changed[inst.col]]] = True
-        sets.append(rows)
+        sets.append((rows, changed))
         prev = cur[:2]
     return sets
 
@@ -157,7 +157,10 @@ def _changed_row_sets(inst, oracle, rounds):
 def test_work_profile(gpu, oracle, case):
     """Round 1 processes every nonempty row; a later round processes every
     row (marking gave way to a full sweep) or exactly the rows holding a
-    column the previous round changed.  The run itself is the propagation."""
+    column the previous round changed.  A row longer than 256 entries is
+    processed in 256-entry segments, and in a marked round only its segments
+    holding a changed column (or not computed before in the run) are summed
+    again.  The run itself is the propagation."""
     from test_gpu_parity import _case
     inst = _case(case)
     exp = oracle.jacobi(inst)
@@ -172,16 +175,35 @@ def test_work_profile(gpu, oracle, case):
     assert len(rows) == exp[2]
     all_cols = np.unique(inst.col).size
     assert (rows[0], nnz[0], cols[0]) == (nonempty.sum(), inst.nnz, all_cols)
+    seg = 256
+    computed = set()  # (row, segment) summed earlier in the run
+    for r in np.flatnonzero(lens > seg):
+        computed.update((r, g) for g in range((lens[r] + seg - 1) // seg))
     n_marked = 0
-    for k, marked in enumerate(_changed_row_sets(inst, oracle, exp[2]), start=1):
+    for k, (marked, changed) in enumerate(_changed_row_sets(inst, oracle, exp[2]), start=1):
         if rows[k] == nonempty.sum() and nnz[k] == inst.nnz:
+            for r in np.flatnonzero(lens > seg):
+                computed.update((r, g) for g in range((lens[r] + seg - 1) // seg))
             continue
         n_marked += 1
         sel = marked & nonempty
         assert rows[k] == sel.sum(), f"round {k + 1}"
-        assert nnz[k] == lens[sel].sum(), f"round {k + 1}"
-        assert cols[k] == np.unique(np.concatenate([inst.col[inst.row_ptr[r]:inst.row_ptr[r + 1]]
-                                                    for r in np.flatnonzero(sel)] or [[]])).size
+        want_nnz, want_cols = 0, set()
+        for r in np.flatnonzero(sel):
+            c = inst.col[inst.row_ptr[r]:inst.row_ptr[r + 1]]
+            if lens[r] <= seg:
+                want_nnz += lens[r]
+                want_cols.update(c.tolist())
+                continue
+            for g in range((lens[r] + seg - 1) // seg):
+                cg = c[g * seg:(g + 1) * seg]
+                if (r, g) in computed and not changed[cg].any():
+                    continue
+                computed.add((r, g))
+                want_nnz += cg.size
+                want_cols.update(cg.tolist())
+        assert nnz[k] == want_nnz, f"round {k + 1}"
+        assert cols[k] == len(want_cols), f"round {k + 1}"
     assert n_marked > 0, "no round used row marks"
     p = I.Problem(inst, {"row_marking": 0})
     try:
