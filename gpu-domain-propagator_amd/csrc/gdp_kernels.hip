@@ -789,7 +789,9 @@ __device__ __forceinline__ void publish_word(const DevMatrix &M, const DevState 
     const DevState V = unit_view(M, D, u);
     if (valid && b == 0) {
         coherent_store(&V.bm[w], 0u);
-        if (!M.replicas) atomic_max_i(&D.units[u].changed[step & 1], step);  // the unit moved a bound
+        // the unit moved a bound (read first: many words of a unit set it)
+        if (!M.replicas && coherent_load(&D.units[u].changed[step & 1]) != step)
+            atomic_max_i(&D.units[u].changed[step & 1], step);
     }
     const bool all_rows = all_rows_known >= 0 ? all_rows_known != 0 : decide_all_rows(M, D, u, valid, step);
     const bool moved = valid && ((bits >> b) & 1u);
@@ -853,8 +855,12 @@ __device__ __forceinline__ void publish_phase(const DevMatrix &M, const DevState
         unsigned long long pending = __ballot(bits != 0u);
         if (!pending) continue;
         if (lane == 0) {  // the chunk's unit (replicas) moved a bound in this step
-            if (M.replicas) atomic_max_i(&D.units[u].changed[step & 1], step);
-            atomic_or_i(&status_row(D, step)[kStatusChanged], 1);
+            // read first: most chunks find the flags set, and same-address
+            // atomics from every chunk serialise
+            if (M.replicas && coherent_load(&D.units[u].changed[step & 1]) != step)
+                atomic_max_i(&D.units[u].changed[step & 1], step);
+            if (coherent_load(&status_row(D, step)[kStatusChanged]) == 0)
+                atomic_or_i(&status_row(D, step)[kStatusChanged], 1);
         }
         // replicas: one decision for the chunk's unit
         const int known = M.replicas ? (decide_all_rows(M, D, u, true, step) ? 1 : 0) : -1;
