@@ -169,8 +169,9 @@ def load_traffic(inst, rounds, kernel):
 # ---------------------------------------------------------------------------
 
 def batched_leg(sync):
-    """256 uneven MIPs: one shared run vs back-to-back plain launches.
-    Returns the result record and the outputs for the parity check."""
+    """256 uneven MIPs: one shared run (with the default tuning, then with
+    the autotuner's pick) vs back-to-back plain launches.  Returns the result
+    record and the outputs for the parity check."""
     import numpy as np
     from gdp_amd import interface as I
     from gdp_amd.instances import batch_family, small_mip
@@ -188,8 +189,26 @@ def batched_leg(sync):
         infos = b.propagate_device(dl.ptr, du.ptr, sync)
         if rep:
             times.append(infos[0].kernel_ms)
-    batched_ms = statistics.median(times)
+    default_ms = statistics.median(times)
     out_l, out_u = dl.numpy(), du.numpy()
+    b.close()
+    # autotuned layout and persistent grid: a coordinate search on a sample of
+    # the batch (every 8th instance), applied to the whole batch
+    print("bench: autotuning the batch layout on a sample of 32 instances", file=sys.stderr, flush=True)
+    t_tune = time.perf_counter()
+    tuning, _ = I.batch_autotune(insts[::8], sync)
+    tune_s = time.perf_counter() - t_tune
+    b = I.Batch(insts, tuning)
+    times = []
+    for rep in range(4):
+        dl.copy_from(root_l)
+        du.copy_from(root_u)
+        tuned_infos = b.propagate_device(dl.ptr, du.ptr, sync)
+        if rep:
+            times.append(tuned_infos[0].kernel_ms)
+    tuned_ms = statistics.median(times)
+    assert np.array_equal(dl.numpy(), out_l) and np.array_equal(du.numpy(), out_u), "tuned batch differs"
+    batched_ms = tuned_ms
     # back-to-back plain launches: one device-resident problem per instance
     probs = [I.Problem(x) for x in insts]
     arrs = [(I.DeviceArray(x.lb), I.DeviceArray(x.ub)) for x in insts]
@@ -201,6 +220,7 @@ def batched_leg(sync):
                        f"[3, 30], {nnz} nonzeros in total, root bounds to the fixpoint",
            "sync": SYNC_NAMES[sync], "instances": len(insts), "nnz_total": nnz,
            "batched_ms": batched_ms, "nodes_per_s": len(insts) / (batched_ms * 1e-3),
+           "default_tuning_ms": default_ms, "autotuned": tuning, "autotune_s": tune_s,
            "plain_ms": plain_ms, "plain_nodes_per_s": len(insts) / (plain_ms * 1e-3),
            "speedup": plain_ms / batched_ms, "max_rounds": max(i.rounds for i in infos)}
     for p in probs:

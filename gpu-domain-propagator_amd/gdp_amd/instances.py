@@ -18,7 +18,12 @@ Generators (all deterministic in their seed, numpy PCG64):
                    feasible point plus slack U[0, 0.1*activity range].
   cover_knapsack   m=4e5 half covering rows (0/1, length U{3..40}, >= 1), half
                    knapsack rows (weights U{1..1000}, length U{10..200},
-                   capacity 0.3*sum of weights); n=2e6 binaries, 2% fixed.
+                   capacity 0.3*sum of weights); n=2e6 binaries, 2% fixed
+                   (a fraction one_frac of them to 1, the rest to 0).  With
+                   the default seed the fixings to 1 overfill some knapsack
+                   rows: round 1 proves the instance infeasible; with
+                   one_frac=0 it is feasible and reaches its fixpoint in 2
+                   rounds.
 """
 from __future__ import annotations
 
@@ -191,7 +196,7 @@ def powerlaw_mip(seed: int = 1, m: int = 1_000_000, n: int = 500_000, s: float =
 
 
 def cover_knapsack(seed: int = 2, m: int = 400_000, n: int = 2_000_000, fixed_frac: float = 0.02,
-                   name: str = "cover_knapsack") -> Instance:
+                   one_frac: float = 0.5, name: str = "cover_knapsack") -> Instance:
     rng = np.random.default_rng(seed)
     mc = m // 2
     mk = m - mc
@@ -209,7 +214,7 @@ def cover_knapsack(seed: int = 2, m: int = 400_000, n: int = 2_000_000, fixed_fr
     lb = np.zeros(n)
     ub = np.ones(n)
     fixed = rng.random(n) < fixed_frac
-    to_one = rng.random(n) < 0.5
+    to_one = rng.random(n) < one_frac
     lb[fixed & to_one] = 1.0
     ub[fixed & ~to_one] = 0.0
     return _finalize(m, n, row_ptr, c, val, lhs, rhs, lb, ub, vartype, name,

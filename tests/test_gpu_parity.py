@@ -14,7 +14,7 @@ import pytest
 
 import fixtures
 from gdp_amd import interface as I
-from gdp_amd.instances import INF, powerlaw_mip, small_mip
+from gdp_amd.instances import INF, cover_knapsack, powerlaw_mip, small_mip
 
 pytestmark = pytest.mark.gpu
 
@@ -180,3 +180,23 @@ def test_tunings_bit_exact(gpu, oracle, case, tuning):
             assert_same(p.propagate(inst.lb, inst.ub, sync), exp, f"{case} {tuning} sync={sync}")
     finally:
         p.close()
+
+
+@pytest.mark.parametrize("one_frac", [0.5, 0.0])
+def test_cover_knapsack_full_size(gpu, oracle, one_frac):
+    """The structured binary instance at full size (m=400,000, n=2,000,000,
+    25.3M nonzeros): as generated (fixings to 1 overfill knapsack rows, so
+    round 1 proves infeasibility) and with fixings to 0 only (feasible)."""
+    inst = cover_knapsack(one_frac=one_frac)
+    exp = oracle.jacobi(inst)
+    seq = oracle.seq(inst)
+    assert exp[3] == seq[3] == (one_frac > 0)
+    p = I.Problem(inst)
+    try:
+        for sync in (I.MEGAKERNEL, I.CPU_LOOP):
+            assert_same(p.propagate(inst.lb, inst.ub, sync), exp, f"one_frac={one_frac} sync={sync}")
+    finally:
+        p.close()
+    if not exp[3]:
+        np.testing.assert_array_equal(exp[0], seq[0])
+        np.testing.assert_array_equal(exp[1], seq[1])
