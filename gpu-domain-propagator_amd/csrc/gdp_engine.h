@@ -152,6 +152,14 @@ int server_slots(const Server *s);
 // propagated fixpoint, may be empty); returns its ticket.
 int64_t server_submit(Server *s, const double *lbs, const double *ubs, const int *hints, int n_hints,
                       int max_rounds);
+// Zero-copy request: lbs/ubs and out_lbs/out_ubs are pinned host buffers
+// owned by the caller and left untouched until the ticket completes; the
+// vectors are transferred from lbs/ubs at admission and the result straight
+// into out_lbs/out_ubs (server_result then copies nothing).  In place
+// (out_lbs == lbs, out_ubs == ubs), only the columns that changed come back
+// over PCIe and are written into the buffers.
+int64_t server_submit_pinned(Server *s, const double *lbs, const double *ubs, double *out_lbs, double *out_ubs,
+                             const int *hints, int n_hints, int max_rounds);
 // Base vector of delta requests (kept in device memory), e.g. the
 // propagated root of a branch-and-bound tree.
 void server_set_base(Server *s, const double *lbs, const double *ubs);

@@ -41,8 +41,8 @@ __global__ void unpack_kernel(DevMatrix M, DevState D, int n_list, const int *li
                               double *const *ubp);
 __global__ void delta_kernel(int n_list, double *const *lbp, double *const *ubp, const int *ptr, const int *cols,
                              const double *lv, const double *uv);
-__global__ void diff_kernel(int n, int n_list, double *const *lbp, double *const *ubp, const double *base_l,
-                            const double *base_u, int cap, int *cnt, int *cols, double *vl, double *vu);
+__global__ void diff_kernel(int n, int n_list, double *const *lbp, double *const *ubp, const double *const *blp,
+                            const double *const *bup, int cap, int *cnt, int *cols, double *vl, double *vu);
 
 namespace {
 
@@ -1041,6 +1041,11 @@ struct Server {
         std::vector<int> cols;           // changed columns (delta) / hints (full)
         std::vector<double> dlb, dub;    // new bounds of the changed columns (delta)
         int max_rounds = kMaxRounds;
+        // caller-owned pinned buffers (zero-copy requests): the vectors are
+        // transferred from in_* at admission and the result into out_*
+        const double *pin_lb = nullptr, *pin_ub = nullptr;
+        double *out_lb = nullptr, *out_ub = nullptr;
+        int stage = -1;                  // device staging entry the vectors were prefetched into
     };
     struct Done {
         int outbox = -1;                 // pinned outbox entry holding the result, or -1
@@ -1048,6 +1053,7 @@ struct Server {
         bool changes = false;            // the result as columns that differ from the base
         std::vector<int> cols;
         std::vector<double> clb, cub;
+        bool in_place = false;           // the result went to the caller's pinned buffers
         RunResult r;
     };
     static constexpr int kDiffCap = 4096;  // changed columns returned without the full vector
@@ -1065,6 +1071,11 @@ struct Server {
     bool has_base = false;
     std::vector<double> base_lb, base_ub;  // host copy of the base
     std::vector<char> slot_delta;        // slot holds a delta request
+    std::vector<double *> slot_out_lb, slot_out_ub;  // caller's pinned result buffers, or null
+    std::vector<char> slot_inplace;      // zero-copy request whose result replaces its input:
+                                         // only the changed columns come back
+    double *d_orig = nullptr;            // slots x (lb, ub): inputs of in-place requests
+    double **d_blp = nullptr, **d_bup = nullptr;  // base vector of each diffed slot
     // changed columns of finished delta requests (device and pinned host)
     int *d_ccnt = nullptr, *d_ccols = nullptr;
     double *d_cl = nullptr, *d_cu = nullptr;
@@ -1074,6 +1085,13 @@ struct Server {
     std::vector<double *> slot_lb, slot_ub;
     double *inbox = nullptr, *outbox = nullptr;  // pinned host memory
     std::vector<int> inbox_free, outbox_free;
+    // prefetch: request vectors from pinned memory are copied at submission
+    // into device staging entries on a copy stream, overlapping the steps
+    hipStream_t cs = nullptr;
+    double *d_stage = nullptr;
+    std::vector<hipEvent_t> stage_ev;
+    std::vector<int> stage_free;
+    double *stage_lb(int e) { return d_stage + 2 * n * e; }
     // device buffers for the changes of delta requests
     int *d_dptr = nullptr, *d_dcols = nullptr;
     double *d_dlb = nullptr, *d_dub = nullptr;
@@ -1093,7 +1111,15 @@ struct Server {
         for (void *q : {static_cast<void *>(d_dptr), static_cast<void *>(d_dcols), static_cast<void *>(d_dlb),
                         static_cast<void *>(d_dub), static_cast<void *>(d_dl), static_cast<void *>(d_du)})
             if (q) (void)hipFree(q);
+        for (void *q : {static_cast<void *>(d_orig), static_cast<void *>(d_blp), static_cast<void *>(d_bup)})
+            if (q) (void)hipFree(q);
+        if (cs) (void)hipStreamSynchronize(cs);
+        for (hipEvent_t e : stage_ev) (void)hipEventDestroy(e);
+        if (d_stage) (void)hipFree(d_stage);
+        if (cs) (void)hipStreamDestroy(cs);
     }
+    double *orig_lb(int k) { return d_orig + 2 * n * k; }
+    double *orig_ub(int k) { return d_orig + 2 * n * k + n; }
     double *in_lb(int e) { return inbox + 2 * n * e; }
     double *in_ub(int e) { return inbox + 2 * n * e + n; }
     double *out_lb(int e) { return outbox + 2 * n * e; }
@@ -1103,7 +1129,7 @@ struct Server {
 Server *server_create(Problem *P, int64_t hbm_budget, int max_slots) {
     if (!P->replicas) throw std::runtime_error("serving needs a single matrix");
     if (max_slots < 1) throw std::runtime_error("max_slots must be positive");
-    const int64_t per_slot = P->stats.unit_state_bytes + 16LL * P->n;
+    const int64_t per_slot = P->stats.unit_state_bytes + 32LL * P->n;  // state, vectors, in-place inputs
     int slots = static_cast<int>(std::min<int64_t>(max_slots, hbm_budget / std::max<int64_t>(1, per_slot)));
     if (slots < 1) throw std::runtime_error("HBM budget below one node's state");
     auto *S = new Server();
@@ -1120,6 +1146,12 @@ Server *server_create(Problem *P, int64_t hbm_budget, int max_slots) {
         }
         S->slot_ticket.assign(slots, -1);
         S->slot_delta.assign(slots, 0);
+        S->slot_out_lb.assign(slots, nullptr);
+        S->slot_out_ub.assign(slots, nullptr);
+        S->slot_inplace.assign(slots, 0);
+        check(hipMalloc(&S->d_orig, sizeof(double) * 2 * S->n * slots), "hipMalloc inputs");
+        check(hipMalloc(&S->d_blp, sizeof(double *) * slots), "hipMalloc");
+        check(hipMalloc(&S->d_bup, sizeof(double *) * slots), "hipMalloc");
         // changed-column regions for finished delta requests (kDiffCap each)
         S->diff_slots = slots;
         const size_t dc = static_cast<size_t>(slots) * Server::kDiffCap;
@@ -1142,7 +1174,19 @@ Server *server_create(Problem *P, int64_t hbm_budget, int max_slots) {
             S->inbox_free.push_back(e);
             S->outbox_free.push_back(e);
         }
-        // fresh status ring for the server's steps
+        // device staging for prefetched request vectors (up to 512 MiB)
+        const int n_stage = static_cast<int>(
+            std::max<int64_t>(1, std::min<int64_t>(std::min(slots, 256), (512LL << 20) / static_cast<int64_t>(16 * S->n))));
+        check(hipStreamCreateWithFlags(&S->cs, hipStreamNonBlocking), "hipStreamCreate");
+        check(hipMalloc(&S->d_stage, sizeof(double) * 2 * S->n * n_stage), "hipMalloc staging");
+        for (int e = n_stage - 1; e >= 0; e--) {
+            hipEvent_t ev;
+            check(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "hipEventCreate");
+            S->stage_ev.push_back(ev);
+            S->stage_free.push_back(e);
+        }
+        std::reverse(S->stage_ev.begin(), S->stage_ev.end());
+                // fresh status ring for the server's steps
         std::lock_guard<std::mutex> lock(P->run_mutex);
         hipLaunchKernelGGL(unit_reset_kernel, dim3(1), dim3(256), 0, P->stream, P->D, 0, P->d_params, 0, 1);
         check(hipStreamSynchronize(P->stream), "server init");
@@ -1165,6 +1209,20 @@ void server_set_base(Server *S, const double *lbs, const double *ubs) {
     S->has_base = true;
 }
 
+namespace {
+// Starts copying a request's vectors (pinned host memory) into a free device
+// staging entry on the copy stream.
+void prefetch(Server *S, Server::Request &q, const double *lb, const double *ub) {
+    if (S->stage_free.empty()) return;
+    const int e = S->stage_free.back();
+    S->stage_free.pop_back();
+    check(hipMemcpyAsync(S->stage_lb(e), lb, S->n * sizeof(double), hipMemcpyHostToDevice, S->cs), "H2D");
+    check(hipMemcpyAsync(S->stage_lb(e) + S->n, ub, S->n * sizeof(double), hipMemcpyHostToDevice, S->cs), "H2D");
+    check(hipEventRecord(S->stage_ev[e], S->cs), "event");
+    q.stage = e;
+}
+}  // namespace
+
 int64_t server_submit(Server *S, const double *lbs, const double *ubs, const int *hints, int n_hints,
                       int max_rounds) {
     if (max_rounds < 1 || max_rounds > kMaxRounds) throw std::runtime_error("max_rounds out of range");
@@ -1175,10 +1233,32 @@ int64_t server_submit(Server *S, const double *lbs, const double *ubs, const int
         S->inbox_free.pop_back();
         std::memcpy(S->in_lb(q.inbox), lbs, sizeof(double) * S->n);
         std::memcpy(S->in_ub(q.inbox), ubs, sizeof(double) * S->n);
+        prefetch(S, q, S->in_lb(q.inbox), S->in_ub(q.inbox));
     } else {
         q.lb.assign(lbs, lbs + S->n);
         q.ub.assign(ubs, ubs + S->n);
     }
+    if (hints && n_hints > 0) {
+        for (int k = 0; k < n_hints; k++)
+            if (hints[k] < 0 || hints[k] >= static_cast<int>(S->n)) throw std::runtime_error("hint column out of range");
+        q.cols.assign(hints, hints + n_hints);
+    }
+    q.max_rounds = max_rounds;
+    S->queue.push_back(std::move(q));
+    return S->queue.back().ticket;
+}
+
+int64_t server_submit_pinned(Server *S, const double *lbs, const double *ubs, double *out_lbs, double *out_ubs,
+                             const int *hints, int n_hints, int max_rounds) {
+    if (max_rounds < 1 || max_rounds > kMaxRounds) throw std::runtime_error("max_rounds out of range");
+    if (!lbs || !ubs || !out_lbs || !out_ubs) throw std::runtime_error("null buffer");
+    Server::Request q;
+    q.ticket = S->next_ticket++;
+    q.pin_lb = lbs;
+    q.pin_ub = ubs;
+    q.out_lb = out_lbs;
+    q.out_ub = out_ubs;
+    prefetch(S, q, lbs, ubs);
     if (hints && n_hints > 0) {
         for (int k = 0; k < n_hints; k++)
             if (hints[k] < 0 || hints[k] >= static_cast<int>(S->n)) throw std::runtime_error("hint column out of range");
@@ -1233,7 +1313,7 @@ int server_step(Server *S) {
     hipStream_t s = P->stream;
     const size_t n = S->n;
     // ---- admit queued requests into free slots; they start in this step ----
-    std::vector<int> adm, caps, hint_ptr(1, 0), hints, used_inbox;
+    std::vector<int> adm, caps, hint_ptr(1, 0), hints, used_inbox, used_stage;
     std::vector<double *> adm_lb, adm_ub;
     std::vector<double *> del_lb, del_ub;
     std::vector<int> dptr(1, 0), dcols;
@@ -1252,6 +1332,25 @@ int server_step(Server *S) {
             dlv.insert(dlv.end(), q.dlb.begin(), q.dlb.end());
             duv.insert(duv.end(), q.dub.begin(), q.dub.end());
             dptr.push_back(static_cast<int>(dcols.size()));
+        } else if (q.stage >= 0) {
+            // prefetched: the slot's (lb, ub) pair is contiguous, like the entry's
+            check(hipStreamWaitEvent(s, S->stage_ev[q.stage], 0), "wait");
+            check(hipMemcpyAsync(S->slot_lb[k], S->stage_lb(q.stage), 2 * n * sizeof(double), hipMemcpyDeviceToDevice,
+                                 s),
+                  "D2D");
+            used_stage.push_back(q.stage);
+            if (q.inbox >= 0) used_inbox.push_back(q.inbox);
+            if (q.pin_lb && q.out_lb == q.pin_lb && q.out_ub == q.pin_ub)
+                check(hipMemcpyAsync(S->orig_lb(k), S->slot_lb[k], 2 * n * sizeof(double), hipMemcpyDeviceToDevice,
+                                     s),
+                      "D2D");
+        } else if (q.pin_lb) {
+            check(hipMemcpyAsync(S->slot_lb[k], q.pin_lb, n * sizeof(double), hipMemcpyHostToDevice, s), "H2D");
+            check(hipMemcpyAsync(S->slot_ub[k], q.pin_ub, n * sizeof(double), hipMemcpyHostToDevice, s), "H2D");
+            if (q.out_lb == q.pin_lb && q.out_ub == q.pin_ub)
+                check(hipMemcpyAsync(S->orig_lb(k), S->slot_lb[k], 2 * n * sizeof(double), hipMemcpyDeviceToDevice,
+                                     s),
+                      "D2D");
         } else if (q.inbox >= 0) {
             check(hipMemcpyAsync(S->slot_lb[k], S->in_lb(q.inbox), n * sizeof(double), hipMemcpyHostToDevice, s),
                   "H2D");
@@ -1272,6 +1371,9 @@ int server_step(Server *S) {
         caps.push_back(q.max_rounds);
         S->slot_ticket[k] = q.ticket;
         S->slot_delta[k] = q.delta ? 1 : 0;
+        S->slot_out_lb[k] = q.out_lb;
+        S->slot_out_ub[k] = q.out_ub;
+        S->slot_inplace[k] = q.pin_lb && q.out_lb == q.pin_lb && q.out_ub == q.pin_ub ? 1 : 0;
         S->queue.pop_front();
     }
     if (!del_lb.empty()) {
@@ -1309,6 +1411,7 @@ int server_step(Server *S) {
     check(hipGetLastError(), "step launch");
     std::vector<UnitState> st = read_units(P, static_cast<int>(active.size()), active.data());  // synchronizes
     for (int e : used_inbox) S->inbox_free.push_back(e);
+    for (int e : used_stage) S->stage_free.push_back(e);
     keep.clear();
     int err = 0;
     check(hipMemcpy(&err, P->D.global + kGlobalError, sizeof(int), hipMemcpyDeviceToHost), "D2H");
@@ -1338,22 +1441,30 @@ int server_step(Server *S) {
                            dim3(256), 0, s, P->M, P->D, static_cast<int>(fin.size()), P->d_list, P->d_lbp, P->d_ubp);
         check(hipGetLastError(), "unpack launch");
         std::vector<Server::Done> out(fin.size());
-        // delta requests: the columns that differ from the base, on the device
-        std::vector<int> dq;  // indices into fin of delta requests
+        // delta requests (against the base vector) and in-place zero-copy
+        // requests (against their own input): the columns that differ, found
+        // on the device, so that only they cross PCIe
+        std::vector<int> dq;  // indices into fin of diffed requests
         std::vector<double *> dl, du;
+        std::vector<const double *> bl, bu;
         for (size_t q = 0; q < fin.size(); q++)
-            if (S->slot_delta[fin[q]]) {
+            if (S->slot_delta[fin[q]] || S->slot_inplace[fin[q]]) {
+                const int k = fin[q];
                 dq.push_back(static_cast<int>(q));
-                dl.push_back(S->slot_lb[fin[q]]);
-                du.push_back(S->slot_ub[fin[q]]);
+                dl.push_back(S->slot_lb[k]);
+                du.push_back(S->slot_ub[k]);
+                bl.push_back(S->slot_delta[k] ? S->d_base : S->orig_lb(k));
+                bu.push_back(S->slot_delta[k] ? S->d_base + n : S->orig_ub(k));
             }
         if (!dq.empty()) {
             const int nd = static_cast<int>(dq.size());
             upload(P->d_lbp, dl.data(), nd, s);
             upload(P->d_ubp, du.data(), nd, s);
+            upload(S->d_blp, const_cast<double **>(bl.data()), nd, s);
+            upload(S->d_bup, const_cast<double **>(bu.data()), nd, s);
             check(hipMemsetAsync(S->d_ccnt, 0, sizeof(int) * nd, s), "memset");
             hipLaunchKernelGGL(diff_kernel, dim3(gx, std::max(1, std::min(nd, 1024))), dim3(256), 0, s,
-                               static_cast<int>(n), nd, P->d_lbp, P->d_ubp, S->d_base, S->d_base + n,
+                               static_cast<int>(n), nd, P->d_lbp, P->d_ubp, S->d_blp, S->d_bup,
                                Server::kDiffCap, S->d_ccnt, S->d_ccols, S->d_cl, S->d_cu);
             check(hipGetLastError(), "diff launch");
             check(hipMemcpyAsync(S->h_ccnt, S->d_ccnt, sizeof(int) * nd, hipMemcpyDeviceToHost, s), "D2H");
@@ -1365,9 +1476,19 @@ int server_step(Server *S) {
             for (int j = 0; j < nd; j++) {
                 const int c = S->h_ccnt[j];
                 if (c > Server::kDiffCap) continue;  // too many: the full vector below
+                const int k = fin[dq[j]];
                 Server::Done &d = out[dq[j]];
-                d.changes = true;
                 const size_t o = static_cast<size_t>(j) * Server::kDiffCap;
+                if (S->slot_inplace[k]) {
+                    // the caller's buffers hold the input: write the changes
+                    for (int q = 0; q < c; q++) {
+                        S->slot_out_lb[k][S->h_ccols[o + q]] = S->h_cl[o + q];
+                        S->slot_out_ub[k][S->h_ccols[o + q]] = S->h_cu[o + q];
+                    }
+                    d.in_place = true;
+                    continue;
+                }
+                d.changes = true;
                 d.cols.assign(S->h_ccols + o, S->h_ccols + o + c);
                 d.clb.assign(S->h_cl + o, S->h_cl + o + c);
                 d.cub.assign(S->h_cu + o, S->h_cu + o + c);
@@ -1377,9 +1498,13 @@ int server_step(Server *S) {
             const int k = fin[q];
             Server::Done &d = out[q];
             d.r = fin_r[q];
-            if (d.changes) continue;
+            if (d.changes || d.in_place) continue;
             double *hl, *hu;
-            if (!S->outbox_free.empty()) {
+            if (S->slot_out_lb[k]) {
+                d.in_place = true;
+                hl = S->slot_out_lb[k];
+                hu = S->slot_out_ub[k];
+            } else if (!S->outbox_free.empty()) {
                 d.outbox = S->outbox_free.back();
                 S->outbox_free.pop_back();
                 hl = S->out_lb(d.outbox);
@@ -1399,6 +1524,9 @@ int server_step(Server *S) {
             S->done[S->slot_ticket[k]] = std::move(out[q]);
             S->last_completed.push_back(S->slot_ticket[k]);
             S->slot_ticket[k] = -1;
+            S->slot_out_lb[k] = nullptr;
+            S->slot_out_ub[k] = nullptr;
+            S->slot_inplace[k] = 0;
         }
     }
     S->step++;
@@ -1409,7 +1537,10 @@ bool server_result(Server *S, int64_t ticket, double *lbs, double *ubs, RunResul
     auto it = S->done.find(ticket);
     if (it == S->done.end()) return false;
     Server::Done &d = it->second;
-    if (d.changes) {
+    if (d.in_place || (!lbs && !ubs)) {
+        // the bounds are in the caller's pinned buffers (or not wanted)
+        if (d.outbox >= 0) S->outbox_free.push_back(d.outbox);
+    } else if (d.changes) {
         std::memcpy(lbs, S->base_lb.data(), sizeof(double) * S->n);
         std::memcpy(ubs, S->base_ub.data(), sizeof(double) * S->n);
         for (size_t q = 0; q < d.cols.size(); q++) {

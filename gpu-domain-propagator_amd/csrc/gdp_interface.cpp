@@ -536,6 +536,18 @@ GDP_RETCODE gdpServerSubmit(gdp_server s, const double *lbs, const double *ubs, 
     }
 }
 
+GDP_RETCODE gdpServerSubmitPinned(gdp_server s, const double *lbs, const double *ubs, double *out_lbs,
+                                  double *out_ubs, const int *hints, const int n_hints, const int max_rounds,
+                                  long long *ticket) {
+    if (!s || !lbs || !ubs || !out_lbs || !out_ubs || !ticket) return GDP_ERROR;
+    try {
+        *ticket = gdp::server_submit_pinned(s->s, lbs, ubs, out_lbs, out_ubs, hints, n_hints, max_rounds);
+        return GDP_OKAY;
+    } catch (const std::exception &e) {
+        return fail("gdpServerSubmitPinned", e);
+    }
+}
+
 GDP_RETCODE gdpServerSetBase(gdp_server s, const double *lbs, const double *ubs) {
     if (!s || !lbs || !ubs) return GDP_ERROR;
     try {
@@ -608,7 +620,7 @@ GDP_RETCODE gdpServerResultChanges(gdp_server s, const long long ticket, int *co
 
 GDP_RETCODE gdpServerResult(gdp_server s, const long long ticket, double *lbs, double *ubs, gdp_run_info *info,
                             int *ready) {
-    if (!s || !lbs || !ubs || !ready) return GDP_ERROR;
+    if (!s || !ready || (!lbs) != (!ubs)) return GDP_ERROR;
     RunResult r;
     *ready = gdp::server_result(s->s, ticket, lbs, ubs, &r) ? 1 : 0;
     if (*ready) fill_info(info, r.rounds, r.infeasible, r.capped, r.kernel_ms);
@@ -698,6 +710,13 @@ GDP_RETCODE gdpDeviceMalloc(void **ptr, const size_t bytes) {
 }
 
 GDP_RETCODE gdpDeviceFree(void *ptr) { return hip_rc(hipFree(ptr), "hipFree"); }
+
+GDP_RETCODE gdpHostAlloc(void **ptr, const size_t bytes) {
+    if (!ptr) return GDP_ERROR;
+    return hip_rc(hipHostMalloc(ptr, bytes ? bytes : 1, 0), "hipHostMalloc");
+}
+
+GDP_RETCODE gdpHostFree(void *ptr) { return hip_rc(hipHostFree(ptr), "hipHostFree"); }
 
 // hipMemcpy may return before a device-to-device copy has finished; these
 // helpers promise completion, since the propagation runs on its own stream.

@@ -1159,11 +1159,13 @@ __global__ __launch_bounds__(256) void delta_kernel(int n_list, double *const *l
 // cols / vl / vu, their number to cnt[k] (which may exceed cap).  cnt must be
 // zero on entry.
 __global__ __launch_bounds__(256) void diff_kernel(int n, int n_list, double *const *lbp, double *const *ubp,
-                                                   const double *base_l, const double *base_u, int cap, int *cnt,
-                                                   int *cols, double *vl, double *vu) {
+                                                   const double *const *blp, const double *const *bup, int cap,
+                                                   int *cnt, int *cols, double *vl, double *vu) {
     for (int k = blockIdx.y; k < n_list; k += gridDim.y) {
         const double *l = lbp[k];
         const double *u = ubp[k];
+        const double *base_l = blp[k];
+        const double *base_u = bup[k];
         for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
             const double a = l[j], b = u[j];
             if (__double_as_longlong(a) == __double_as_longlong(base_l[j]) &&

@@ -10,6 +10,6 @@ from .mps import parse_mps, read_mps  # noqa: F401
 from .interface import (  # noqa: F401
     CPU_LOOP, GPU_LOOP, MEGAKERNEL, GDP_OKAY, GDP_ERROR, GDP_NOTIMPLEMENTED, GDP_BINARY, GDP_INTEGER,
     GDP_CONTINUOUS, MAX_NUM_ROUNDS, GDP_INF, GDP_EPS, LIB_PATH, LEGACY_SYMBOLS, EXT_SYMBOLS, Problem, Outcome,
-    load_library, device_count, device_name, DeviceArray, Batch, Server, gpu_propagate, host_propagate, propagateGPUAtomic, propagateGPUReduction,
+    load_library, device_count, device_name, DeviceArray, PinnedArray, Batch, Server, gpu_propagate, host_propagate, propagateGPUAtomic, propagateGPUReduction,
     propagateSequential, propagateFullOMP, propagateSequentialDisjoint,
 )

@@ -68,6 +68,7 @@ EXT_SYMBOLS = (
     "gdpServerDestroy",
     "gdpServerSlots",
     "gdpServerSubmit",
+    "gdpServerSubmitPinned",
     "gdpServerSetBase",
     "gdpServerSubmitDelta",
     "gdpServerStep",
@@ -82,6 +83,8 @@ EXT_SYMBOLS = (
     "gdpDeviceCount",
     "gdpDeviceMalloc",
     "gdpDeviceFree",
+    "gdpHostAlloc",
+    "gdpHostFree",
     "gdpMemcpyHtoD",
     "gdpMemcpyDtoH",
     "gdpMemcpyDtoD",
@@ -197,6 +200,8 @@ def load_library(path: Optional[str] = None) -> ctypes.CDLL:
     lib.gdpServerSlots.restype = c_int
     lib.gdpServerSubmit.argtypes = [vp, vp, vp, vp, c_int, c_int, POINTER(c_longlong)]
     lib.gdpServerSubmit.restype = c_int
+    lib.gdpServerSubmitPinned.argtypes = [vp, vp, vp, vp, vp, vp, c_int, c_int, POINTER(c_longlong)]
+    lib.gdpServerSubmitPinned.restype = c_int
     lib.gdpServerSetBase.argtypes = [vp, vp, vp]
     lib.gdpServerSetBase.restype = c_int
     lib.gdpServerSubmitDelta.argtypes = [vp, c_int, vp, vp, vp, c_int, POINTER(c_longlong)]
@@ -225,6 +230,10 @@ def load_library(path: Optional[str] = None) -> ctypes.CDLL:
     lib.gdpDeviceCount.argtypes = [POINTER(c_int)]
     lib.gdpDeviceMalloc.argtypes = [POINTER(vp), ctypes.c_size_t]
     lib.gdpDeviceFree.argtypes = [vp]
+    lib.gdpHostAlloc.argtypes = [POINTER(vp), ctypes.c_size_t]
+    lib.gdpHostAlloc.restype = c_int
+    lib.gdpHostFree.argtypes = [vp]
+    lib.gdpHostFree.restype = c_int
     for f in ("gdpMemcpyHtoD", "gdpMemcpyDtoH", "gdpMemcpyDtoD"):
         getattr(lib, f).argtypes = [vp, vp, ctypes.c_size_t]
     lib.gdpDeviceName.argtypes = [ctypes.c_char_p, c_int]
@@ -253,6 +262,31 @@ def device_name() -> str:
     if load_library().gdpDeviceName(buf, 256) != GDP_OKAY:
         raise RuntimeError(last_error())
     return buf.value.decode()
+
+
+class PinnedArray:
+    """A float64 array in pinned host memory (gdpHostAlloc); `.array` is a
+    numpy view of it.  For zero-copy server requests."""
+
+    def __init__(self, size: int):
+        self.lib = load_library()
+        p = c_void_p()
+        if self.lib.gdpHostAlloc(ctypes.byref(p), 8 * max(1, int(size))) != GDP_OKAY:
+            raise RuntimeError(last_error())
+        self.ptr = p.value
+        self.array = np.ctypeslib.as_array(ctypes.cast(p, POINTER(c_double)), shape=(int(size),))
+
+    def free(self):
+        if self.ptr:
+            self.array = None
+            self.lib.gdpHostFree(c_void_p(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 class DeviceArray:
@@ -634,6 +668,21 @@ class Server:
             raise RuntimeError(f"gdpServerSubmit failed: {last_error()}")
         return t.value
 
+    def submit_pinned(self, lb: "PinnedArray", ub: "PinnedArray", out_lb: "PinnedArray", out_ub: "PinnedArray",
+                      hints=None, max_rounds: int = MAX_NUM_ROUNDS) -> int:
+        """Zero-copy request (gdpServerSubmitPinned): the bounds are read from
+        the pinned arrays lb / ub when the request is admitted and the result
+        is written into out_lb / out_ub; keep all four alive and untouched
+        until the ticket completes, then call result(ticket, copy=False)."""
+        hs = _arr(hints, np.int32) if hints is not None and len(hints) else None
+        t = c_longlong()
+        rc = self.lib.gdpServerSubmitPinned(self.handle, c_void_p(lb.ptr), c_void_p(ub.ptr), c_void_p(out_lb.ptr),
+                                            c_void_p(out_ub.ptr), _ptr(hs) if hs is not None else None,
+                                            0 if hs is None else len(hs), max_rounds, ctypes.byref(t))
+        if rc != GDP_OKAY:
+            raise RuntimeError(f"gdpServerSubmitPinned failed: {last_error()}")
+        return t.value
+
     def set_base(self, lb, ub):
         """Base vector of delta requests (a propagated fixpoint)."""
         lb = _arr(lb, np.float64)
@@ -674,13 +723,15 @@ class Server:
             raise RuntimeError(f"gdpServerCompleted failed: {last_error()}")
         return [buf[i] for i in range(min(cnt.value, cap))]
 
-    def result(self, ticket: int) -> Optional["Outcome"]:
-        lb = np.empty(self.n, np.float64)
-        ub = np.empty(self.n, np.float64)
+    def result(self, ticket: int, copy: bool = True) -> Optional["Outcome"]:
+        """Completed request (None if not complete).  copy=False: only the
+        outcome (a zero-copy request's bounds are in its pinned arrays)."""
+        lb = np.empty(self.n if copy else 0, np.float64)
+        ub = np.empty(self.n if copy else 0, np.float64)
         info = RunInfo()
         ready = c_int()
-        if self.lib.gdpServerResult(self.handle, ticket, _ptr(lb), _ptr(ub), ctypes.byref(info),
-                                    ctypes.byref(ready)) != GDP_OKAY:
+        if self.lib.gdpServerResult(self.handle, ticket, _ptr(lb) if copy else None, _ptr(ub) if copy else None,
+                                    ctypes.byref(info), ctypes.byref(ready)) != GDP_OKAY:
             raise RuntimeError(f"gdpServerResult failed: {last_error()}")
         return _outcome(lb, ub, info) if ready.value else None
 

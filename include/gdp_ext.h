@@ -159,7 +159,8 @@ GDP_RETCODE gdpBatchPropagateDevice(gdp_problem p, double *d_lbs, double *d_ubs,
 
 /* ---- serving node requests with continuous batching ----
  * A server admits submitted requests into free slots of the problem's
- * device state (as many slots as hbm_budget bytes allow, at most max_slots),
+ * device state (as many slots as hbm_budget bytes allow, at most max_slots;
+ * a slot takes the unit state of gdpProblemGetStats plus 32 bytes a column),
  * and each gdpServerStep advances every running request by one round in
  * shared launches: requests that arrive while others run join at the next
  * step.  A request's result is bit-identical to propagating it alone. */
@@ -171,6 +172,17 @@ GDP_RETCODE gdpServerSlots(gdp_server s, int *slots);
  * gdpPropagateNodesDevice, may be NULL) and returns its ticket. */
 GDP_RETCODE gdpServerSubmit(gdp_server s, const double *lbs, const double *ubs, const int *hints,
                             const int n_hints, const int max_rounds, long long *ticket);
+/* Zero-copy request: lbs / ubs (input) and out_lbs / out_ubs (result) are
+ * pinned host buffers of n_vars doubles (gdpHostAlloc) owned by the caller,
+ * left untouched by it until the ticket completes.  The vectors are
+ * transferred straight from lbs / ubs when the request is admitted and the
+ * result straight into out_lbs / out_ubs when it completes; no host copy is
+ * made.  In place (out_lbs == lbs and out_ubs == ubs) only the columns that
+ * changed cross PCIe on the way back and are written into the buffers.
+ * gdpServerResult with NULL buffers then returns the outcome. */
+GDP_RETCODE gdpServerSubmitPinned(gdp_server s, const double *lbs, const double *ubs, double *out_lbs,
+                                  double *out_ubs, const int *hints, const int n_hints, const int max_rounds,
+                                  long long *ticket);
 /* Base vector of delta requests, kept in device memory: a propagated
  * fixpoint, e.g. the root of a branch-and-bound tree. */
 GDP_RETCODE gdpServerSetBase(gdp_server s, const double *lbs, const double *ubs);
@@ -186,8 +198,10 @@ GDP_RETCODE gdpServerPending(gdp_server s, int *pending);
  * them into tickets, their number into *count. */
 GDP_RETCODE gdpServerCompleted(gdp_server s, long long *tickets, const int cap, int *count);
 /* *ready = 1 and the result copied out (and forgotten) if the request has
- * completed; *ready = 0 otherwise.  info->rounds is -1 for a request whose
- * bounds left the problem's bound envelope. */
+ * completed; *ready = 0 otherwise.  lbs and ubs may both be NULL: only the
+ * outcome is returned (zero-copy requests, or results not wanted).
+ * info->rounds is -1 for a request whose bounds left the problem's bound
+ * envelope. */
 GDP_RETCODE gdpServerResult(gdp_server s, const long long ticket, double *lbs, double *ubs,
                             gdp_run_info *info, int *ready);
 
@@ -235,6 +249,9 @@ GDP_RETCODE gdpOMPPropagate(const int n_cons, const int n_vars, const int nnz,
 GDP_RETCODE gdpDeviceCount(int *count);
 GDP_RETCODE gdpDeviceMalloc(void **ptr, const size_t bytes);
 GDP_RETCODE gdpDeviceFree(void *ptr);
+/* Pinned (page-locked) host memory for zero-copy server requests. */
+GDP_RETCODE gdpHostAlloc(void **ptr, const size_t bytes);
+GDP_RETCODE gdpHostFree(void *ptr);
 GDP_RETCODE gdpMemcpyHtoD(void *dst, const void *src, const size_t bytes);
 GDP_RETCODE gdpMemcpyDtoH(void *dst, const void *src, const size_t bytes);
 GDP_RETCODE gdpMemcpyDtoD(void *dst, const void *src, const size_t bytes);

@@ -252,7 +252,7 @@ def test_server_budget_limits_slots(gpu):
     inst = small_mip(seed=1)
     p = I.Problem(inst)
     try:
-        per = p.stats().unit_state_bytes + 16 * inst.n
+        per = p.stats().unit_state_bytes + 32 * inst.n  # state, bound vectors, inputs of in-place requests
         srv = I.Server(p, hbm_budget=3 * per + 1, max_slots=100)
         assert srv.slots == 3
         srv.close()
@@ -260,6 +260,65 @@ def test_server_budget_limits_slots(gpu):
             I.Server(p, hbm_budget=per // 2, max_slots=4)
     finally:
         p.close()
+
+
+def test_server_zero_copy_requests(gpu, oracle):
+    """Requests in caller-owned pinned memory (separate output arrays, or in
+    place), mixed with copied and delta requests: results land in the
+    caller's arrays, bit-identical to the oracle, and separate input arrays
+    are left unchanged."""
+    inst = powerlaw_mip(seed=6, m=20000, n=10000, len_hi=5000, s=1.8)
+    base = oracle.jacobi(inst)
+    nodes = node_vectors(inst, oracle, 24, seed=8)
+    p = I.Problem(inst)
+    pins = []
+    try:
+        srv = I.Server(p, hbm_budget=256 << 20, max_slots=6)
+        srv.set_base(base[0], base[1])
+        tickets, outs = {}, {}
+        for k, (lb, ub, cols) in enumerate(nodes):
+            if k % 6 == 0:  # separate output arrays
+                a, b, ol, ou = (I.PinnedArray(inst.n) for _ in range(4))
+                a.array[:] = lb
+                b.array[:] = ub
+                ol.array[:] = np.nan
+                ou.array[:] = np.nan
+                pins += [a, b, ol, ou]
+                tickets[k] = srv.submit_pinned(a, b, ol, ou, hints=cols)
+                outs[k] = (a, b, ol, ou)
+            elif k % 6 == 3:  # in place: only the changed columns come back
+                a, b = I.PinnedArray(inst.n), I.PinnedArray(inst.n)
+                a.array[:] = lb
+                b.array[:] = ub
+                pins += [a, b]
+                tickets[k] = srv.submit_pinned(a, b, a, b, hints=cols)
+                outs[k] = (None, None, a, b)
+            elif k % 3 == 1:
+                tickets[k] = srv.submit(lb, ub, hints=cols)
+            else:
+                tickets[k] = srv.submit_delta(cols, lb[cols], ub[cols])
+            if k % 5 == 4:
+                srv.step()
+        while srv.pending():
+            srv.step()
+        for k, t in tickets.items():
+            exp = oracle.jacobi(inst, lb=nodes[k][0], ub=nodes[k][1])
+            if k in outs:
+                a, b, ol, ou = outs[k]
+                got = srv.result(t, copy=False)
+                assert got is not None and got.rounds == exp[2], f"request {k}"
+                np.testing.assert_array_equal(ol.array, exp[0], err_msg=f"request {k}")
+                np.testing.assert_array_equal(ou.array, exp[1], err_msg=f"request {k}")
+                if a is not None:
+                    np.testing.assert_array_equal(a.array, nodes[k][0])
+                    np.testing.assert_array_equal(b.array, nodes[k][1])
+            else:
+                assert_same(srv.result(t), exp, f"request {k}")
+        srv.close()
+    finally:
+        p.close()
+        for x in pins:
+            x.free()
 
 
 def test_server_delta_requests(gpu, oracle):
