@@ -110,6 +110,16 @@ struct RoundWork {
 };
 RunResult problem_work_profile(Problem *p, double *lbs, double *ubs, int max_rounds, std::vector<RoundWork> &work);
 
+// Progress measure of every round of one propagation of host vectors
+// lbs/ubs (in/out, the CPU_LOOP launches, same result): after each step a
+// kernel compares the round's bounds with the weakest bounds wl/wu, the
+// limit bounds ll/lu and the previous round's bounds (host arrays of
+// length n).  fin[r], inf[r]: the unnormalised finite progress and the
+// bounds that became finite in round r + 1.
+RunResult problem_propagate_measured(Problem *p, double *lbs, double *ubs, int max_rounds, const double *wl,
+                                     const double *wu, const double *ll, const double *lu, std::vector<double> &fin,
+                                     std::vector<int> &inf);
+
 // Replicas mode: `count` device-resident bound vectors at once (vector k's
 // at d_lbs[k], d_ubs[k], host arrays of device pointers).  With hints (host
 // CSR: vector k's columns hints[hint_ptr[k] .. hint_ptr[k+1])), the vectors

@@ -43,6 +43,10 @@ __global__ void delta_kernel(int n_list, double *const *lbp, double *const *ubp,
                              const double *lv, const double *uv);
 __global__ void diff_kernel(int n, int n_list, double *const *lbp, double *const *ubp, const double *const *blp,
                             const double *const *bup, int cap, int *cnt, int *cols, double *vl, double *vu);
+__global__ void measure_kernel(int n, const double *lb, const double *ub, const double *wl, const double *wu,
+                               const double *ll, const double *lu, double *pl, double *pu, double *fin_part,
+                               int *inf_part);
+__global__ void measure_total_kernel(int parts, const double *fin_part, const int *inf_part, double *fin, int *inf);
 
 namespace {
 
@@ -898,6 +902,80 @@ RunResult problem_work_profile(Problem *P, double *lbs, double *ubs, int max_rou
         throw;
     }
     check(hipFree(d_b), "hipFree");
+    return res;
+}
+
+RunResult problem_propagate_measured(Problem *P, double *lbs, double *ubs, int max_rounds, const double *wl,
+                                     const double *wu, const double *ll, const double *lu, std::vector<double> &fin,
+                                     std::vector<int> &inf) {
+    if (!P->replicas) throw std::runtime_error("progress measures are for a single instance");
+    if (max_rounds < 1 || max_rounds > kMaxRounds) throw std::runtime_error("max_rounds out of range");
+    std::lock_guard<std::mutex> lock(P->run_mutex);
+    const DevMatrix &M = P->M;
+    hipStream_t s = P->stream;
+    const int n = P->n;
+    const size_t nb = sizeof(double) * static_cast<size_t>(n);
+    const int gx = std::max(1, std::min(1024, (n + 255) / 256));
+    // bounds of the round | previous round | weakest | limit, each lb then ub;
+    // then the workgroup partials and the per-round sums
+    const size_t bytes = 8 * nb + gx * (sizeof(double) + sizeof(int)) + kMaxRounds * (sizeof(double) + sizeof(int));
+    char *buf = nullptr;
+    check(hipMalloc(&buf, bytes), "hipMalloc measure buffers");
+    double *d_b = reinterpret_cast<double *>(buf);
+    double *d_prev = d_b + 2 * static_cast<size_t>(n);
+    double *d_w = d_prev + 2 * static_cast<size_t>(n);
+    double *d_l = d_w + 2 * static_cast<size_t>(n);
+    double *part_f = d_l + 2 * static_cast<size_t>(n);
+    double *out_f = part_f + gx;
+    int *part_i = reinterpret_cast<int *>(out_f + kMaxRounds);
+    int *out_i = part_i + gx;
+    RunResult res;
+    fin.clear();
+    inf.clear();
+    try {
+        const double *src[8] = {lbs, ubs, lbs, ubs, wl, wu, ll, lu};
+        for (int q = 0; q < 8; q++)
+            check(hipMemcpy(d_b + q * static_cast<size_t>(n), src[q], nb, hipMemcpyHostToDevice), "H2D measure");
+        const int list[1] = {0};
+        double *lbp = d_b, *ubp = d_b + n;
+        const int caps[1] = {max_rounds};
+        P->D.n_active = 1;
+        load_units(P, 1, list, &lbp, &ubp, nullptr, nullptr, caps, 1, true);
+        std::vector<UnitState> st = read_units(P, 1, list);
+        if (st[0].envelope) throw std::runtime_error("bounds outside the problem's bound envelope");
+        int rounds = 0;
+        for (int step = 1; step <= max_rounds; step++) {
+            if (step == 1 ? st[0].crossed != 0 : !active_after(st[0], step - 1)) break;
+            launch_step(P, step, 0, 1);
+            check(hipGetLastError(), "step launch");
+            hipLaunchKernelGGL(unpack_kernel, dim3(gx, 1), dim3(256), 0, s, P->M, P->D, 1, P->d_list, P->d_lbp,
+                               P->d_ubp);
+            hipLaunchKernelGGL(measure_kernel, dim3(gx), dim3(256), 0, s, n, d_b, d_b + n, d_w, d_w + n, d_l,
+                               d_l + n, d_prev, d_prev + n, part_f, part_i);
+            hipLaunchKernelGGL(measure_total_kernel, dim3(1), dim3(64), 0, s, gx, part_f, part_i, out_f + step - 1,
+                               out_i + step - 1);
+            check(hipGetLastError(), "measure launch");
+            st = read_units(P, 1, list);
+            rounds = step;
+        }
+        res = outcome(st[0]);
+        hipLaunchKernelGGL(unpack_kernel, dim3(gx, 1), dim3(256), 0, s, P->M, P->D, 1, P->d_list, P->d_lbp,
+                           P->d_ubp);
+        check(hipGetLastError(), "unpack launch");
+        check(hipStreamSynchronize(s), "unpack sync");
+        fin.resize(rounds);
+        inf.resize(rounds);
+        if (rounds > 0) {
+            check(hipMemcpy(fin.data(), out_f, sizeof(double) * rounds, hipMemcpyDeviceToHost), "D2H measure");
+            check(hipMemcpy(inf.data(), out_i, sizeof(int) * rounds, hipMemcpyDeviceToHost), "D2H measure");
+        }
+        check(hipMemcpy(lbs, d_b, nb, hipMemcpyDeviceToHost), "D2H lbs");
+        check(hipMemcpy(ubs, d_b + n, nb, hipMemcpyDeviceToHost), "D2H ubs");
+    } catch (...) {
+        (void)hipFree(buf);
+        throw;
+    }
+    check(hipFree(buf), "hipFree");
     return res;
 }
 

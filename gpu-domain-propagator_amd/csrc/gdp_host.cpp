@@ -29,6 +29,22 @@
 
 namespace gdp {
 
+// computeActivities (src/propagation_methods.cuh:161-221): CSR order.
+RowActs row_activities(int i, const int *col, const int *row_ptr, const double *val, const double *lb,
+                       const double *ub) {
+    RowActs r{0.0, 0.0, 0.0, 0, 0};
+    for (int k = row_ptr[i]; k < row_ptr[i + 1]; k++) {
+        const int j = col[k];
+        const Contribution c = contribution(val[k], lb[j], ub[j]);
+        r.delta = eps_max_step(r.delta, c.delta);
+        r.min_inf += c.min_inf;
+        r.max_inf += c.max_inf;
+        if (!c.min_inf) r.minact += c.min_part;
+        if (!c.max_inf) r.maxact += c.max_part;
+    }
+    return r;
+}
+
 namespace {
 
 struct Csc {
@@ -47,27 +63,6 @@ Csc build_csc(int m, int n, const int *col, const int *row_ptr) {
     for (int i = 0; i < m; i++)
         for (int k = row_ptr[i]; k < row_ptr[i + 1]; k++) c.rowidx[fill[col[k]]++] = i;
     return c;
-}
-
-struct RowActs {
-    double minact, maxact, delta;
-    int min_inf, max_inf;
-};
-
-// computeActivities (src/propagation_methods.cuh:161-221): CSR order.
-RowActs row_activities(int i, const int *col, const int *row_ptr, const double *val, const double *lb,
-                       const double *ub) {
-    RowActs r{0.0, 0.0, 0.0, 0, 0};
-    for (int k = row_ptr[i]; k < row_ptr[i + 1]; k++) {
-        const int j = col[k];
-        const Contribution c = contribution(val[k], lb[j], ub[j]);
-        r.delta = eps_max_step(r.delta, c.delta);
-        r.min_inf += c.min_inf;
-        r.max_inf += c.max_inf;
-        if (!c.min_inf) r.minact += c.min_part;
-        if (!c.max_inf) r.maxact += c.max_part;
-    }
-    return r;
 }
 
 void mark_column(int j, const Csc &c, int *marked) {
@@ -114,7 +109,7 @@ bool seq_round(int m, const int *col, const int *row_ptr, const double *val, con
 
 HostResult seq_propagate(int m, int n, const int *col, const int *row_ptr, const double *val,
                          const double *lhs, const double *rhs, double *lb, double *ub, const int *vartype,
-                         int max_rounds) {
+                         int max_rounds, const RoundObserver *observer) {
     HostResult res;
     if (crossed_any(n, lb, ub, nullptr)) { res.infeasible = 1; return res; }
     const Csc csc = build_csc(m, n, col, row_ptr);
@@ -125,9 +120,15 @@ HostResult seq_propagate(int m, int n, const int *col, const int *row_ptr, const
     for (round = 1; round <= max_rounds && change; round++) {
         std::fill(chg.begin(), chg.end(), 0);
         change = seq_round(m, col, row_ptr, val, lhs, rhs, lb, ub, vartype, csc, marked, next, chg, nullptr);
+        const bool go_on = !observer || (*observer)(round, lb, ub);
         if (change && crossed_any(n, lb, ub, chg.data())) {
             res.infeasible = 1;
             res.rounds = round;
+            return res;
+        }
+        if (!go_on) {
+            res.rounds = round;
+            res.capped = change ? 1 : 0;
             return res;
         }
     }

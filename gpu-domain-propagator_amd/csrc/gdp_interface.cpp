@@ -125,6 +125,106 @@ GDP_RETCODE host_double(int which, int m, int n, int nnz, const int *col, const 
     }
 }
 
+// The progress-measure runs (progressPropagators.cuh:11-131 for cpu_seq,
+// :134-272 for gpu_atomic): the weakest bounds of the input
+// (initMeasureData, progress_function.h:19-53), the cpu_seq fixpoint as the
+// limit, the largest attainable measure, then the rounds of the propagation
+// with the measure of each (measureAndPrintProgress, :152-195).  The
+// reference's closing run without the measure (timing only, its result
+// discarded) is not repeated.
+GDP_RETCODE with_measure(bool gpu, int m, int n, int nnz, const int *col, const int *rp, const double *val,
+                         const double *lhs, const double *rhs, double *lb, double *ub, const int *vt,
+                         int max_rounds, bool print, double *p_fin, double *p_inf, int *n_rounds,
+                         gdp_run_info *info) {
+    if (n_rounds) *n_rounds = 0;
+    if (!sizes_ok(m, n, nnz)) {
+        gdp::set_last_error("negative size");
+        return GDP_ERROR;
+    }
+    if (max_rounds < 1 || max_rounds > MAX_NUM_ROUNDS) {
+        gdp::set_last_error("max_rounds out of range");
+        return GDP_ERROR;
+    }
+    if (empty_problem(m, n, nnz)) {
+        if (print) std::printf("propagation of 0 size problem. Nothing to propagate.\n");
+        return GDP_OKAY;
+    }
+    gdp::Problem *p = nullptr;
+    try {
+        if (print)
+            std::printf(gpu ? "\n=== gpu_atomic execution with measure of progress ===\n"
+                            : "\n=== cpu_seq execution with measure of progress ===\n");
+        const std::vector<double> lb_in(lb, lb + n), ub_in(ub, ub + n);
+        std::vector<double> wl(lb_in), wu(ub_in), ll(lb_in), lu(ub_in);
+        if (print) std::printf("\n====   Running the weakest bounds procedure  ====");
+        gdp::weakest_bounds(m, n, col, rp, val, lhs, rhs, vt, wl.data(), wu.data());
+        if (print) std::printf("====   end weakest bounds procedure  ====");
+        if (print) std::printf("\n\n==== propagator run for limit poitns ====");
+        gdp::seq_propagate(m, n, col, rp, val, lhs, rhs, ll.data(), lu.data(), vt, MAX_NUM_ROUNDS);
+        if (print) std::printf("==== end limit points ====\n");
+        const int bad = gdp::measure_defect(n, wu.data(), lu.data());
+        if (bad >= 0) {
+            if (print) std::printf("varidx: %d\n", bad);
+            std::fflush(stdout);
+            gdp::set_last_error("variable " + std::to_string(bad) +
+                                " has a finite limit upper bound but an infinite weakest upper bound");
+            return GDP_ERROR;
+        }
+        const gdp::Measure mx = gdp::measure_max(n, lb_in.data(), ub_in.data(), wl.data(), wu.data(), ll.data(),
+                                                 lu.data());
+        std::vector<double> fin;
+        std::vector<int> inf;
+        int rounds = 0, infeasible = 0, capped = 0;
+        double ms = 0.0;
+        if (gpu) {
+            p = gdp::problem_create(m, n, nnz, col, rp, val, lhs, rhs, vt, lb, ub, nullptr);
+            std::vector<double> l(lb_in), u(ub_in);
+            const RunResult r = gdp::problem_propagate_measured(p, l.data(), u.data(), max_rounds, wl.data(),
+                                                                wu.data(), ll.data(), lu.data(), fin, inf);
+            gdp::problem_destroy(p);
+            p = nullptr;
+            std::memcpy(lb, l.data(), n * sizeof(double));
+            std::memcpy(ub, u.data(), n * sizeof(double));
+            rounds = r.rounds;
+            infeasible = r.infeasible;
+            capped = r.capped;
+            ms = r.kernel_ms;
+        } else {
+            std::vector<double> pl(lb_in), pu(ub_in);
+            const gdp::RoundObserver obs = [&](int, const double *l, const double *u) {
+                const gdp::Measure x = gdp::measure_round(n, l, u, wl.data(), wu.data(), ll.data(), lu.data(),
+                                                          pl.data(), pu.data());
+                fin.push_back(x.fin);
+                inf.push_back(x.inf);
+                pl.assign(l, l + n);
+                pu.assign(u, u + n);
+                return true;
+            };
+            const HostResult r = gdp::seq_propagate(m, n, col, rp, val, lhs, rhs, lb, ub, vt, max_rounds, &obs);
+            rounds = r.rounds;
+            infeasible = r.infeasible;
+            capped = r.capped;
+        }
+        int inf_total = 0;
+        for (size_t k = 0; k < fin.size(); k++) {
+            const double pf = gdp::measure_normalize(fin[k], mx.fin);
+            const double pi = gdp::measure_normalize(static_cast<double>(inf[k] + inf_total),
+                                                     static_cast<double>(mx.inf));
+            inf_total += inf[k];
+            if (print) std::printf("round=%d, P_fin=%.10f, P_inf=%.2f\n", static_cast<int>(k) + 1, pf, pi);
+            if (p_fin) p_fin[k] = pf;
+            if (p_inf) p_inf[k] = pi;
+        }
+        if (print) std::fflush(stdout);
+        if (n_rounds) *n_rounds = static_cast<int>(fin.size());
+        fill_info(info, rounds, infeasible, capped, ms);
+        return GDP_OKAY;
+    } catch (const std::exception &e) {
+        if (p) gdp::problem_destroy(p);
+        return fail("propagation with measure", e);
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -224,21 +324,49 @@ GDP_RETCODE propagateConstraintsSequentialDisjointDouble(const int n_cons, const
                        MAX_NUM_ROUNDS, 0, nullptr);
 }
 
-GDP_RETCODE sequentialPropagateWithMeasureDouble(const int, const int, const int, const int *, const int *,
-                                                 const double *, const double *, const double *, double *,
-                                                 double *, const int *) {
-    gdp::set_last_error("progress-measure propagation is not provided");
-    return GDP_NOTIMPLEMENTED;
+GDP_RETCODE sequentialPropagateWithMeasureDouble(const int n_cons, const int n_vars, const int nnz,
+                                                 const int *col_indices, const int *row_indices,
+                                                 const double *vals, const double *lhss, const double *rhss,
+                                                 double *lbs, double *ubs, const int *vartypes) {
+    return with_measure(false, n_cons, n_vars, nnz, col_indices, row_indices, vals, lhss, rhss, lbs, ubs, vartypes,
+                        MAX_NUM_ROUNDS, true, nullptr, nullptr, nullptr, nullptr);
 }
 
-GDP_RETCODE atomicPropagateWithMeasureDouble(const int, const int, const int, const int *, const int *,
-                                             const double *, const double *, const double *, double *, double *,
-                                             const int *) {
-    gdp::set_last_error("progress-measure propagation is not provided");
-    return GDP_NOTIMPLEMENTED;
+GDP_RETCODE atomicPropagateWithMeasureDouble(const int n_cons, const int n_vars, const int nnz,
+                                             const int *col_indices, const int *row_indices, const double *vals,
+                                             const double *lhss, const double *rhss, double *lbs, double *ubs,
+                                             const int *vartypes) {
+    return with_measure(true, n_cons, n_vars, nnz, col_indices, row_indices, vals, lhss, rhss, lbs, ubs, vartypes,
+                        MAX_NUM_ROUNDS, true, nullptr, nullptr, nullptr, nullptr);
 }
 
 // ---- extension API (gdp_ext.h) ----
+
+GDP_RETCODE gdpPropagateWithMeasure(const int n_cons, const int n_vars, const int nnz, const int *col_indices,
+                                    const int *row_ptrs, const double *vals, const double *lhss, const double *rhss,
+                                    double *lbs, double *ubs, const int *vartypes, const int method,
+                                    const int max_rounds, const int print, double *p_fin, double *p_inf,
+                                    int *n_rounds, gdp_run_info *info) {
+    if (method != GDP_MEASURE_CPU_SEQ && method != GDP_MEASURE_GPU) {
+        gdp::set_last_error("unknown measure method");
+        return GDP_ERROR;
+    }
+    return with_measure(method == GDP_MEASURE_GPU, n_cons, n_vars, nnz, col_indices, row_ptrs, vals, lhss, rhss,
+                        lbs, ubs, vartypes, max_rounds, print != 0, p_fin, p_inf, n_rounds, info);
+}
+
+GDP_RETCODE gdpWeakestBounds(const int n_cons, const int n_vars, const int nnz, const int *col_indices,
+                             const int *row_ptrs, const double *vals, const double *lhss, const double *rhss,
+                             double *lbs, double *ubs, const int *vartypes) {
+    if (!sizes_ok(n_cons, n_vars, nnz)) return GDP_ERROR;
+    if (empty_problem(n_cons, n_vars, nnz)) return GDP_OKAY;
+    try {
+        gdp::weakest_bounds(n_cons, n_vars, col_indices, row_ptrs, vals, lhss, rhss, vartypes, lbs, ubs);
+        return GDP_OKAY;
+    } catch (const std::exception &e) {
+        return fail("weakest bounds", e);
+    }
+}
 
 void gdpTuningDefaults(gdp_tuning *t) {
     if (!t) return;

@@ -1182,4 +1182,64 @@ __global__ __launch_bounds__(256) void diff_kernel(int n, int n_list, double *co
     }
 }
 
+// Progress measure of one round (src/progressMeasureMethods/progress_function.h:45-116,
+// :152-195): for every column the finite progress of its bounds between the
+// weakest bounds (wl, wu) and the limit bounds (ll, lu), and the bounds that
+// became finite since the previous round (pl, pu, then overwritten with this
+// round's).  Bounds in the caller's column order.  Each workgroup writes the
+// sums of its columns to fin_part / inf_part[blockIdx.x] in a fixed order.
+__global__ __launch_bounds__(256) void measure_kernel(int n, const double *lb, const double *ub, const double *wl,
+                                                      const double *wu, const double *ll, const double *lu,
+                                                      double *pl, double *pu, double *fin_part, int *inf_part) {
+    __shared__ double sf[4];
+    __shared__ int si[4];
+    double f = 0.0;
+    int k = 0;
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+        const double l = lb[j], u = ub[j], a = wl[j], b = wu[j], c = ll[j], d = lu[j];
+        double v = 0.0;
+        if (eps_gt(a, -kInf) && eps_gt(l, -kInf) && eps_ge(l, a) && !(fabs(c - a) <= kEps)) v += (l - a) / (c - a);
+        if (eps_lt(b, kInf) && eps_lt(u, kInf) && eps_le(u, b) && !(fabs(b - d) <= kEps)) v += (b - u) / (b - d);
+        f += v;
+        k += (is_neg_inf(pl[j]) && eps_gt(l, -kInf)) ? 1 : 0;
+        k += (is_pos_inf(pu[j]) && eps_lt(u, kInf)) ? 1 : 0;
+        pl[j] = l;
+        pu[j] = u;
+    }
+    for (int h = 32; h >= 1; h >>= 1) {
+        f += shfl_xor_d(f, h);
+        k += __shfl_xor(k, h, 64);
+    }
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        sf[wave] = f;
+        si[wave] = k;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        fin_part[blockIdx.x] = (sf[0] + sf[1]) + (sf[2] + sf[3]);
+        inf_part[blockIdx.x] = si[0] + si[1] + si[2] + si[3];
+    }
+}
+
+// Sums the workgroup partials of measure_kernel in index order into
+// fin[0], inf[0] (one wave).
+__global__ __launch_bounds__(64) void measure_total_kernel(int parts, const double *fin_part, const int *inf_part,
+                                                           double *fin, int *inf) {
+    double f = 0.0;
+    int k = 0;
+    for (int q = threadIdx.x; q < parts; q += 64) {
+        f += fin_part[q];
+        k += inf_part[q];
+    }
+    for (int h = 32; h >= 1; h >>= 1) {
+        f += shfl_xor_d(f, h);
+        k += __shfl_xor(k, h, 64);
+    }
+    if (threadIdx.x == 0) {
+        fin[0] = f;
+        inf[0] = k;
+    }
+}
+
 }  // namespace gdp

@@ -78,6 +78,8 @@ EXT_SYMBOLS = (
     "gdpServerResult",
     "gdpSequentialPropagate",
     "gdpOMPPropagate",
+    "gdpPropagateWithMeasure",
+    "gdpWeakestBounds",
     "gdpParseMPS",
     "gdpMpsFree",
     "gdpDeviceCount",
@@ -223,6 +225,11 @@ def load_library(path: Optional[str] = None) -> ctypes.CDLL:
     lib.gdpOMPPropagate.argtypes = [c_int, c_int, c_int, vp, vp, vp, vp, vp, vp, vp, vp, c_int, c_int,
                                     POINTER(RunInfo)]
     lib.gdpOMPPropagate.restype = c_int
+    lib.gdpPropagateWithMeasure.argtypes = [c_int, c_int, c_int, vp, vp, vp, vp, vp, vp, vp, vp, c_int, c_int,
+                                            c_int, vp, vp, POINTER(c_int), POINTER(RunInfo)]
+    lib.gdpPropagateWithMeasure.restype = c_int
+    lib.gdpWeakestBounds.argtypes = [c_int, c_int, c_int, vp, vp, vp, vp, vp, vp, vp, vp]
+    lib.gdpWeakestBounds.restype = c_int
     lib.gdpParseMPS.argtypes = [ctypes.c_char_p, ctypes.c_size_t, POINTER(MpsModel)]
     lib.gdpParseMPS.restype = c_int
     lib.gdpMpsFree.argtypes = [POINTER(MpsModel)]
@@ -398,6 +405,21 @@ def propagateFullOMP(n_vars, n_cons, nnz, csr_col_indices, csr_row_ptrs, csr_val
                      vartypes, datatype=c_double):
     return _call_list("propagateConstraintsFullOMPDouble", "propagateConstraintsFullOMPFloat", n_vars, n_cons, nnz,
                       csr_col_indices, csr_row_ptrs, csr_vals, lhss, rhss, lbs, ubs, vartypes, datatype)
+
+
+def propagateSequentialWithMeasure(n_vars, n_cons, nnz, csr_col_indices, csr_row_ptrs, csr_vals, lhss, rhss, lbs,
+                                   ubs, vartypes):
+    """cpu_seq to the fixpoint, printing the progress measure of every round."""
+    return _call_list("sequentialPropagateWithMeasureDouble", None, n_vars, n_cons, nnz, csr_col_indices,
+                      csr_row_ptrs, csr_vals, lhss, rhss, lbs, ubs, vartypes, c_double)
+
+
+def propagateGPUAtomicWithMeasure(n_vars, n_cons, nnz, csr_col_indices, csr_row_ptrs, csr_vals, lhss, rhss, lbs,
+                                  ubs, vartypes):
+    """The GPU's rounds to the fixpoint, printing the progress measure of
+    every round."""
+    return _call_list("atomicPropagateWithMeasureDouble", None, n_vars, n_cons, nnz, csr_col_indices,
+                      csr_row_ptrs, csr_vals, lhss, rhss, lbs, ubs, vartypes, c_double)
 
 
 def propagateSequentialDisjoint(n_vars, n_cons, nnz, csr_col_indices, csr_row_ptrs, csr_vals, lhss, rhss, lbs, ubs,
@@ -845,3 +867,48 @@ def host_propagate(inst, method: str = "seq", max_rounds: int = MAX_NUM_ROUNDS, 
     if rc != GDP_OKAY:
         raise RuntimeError(f"host propagation failed: {last_error()}")
     return Outcome(lb, ub, info.rounds, bool(info.infeasible), bool(info.capped), info.kernel_ms)
+
+
+MEASURE_CPU_SEQ = 0
+MEASURE_GPU = 1
+
+
+def _inst_arrays(inst):
+    return (_arr(inst.col, np.int32), _arr(inst.row_ptr, np.int32), _arr(inst.val, np.float64),
+            _arr(inst.lhs, np.float64), _arr(inst.rhs, np.float64), _arr(inst.vartype, np.int32))
+
+
+def propagate_with_measure(inst, method: int = MEASURE_CPU_SEQ, max_rounds: int = MAX_NUM_ROUNDS,
+                           print_measure: bool = False):
+    """Propagation to the fixpoint with the progress measure of every round
+    (gdpPropagateWithMeasure): returns (Outcome, p_fin, p_inf), the measures
+    normalised to 0..100 as the reference prints them."""
+    lib = load_library()
+    col, rp, val, lhs, rhs, vt = _inst_arrays(inst)
+    lb = np.array(inst.lb, dtype=np.float64, copy=True)
+    ub = np.array(inst.ub, dtype=np.float64, copy=True)
+    pf = np.zeros(max_rounds)
+    pi = np.zeros(max_rounds)
+    nr = c_int()
+    info = RunInfo()
+    rc = lib.gdpPropagateWithMeasure(int(inst.m), int(inst.n), len(col), _ptr(col), _ptr(rp), _ptr(val), _ptr(lhs),
+                                     _ptr(rhs), _ptr(lb), _ptr(ub), _ptr(vt), method, max_rounds,
+                                     int(print_measure), _ptr(pf), _ptr(pi), ctypes.byref(nr), ctypes.byref(info))
+    if rc != GDP_OKAY:
+        raise RuntimeError(f"gdpPropagateWithMeasure failed: {last_error()}")
+    out = Outcome(lb, ub, info.rounds, bool(info.infeasible), bool(info.capped), info.kernel_ms)
+    return out, pf[:nr.value].copy(), pi[:nr.value].copy()
+
+
+def weakest_bounds(inst, lb=None, ub=None):
+    """Weakest finite bounds of the bounds that start infinite
+    (gdpWeakestBounds): returns (lb, ub)."""
+    lib = load_library()
+    col, rp, val, lhs, rhs, vt = _inst_arrays(inst)
+    lb = np.array(inst.lb if lb is None else lb, dtype=np.float64, copy=True)
+    ub = np.array(inst.ub if ub is None else ub, dtype=np.float64, copy=True)
+    rc = lib.gdpWeakestBounds(int(inst.m), int(inst.n), len(col), _ptr(col), _ptr(rp), _ptr(val), _ptr(lhs),
+                              _ptr(rhs), _ptr(lb), _ptr(ub), _ptr(vt))
+    if rc != GDP_OKAY:
+        raise RuntimeError(f"gdpWeakestBounds failed: {last_error()}")
+    return lb, ub

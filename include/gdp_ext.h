@@ -243,6 +243,31 @@ GDP_RETCODE gdpOMPPropagate(const int n_cons, const int n_vars, const int nnz,
                             const int *vartypes, const int max_rounds, const int threads,
                             gdp_run_info *info);
 
+/* ---- progress measure ----
+ * The computation of sequentialPropagateWithMeasureDouble (method
+ * GDP_MEASURE_CPU_SEQ) and atomicPropagateWithMeasureDouble (method
+ * GDP_MEASURE_GPU) with the measures returned: p_fin[k] and p_inf[k] (arrays
+ * of max_rounds entries, max_rounds in 1..100) receive round k+1's
+ * normalised measures (0..100, as printed), *n_rounds the rounds run; lbs/ubs
+ * in/out.  print != 0 also prints what the reference prints.
+ * Reference: src/progressMeasureMethods/progressPropagators.cuh:11-272,
+ * progress_function.h:10-195. */
+#define GDP_MEASURE_CPU_SEQ 0
+#define GDP_MEASURE_GPU 1
+GDP_RETCODE gdpPropagateWithMeasure(const int n_cons, const int n_vars, const int nnz,
+                                    const int *col_indices, const int *row_ptrs, const double *vals,
+                                    const double *lhss, const double *rhss, double *lbs, double *ubs,
+                                    const int *vartypes, const int method, const int max_rounds,
+                                    const int print, double *p_fin, double *p_inf, int *n_rounds,
+                                    gdp_run_info *info);
+/* Weakest bounds of lbs/ubs (in/out): every bound that starts infinite gets
+ * the weakest finite value the search of computeWeakestBounds
+ * (src/progressMeasureMethods/weakest_bounds.h:772-829) derives for it. */
+GDP_RETCODE gdpWeakestBounds(const int n_cons, const int n_vars, const int nnz,
+                             const int *col_indices, const int *row_ptrs, const double *vals,
+                             const double *lhss, const double *rhss, double *lbs, double *ubs,
+                             const int *vartypes);
+
 /* Device memory plumbing, so that callers need no second HIP runtime (a
  * framework bundling its own ROCm runtime cannot share a process with this
  * library's).  Synchronous; device 0 unless the caller selected another. */

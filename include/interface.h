@@ -109,8 +109,14 @@ GDP_RETCODE propagateConstraintsSequentialDisjointDouble(
         const int *vartypes);
 
 /* Replace include/interface.h:222-236 and :246-260 (src/interface.cu:293-353,
- * progress-measure research instrumentation).  Not provided: both return
- * GDP_NOTIMPLEMENTED and leave the bounds untouched. */
+ * src/progressMeasureMethods/progressPropagators.cuh:11-272): propagation to
+ * the fixpoint that prints a progress measure after every round
+ * ("round=%d, P_fin=%.10f, P_inf=%.2f"), measured against the weakest bounds
+ * of the input and the cpu_seq fixpoint.  The sequential entry propagates
+ * with cpu_seq on the host; the atomic entry with the GPU's rounds, the
+ * measure of each round computed on the device.  lbs/ubs receive the
+ * fixpoint.  Where the reference exit()s (a finite limit upper bound whose
+ * weakest bound is infinite) these return GDP_ERROR. */
 GDP_RETCODE sequentialPropagateWithMeasureDouble(
         const int n_cons, const int n_vars, const int nnz,
         const int *col_indices, const int *row_indices, const double *vals,

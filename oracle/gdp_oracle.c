@@ -426,3 +426,176 @@ int oracle_jacobi_propagate(int m, int n, int nnz, const int *col, const int *ro
     work_free(&w);
     return 1;
 }
+
+/* ------------------------------------------------------------------------ */
+/* progress measure: src/progressMeasureMethods/                             */
+/* ------------------------------------------------------------------------ */
+
+/* One candidate weakest bound (class NewWeakestBound, weakest_bounds.h:13-35):
+ * variable, 'l' or 'u', value, CSR position of the entry that produced it. */
+typedef struct {
+    int var;
+    char type;
+    double newb;
+    int pos;
+} wbd_t;
+
+typedef struct {
+    int m, n;
+    const int *col, *rowptr, *vt;
+    const double *val, *lhs, *rhs;
+    double *glb, *gub;        /* global weakest bounds (Bounds global_wbds) */
+    unsigned char *expanded;  /* can_tighten_arr: one flag per nonzero */
+} wb_ctx;
+
+/* NEWcomputeInfiniteReduction (weakest_bounds.h:638-688): the candidates
+ * of one row; the infinity count passed to both tightenings is the
+ * minimum activity's, as in the reference. */
+static int wb_row(const wb_ctx *c, int row, const double *lb, const double *ub, const double *act,
+                  const int *inf, wbd_t *out) {
+    int cnt = 0;
+    const double minact = act[0], maxact = act[1];
+    const int mininf = inf[0], maxinf = inf[1];
+    const double lhs = c->lhs[row], rhs = c->rhs[row];
+    for (int k = c->rowptr[row]; k < c->rowptr[row + 1]; k++) {
+        const int j = c->col[k];
+        const double a = c->val[k];
+        const double slack = rhs - minact, surplus = lhs - maxact;
+        const int cont = c->vt[j] == 3;
+        if (isposinf(ub[j])) {
+            const int c1 = a > 0 && !isposinf(rhs) && (mininf == 0 || (mininf == 1 && isneginf(lb[j])));
+            const int c2 = a < 0 && !isneginf(lhs) && (maxinf == 0 || (maxinf == 1 && isneginf(lb[j])));
+            if (c1 || c2) {
+                double nb = ub[j];
+                oracle_tighten_var_upper(a, slack, surplus, mininf, lb[j], ub[j], cont, &nb);
+                out[cnt].var = j; out[cnt].type = 'u'; out[cnt].newb = nb; out[cnt].pos = k; cnt++;
+            }
+        }
+        if (isneginf(lb[j])) {
+            const int c1 = a > 0 && !isneginf(lhs) && (maxinf == 0 || (maxinf == 1 && isneginf(ub[j])));
+            const int c2 = a < 0 && !isposinf(rhs) && (mininf == 0 || (mininf == 1 && isneginf(ub[j])));
+            if (c1 || c2) {
+                double nb = lb[j];
+                oracle_tighten_var_lower(a, slack, surplus, mininf, lb[j], ub[j], cont, &nb);
+                out[cnt].var = j; out[cnt].type = 'l'; out[cnt].newb = nb; out[cnt].pos = k; cnt++;
+            }
+        }
+    }
+    return cnt;
+}
+
+/* weakestBoundsAlgorithm (weakest_bounds.h:691-745): the local bounds are
+ * a copy per call, the candidates of the call are collected over all rows
+ * (each folded into the global weakest bounds, Bounds::updateWeakestBounds
+ * :513-529) and expanded in order unless their entry was expanded before. */
+static int wb_node(wb_ctx *c, const double *plb, const double *pub, const wbd_t *nw) {
+    const int m = c->m, n = c->n;
+    double *lb = malloc(sizeof(double) * (size_t)n);
+    double *ub = malloc(sizeof(double) * (size_t)n);
+    const int nnz = c->rowptr[m];
+    wbd_t *all = malloc(sizeof(wbd_t) * (size_t)(2 * nnz + 1));
+    if (!lb || !ub || !all) { free(lb); free(ub); free(all); return -1; }
+    memcpy(lb, plb, sizeof(double) * (size_t)n);
+    memcpy(ub, pub, sizeof(double) * (size_t)n);
+    if (nw) {
+        /* Bounds::updateLocalWeakestBounds (:496-510) */
+        c->expanded[nw->pos] = 1;
+        if (nw->type == 'l' && isneginf(lb[nw->var])) lb[nw->var] = nw->newb;
+        if (nw->type == 'u' && isposinf(ub[nw->var])) ub[nw->var] = nw->newb;
+    }
+    int total = 0;
+    for (int i = 0; i < m; i++) {
+        double act[3];
+        int inf[2];
+        oracle_row_activities_seq(i, c->col, c->rowptr, c->val, lb, ub, act, inf);
+        if ((inf[0] <= 1 && !isposinf(c->rhs[i])) || (inf[1] <= 1 && !isneginf(c->lhs[i]))) {
+            const int k = wb_row(c, i, lb, ub, act, inf, all + total);
+            for (int q = total; q < total + k; q++) {
+                const int j = all[q].var;
+                const double v = all[q].newb;
+                if (all[q].type == 'l') {
+                    if (isneginf(c->glb[j]) || (!isneginf(c->glb[j]) && epslt(v, c->glb[j]))) c->glb[j] = v;
+                } else {
+                    if (isposinf(c->gub[j]) || (!isposinf(c->gub[j]) && epsgt(v, c->gub[j]))) c->gub[j] = v;
+                }
+            }
+            total += k;
+        }
+    }
+    int rc = 0;
+    for (int q = 0; q < total && rc == 0; q++)
+        if (!c->expanded[all[q].pos]) rc = wb_node(c, lb, ub, &all[q]);
+    free(lb);
+    free(ub);
+    free(all);
+    return rc;
+}
+
+/* computeWeakestBounds (weakest_bounds.h:772-829). */
+int oracle_weakest_bounds(int m, int n, int nnz, const int *col, const int *rowptr, const double *val,
+                          const double *lhs, const double *rhs, double *lb, double *ub, const int *vartype) {
+    wb_ctx c;
+    c.m = m; c.n = n; c.col = col; c.rowptr = rowptr; c.vt = vartype;
+    c.val = val; c.lhs = lhs; c.rhs = rhs;
+    c.glb = malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    c.gub = malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    c.expanded = calloc((size_t)(nnz > 0 ? nnz : 1), 1);
+    if (!c.glb || !c.gub || !c.expanded) { free(c.glb); free(c.gub); free(c.expanded); return -1; }
+    memcpy(c.glb, lb, sizeof(double) * (size_t)n);
+    memcpy(c.gub, ub, sizeof(double) * (size_t)n);
+    const int rc = wb_node(&c, lb, ub, NULL);
+    if (rc == 0) {
+        memcpy(lb, c.glb, sizeof(double) * (size_t)n);
+        memcpy(ub, c.gub, sizeof(double) * (size_t)n);
+    }
+    free(c.glb);
+    free(c.gub);
+    free(c.expanded);
+    return rc;
+}
+
+/* calcMaxMeasureValues (progress_function.h:118-150). */
+void oracle_progress_max(int n, const double *lb_in, const double *ub_in, const double *wl, const double *wu,
+                         const double *ll, const double *lu, double *p_fin, int *p_inf) {
+    double f = 0.0;
+    int k = 0;
+    for (int j = 0; j < n; j++) {
+        if (epsgt(ll[j], -OR_INF) && !(fabs(ll[j] - wl[j]) <= OR_EPS)) f += 1.0;
+        if (epslt(lu[j], OR_INF) && !(fabs(lu[j] - wu[j]) <= OR_EPS)) f += 1.0;
+        if (epsgt(ll[j], -OR_INF) && epsle(lb_in[j], -OR_INF)) k += 1;
+        if (epslt(lu[j], OR_INF) && epsge(ub_in[j], OR_INF)) k += 1;
+    }
+    *p_fin = f;
+    *p_inf = k;
+}
+
+/* measureAndPrintProgress with calcProgressMeasureVar
+ * (progress_function.h:152-195, :45-116): the unnormalised sums of one
+ * round, variables in index order.  Returns 0, or -1 - j for the first
+ * variable j whose limit upper bound is finite while its weakest upper bound
+ * is infinite (the reference exits there). */
+int oracle_progress_round(int n, const double *lb, const double *ub, const double *wl, const double *wu,
+                          const double *ll, const double *lu, const double *pl, const double *pu,
+                          double *p_fin, int *p_inf) {
+    double f = 0.0;
+    int k = 0;
+    for (int j = 0; j < n; j++) {
+        if (epslt(lu[j], OR_INF) && epsge(wu[j], OR_INF)) return -1 - j;
+        double v = 0.0;
+        if (epsgt(wl[j], -OR_INF) && epsgt(lb[j], -OR_INF) && epsge(lb[j], wl[j]) && !(fabs(ll[j] - wl[j]) <= OR_EPS))
+            v += (lb[j] - wl[j]) / (ll[j] - wl[j]);
+        if (epslt(wu[j], OR_INF) && epslt(ub[j], OR_INF) && epsle(ub[j], wu[j]) && !(fabs(wu[j] - lu[j]) <= OR_EPS))
+            v += (wu[j] - ub[j]) / (wu[j] - lu[j]);
+        f += v;
+        if (epsle(pl[j], -OR_INF) && epsgt(lb[j], -OR_INF)) k += 1;
+        if (epsge(pu[j], OR_INF) && epslt(ub[j], OR_INF)) k += 1;
+    }
+    *p_fin = f;
+    *p_inf = k;
+    return 0;
+}
+
+/* normalizeTo0_100 (progress_function.h:10-16). */
+double oracle_progress_normalize(double val, double max_val) {
+    return epsgt(max_val, 0.0) ? (100.0 * val) / max_val : 0.0;
+}

@@ -89,6 +89,24 @@ int oracle_seq_disjoint_propagate(int m, int n, int nnz, const int *col, const i
                                   double *lb, double *ub, const int *vartype,
                                   int max_rounds, oracle_result *res);
 
+/* ---- progress measure (src/progressMeasureMethods/) ---- */
+
+/* computeWeakestBounds (weakest_bounds.h:772-829): lb/ub in/out, the
+ * weakest finite bounds propagation can derive for bounds that start
+ * infinite.  Returns 0, or -1 when out of memory. */
+int oracle_weakest_bounds(int m, int n, int nnz, const int *col, const int *rowptr, const double *val,
+                          const double *lhs, const double *rhs, double *lb, double *ub, const int *vartype);
+/* calcMaxMeasureValues (progress_function.h:118-150) */
+void oracle_progress_max(int n, const double *lb_in, const double *ub_in, const double *wl, const double *wu,
+                         const double *ll, const double *lu, double *p_fin, int *p_inf);
+/* one round's unnormalised measure (progress_function.h:45-116, :152-195);
+ * returns 0 or -1 - j for a variable j the reference exits on */
+int oracle_progress_round(int n, const double *lb, const double *ub, const double *wl, const double *wu,
+                          const double *ll, const double *lu, const double *pl, const double *pu,
+                          double *p_fin, int *p_inf);
+/* normalizeTo0_100 (progress_function.h:10-16) */
+double oracle_progress_normalize(double val, double max_val);
+
 /* CSR -> CSC by counting sort (structure only; rows ascending in a column). */
 void oracle_csr_to_csc(int m, int n, int nnz, const int *col, const int *rowptr,
                        const double *val, int *colptr, int *rowidx, double *cval);

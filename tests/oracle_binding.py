@@ -49,6 +49,12 @@ def lib():
         L.oracle_canonical_sum.restype = c_double
         L.oracle_row_activities_seq.argtypes = [c_int, vp, vp, vp, vp, vp, vp, vp]
         L.oracle_row_activities_canon.argtypes = [c_int, vp, vp, vp, vp, vp, c_int, vp, vp]
+        L.oracle_weakest_bounds.argtypes = [c_int, c_int, c_int, vp, vp, vp, vp, vp, vp, vp, vp]
+        L.oracle_progress_max.argtypes = [c_int, vp, vp, vp, vp, vp, vp, POINTER(c_double), POINTER(c_int)]
+        L.oracle_progress_round.argtypes = [c_int, vp, vp, vp, vp, vp, vp, vp, vp, POINTER(c_double),
+                                            POINTER(c_int)]
+        L.oracle_progress_normalize.argtypes = [c_double, c_double]
+        L.oracle_progress_normalize.restype = c_double
         _lib = L
     return _lib
 
@@ -123,3 +129,51 @@ def tighten_lower(coeff, slack, surplus, num_inf, lb, ub, is_cont):
     nb = c_double()
     t = lib().oracle_tighten_var_lower(coeff, slack, surplus, num_inf, lb, ub, int(is_cont), ctypes.byref(nb))
     return bool(t), nb.value
+
+
+def weakest_bounds(inst, lb=None, ub=None):
+    """Weakest finite bounds of the bounds that start infinite
+    (computeWeakestBounds): returns (lb, ub)."""
+    col, rp, val, lhs, rhs, vt = _arrays(inst)
+    lb = np.array(inst.lb if lb is None else lb, np.float64)
+    ub = np.array(inst.ub if ub is None else ub, np.float64)
+    rc = lib().oracle_weakest_bounds(inst.m, inst.n, len(col), _p(col), _p(rp), _p(val), _p(lhs), _p(rhs), _p(lb),
+                                     _p(ub), _p(vt))
+    assert rc == 0
+    return lb, ub
+
+
+def progress_series(inst, driver="seq"):
+    """The progress measure of every round of a propagation to the fixpoint,
+    as the reference's measure entry points print it: the weakest bounds of
+    the input, the cpu_seq fixpoint as the limit, and per round the
+    normalised finite and infinite measures.  driver "seq" propagates with
+    cpu_seq, "jacobi" round-synchronously.  Returns (rows, lb, ub) with rows
+    a list of (round, p_fin, p_inf), or (None, j) if variable j has a finite
+    limit upper bound but an infinite weakest one (the reference exits)."""
+    L = lib()
+    n = inst.n
+    lb_in = np.array(inst.lb, np.float64)
+    ub_in = np.array(inst.ub, np.float64)
+    wl, wu = weakest_bounds(inst)
+    ll, lu, _, _, _ = seq(inst)
+    fmax, imax = c_double(), c_int()
+    L.oracle_progress_max(n, _p(lb_in), _p(ub_in), _p(wl), _p(wu), _p(ll), _p(lu), ctypes.byref(fmax),
+                          ctypes.byref(imax))
+    run = seq if driver == "seq" else jacobi
+    lb, ub, rounds, _, _ = run(inst)
+    rows = []
+    pl, pu = lb_in, ub_in
+    inf_total = 0
+    for r in range(1, rounds + 1):
+        cl, cu, _, _, _ = run(inst, max_rounds=r)
+        f, k = c_double(), c_int()
+        rc = L.oracle_progress_round(n, _p(cl), _p(cu), _p(wl), _p(wu), _p(ll), _p(lu), _p(pl), _p(pu),
+                                     ctypes.byref(f), ctypes.byref(k))
+        if rc != 0:
+            return None, -1 - rc
+        rows.append((r, L.oracle_progress_normalize(f.value, fmax.value),
+                     L.oracle_progress_normalize(float(k.value + inf_total), float(imax.value))))
+        inf_total += k.value
+        pl, pu = cl, cu
+    return rows, lb, ub

@@ -76,12 +76,13 @@ def test_unknown_sync_type_is_an_error(gdp):
     assert rc == gdp.GDP_ERROR
 
 
-def test_measure_entry_points_not_implemented(gdp):
-    inst = fixtures.analytic("achterberg")
-    for name in ("sequentialPropagateWithMeasureDouble", "atomicPropagateWithMeasureDouble"):
-        rc, lb, ub = _call(gdp, name, inst)
-        assert rc == gdp.GDP_NOTIMPLEMENTED
-        np.testing.assert_array_equal(lb, inst.lb)
+@pytest.mark.parametrize("name", fixtures.ANALYTIC)
+def test_sequential_measure_entry_reaches_the_analytic_fixpoint(gdp, name):
+    inst = fixtures.analytic(name)
+    rc, lb, ub = _call(gdp, "sequentialPropagateWithMeasureDouble", inst)
+    assert rc == gdp.GDP_OKAY
+    np.testing.assert_allclose(lb, inst.meta["lb_expected"], rtol=0, atol=1e-4)
+    np.testing.assert_allclose(ub, inst.meta["ub_expected"], rtol=0, atol=1e-4)
 
 
 def _instances():
