@@ -146,18 +146,35 @@ struct Sink {
     DevState V;
 };
 
+// Sentinels of the integer accumulator: a candidate beyond the int32 range
+// goes to AX, and AI's side is pushed to its sentinel so that readers know
+// to consult AX (AI.x == kAiHigh may also be a genuine bound; reading AX
+// then is merely redundant).  Integer candidates are integral, so those
+// beyond the range are >= 2^31 or <= -2^31 and the max / min with the
+// sentinel is the candidate itself.
+constexpr int kAiHigh = 2147483647;
+constexpr int kAiLow = -2147483647 - 1;
+
 // Folds an accepted lower / upper candidate into the accumulator and marks
 // the column in the unit's bitmap of moved columns.
 __device__ __forceinline__ void accumulate(const DevMatrix &M, const Sink &K, int id, int t, double nl, double nu) {
     const DevState &D = K.V;
     if (id < M.n_int) {
         if (t & 1) {
-            if (fits_int32(nl)) atomic_max_i(&D.AI[id].x, static_cast<int>(nl));
-            else atomic_max_d(&D.AX[id].x, nl);
+            if (fits_int32(nl)) {
+                atomic_max_i(&D.AI[id].x, static_cast<int>(nl));
+            } else {
+                atomic_max_d(&D.AX[id].x, nl);
+                atomic_max_i(&D.AI[id].x, kAiHigh);
+            }
         }
         if (t & 2) {
-            if (fits_int32(nu)) atomic_min_i(&D.AI[id].y, static_cast<int>(nu));
-            else atomic_min_d(&D.AX[id].y, nu);
+            if (fits_int32(nu)) {
+                atomic_min_i(&D.AI[id].y, static_cast<int>(nu));
+            } else {
+                atomic_min_d(&D.AX[id].y, nu);
+                atomic_min_i(&D.AI[id].y, kAiLow);
+            }
         }
     } else {
         double2 *a = &D.AF[id - M.n_int];
@@ -170,13 +187,18 @@ __device__ __forceinline__ void accumulate(const DevMatrix &M, const Sink &K, in
 // Current accumulated bounds of internal column `id` (device-coherent).
 __device__ __forceinline__ void accumulated(const DevMatrix &M, const DevState &D, int id, double &l, double &u) {
     if (id < M.n_int) {
-        const int *ai = reinterpret_cast<const int *>(&D.AI[id]);
-        l = static_cast<double>(coherent_load(ai));
-        u = static_cast<double>(coherent_load(ai + 1));
-        const double xl = coherent_load_d(&D.AX[id].x);
-        const double xu = coherent_load_d(&D.AX[id].y);
-        l = xl > l ? xl : l;
-        u = xu < u ? xu : u;
+        // both int32 sides in one device-coherent access; AX only behind a sentinel
+        const long long w = coherent_load(reinterpret_cast<const long long *>(&D.AI[id]));
+        const int lo = static_cast<int>(static_cast<unsigned long long>(w) & 0xffffffffull);
+        const int hi = static_cast<int>(static_cast<unsigned long long>(w) >> 32);
+        l = static_cast<double>(lo);
+        u = static_cast<double>(hi);
+        if (lo == kAiHigh || hi == kAiLow) {
+            const double xl = coherent_load_d(&D.AX[id].x);
+            const double xu = coherent_load_d(&D.AX[id].y);
+            l = xl > l ? xl : l;
+            u = xu < u ? xu : u;
+        }
     } else {
         const double2 *a = &D.AF[id - M.n_int];
         l = coherent_load_d(&a->x);
@@ -199,9 +221,12 @@ __device__ __forceinline__ void store_snapshot(const DevMatrix &M, const DevStat
     if (id < M.n16) {
         coherent_store(&D.S16[id], pack16(l, u));
     } else if (id < M.n_int) {
-        int *s = reinterpret_cast<int *>(&D.S32[id - M.n16]);
-        coherent_store(s, fits_int32(l) ? static_cast<int>(l) : -2147483647);
-        coherent_store(s + 1, fits_int32(u) ? static_cast<int>(u) : 2147483647);
+        // both sides in one device-coherent 8-byte store (x in the low word)
+        const int lo = fits_int32(l) ? static_cast<int>(l) : -2147483647;
+        const int hi = fits_int32(u) ? static_cast<int>(u) : 2147483647;
+        const unsigned long long w = static_cast<unsigned long long>(static_cast<unsigned>(lo)) |
+                                     (static_cast<unsigned long long>(static_cast<unsigned>(hi)) << 32);
+        coherent_store(reinterpret_cast<long long *>(&D.S32[id - M.n16]), static_cast<long long>(w));
     } else {
         double2 *s = &D.SF[id - M.n_int];
         coherent_store_d(&s->x, l);

@@ -124,6 +124,16 @@ def infeasible_instance() -> Instance:
                      [0, 0, 0], [5, 5, 5], [1, 1, 3], "infeasible")
 
 
+def int32_overflow() -> Instance:
+    """Integer columns with compact bound records (16-bit range: x, y; 32-bit
+    range: v, t) receive candidates beyond the int32 range: x - w >= 3e9,
+    y + w <= -3e9, v - w >= 5e9, t + w <= -5e9 with w in [0, 1].  The bounds
+    cross in round 1 and the returned bounds are the exact candidates."""
+    return from_rows(4, 5, [[0, 4], [1, 4], [2, 4], [3, 4]], [[1.0, -1.0], [1.0, 1.0], [1.0, -1.0], [1.0, 1.0]],
+                     [3e9, -INF, 5e9, -INF], [INF, -3e9, INF, -5e9],
+                     [0, 0, 0, -100000, 0], [5, 5, 100000, 0, 1], [1, 1, 1, 1, 3], "int32_overflow")
+
+
 # Row patterns of gate_ties: offsets (in units of 1e-9) of the tie entries
 # above R, in row order, and whether the sequential epsilon-maximum passes the
 # row gate.  The running maximum starts at 0 and takes a value only when it

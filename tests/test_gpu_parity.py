@@ -37,6 +37,8 @@ def _case(name):
         return fixtures.gate_ties()
     if kind == "infeasible":
         return fixtures.infeasible_instance()
+    if kind == "int32_overflow":
+        return fixtures.int32_overflow()
     if kind == "tiny":
         return fixtures.tiny_integer(int(arg))
     if kind == "powerlaw":
@@ -45,8 +47,8 @@ def _case(name):
 
 
 CASES = ([f"analytic:{n}" for n in fixtures.ANALYTIC] + [f"small:{s}" for s in range(3)] +
-         ["small_int:7", "varied:0", "varied:1", "varied:2", "gcfp:30", "infeasible", "tiny:3", "tiny:11",
-          "powerlaw:5", "gate_ties"])
+         ["small_int:7", "varied:0", "varied:1", "varied:2", "gcfp:30", "infeasible", "int32_overflow", "tiny:3",
+          "tiny:11", "powerlaw:5", "gate_ties"])
 
 
 def assert_same(got, exp, label=""):
