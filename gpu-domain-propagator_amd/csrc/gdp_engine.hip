@@ -43,6 +43,11 @@ __global__ void delta_kernel(int n_list, double *const *lbp, double *const *ubp,
                              const double *lv, const double *uv);
 __global__ void diff_kernel(int n, int n_list, double *const *lbp, double *const *ubp, const double *const *blp,
                             const double *const *bup, int cap, int *cnt, int *cols, double *vl, double *vu);
+__global__ void layout_rows_kernel(int mp, const int *src_row, const int *row_ptr_in, const int *col_in,
+                                   const double *val_in, const int *row_ptr_out, const int *internal,
+                                   const int *vartype, int n, int *col_out, double *val_out, int *col_cnt, int *bad);
+__global__ void layout_csc_kernel(int mp, const int *row_ptr_out, const int *col_out, const int *row_part, int *fill,
+                                  int *col_rows, int *col_seg);
 __global__ void measure_kernel(int n, const double *lb, const double *ub, const double *wl, const double *wu,
                                const double *ll, const double *lu, double *pl, double *pu, double *fin_part,
                                int *inf_part);
@@ -88,14 +93,17 @@ int column_class(int vartype, const double *env_lb, const double *env_ub, int j)
     return 2;
 }
 
-void validate(const InstanceView &v) {
+// Input contract; with `columns` false the column indices are checked later
+// (on the device, while the layout is built there).
+void validate(const InstanceView &v, bool columns) {
     if (v.m <= 0 || v.n <= 0 || v.nnz <= 0) throw std::runtime_error("empty problem");
     if (!v.col || !v.row_ptr || !v.val || !v.lhs || !v.rhs || !v.vartype) throw std::runtime_error("null input array");
     if (v.row_ptr[0] != 0 || v.row_ptr[v.m] != v.nnz) throw std::runtime_error("row_ptrs do not span nnz");
     for (int i = 0; i < v.m; i++)
         if (v.row_ptr[i + 1] < v.row_ptr[i]) throw std::runtime_error("row_ptrs not monotone");
-    for (int k = 0; k < v.nnz; k++)
-        if (v.col[k] < 0 || v.col[k] >= v.n) throw std::runtime_error("column index out of range");
+    if (columns)
+        for (int k = 0; k < v.nnz; k++)
+            if (v.col[k] < 0 || v.col[k] >= v.n) throw std::runtime_error("column index out of range");
 }
 
 // Outcome of one unit from its state words (the reference's round-count
@@ -245,9 +253,61 @@ void alloc_units(Problem *P, int cap) {
     P->stats.n_units = units;
 }
 
+// The passes over the nonzeros of a single instance, on the device: the
+// source CSR is uploaded as given, the entries are moved into the padded
+// layout (rows in the order src_row, at the starts of d_rp), the columns'
+// entries are counted, and the column-to-row index is filled.  Column
+// indices outside [0, n) are reported here.  Synchronous.
+void device_layout_build(const InstanceView &v, int mp, int n_ids, int n_padded, const std::vector<int> &src_row,
+                         const std::vector<int> &row_part, const int *d_rp, const int *d_internal, int *d_col,
+                         double *d_val, int *d_cptr, int *d_crows, int *d_cseg, hipStream_t s) {
+    std::vector<void *> tmp;
+    try {
+        int *t_col = dev_alloc<int>(v.nnz, tmp);
+        double *t_val = dev_alloc<double>(v.nnz, tmp);
+        int *t_rp = dev_alloc<int>(v.m + 1, tmp);
+        int *t_src = dev_alloc<int>(mp, tmp);
+        int *t_part = dev_alloc<int>(mp, tmp);
+        int *t_vt = dev_alloc<int>(v.n, tmp);
+        int *t_cnt = dev_alloc<int>(n_ids + 1, tmp);  // counts, then the fill positions
+        upload(t_col, v.col, v.nnz, s);
+        upload(t_val, v.val, v.nnz, s);
+        upload(t_rp, v.row_ptr, v.m + 1, s);
+        upload(t_src, src_row.data(), mp, s);
+        upload(t_part, row_part.data(), mp, s);
+        upload(t_vt, v.vartype, v.n, s);
+        check(hipMemsetAsync(d_col, 0, sizeof(int) * n_padded, s), "memset layout");
+        check(hipMemsetAsync(d_val, 0, sizeof(double) * n_padded, s), "memset layout");
+        check(hipMemsetAsync(t_cnt, 0, sizeof(int) * (n_ids + 1), s), "memset counts");
+        const int grid = std::max(1, std::min(2048, (mp + 3) / 4));
+        hipLaunchKernelGGL(layout_rows_kernel, dim3(grid), dim3(256), 0, s, mp, t_src, t_rp, t_col, t_val, d_rp,
+                           d_internal, t_vt, v.n, d_col, d_val, t_cnt, t_cnt + n_ids);
+        check(hipGetLastError(), "layout launch");
+        std::vector<int> cnt(n_ids + 1);
+        check(hipMemcpyAsync(cnt.data(), t_cnt, sizeof(int) * (n_ids + 1), hipMemcpyDeviceToHost, s), "D2H counts");
+        check(hipStreamSynchronize(s), "layout");
+        if (cnt[n_ids]) throw std::runtime_error("column index out of range");
+        std::vector<int> cptr(n_ids + 1, 0);
+        for (int c = 0; c < n_ids; c++) cptr[c + 1] = cptr[c] + cnt[c];
+        upload(d_cptr, cptr.data(), n_ids + 1, s);
+        upload(t_cnt, cptr.data(), n_ids, s);
+        hipLaunchKernelGGL(layout_csc_kernel, dim3(grid), dim3(256), 0, s, mp, d_rp, d_col, t_part, t_cnt, d_crows,
+                           d_cseg);
+        check(hipGetLastError(), "layout launch");
+        check(hipStreamSynchronize(s), "layout");
+    } catch (...) {
+        for (void *p : tmp) (void)hipFree(p);
+        throw;
+    }
+    for (void *p : tmp) check(hipFree(p), "hipFree");
+}
+
 Problem *build(const std::vector<InstanceView> &inst, bool replicas, const Tuning *tuning) {
     if (inst.empty()) throw std::runtime_error("no instance");
-    for (const auto &v : inst) validate(v);
+    // A single instance has its nonzeros laid out on the device (the host
+    // orders rows and columns only); a batch is laid out on the host.
+    const bool device_layout = replicas && inst.size() == 1;
+    for (const auto &v : inst) validate(v, !device_layout);
     auto *P = new Problem();
     try {
         P->replicas = replicas;
@@ -353,8 +413,8 @@ Problem *build(const std::vector<InstanceView> &inst, bool replicas, const Tunin
         if (padded > 0x7fffff00LL) throw std::runtime_error("too many nonzeros");
         std::vector<int> h_rp(mp + 1);
         std::vector<double2> h_sides(mp);
-        std::vector<int> h_col(padded, 0);
-        std::vector<double> h_val(padded, 0.0);
+        std::vector<int> h_col(device_layout ? 0 : padded, 0);
+        std::vector<double> h_val(device_layout ? 0 : padded, 0.0);
         std::vector<int> col_cnt(n_ids + 1, 0);
         int pos = 0;
         for (int r = 0; r < mp; r++) {
@@ -363,6 +423,10 @@ Problem *build(const std::vector<InstanceView> &inst, bool replicas, const Tunin
             const int pad = (kRowAlign - h_len[r] % kRowAlign) % kRowAlign;
             h_rp[r] = pos | pad;
             h_sides[r] = make_double2(v.lhs[row], v.rhs[row]);
+            if (device_layout) {
+                pos += h_len[r] + pad;
+                continue;
+            }
             for (int k = v.row_ptr[row]; k < v.row_ptr[row + 1]; k++) {
                 const int id = internal[col_off[i] + v.col[k]];
                 h_col[pos] = id | (v.vartype[v.col[k]] == kVarContinuous ? 0 : kIntFlag);
@@ -377,9 +441,9 @@ Problem *build(const std::vector<InstanceView> &inst, bool replicas, const Tunin
 
         // column-to-row index of the internal layout (row marking)
         std::vector<int> h_cptr(n_ids + 1, 0);
-        for (int c = 0; c < n_ids; c++) h_cptr[c + 1] = h_cptr[c] + col_cnt[c + 1];
-        std::vector<int> h_crows(std::max<long long>(1, nnz_total));
-        {
+        std::vector<int> h_crows(device_layout ? 1 : std::max<long long>(1, nnz_total));
+        if (!device_layout) {
+            for (int c = 0; c < n_ids; c++) h_cptr[c + 1] = h_cptr[c] + col_cnt[c + 1];
             std::vector<int> fill(h_cptr.begin(), h_cptr.end() - 1);
             for (int r = 0; r < mp; r++) {
                 const int st = h_rp[r] & ~(kRowAlign - 1);
@@ -454,9 +518,9 @@ Problem *build(const std::vector<InstanceView> &inst, bool replicas, const Tunin
         // segment of each csc entry that lies in a split row (same fill order
         // as col_rows)
         std::vector<int> h_cseg(h_crows.size(), -1);
-        {
-            std::vector<int> row_part(mp, -1);
-            for (const int4 &sr : h_seg_rows) row_part[sr.x] = sr.y;
+        std::vector<int> row_part(mp, -1);
+        for (const int4 &sr : h_seg_rows) row_part[sr.x] = sr.y;
+        if (!device_layout) {
             std::vector<int> fill(h_cptr.begin(), h_cptr.end() - 1);
             for (int rr = 0; rr < mp; rr++) {
                 const int st = h_rp[rr] & ~(kRowAlign - 1);
@@ -501,24 +565,32 @@ Problem *build(const std::vector<InstanceView> &inst, bool replicas, const Tunin
         int4 *d_seg_rows = dev_alloc<int4>(h_seg_rows.size(), o);
         int *d_part_row = dev_alloc<int>(h_part_row.size(), o);
         int *d_cptr = dev_alloc<int>(n_ids + 1, o);
-        int *d_crows = dev_alloc<int>(h_crows.size(), o);
-        int *d_cseg = dev_alloc<int>(h_cseg.size(), o);
+        const size_t n_csc = device_layout ? std::max<long long>(1, nnz_total) : h_crows.size();
+        int *d_crows = dev_alloc<int>(n_csc, o);
+        int *d_cseg = dev_alloc<int>(n_csc, o);
         int *d_perm = dev_alloc<int>(n_ids, o);
         int *d_word_unit = dev_alloc<int>(word_unit.size(), o);
         P->d_inv_perm = dev_alloc<int>(P->n, o);
         upload(d_rp, h_rp.data(), mp + 1, s);
         upload(d_sides, h_sides.data(), mp, s);
-        upload(d_col, h_col.data(), n_padded, s);
-        upload(d_val, h_val.data(), n_padded, s);
         upload(d_items, h_items.data(), h_items.size(), s);
         upload(d_seg_rows, h_seg_rows.data(), h_seg_rows.size(), s);
         upload(d_part_row, h_part_row.data(), h_part_row.size(), s);
-        upload(d_cptr, h_cptr.data(), n_ids + 1, s);
-        upload(d_crows, h_crows.data(), h_crows.size(), s);
-        upload(d_cseg, h_cseg.data(), h_cseg.size(), s);
         upload(d_perm, perm.data(), n_ids, s);
         upload(d_word_unit, word_unit.data(), word_unit.size(), s);
         upload(P->d_inv_perm, internal.data(), P->n, s);
+        if (device_layout) {
+            std::vector<int> src_row(mp);
+            for (int r = 0; r < mp; r++) src_row[r] = order[r].second;
+            device_layout_build(inst[0], mp, n_ids, n_padded, src_row, row_part, d_rp, P->d_inv_perm, d_col, d_val,
+                                d_cptr, d_crows, d_cseg, s);
+        } else {
+            upload(d_col, h_col.data(), n_padded, s);
+            upload(d_val, h_val.data(), n_padded, s);
+            upload(d_cptr, h_cptr.data(), n_ids + 1, s);
+            upload(d_crows, h_crows.data(), h_crows.size(), s);
+            upload(d_cseg, h_cseg.data(), h_cseg.size(), s);
+        }
         M.row_ptr = d_rp;
         M.sides = d_sides;
         M.col = d_col;
@@ -911,7 +983,6 @@ RunResult problem_propagate_measured(Problem *P, double *lbs, double *ubs, int m
     if (!P->replicas) throw std::runtime_error("progress measures are for a single instance");
     if (max_rounds < 1 || max_rounds > kMaxRounds) throw std::runtime_error("max_rounds out of range");
     std::lock_guard<std::mutex> lock(P->run_mutex);
-    const DevMatrix &M = P->M;
     hipStream_t s = P->stream;
     const int n = P->n;
     const size_t nb = sizeof(double) * static_cast<size_t>(n);

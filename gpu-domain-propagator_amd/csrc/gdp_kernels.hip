@@ -1267,4 +1267,65 @@ __global__ __launch_bounds__(64) void measure_total_kernel(int parts, const doub
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// problem layout on the device (single instance): the passes over the
+// nonzeros.  The host orders the rows (tree-width bins), pads them and
+// relabels the columns; these kernels move the entries.
+// ---------------------------------------------------------------------------
+
+// Row r of the layout is source row src_row[r]: its entries go to
+// dst = row_ptr_out[r] rounded down to kRowAlign, as (internal id | integer
+// flag, coefficient), and every entry counts one for its column in col_cnt
+// (by internal id).  Padding entries are left as written before (zeros).  A
+// column index outside [0, n) sets *bad and the entry is skipped.  One wave
+// per row, grid-stride.
+__global__ __launch_bounds__(256) void layout_rows_kernel(int mp, const int *src_row, const int *row_ptr_in,
+                                                          const int *col_in, const double *val_in,
+                                                          const int *row_ptr_out, const int *internal,
+                                                          const int *vartype, int n, int *col_out, double *val_out,
+                                                          int *col_cnt, int *bad) {
+    const int lane = threadIdx.x & 63;
+    const int waves = gridDim.x * (blockDim.x >> 6);
+    for (int r = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); r < mp; r += waves) {
+        const int src = src_row[r];
+        const int k0 = row_ptr_in[src], len = row_ptr_in[src + 1] - k0;
+        const int dst = row_ptr_out[r] & ~(kRowAlign - 1);
+        for (int e = lane; e < len; e += 64) {
+            const int j = col_in[k0 + e];
+            if (j < 0 || j >= n) {
+                atomic_or_i(bad, 1);
+                continue;
+            }
+            const int id = internal[j];
+            col_out[dst + e] = id | (vartype[j] == kVarContinuous ? 0 : kIntFlag);
+            val_out[dst + e] = val_in[k0 + e];
+            atomic_add_i(&col_cnt[id], 1);
+        }
+    }
+}
+
+// Column-to-row index: every entry of layout row r appends r to its column's
+// list (fill[] starts at col_ptr[]) and, for split rows (row_part[r] >= 0),
+// the entry's segment.  The order inside a column's list is not fixed; the
+// lists are used as sets (row and segment marking).  One wave per row.
+__global__ __launch_bounds__(256) void layout_csc_kernel(int mp, const int *row_ptr_out, const int *col_out,
+                                                         const int *row_part, int *fill, int *col_rows,
+                                                         int *col_seg) {
+    const int lane = threadIdx.x & 63;
+    const int waves = gridDim.x * (blockDim.x >> 6);
+    for (int r = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); r < mp; r += waves) {
+        const int a = row_ptr_out[r], b = row_ptr_out[r + 1];
+        const int st = a & ~(kRowAlign - 1);
+        const int len = (b & ~(kRowAlign - 1)) - st - (a & (kRowAlign - 1));
+        const int part = row_part[r];
+        for (int e = lane; e < len; e += 64) {
+            const int id = col_out[st + e] & kIdMask;
+            const int slot = atomic_add_i(&fill[id], 1);
+            col_rows[slot] = r;
+            col_seg[slot] = part >= 0 ? part + e / kSegLen : -1;
+        }
+    }
+}
+
 }  // namespace gdp
