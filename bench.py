@@ -460,6 +460,8 @@ def end_to_end_leg(inst, out_lb, out_ub, sync, runs=3):
     (propagateConstraintsGPUAtomicDouble): host arrays in, host arrays out,
     the device layout built and freed inside the call.  Best of `runs`; the
     result is checked bit for bit against the timed propagation."""
+    import numpy as np
+    from gdp_amd import interface as I
     lib = I.load_library()
     col = np.ascontiguousarray(inst.col, np.int32)
     rp = np.ascontiguousarray(inst.row_ptr, np.int32)

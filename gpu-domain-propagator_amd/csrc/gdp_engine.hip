@@ -43,9 +43,10 @@ __global__ void delta_kernel(int n_list, double *const *lbp, double *const *ubp,
                              const double *lv, const double *uv);
 __global__ void diff_kernel(int n, int n_list, double *const *lbp, double *const *ubp, const double *const *blp,
                             const double *const *bup, int cap, int *cnt, int *cols, double *vl, double *vu);
-__global__ void layout_rows_kernel(int mp, const int *src_row, const int *row_ptr_in, const int *col_in,
-                                   const double *val_in, const int *row_ptr_out, const int *internal,
-                                   const int *vartype, int n, int *col_out, double *val_out, int *col_cnt, int *bad);
+__global__ void layout_rows_kernel(int mp, const long long *src_start, const int *row_inst, const int *inst_col_off,
+                                   const int *col_in, const double *val_in, const int *row_ptr_out,
+                                   const int *internal, const int *vartype, int *col_out, double *val_out,
+                                   int *col_cnt, int *bad);
 __global__ void layout_csc_kernel(int mp, const int *row_ptr_out, const int *col_out, const int *row_part, int *fill,
                                   int *col_rows, int *col_seg);
 __global__ void measure_kernel(int n, const double *lb, const double *ub, const double *wl, const double *wu,
@@ -93,17 +94,14 @@ int column_class(int vartype, const double *env_lb, const double *env_ub, int j)
     return 2;
 }
 
-// Input contract; with `columns` false the column indices are checked later
-// (on the device, while the layout is built there).
-void validate(const InstanceView &v, bool columns) {
+// Input contract of the row structure (the column indices are checked on
+// the device while the layout is built).
+void validate(const InstanceView &v) {
     if (v.m <= 0 || v.n <= 0 || v.nnz <= 0) throw std::runtime_error("empty problem");
     if (!v.col || !v.row_ptr || !v.val || !v.lhs || !v.rhs || !v.vartype) throw std::runtime_error("null input array");
     if (v.row_ptr[0] != 0 || v.row_ptr[v.m] != v.nnz) throw std::runtime_error("row_ptrs do not span nnz");
     for (int i = 0; i < v.m; i++)
         if (v.row_ptr[i + 1] < v.row_ptr[i]) throw std::runtime_error("row_ptrs not monotone");
-    if (columns)
-        for (int k = 0; k < v.nnz; k++)
-            if (v.col[k] < 0 || v.col[k] >= v.n) throw std::runtime_error("column index out of range");
 }
 
 // Outcome of one unit from its state words (the reference's round-count
@@ -253,35 +251,46 @@ void alloc_units(Problem *P, int cap) {
     P->stats.n_units = units;
 }
 
-// The passes over the nonzeros of a single instance, on the device: the
-// source CSR is uploaded as given, the entries are moved into the padded
-// layout (rows in the order src_row, at the starts of d_rp), the columns'
-// entries are counted, and the column-to-row index is filled.  Column
-// indices outside [0, n) are reported here.  Synchronous.
-void device_layout_build(const InstanceView &v, int mp, int n_ids, int n_padded, const std::vector<int> &src_row,
+// The passes over the nonzeros, on the device: the instances' CSR arrays are
+// uploaded as given (concatenated), the entries are moved into the padded
+// layout (layout row r from source position src_start[r] of instance
+// row_inst[r], at the starts of d_rp), the columns' entries are counted and
+// the column-to-row index is filled.  Column indices outside an instance's
+// range are reported here.  Synchronous.
+void device_layout_build(const std::vector<InstanceView> &inst, const std::vector<int> &col_off, int mp, int n_ids,
+                         int n_padded, const std::vector<long long> &src_start, const std::vector<int> &row_inst,
                          const std::vector<int> &row_part, const int *d_rp, const int *d_internal, int *d_col,
                          double *d_val, int *d_cptr, int *d_crows, int *d_cseg, hipStream_t s) {
+    const int NI = static_cast<int>(inst.size());
+    long long nnz = 0;
+    for (const auto &v : inst) nnz += v.nnz;
     std::vector<void *> tmp;
     try {
-        int *t_col = dev_alloc<int>(v.nnz, tmp);
-        double *t_val = dev_alloc<double>(v.nnz, tmp);
-        int *t_rp = dev_alloc<int>(v.m + 1, tmp);
-        int *t_src = dev_alloc<int>(mp, tmp);
+        int *t_col = dev_alloc<int>(nnz, tmp);
+        double *t_val = dev_alloc<double>(nnz, tmp);
+        long long *t_src = dev_alloc<long long>(mp, tmp);
+        int *t_inst = NI > 1 ? dev_alloc<int>(mp, tmp) : nullptr;
+        int *t_coff = dev_alloc<int>(NI + 1, tmp);
         int *t_part = dev_alloc<int>(mp, tmp);
-        int *t_vt = dev_alloc<int>(v.n, tmp);
+        int *t_vt = dev_alloc<int>(col_off[NI], tmp);
         int *t_cnt = dev_alloc<int>(n_ids + 1, tmp);  // counts, then the fill positions
-        upload(t_col, v.col, v.nnz, s);
-        upload(t_val, v.val, v.nnz, s);
-        upload(t_rp, v.row_ptr, v.m + 1, s);
-        upload(t_src, src_row.data(), mp, s);
+        long long at = 0;
+        for (int i = 0; i < NI; i++) {
+            upload(t_col + at, inst[i].col, inst[i].nnz, s);
+            upload(t_val + at, inst[i].val, inst[i].nnz, s);
+            upload(t_vt + col_off[i], inst[i].vartype, inst[i].n, s);
+            at += inst[i].nnz;
+        }
+        upload(t_src, src_start.data(), mp, s);
+        if (t_inst) upload(t_inst, row_inst.data(), mp, s);
+        upload(t_coff, col_off.data(), NI + 1, s);
         upload(t_part, row_part.data(), mp, s);
-        upload(t_vt, v.vartype, v.n, s);
         check(hipMemsetAsync(d_col, 0, sizeof(int) * n_padded, s), "memset layout");
         check(hipMemsetAsync(d_val, 0, sizeof(double) * n_padded, s), "memset layout");
         check(hipMemsetAsync(t_cnt, 0, sizeof(int) * (n_ids + 1), s), "memset counts");
         const int grid = std::max(1, std::min(2048, (mp + 3) / 4));
-        hipLaunchKernelGGL(layout_rows_kernel, dim3(grid), dim3(256), 0, s, mp, t_src, t_rp, t_col, t_val, d_rp,
-                           d_internal, t_vt, v.n, d_col, d_val, t_cnt, t_cnt + n_ids);
+        hipLaunchKernelGGL(layout_rows_kernel, dim3(grid), dim3(256), 0, s, mp, t_src, t_inst, t_coff, t_col, t_val,
+                           d_rp, d_internal, t_vt, d_col, d_val, t_cnt, t_cnt + n_ids);
         check(hipGetLastError(), "layout launch");
         std::vector<int> cnt(n_ids + 1);
         check(hipMemcpyAsync(cnt.data(), t_cnt, sizeof(int) * (n_ids + 1), hipMemcpyDeviceToHost, s), "D2H counts");
@@ -304,10 +313,9 @@ void device_layout_build(const InstanceView &v, int mp, int n_ids, int n_padded,
 
 Problem *build(const std::vector<InstanceView> &inst, bool replicas, const Tuning *tuning) {
     if (inst.empty()) throw std::runtime_error("no instance");
-    // A single instance has its nonzeros laid out on the device (the host
-    // orders rows and columns only); a batch is laid out on the host.
-    const bool device_layout = replicas && inst.size() == 1;
-    for (const auto &v : inst) validate(v, !device_layout);
+    // The host orders rows and columns; the nonzeros are laid out on the
+    // device, which also checks the column indices.
+    for (const auto &v : inst) validate(v);
     auto *P = new Problem();
     try {
         P->replicas = replicas;
@@ -413,43 +421,23 @@ Problem *build(const std::vector<InstanceView> &inst, bool replicas, const Tunin
         if (padded > 0x7fffff00LL) throw std::runtime_error("too many nonzeros");
         std::vector<int> h_rp(mp + 1);
         std::vector<double2> h_sides(mp);
-        std::vector<int> h_col(device_layout ? 0 : padded, 0);
-        std::vector<double> h_val(device_layout ? 0 : padded, 0.0);
-        std::vector<int> col_cnt(n_ids + 1, 0);
-        int pos = 0;
-        for (int r = 0; r < mp; r++) {
-            const int i = order[r].first, row = order[r].second;
-            const auto &v = inst[i];
-            const int pad = (kRowAlign - h_len[r] % kRowAlign) % kRowAlign;
-            h_rp[r] = pos | pad;
-            h_sides[r] = make_double2(v.lhs[row], v.rhs[row]);
-            if (device_layout) {
-                pos += h_len[r] + pad;
-                continue;
-            }
-            for (int k = v.row_ptr[row]; k < v.row_ptr[row + 1]; k++) {
-                const int id = internal[col_off[i] + v.col[k]];
-                h_col[pos] = id | (v.vartype[v.col[k]] == kVarContinuous ? 0 : kIntFlag);
-                h_val[pos] = v.val[k];
-                col_cnt[id + 1]++;
-                pos++;
-            }
-            pos += pad;
-        }
-        h_rp[mp] = pos;
-        const int n_padded = pos;
-
-        // column-to-row index of the internal layout (row marking)
-        std::vector<int> h_cptr(n_ids + 1, 0);
-        std::vector<int> h_crows(device_layout ? 1 : std::max<long long>(1, nnz_total));
-        if (!device_layout) {
-            for (int c = 0; c < n_ids; c++) h_cptr[c + 1] = h_cptr[c] + col_cnt[c + 1];
-            std::vector<int> fill(h_cptr.begin(), h_cptr.end() - 1);
+        std::vector<long long> src_start(mp);  // position of the row's entries in the concatenated inputs
+        {
+            std::vector<long long> nnz_off(NI + 1, 0);
+            for (int i = 0; i < NI; i++) nnz_off[i + 1] = nnz_off[i] + inst[i].nnz;
+            int pos = 0;
             for (int r = 0; r < mp; r++) {
-                const int st = h_rp[r] & ~(kRowAlign - 1);
-                for (int e = 0; e < h_len[r]; e++) h_crows[fill[h_col[st + e] & kIdMask]++] = r;
+                const int i = order[r].first, row = order[r].second;
+                const auto &v = inst[i];
+                const int pad = (kRowAlign - h_len[r] % kRowAlign) % kRowAlign;
+                h_rp[r] = pos | pad;
+                h_sides[r] = make_double2(v.lhs[row], v.rhs[row]);
+                src_start[r] = nnz_off[i] + v.row_ptr[row];
+                pos += h_len[r] + pad;
             }
+            h_rp[mp] = pos;
         }
+        const int n_padded = h_rp[mp];
 
         // ---- sweep items ----
         // larger group items for large instances (fewer, longer waves)
@@ -515,19 +503,10 @@ Problem *build(const std::vector<InstanceView> &inst, bool replicas, const Tunin
         }
         if (NI >= (1 << (31 - kItemUnitShift))) throw std::runtime_error("too many instances in a batch");
 
-        // segment of each csc entry that lies in a split row (same fill order
-        // as col_rows)
-        std::vector<int> h_cseg(h_crows.size(), -1);
+        // first segment of each split row (-1: not split), for the segment
+        // of every column-to-row entry
         std::vector<int> row_part(mp, -1);
         for (const int4 &sr : h_seg_rows) row_part[sr.x] = sr.y;
-        if (!device_layout) {
-            std::vector<int> fill(h_cptr.begin(), h_cptr.end() - 1);
-            for (int rr = 0; rr < mp; rr++) {
-                const int st = h_rp[rr] & ~(kRowAlign - 1);
-                for (int e = 0; e < h_len[rr]; e++)
-                    h_cseg[fill[h_col[st + e] & kIdMask]++] = row_part[rr] >= 0 ? row_part[rr] + e / kSegLen : -1;
-            }
-        }
 
         P->stats.m = P->m;
         P->stats.n = P->n;
@@ -565,7 +544,7 @@ Problem *build(const std::vector<InstanceView> &inst, bool replicas, const Tunin
         int4 *d_seg_rows = dev_alloc<int4>(h_seg_rows.size(), o);
         int *d_part_row = dev_alloc<int>(h_part_row.size(), o);
         int *d_cptr = dev_alloc<int>(n_ids + 1, o);
-        const size_t n_csc = device_layout ? std::max<long long>(1, nnz_total) : h_crows.size();
+        const size_t n_csc = std::max<long long>(1, nnz_total);
         int *d_crows = dev_alloc<int>(n_csc, o);
         int *d_cseg = dev_alloc<int>(n_csc, o);
         int *d_perm = dev_alloc<int>(n_ids, o);
@@ -579,18 +558,8 @@ Problem *build(const std::vector<InstanceView> &inst, bool replicas, const Tunin
         upload(d_perm, perm.data(), n_ids, s);
         upload(d_word_unit, word_unit.data(), word_unit.size(), s);
         upload(P->d_inv_perm, internal.data(), P->n, s);
-        if (device_layout) {
-            std::vector<int> src_row(mp);
-            for (int r = 0; r < mp; r++) src_row[r] = order[r].second;
-            device_layout_build(inst[0], mp, n_ids, n_padded, src_row, row_part, d_rp, P->d_inv_perm, d_col, d_val,
-                                d_cptr, d_crows, d_cseg, s);
-        } else {
-            upload(d_col, h_col.data(), n_padded, s);
-            upload(d_val, h_val.data(), n_padded, s);
-            upload(d_cptr, h_cptr.data(), n_ids + 1, s);
-            upload(d_crows, h_crows.data(), h_crows.size(), s);
-            upload(d_cseg, h_cseg.data(), h_cseg.size(), s);
-        }
+        device_layout_build(inst, col_off, mp, n_ids, n_padded, src_start, h_unit, row_part, d_rp, P->d_inv_perm, d_col,
+                            d_val, d_cptr, d_crows, d_cseg, s);
         M.row_ptr = d_rp;
         M.sides = d_sides;
         M.col = d_col;

@@ -1274,31 +1274,37 @@ __global__ __launch_bounds__(64) void measure_total_kernel(int parts, const doub
 // relabels the columns; these kernels move the entries.
 // ---------------------------------------------------------------------------
 
-// Row r of the layout is source row src_row[r]: its entries go to
-// dst = row_ptr_out[r] rounded down to kRowAlign, as (internal id | integer
-// flag, coefficient), and every entry counts one for its column in col_cnt
-// (by internal id).  Padding entries are left as written before (zeros).  A
-// column index outside [0, n) sets *bad and the entry is skipped.  One wave
-// per row, grid-stride.
-__global__ __launch_bounds__(256) void layout_rows_kernel(int mp, const int *src_row, const int *row_ptr_in,
-                                                          const int *col_in, const double *val_in,
-                                                          const int *row_ptr_out, const int *internal,
-                                                          const int *vartype, int n, int *col_out, double *val_out,
-                                                          int *col_cnt, int *bad) {
+// Row r of the layout takes the entries col_in / val_in [src_start[r],
+// src_start[r] + len), len from the padded row starts row_ptr_out, of
+// instance row_inst[r] (instance 0 when row_inst is null), whose columns are
+// global columns inst_col_off[i] .. inst_col_off[i + 1] - 1.  Each entry goes
+// to its padded place as (internal id | integer flag, coefficient) and counts
+// one for its column in col_cnt (by internal id).  Padding entries are left
+// as written before (zeros).  A column index outside the instance's range
+// sets *bad and the entry is skipped.  One wave per row, grid-stride.
+__global__ __launch_bounds__(256) void layout_rows_kernel(int mp, const long long *src_start, const int *row_inst,
+                                                          const int *inst_col_off, const int *col_in,
+                                                          const double *val_in, const int *row_ptr_out,
+                                                          const int *internal, const int *vartype, int *col_out,
+                                                          double *val_out, int *col_cnt, int *bad) {
     const int lane = threadIdx.x & 63;
     const int waves = gridDim.x * (blockDim.x >> 6);
     for (int r = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); r < mp; r += waves) {
-        const int src = src_row[r];
-        const int k0 = row_ptr_in[src], len = row_ptr_in[src + 1] - k0;
-        const int dst = row_ptr_out[r] & ~(kRowAlign - 1);
+        const long long k0 = src_start[r];
+        const int a = row_ptr_out[r], b = row_ptr_out[r + 1];
+        const int dst = a & ~(kRowAlign - 1);
+        const int len = (b & ~(kRowAlign - 1)) - dst - (a & (kRowAlign - 1));
+        const int inst = row_inst ? row_inst[r] : 0;
+        const int base = inst_col_off[inst], lim = inst_col_off[inst + 1] - base;
         for (int e = lane; e < len; e += 64) {
             const int j = col_in[k0 + e];
-            if (j < 0 || j >= n) {
+            if (j < 0 || j >= lim) {
                 atomic_or_i(bad, 1);
                 continue;
             }
-            const int id = internal[j];
-            col_out[dst + e] = id | (vartype[j] == kVarContinuous ? 0 : kIntFlag);
+            const int g = base + j;
+            const int id = internal[g];
+            col_out[dst + e] = id | (vartype[g] == kVarContinuous ? 0 : kIntFlag);
             val_out[dst + e] = val_in[k0 + e];
             atomic_add_i(&col_cnt[id], 1);
         }
