@@ -124,6 +124,39 @@ def infeasible_instance() -> Instance:
                      [0, 0, 0], [5, 5, 5], [1, 1, 3], "infeasible")
 
 
+def layout_edges(seed: int = 0, n: int = 90000) -> Instance:
+    """Shapes at the edges of the matrix layout: empty rows first, in the
+    middle and last; rows of 1, 3, 4, 5, 255, 256, 257, 512, 513 and 1024
+    entries (padding and segment boundaries); one row of 70000 entries (274
+    segments); most columns referenced by no row; binary, small-integer,
+    wide-integer (beyond 16 bits) and continuous columns, some infinite
+    bounds."""
+    rng = np.random.default_rng(seed)
+    lens = [0, 1, 3, 4, 5, 0, 255, 256, 257, 512, 513, 1024, 70000, 0, 2, 7, 0]
+    lens += rng.integers(1, 40, size=300).tolist() + [0]
+    used = rng.choice(n, size=75000, replace=False)  # columns any row may use
+    rows, vals = [], []
+    for L in lens:
+        cols = np.sort(rng.choice(used, size=L, replace=False)) if L else np.zeros(0, np.int64)
+        v = rng.integers(1, 9, size=L).astype(np.float64) * np.where(rng.random(L) < 0.5, -1.0, 1.0)
+        rows.append(cols.tolist())
+        vals.append(v.tolist())
+    kind = rng.integers(0, 4, size=n)  # 0 binary, 1 small integer, 2 wide integer, 3 continuous
+    lb = np.select([kind == 0, kind == 1, kind == 2], [0.0, -5.0, -100000.0], -50.0)
+    ub = np.select([kind == 0, kind == 1, kind == 2], [1.0, 20.0, 200000.0], 50.0)
+    cont = kind == 3
+    lb[cont & (rng.random(n) < 0.1)] = -INF
+    ub[cont & (rng.random(n) < 0.1)] = INF
+    vartype = np.where(cont, 3, np.where(kind == 0, 0, 1)).astype(np.int32)
+    x0 = np.where(kind == 3, rng.uniform(-10, 10, size=n), np.minimum(np.maximum(0.0, lb), ub))
+    lhs, rhs = [], []
+    for cols, v in zip(rows, vals):
+        act = float(np.dot(v, x0[cols])) if cols else 0.0
+        lhs.append(act - float(rng.uniform(0, 20)) if rng.random() < 0.7 else -INF)
+        rhs.append(act + float(rng.uniform(0, 20)) if rng.random() < 0.7 else INF)
+    return from_rows(len(lens), n, rows, vals, lhs, rhs, lb, ub, vartype, f"layout_edges{seed}")
+
+
 def int32_overflow() -> Instance:
     """Integer columns with compact bound records (16-bit range: x, y; 32-bit
     range: v, t) receive candidates beyond the int32 range: x - w >= 3e9,

@@ -39,6 +39,8 @@ def _case(name):
         return fixtures.infeasible_instance()
     if kind == "int32_overflow":
         return fixtures.int32_overflow()
+    if kind == "layout_edges":
+        return fixtures.layout_edges(int(arg))
     if kind == "tiny":
         return fixtures.tiny_integer(int(arg))
     if kind == "powerlaw":
@@ -48,7 +50,7 @@ def _case(name):
 
 CASES = ([f"analytic:{n}" for n in fixtures.ANALYTIC] + [f"small:{s}" for s in range(3)] +
          ["small_int:7", "varied:0", "varied:1", "varied:2", "gcfp:30", "infeasible", "int32_overflow", "tiny:3",
-          "tiny:11", "powerlaw:5", "gate_ties"])
+          "tiny:11", "powerlaw:5", "gate_ties", "layout_edges:0", "layout_edges:1"])
 
 
 def assert_same(got, exp, label=""):
@@ -202,3 +204,21 @@ def test_cover_knapsack_full_size(gpu, oracle, one_frac):
     if not exp[3]:
         np.testing.assert_array_equal(exp[0], seq[0])
         np.testing.assert_array_equal(exp[1], seq[1])
+
+
+def test_column_index_out_of_range_is_an_error(gpu):
+    # checked on the device while the layout is built; no problem is created
+    inst = fixtures.layout_edges(2)
+    bad = inst.col.copy()
+    bad[len(bad) // 2] = inst.n
+    broken = type(inst)(inst.m, inst.n, bad, inst.row_ptr, inst.val, inst.lhs, inst.rhs, inst.lb, inst.ub,
+                        inst.vartype, "bad_column")
+    with pytest.raises(RuntimeError, match="column index out of range"):
+        I.Problem(broken)
+    bad[len(bad) // 2] = -1
+    with pytest.raises(RuntimeError, match="column index out of range"):
+        I.Batch([fixtures.analytic("achterberg"), broken])
+    # the device is still usable afterwards
+    exp = fixtures.analytic("achterberg")
+    got = I.gpu_propagate(exp, I.MEGAKERNEL)
+    np.testing.assert_allclose(got.lbs, exp.meta["lb_expected"], atol=1e-4)

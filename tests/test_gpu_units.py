@@ -27,7 +27,7 @@ def batch_instances():
     insts += [small_mip(seed=s, m=200 + 150 * s, n=100 + 70 * s) for s in range(5)]
     insts += [small_mip(seed=7, int_frac=1.0), fixtures.generalized_cfp(12), fixtures.gate_ties(),
               fixtures.infeasible_instance(), fixtures.varied(1),
-              powerlaw_mip(seed=5, m=6000, n=3000, len_hi=3000, s=1.8)]
+              powerlaw_mip(seed=5, m=6000, n=3000, len_hi=3000, s=1.8), fixtures.layout_edges(3)]
     return insts
 
 
