@@ -455,6 +455,24 @@ def summarize(d, kernel="mega_kernel"):
 
 # ---------------------------------------------------------------------------
 
+def measured_copy_gbs(nbytes=1 << 30, reps=5):
+    """HBM bandwidth this device reaches in a plain device-to-device copy of
+    nbytes: (read + write bytes) / time, best of reps (STREAM-style copy)."""
+    import numpy as np
+    from gdp_amd import interface as I
+    src = I.DeviceArray(np.zeros(nbytes // 8))
+    dst = I.DeviceArray(np.zeros(nbytes // 8))
+    dst.copy_from(src)
+    best = float("inf")
+    for _ in range(reps):
+        t = time.perf_counter()
+        dst.copy_from(src)  # synchronous
+        best = min(best, time.perf_counter() - t)
+    src.free()
+    dst.free()
+    return 2.0 * nbytes / best / 1e9
+
+
 def end_to_end_leg(inst, out_lb, out_ub, sync, runs=3):
     """The drop-in call as a caller of the reference's C interface makes it
     (propagateConstraintsGPUAtomicDouble): host arrays in, host arrays out,
@@ -582,6 +600,8 @@ def main():
                        "warmup": args.warmup, "steps": args.steps, "bench_launch_ms": kernel_ms}, fh)
 
     traffic = load_traffic(inst, rounds, kernel) if args.sync == 2 else None
+    print("bench: device copy bandwidth", file=sys.stderr, flush=True)
+    copy_gbs = measured_copy_gbs()
     result = {
         "metric": METRIC,
         "value": value,
@@ -605,7 +625,8 @@ def main():
                      "processed_nnz_per_launch": int(w_nnz.sum()), "processed_rows_per_launch": int(w_rows.sum()),
                      "accepted_candidates_per_launch": int(w_acc.sum()),
                      "dense_sweep_bytes_per_launch": dense,
-                     "dense_equivalent_gbs": dense / (kernel_ms * 1e-3) / 1e9},
+                     "dense_equivalent_gbs": dense / (kernel_ms * 1e-3) / 1e9,
+                     "measured_copy_gbs": copy_gbs, "frac_of_measured_copy": achieved / copy_gbs},
         "wall_ms_per_step": wall_ms,
     }
     print("bench: end-to-end drop-in call", file=sys.stderr, flush=True)
@@ -621,7 +642,10 @@ def main():
         parity, base, bparity = cpu_leg(inst, out_lb, out_ub, rounds, args.cpu_runs, batched_out)
         result["parity"] = parity
         result["cpu_baseline"] = base
+        # the host cores of this GPU's share, and the reference's compile-time
+        # 8 threads (SHARED_MEM_THREADS)
         result["cpu_omp"] = omp_baseline(inst, min(16, os.cpu_count() or 8))
+        result["cpu_omp_8_threads"] = omp_baseline(inst, 8)
         if bparity is not None:
             result["batched"]["parity"] = bparity
     print(json.dumps(result), flush=True)
