@@ -283,7 +283,11 @@ def serving_leg(sync):
             start = time.perf_counter()
             while got < N:
                 now = time.perf_counter() - start
-                while k < N and arr[k] <= now and (mode not in ("pinned", "inplace") or free_out):
+                # at most 64 submissions between two steps: the driver is one
+                # thread, and a burst of submissions must not starve the steps
+                burst = 0
+                while k < N and arr[k] <= now and burst < 64 and (mode not in ("pinned", "inplace") or free_out):
+                    burst += 1
                     lb, ub, cols = pool[k % len(pool)]
                     if mode == "full":
                         t_ = srv.submit(lb, ub, hints=cols)

@@ -1210,6 +1210,19 @@ struct Server {
     std::vector<hipEvent_t> stage_ev;
     std::vector<int> stage_free;
     double *stage_lb(int e) { return d_stage + 2 * n * e; }
+    // host vectors of n doubles for requests and results that found no
+    // pinned entry, kept for reuse (up to 1 GiB): a fresh allocation each
+    // time pays page faults on every page, which stalls the submitting thread
+    std::vector<std::vector<double>> spare;
+    std::vector<double> take_vec() {
+        if (spare.empty()) return std::vector<double>(n);
+        std::vector<double> v = std::move(spare.back());
+        spare.pop_back();
+        return v;
+    }
+    void give_vec(std::vector<double> &&v) {
+        if (v.size() == n && (spare.size() + 1) * n * sizeof(double) <= (1ull << 30)) spare.push_back(std::move(v));
+    }
     // device buffers for the changes of delta requests
     int *d_dptr = nullptr, *d_dcols = nullptr;
     double *d_dlb = nullptr, *d_dub = nullptr;
@@ -1353,8 +1366,10 @@ int64_t server_submit(Server *S, const double *lbs, const double *ubs, const int
         std::memcpy(S->in_ub(q.inbox), ubs, sizeof(double) * S->n);
         prefetch(S, q, S->in_lb(q.inbox), S->in_ub(q.inbox));
     } else {
-        q.lb.assign(lbs, lbs + S->n);
-        q.ub.assign(ubs, ubs + S->n);
+        q.lb = S->take_vec();
+        q.ub = S->take_vec();
+        std::memcpy(q.lb.data(), lbs, sizeof(double) * S->n);
+        std::memcpy(q.ub.data(), ubs, sizeof(double) * S->n);
     }
     if (hints && n_hints > 0) {
         for (int k = 0; k < n_hints; k++)
@@ -1530,6 +1545,7 @@ int server_step(Server *S) {
     std::vector<UnitState> st = read_units(P, static_cast<int>(active.size()), active.data());  // synchronizes
     for (int e : used_inbox) S->inbox_free.push_back(e);
     for (int e : used_stage) S->stage_free.push_back(e);
+    for (auto &v : keep) S->give_vec(std::move(v));
     keep.clear();
     int err = 0;
     check(hipMemcpy(&err, P->D.global + kGlobalError, sizeof(int), hipMemcpyDeviceToHost), "D2H");
@@ -1628,8 +1644,8 @@ int server_step(Server *S) {
                 hl = S->out_lb(d.outbox);
                 hu = S->out_ub(d.outbox);
             } else {
-                d.lb.resize(n);
-                d.ub.resize(n);
+                d.lb = S->take_vec();
+                d.ub = S->take_vec();
                 hl = d.lb.data();
                 hu = d.ub.data();
             }
@@ -1673,6 +1689,8 @@ bool server_result(Server *S, int64_t ticket, double *lbs, double *ubs, RunResul
         std::memcpy(lbs, d.lb.data(), sizeof(double) * S->n);
         std::memcpy(ubs, d.ub.data(), sizeof(double) * S->n);
     }
+    S->give_vec(std::move(d.lb));
+    S->give_vec(std::move(d.ub));
     *r = d.r;
     S->done.erase(it);
     return true;
@@ -1701,6 +1719,8 @@ int server_result_changes(Server *S, int64_t ticket, std::vector<int> &cols, std
                 ubs.push_back(fu[j]);
             }
         if (d.outbox >= 0) S->outbox_free.push_back(d.outbox);
+        S->give_vec(std::move(d.lb));
+        S->give_vec(std::move(d.ub));
     }
     *r = d.r;
     S->done.erase(it);
