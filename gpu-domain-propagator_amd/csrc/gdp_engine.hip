@@ -721,7 +721,18 @@ double run_steps(Problem *P, int sync_type, int max_rounds) {
     } else {
         int first = 1, steps = max_rounds, n_units = P->n_units;
         void *args[] = {const_cast<DevMatrix *>(&M), &D, &first, &steps, &n_units};
-        check(hipLaunchCooperativeKernel(P->mega_fn, dim3(P->mega_grid), dim3(P->mega_threads), args, 0, s),
+        // as many workgroups as the largest phase has waves of work (sweep
+        // items, bitmap chunks, split-row segments), up to the resident
+        // maximum: a small problem then pays its grid barriers over a few
+        // workgroups instead of the whole chip
+        const long long units = P->replicas ? D.n_active : 1;
+        const long long need = units * std::max<long long>({static_cast<long long>(M.n_items),
+                                                            static_cast<long long>((M.nwords + 7) / 8),
+                                                            static_cast<long long>(M.n_seg_parts)});
+        const int waves_per_group = P->mega_threads / 64;
+        const int grid = static_cast<int>(
+            std::max(1LL, std::min<long long>(P->mega_grid, (need + waves_per_group - 1) / waves_per_group)));
+        check(hipLaunchCooperativeKernel(P->mega_fn, dim3(grid), dim3(P->mega_threads), args, 0, s),
               "mega_kernel launch");
         int g[kGlobalWords];
         check(hipMemcpyAsync(g, D.global, sizeof(g), hipMemcpyDeviceToHost, s), "D2H");
