@@ -273,67 +273,110 @@ def serving_leg(sync):
         a.array[:] = pool[q][0]
         c.array[:] = pool[q][1]
     pin_out = [(I.PinnedArray(inst.n), I.PinnedArray(inst.n)) for _ in range(n_out)]
+    # The server steps on its own thread.  Four driver threads produce the
+    # requests at their Poisson arrival times (request k on thread k mod 4),
+    # submit them and take completed results, whoever sees them first: the
+    # vector copies run inside the library and numpy, which release the GIL,
+    # so the threads overlap where one thread would bound the rate.
+    import threading
+    n_drv = 4
+    srv.start()
     rates = []
     for mode in ("full", "pinned", "inplace", "delta"):
         for rate in (1000, 2000, 5000, 10000, 20000, 50000):
             N = int(min(4000, max(600, rate)))
             arr = np.cumsum(rng.exponential(1.0 / rate, size=N))
-            idx, done, got, k = {}, np.zeros(N), 0, 0
-            free_out, out_of = list(range(n_out)), {}
+            # per ticket: request index, completion time; output pairs of
+            # zero-copy requests (a result may be taken by another thread
+            # before the submitting thread has recorded its ticket)
+            st = {"idx": {}, "done_t": {}, "got": 0, "free_out": list(range(n_out)), "out_of": {},
+                  "early": set()}
+            lock = threading.Lock()
             start = time.perf_counter()
-            while got < N:
-                now = time.perf_counter() - start
-                # at most 64 submissions between two steps: the driver is one
-                # thread, and a burst of submissions must not starve the steps
-                burst = 0
-                while k < N and arr[k] <= now and burst < 64 and (mode not in ("pinned", "inplace") or free_out):
-                    burst += 1
-                    lb, ub, cols = pool[k % len(pool)]
+
+            def take_results():
+                comp = srv.completed()
+                tnow = time.perf_counter() - start
+                for tk in comp:
+                    # full vectors back for full requests, in place for zero-copy
+                    # requests, changed columns for delta requests
                     if mode == "full":
-                        t_ = srv.submit(lb, ub, hints=cols)
-                    elif mode == "pinned":
-                        o = free_out.pop()
-                        src = pin_in[k % n_in]
-                        t_ = srv.submit_pinned(src[0], src[1], pin_out[o][0], pin_out[o][1],
-                                               hints=pool[k % n_in][2])
-                        out_of[t_] = o
-                    elif mode == "inplace":
-                        # the node vector is produced in a free pinned buffer
-                        # pair, which the result then replaces
-                        o = free_out.pop()
-                        pin_out[o][0].array[:] = lb
-                        pin_out[o][1].array[:] = ub
-                        t_ = srv.submit_pinned(pin_out[o][0], pin_out[o][1], pin_out[o][0], pin_out[o][1],
-                                               hints=cols)
-                        out_of[t_] = o
+                        srv.result(tk)
+                    elif mode in ("pinned", "inplace"):
+                        srv.result(tk, copy=False)
                     else:
-                        t_ = srv.submit_delta(cols, lb[cols], ub[cols])
-                    idx[t_] = k
-                    k += 1
-                if srv.pending():
-                    srv.step()
-                    tnow = time.perf_counter() - start
-                    for tk in srv.completed():
-                        done[idx[tk]] = tnow
-                        # full vectors back for full requests, in place for zero-copy
-                        # requests, changed columns for delta requests
-                        if mode == "full":
-                            srv.result(tk)
+                        srv.result_changes(tk)
+                    with lock:
+                        st["done_t"][tk] = tnow
+                        st["got"] += 1
+                        o = st["out_of"].pop(tk, None)
+                        if o is not None:
+                            st["free_out"].append(o)
                         elif mode in ("pinned", "inplace"):
-                            srv.result(tk, copy=False)
-                            free_out.append(out_of.pop(tk))
+                            st["early"].add(tk)
+                return len(comp)
+
+            def drive(h):
+                ks = list(range(h, N, n_drv))
+                i = 0
+                while True:
+                    with lock:
+                        if st["got"] >= N:
+                            return
+                    now = time.perf_counter() - start
+                    sub = 0
+                    while i < len(ks) and arr[ks[i]] <= now:
+                        k = ks[i]
+                        lb, ub, cols = pool[k % len(pool)]
+                        o = None
+                        if mode in ("pinned", "inplace"):
+                            with lock:
+                                o = st["free_out"].pop() if st["free_out"] else None
+                            if o is None:
+                                break  # back-pressure: wait for a free output pair
+                        if mode == "full":
+                            t_ = srv.submit(lb, ub, hints=cols)
+                        elif mode == "pinned":
+                            src = pin_in[k % n_in]
+                            t_ = srv.submit_pinned(src[0], src[1], pin_out[o][0], pin_out[o][1],
+                                                   hints=pool[k % n_in][2])
+                        elif mode == "inplace":
+                            # the node vector is produced in a free pinned buffer
+                            # pair, which the result then replaces
+                            pin_out[o][0].array[:] = lb
+                            pin_out[o][1].array[:] = ub
+                            t_ = srv.submit_pinned(pin_out[o][0], pin_out[o][1], pin_out[o][0], pin_out[o][1],
+                                                   hints=cols)
                         else:
-                            srv.result_changes(tk)
-                        got += 1
-                elif k < N:
-                    while time.perf_counter() - start < arr[k]:
-                        pass
+                            t_ = srv.submit_delta(cols, lb[cols], ub[cols])
+                        with lock:
+                            st["idx"][t_] = k
+                            if o is not None:
+                                if t_ in st["early"]:
+                                    st["early"].discard(t_)
+                                    st["free_out"].append(o)
+                                else:
+                                    st["out_of"][t_] = o
+                        i += 1
+                        sub += 1
+                    if not take_results() and not sub:
+                        time.sleep(0.00005)
+
+            drivers = [threading.Thread(target=drive, args=(h,)) for h in range(n_drv)]
+            for d in drivers:
+                d.start()
+            for d in drivers:
+                d.join()
+            done = np.zeros(N)
+            for tk, k in st["idx"].items():
+                done[k] = st["done_t"][tk]
             lat = (done - arr) * 1e3
             rates.append({"requests": mode, "offered_per_s": rate, "n": N,
                           "achieved_per_s": N / (done.max() - arr[0]),
                           "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99))})
             print(f"bench: serving {mode} {rate}/s -> {rates[-1]['achieved_per_s']:.0f}/s "
                   f"p50 {rates[-1]['p50_ms']:.2f} ms p99 {rates[-1]['p99_ms']:.2f} ms", file=sys.stderr, flush=True)
+    srv.stop()
     srv.close()
     for a, c in pin_in + pin_out:
         a.free()

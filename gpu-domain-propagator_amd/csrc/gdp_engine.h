@@ -178,12 +178,23 @@ void server_set_base(Server *s, const double *lbs, const double *ubs);
 // fixpoint: round 1 processes only the rows of the changed columns.
 int64_t server_submit_delta(Server *s, int n_changes, const int *cols, const double *lbs, const double *ubs,
                             int max_rounds);
-// One step; returns the number of requests completed by it.
+// One step; returns the number of requests completed by it.  Not while the
+// stepping thread runs.
 int server_step(Server *s);
 // Requests queued or running.
-int server_pending(const Server *s);
+int server_pending(Server *s);
 // Tickets completed by the last step.
 const std::vector<int64_t> &server_completed(const Server *s);
+// The server's own stepping thread: while it runs, a step follows whenever
+// requests are queued or running, and callers only submit and take results
+// (from any thread).  Completed tickets are collected for
+// server_take_completed.  server_stop ends the thread after its current
+// step and reports an error the thread met.
+void server_start(Server *s);
+void server_stop(Server *s);
+bool server_running(Server *s);
+// Up to cap tickets completed by the stepping thread and not yet taken.
+int server_take_completed(Server *s, int64_t *tickets, int cap);
 // Completed request: copies its bounds and outcome and forgets it; false if
 // the ticket is not (yet) complete.
 bool server_result(Server *s, int64_t ticket, double *lbs, double *ubs, RunResult *r);

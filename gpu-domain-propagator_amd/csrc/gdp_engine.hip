@@ -13,9 +13,11 @@
 
 #include <algorithm>
 #include <cmath>
+#include <condition_variable>
 #include <cstring>
 #include <deque>
 #include <mutex>
+#include <thread>
 #include <stdexcept>
 #include <string>
 #include <unordered_map>
@@ -43,6 +45,8 @@ __global__ void delta_kernel(int n_list, double *const *lbp, double *const *ubp,
                              const double *lv, const double *uv);
 __global__ void diff_kernel(int n, int n_list, double *const *lbp, double *const *ubp, const double *const *blp,
                             const double *const *bup, int cap, int *cnt, int *cols, double *vl, double *vu);
+__global__ void compact_changes_kernel(int nd, int cap, const int *cnt, const int *cols, const double *vl,
+                                       const double *vu, int *off, int *kcols, double *kl, double *ku);
 __global__ void layout_rows_kernel(int mp, const long long *src_start, const int *row_inst, const int *inst_col_off,
                                    const int *col_in, const double *val_in, const int *row_ptr_out,
                                    const int *internal, const int *vartype, int *col_out, double *val_out,
@@ -1190,6 +1194,17 @@ struct Server {
     int slots = 0;
     size_t n = 0;
     std::deque<Request> queue;
+    // Request-side state shared between the caller's threads and the
+    // stepping thread: the queue, tickets, the free lists of the pinned and
+    // device staging and of the spare vectors, finished requests, busy slot
+    // tickets.  The device work of a step runs without it.
+    std::mutex mu;
+    std::condition_variable cv;
+    std::thread worker;
+    bool running = false, stop = false;
+    std::string worker_error;
+    int busy = 0;                         // slots holding a request
+    std::deque<int64_t> completed_acc;    // completed by the stepping thread, not yet taken
     std::vector<int64_t> slot_ticket;    // -1: free
     std::unordered_map<int64_t, Done> done;
     std::vector<int64_t> last_completed;
@@ -1209,6 +1224,10 @@ struct Server {
     int *d_ccnt = nullptr, *d_ccols = nullptr;
     double *d_cl = nullptr, *d_cu = nullptr;
     int *h_ccnt = nullptr, *h_ccols = nullptr;
+    // the changes packed for the copy back: columns and bounds, each
+    // vector's start (device, then pinned host)
+    int *d_kcols = nullptr, *d_off = nullptr, *h_off = nullptr;
+    double *d_kl = nullptr, *d_ku = nullptr;
     double *h_cl = nullptr, *h_cu = nullptr;
     int diff_slots = 0;
     std::vector<double *> slot_lb, slot_ub;
@@ -1240,10 +1259,22 @@ struct Server {
     double **d_dl = nullptr, **d_du = nullptr;
     size_t dcap = 0;
     ~Server() {
+        if (running) {
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                stop = true;
+            }
+            cv.notify_all();
+            worker.join();
+        }
         if (d_buf) (void)hipFree(d_buf);
         if (d_base) (void)hipFree(d_base);
         if (inbox) (void)hipHostFree(inbox);
         if (outbox) (void)hipHostFree(outbox);
+        for (void *q : {static_cast<void *>(d_kcols), static_cast<void *>(d_off), static_cast<void *>(d_kl),
+                        static_cast<void *>(d_ku)})
+            if (q) (void)hipFree(q);
+        if (h_off) (void)hipHostFree(h_off);
         for (void *q : {static_cast<void *>(h_ccnt), static_cast<void *>(h_ccols), static_cast<void *>(h_cl),
                         static_cast<void *>(h_cu)})
             if (q) (void)hipHostFree(q);
@@ -1271,7 +1302,8 @@ struct Server {
 Server *server_create(Problem *P, int64_t hbm_budget, int max_slots) {
     if (!P->replicas) throw std::runtime_error("serving needs a single matrix");
     if (max_slots < 1) throw std::runtime_error("max_slots must be positive");
-    const int64_t per_slot = P->stats.unit_state_bytes + 32LL * P->n;  // state, vectors, in-place inputs
+    // state, bound vectors, inputs of in-place requests, device staging
+    const int64_t per_slot = P->stats.unit_state_bytes + 48LL * P->n;
     int slots = static_cast<int>(std::min<int64_t>(max_slots, hbm_budget / std::max<int64_t>(1, per_slot)));
     if (slots < 1) throw std::runtime_error("HBM budget below one node's state");
     auto *S = new Server();
@@ -1305,9 +1337,14 @@ Server *server_create(Problem *P, int64_t hbm_budget, int max_slots) {
         check(hipHostMalloc(reinterpret_cast<void **>(&S->h_ccols), sizeof(int) * dc, 0), "hipHostMalloc");
         check(hipHostMalloc(reinterpret_cast<void **>(&S->h_cl), sizeof(double) * dc, 0), "hipHostMalloc");
         check(hipHostMalloc(reinterpret_cast<void **>(&S->h_cu), sizeof(double) * dc, 0), "hipHostMalloc");
-        // pinned staging for request and result vectors (up to 256 MiB each)
+        check(hipMalloc(&S->d_kcols, sizeof(int) * dc), "hipMalloc");
+        check(hipMalloc(&S->d_kl, sizeof(double) * dc), "hipMalloc");
+        check(hipMalloc(&S->d_ku, sizeof(double) * dc), "hipMalloc");
+        check(hipMalloc(&S->d_off, sizeof(int) * (slots + 1)), "hipMalloc");
+        check(hipHostMalloc(reinterpret_cast<void **>(&S->h_off), sizeof(int) * (slots + 1), 0), "hipHostMalloc");
+        // pinned staging for request and result vectors (up to 1 GiB each)
         const int entries = static_cast<int>(
-            std::max<int64_t>(1, std::min<int64_t>(slots, (256LL << 20) / static_cast<int64_t>(16 * S->n))));
+            std::max<int64_t>(1, std::min<int64_t>(slots, (1LL << 30) / static_cast<int64_t>(16 * S->n))));
         check(hipHostMalloc(reinterpret_cast<void **>(&S->inbox), sizeof(double) * 2 * S->n * entries, 0),
               "hipHostMalloc inbox");
         check(hipHostMalloc(reinterpret_cast<void **>(&S->outbox), sizeof(double) * 2 * S->n * entries, 0),
@@ -1316,9 +1353,11 @@ Server *server_create(Problem *P, int64_t hbm_budget, int max_slots) {
             S->inbox_free.push_back(e);
             S->outbox_free.push_back(e);
         }
-        // device staging for prefetched request vectors (up to 512 MiB)
+        // device staging for prefetched request vectors: one per slot, up to
+        // 4 GiB (a small share of the device), so that requests from pinned
+        // memory cross PCIe on the copy stream while steps run
         const int n_stage = static_cast<int>(
-            std::max<int64_t>(1, std::min<int64_t>(std::min(slots, 256), (512LL << 20) / static_cast<int64_t>(16 * S->n))));
+            std::max<int64_t>(1, std::min<int64_t>(slots, (4LL << 30) / static_cast<int64_t>(16 * S->n))));
         check(hipStreamCreateWithFlags(&S->cs, hipStreamNonBlocking), "hipStreamCreate");
         check(hipMalloc(&S->d_stage, sizeof(double) * 2 * S->n * n_stage), "hipMalloc staging");
         for (int e = n_stage - 1; e >= 0; e--) {
@@ -1344,6 +1383,8 @@ void server_destroy(Server *s) { delete s; }
 int server_slots(const Server *s) { return s->slots; }
 
 void server_set_base(Server *S, const double *lbs, const double *ubs) {
+    // between steps: a step reads the device base while it admits requests
+    std::lock_guard<std::mutex> lock(S->P->run_mutex);
     check(hipMemcpy(S->d_base, lbs, sizeof(double) * S->n, hipMemcpyHostToDevice), "H2D base");
     check(hipMemcpy(S->d_base + S->n, ubs, sizeof(double) * S->n, hipMemcpyHostToDevice), "H2D base");
     S->base_lb.assign(lbs, lbs + S->n);
@@ -1352,16 +1393,31 @@ void server_set_base(Server *S, const double *lbs, const double *ubs) {
 }
 
 namespace {
-// Starts copying a request's vectors (pinned host memory) into a free device
-// staging entry on the copy stream.
-void prefetch(Server *S, Server::Request &q, const double *lb, const double *ub) {
-    if (S->stage_free.empty()) return;
+// A free device staging entry (-1 if none); with the server's lock held.
+int take_stage(Server *S) {
+    if (S->stage_free.empty()) return -1;
     const int e = S->stage_free.back();
     S->stage_free.pop_back();
+    return e;
+}
+
+// Starts copying a request's vectors (pinned host memory) into device
+// staging entry e on the copy stream.
+void prefetch(Server *S, int e, const double *lb, const double *ub) {
     check(hipMemcpyAsync(S->stage_lb(e), lb, S->n * sizeof(double), hipMemcpyHostToDevice, S->cs), "H2D");
     check(hipMemcpyAsync(S->stage_lb(e) + S->n, ub, S->n * sizeof(double), hipMemcpyHostToDevice, S->cs), "H2D");
     check(hipEventRecord(S->stage_ev[e], S->cs), "event");
-    q.stage = e;
+}
+
+// Queues a prepared request and wakes the stepping thread.
+int64_t enqueue(Server *S, Server::Request &&q) {
+    const int64_t t = q.ticket;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->queue.push_back(std::move(q));
+    }
+    S->cv.notify_one();
+    return t;
 }
 }  // namespace
 
@@ -1369,27 +1425,38 @@ int64_t server_submit(Server *S, const double *lbs, const double *ubs, const int
                       int max_rounds) {
     if (max_rounds < 1 || max_rounds > kMaxRounds) throw std::runtime_error("max_rounds out of range");
     Server::Request q;
-    q.ticket = S->next_ticket++;
-    if (!S->inbox_free.empty()) {
-        q.inbox = S->inbox_free.back();
-        S->inbox_free.pop_back();
-        std::memcpy(S->in_lb(q.inbox), lbs, sizeof(double) * S->n);
-        std::memcpy(S->in_ub(q.inbox), ubs, sizeof(double) * S->n);
-        prefetch(S, q, S->in_lb(q.inbox), S->in_ub(q.inbox));
-    } else {
-        q.lb = S->take_vec();
-        q.ub = S->take_vec();
-        std::memcpy(q.lb.data(), lbs, sizeof(double) * S->n);
-        std::memcpy(q.ub.data(), ubs, sizeof(double) * S->n);
-    }
     if (hints && n_hints > 0) {
         for (int k = 0; k < n_hints; k++)
             if (hints[k] < 0 || hints[k] >= static_cast<int>(S->n)) throw std::runtime_error("hint column out of range");
         q.cols.assign(hints, hints + n_hints);
     }
     q.max_rounds = max_rounds;
-    S->queue.push_back(std::move(q));
-    return S->queue.back().ticket;
+    int stage = -1;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        q.ticket = S->next_ticket++;
+        if (!S->inbox_free.empty()) {
+            q.inbox = S->inbox_free.back();
+            S->inbox_free.pop_back();
+            stage = take_stage(S);
+        } else {
+            q.lb = S->take_vec();
+            q.ub = S->take_vec();
+        }
+    }
+    // the entries taken above belong to this request: filled without the lock
+    if (q.inbox >= 0) {
+        std::memcpy(S->in_lb(q.inbox), lbs, sizeof(double) * S->n);
+        std::memcpy(S->in_ub(q.inbox), ubs, sizeof(double) * S->n);
+        if (stage >= 0) {
+            prefetch(S, stage, S->in_lb(q.inbox), S->in_ub(q.inbox));
+            q.stage = stage;
+        }
+    } else {
+        std::memcpy(q.lb.data(), lbs, sizeof(double) * S->n);
+        std::memcpy(q.ub.data(), ubs, sizeof(double) * S->n);
+    }
+    return enqueue(S, std::move(q));
 }
 
 int64_t server_submit_pinned(Server *S, const double *lbs, const double *ubs, double *out_lbs, double *out_ubs,
@@ -1397,20 +1464,27 @@ int64_t server_submit_pinned(Server *S, const double *lbs, const double *ubs, do
     if (max_rounds < 1 || max_rounds > kMaxRounds) throw std::runtime_error("max_rounds out of range");
     if (!lbs || !ubs || !out_lbs || !out_ubs) throw std::runtime_error("null buffer");
     Server::Request q;
-    q.ticket = S->next_ticket++;
     q.pin_lb = lbs;
     q.pin_ub = ubs;
     q.out_lb = out_lbs;
     q.out_ub = out_ubs;
-    prefetch(S, q, lbs, ubs);
     if (hints && n_hints > 0) {
         for (int k = 0; k < n_hints; k++)
             if (hints[k] < 0 || hints[k] >= static_cast<int>(S->n)) throw std::runtime_error("hint column out of range");
         q.cols.assign(hints, hints + n_hints);
     }
     q.max_rounds = max_rounds;
-    S->queue.push_back(std::move(q));
-    return S->queue.back().ticket;
+    int stage;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        q.ticket = S->next_ticket++;
+        stage = take_stage(S);
+    }
+    if (stage >= 0) {
+        prefetch(S, stage, lbs, ubs);
+        q.stage = stage;
+    }
+    return enqueue(S, std::move(q));
 }
 
 int64_t server_submit_delta(Server *S, int n_changes, const int *cols, const double *lbs, const double *ubs,
@@ -1419,7 +1493,6 @@ int64_t server_submit_delta(Server *S, int n_changes, const int *cols, const dou
     if (max_rounds < 1 || max_rounds > kMaxRounds) throw std::runtime_error("max_rounds out of range");
     if (n_changes < 0 || (n_changes > 0 && (!cols || !lbs || !ubs))) throw std::runtime_error("bad changes");
     Server::Request q;
-    q.ticket = S->next_ticket++;
     q.delta = true;
     for (int k = 0; k < n_changes; k++)
         if (cols[k] < 0 || cols[k] >= static_cast<int>(S->n)) throw std::runtime_error("changed column out of range");
@@ -1427,17 +1500,34 @@ int64_t server_submit_delta(Server *S, int n_changes, const int *cols, const dou
     q.dlb.assign(lbs, lbs + n_changes);
     q.dub.assign(ubs, ubs + n_changes);
     q.max_rounds = max_rounds;
-    S->queue.push_back(std::move(q));
-    return S->queue.back().ticket;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        q.ticket = S->next_ticket++;
+    }
+    return enqueue(S, std::move(q));
 }
 
-int server_pending(const Server *S) {
-    int busy = 0;
-    for (auto t : S->slot_ticket) busy += t >= 0;
-    return busy + static_cast<int>(S->queue.size());
+int server_pending(Server *S) {
+    std::lock_guard<std::mutex> lk(S->mu);
+    return S->busy + static_cast<int>(S->queue.size());
 }
 
 const std::vector<int64_t> &server_completed(const Server *S) { return S->last_completed; }
+
+int server_take_completed(Server *S, int64_t *tickets, int cap) {
+    std::lock_guard<std::mutex> lk(S->mu);
+    int k = 0;
+    while (k < cap && !S->completed_acc.empty()) {
+        tickets[k++] = S->completed_acc.front();
+        S->completed_acc.pop_front();
+    }
+    return k;
+}
+
+bool server_running(Server *S) {
+    std::lock_guard<std::mutex> lk(S->mu);
+    return S->running;
+}
 
 namespace {
 
@@ -1450,7 +1540,65 @@ void grow(T *&p, size_t count) {
 
 }  // namespace
 
+namespace {
+int step_impl(Server *S);
+}  // namespace
+
 int server_step(Server *S) {
+    if (server_running(S)) throw std::runtime_error("the server steps on its own thread");
+    return step_impl(S);
+}
+
+namespace {
+// The stepping thread: a step whenever requests are queued or running.
+void worker_loop(Server *S) {
+    try {
+        check(hipSetDevice(S->P->device), "hipSetDevice");
+        for (;;) {
+            {
+                std::unique_lock<std::mutex> lk(S->mu);
+                S->cv.wait(lk, [S] { return S->stop || !S->queue.empty() || S->busy > 0; });
+                if (S->stop) return;
+            }
+            step_impl(S);
+        }
+    } catch (const std::exception &e) {
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->worker_error = e.what();
+        S->stop = true;
+    }
+}
+}  // namespace
+
+void server_start(Server *S) {
+    std::lock_guard<std::mutex> lk(S->mu);
+    if (S->running) throw std::runtime_error("the server already runs its stepping thread");
+    S->stop = false;
+    S->worker_error.clear();
+    S->completed_acc.clear();
+    S->running = true;
+    S->worker = std::thread(worker_loop, S);
+}
+
+void server_stop(Server *S) {
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        if (!S->running) return;
+        S->stop = true;
+    }
+    S->cv.notify_all();
+    S->worker.join();
+    std::string err;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->running = false;
+        err = S->worker_error;
+    }
+    if (!err.empty()) throw std::runtime_error("stepping thread: " + err);
+}
+
+namespace {
+int step_impl(Server *S) {
     Problem *P = S->P;
     std::lock_guard<std::mutex> lock(P->run_mutex);
     S->last_completed.clear();
@@ -1463,9 +1611,20 @@ int server_step(Server *S) {
     std::vector<int> dptr(1, 0), dcols;
     std::vector<double> dlv, duv;
     std::vector<std::vector<double>> keep;  // pageable request vectors in flight
-    for (int k = 0; k < S->slots && !S->queue.empty(); k++) {
-        if (S->slot_ticket[k] >= 0) continue;
-        Server::Request &q = S->queue.front();
+    std::vector<std::pair<int, Server::Request>> taken;  // (slot, request) admitted in this step
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        for (int k = 0; k < S->slots && !S->queue.empty(); k++) {
+            if (S->slot_ticket[k] >= 0) continue;
+            S->slot_ticket[k] = S->queue.front().ticket;
+            S->busy++;
+            taken.emplace_back(k, std::move(S->queue.front()));
+            S->queue.pop_front();
+        }
+    }
+    for (auto &kq : taken) {
+        const int k = kq.first;
+        Server::Request &q = kq.second;
         if (q.delta) {
             check(hipMemcpyAsync(S->slot_lb[k], S->d_base, n * sizeof(double), hipMemcpyDeviceToDevice, s), "D2D");
             check(hipMemcpyAsync(S->slot_ub[k], S->d_base + n, n * sizeof(double), hipMemcpyDeviceToDevice, s),
@@ -1513,12 +1672,10 @@ int server_step(Server *S) {
         hints.insert(hints.end(), q.cols.begin(), q.cols.end());
         hint_ptr.push_back(static_cast<int>(hints.size()));
         caps.push_back(q.max_rounds);
-        S->slot_ticket[k] = q.ticket;
         S->slot_delta[k] = q.delta ? 1 : 0;
         S->slot_out_lb[k] = q.out_lb;
         S->slot_out_ub[k] = q.out_ub;
         S->slot_inplace[k] = q.pin_lb && q.out_lb == q.pin_lb && q.out_ub == q.pin_ub ? 1 : 0;
-        S->queue.pop_front();
     }
     if (!del_lb.empty()) {
         const size_t nd = del_lb.size(), nc = dcols.size();
@@ -1554,9 +1711,12 @@ int server_step(Server *S) {
     launch_step(P, S->step, 0, S->step);
     check(hipGetLastError(), "step launch");
     std::vector<UnitState> st = read_units(P, static_cast<int>(active.size()), active.data());  // synchronizes
-    for (int e : used_inbox) S->inbox_free.push_back(e);
-    for (int e : used_stage) S->stage_free.push_back(e);
-    for (auto &v : keep) S->give_vec(std::move(v));
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        for (int e : used_inbox) S->inbox_free.push_back(e);
+        for (int e : used_stage) S->stage_free.push_back(e);
+        for (auto &v : keep) S->give_vec(std::move(v));
+    }
     keep.clear();
     int err = 0;
     check(hipMemcpy(&err, P->D.global + kGlobalError, sizeof(int), hipMemcpyDeviceToHost), "D2H");
@@ -1612,18 +1772,27 @@ int server_step(Server *S) {
                                static_cast<int>(n), nd, P->d_lbp, P->d_ubp, S->d_blp, S->d_bup,
                                Server::kDiffCap, S->d_ccnt, S->d_ccols, S->d_cl, S->d_cu);
             check(hipGetLastError(), "diff launch");
+            // packed on the device, so that only the changes cross PCIe
+            hipLaunchKernelGGL(compact_changes_kernel, dim3(std::max(1, std::min(nd, 1024))), dim3(256), 0, s, nd,
+                               Server::kDiffCap, S->d_ccnt, S->d_ccols, S->d_cl, S->d_cu, S->d_off, S->d_kcols,
+                               S->d_kl, S->d_ku);
+            check(hipGetLastError(), "compact launch");
             check(hipMemcpyAsync(S->h_ccnt, S->d_ccnt, sizeof(int) * nd, hipMemcpyDeviceToHost, s), "D2H");
-            const size_t dc = static_cast<size_t>(nd) * Server::kDiffCap;
-            check(hipMemcpyAsync(S->h_ccols, S->d_ccols, sizeof(int) * dc, hipMemcpyDeviceToHost, s), "D2H");
-            check(hipMemcpyAsync(S->h_cl, S->d_cl, sizeof(double) * dc, hipMemcpyDeviceToHost, s), "D2H");
-            check(hipMemcpyAsync(S->h_cu, S->d_cu, sizeof(double) * dc, hipMemcpyDeviceToHost, s), "D2H");
+            check(hipMemcpyAsync(S->h_off, S->d_off, sizeof(int) * (nd + 1), hipMemcpyDeviceToHost, s), "D2H");
             check(hipStreamSynchronize(s), "diff sync");
+            const size_t total = static_cast<size_t>(S->h_off[nd]);
+            if (total > 0) {
+                check(hipMemcpyAsync(S->h_ccols, S->d_kcols, sizeof(int) * total, hipMemcpyDeviceToHost, s), "D2H");
+                check(hipMemcpyAsync(S->h_cl, S->d_kl, sizeof(double) * total, hipMemcpyDeviceToHost, s), "D2H");
+                check(hipMemcpyAsync(S->h_cu, S->d_ku, sizeof(double) * total, hipMemcpyDeviceToHost, s), "D2H");
+                check(hipStreamSynchronize(s), "diff sync");
+            }
             for (int j = 0; j < nd; j++) {
                 const int c = S->h_ccnt[j];
                 if (c > Server::kDiffCap) continue;  // too many: the full vector below
                 const int k = fin[dq[j]];
                 Server::Done &d = out[dq[j]];
-                const size_t o = static_cast<size_t>(j) * Server::kDiffCap;
+                const size_t o = static_cast<size_t>(S->h_off[j]);
                 if (S->slot_inplace[k]) {
                     // the caller's buffers hold the input: write the changes
                     for (int q = 0; q < c; q++) {
@@ -1649,26 +1818,32 @@ int server_step(Server *S) {
                 d.in_place = true;
                 hl = S->slot_out_lb[k];
                 hu = S->slot_out_ub[k];
-            } else if (!S->outbox_free.empty()) {
-                d.outbox = S->outbox_free.back();
-                S->outbox_free.pop_back();
-                hl = S->out_lb(d.outbox);
-                hu = S->out_ub(d.outbox);
             } else {
-                d.lb = S->take_vec();
-                d.ub = S->take_vec();
-                hl = d.lb.data();
-                hu = d.ub.data();
+                std::lock_guard<std::mutex> lk(S->mu);
+                if (!S->outbox_free.empty()) {
+                    d.outbox = S->outbox_free.back();
+                    S->outbox_free.pop_back();
+                    hl = S->out_lb(d.outbox);
+                    hu = S->out_ub(d.outbox);
+                } else {
+                    d.lb = S->take_vec();
+                    d.ub = S->take_vec();
+                    hl = d.lb.data();
+                    hu = d.ub.data();
+                }
             }
             check(hipMemcpyAsync(hl, S->slot_lb[k], n * sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
             check(hipMemcpyAsync(hu, S->slot_ub[k], n * sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
         }
         check(hipStreamSynchronize(s), "retire sync");
+        std::lock_guard<std::mutex> lk(S->mu);
         for (size_t q = 0; q < fin.size(); q++) {
             const int k = fin[q];
             S->done[S->slot_ticket[k]] = std::move(out[q]);
             S->last_completed.push_back(S->slot_ticket[k]);
+            if (S->running) S->completed_acc.push_back(S->slot_ticket[k]);
             S->slot_ticket[k] = -1;
+            S->busy--;
             S->slot_out_lb[k] = nullptr;
             S->slot_out_ub[k] = nullptr;
             S->slot_inplace[k] = 0;
@@ -1677,14 +1852,33 @@ int server_step(Server *S) {
     S->step++;
     return static_cast<int>(fin.size());
 }
+}  // namespace
 
-bool server_result(Server *S, int64_t ticket, double *lbs, double *ubs, RunResult *r) {
+namespace {
+// Takes a finished request out of the server (false if not finished).
+bool take_done(Server *S, int64_t ticket, Server::Done &d) {
+    std::lock_guard<std::mutex> lk(S->mu);
     auto it = S->done.find(ticket);
     if (it == S->done.end()) return false;
-    Server::Done &d = it->second;
+    d = std::move(it->second);
+    S->done.erase(it);
+    return true;
+}
+
+// Returns a finished request's host buffers to the server.
+void release_done(Server *S, Server::Done &d) {
+    std::lock_guard<std::mutex> lk(S->mu);
+    if (d.outbox >= 0) S->outbox_free.push_back(d.outbox);
+    S->give_vec(std::move(d.lb));
+    S->give_vec(std::move(d.ub));
+}
+}  // namespace
+
+bool server_result(Server *S, int64_t ticket, double *lbs, double *ubs, RunResult *r) {
+    Server::Done d;
+    if (!take_done(S, ticket, d)) return false;
     if (d.in_place || (!lbs && !ubs)) {
         // the bounds are in the caller's pinned buffers (or not wanted)
-        if (d.outbox >= 0) S->outbox_free.push_back(d.outbox);
     } else if (d.changes) {
         std::memcpy(lbs, S->base_lb.data(), sizeof(double) * S->n);
         std::memcpy(ubs, S->base_ub.data(), sizeof(double) * S->n);
@@ -1695,24 +1889,20 @@ bool server_result(Server *S, int64_t ticket, double *lbs, double *ubs, RunResul
     } else if (d.outbox >= 0) {
         std::memcpy(lbs, S->out_lb(d.outbox), sizeof(double) * S->n);
         std::memcpy(ubs, S->out_ub(d.outbox), sizeof(double) * S->n);
-        S->outbox_free.push_back(d.outbox);
     } else {
         std::memcpy(lbs, d.lb.data(), sizeof(double) * S->n);
         std::memcpy(ubs, d.ub.data(), sizeof(double) * S->n);
     }
-    S->give_vec(std::move(d.lb));
-    S->give_vec(std::move(d.ub));
     *r = d.r;
-    S->done.erase(it);
+    release_done(S, d);
     return true;
 }
 
 int server_result_changes(Server *S, int64_t ticket, std::vector<int> &cols, std::vector<double> &lbs,
                           std::vector<double> &ubs, RunResult *r) {
-    auto it = S->done.find(ticket);
-    if (it == S->done.end()) return 0;
-    Server::Done &d = it->second;
     if (!S->has_base) throw std::runtime_error("no base vector set");
+    Server::Done d;
+    if (!take_done(S, ticket, d)) return 0;
     cols.clear();
     lbs.clear();
     ubs.clear();
@@ -1729,12 +1919,9 @@ int server_result_changes(Server *S, int64_t ticket, std::vector<int> &cols, std
                 lbs.push_back(fl[j]);
                 ubs.push_back(fu[j]);
             }
-        if (d.outbox >= 0) S->outbox_free.push_back(d.outbox);
-        S->give_vec(std::move(d.lb));
-        S->give_vec(std::move(d.ub));
     }
     *r = d.r;
-    S->done.erase(it);
+    release_done(S, d);
     return 1;
 }
 

@@ -716,11 +716,38 @@ GDP_RETCODE gdpServerPending(gdp_server s, int *pending) {
 
 GDP_RETCODE gdpServerCompleted(gdp_server s, long long *tickets, const int cap, int *count) {
     if (!s || !count || (cap > 0 && !tickets)) return GDP_ERROR;
+    if (gdp::server_running(s->s)) {
+        std::vector<int64_t> t(std::max(0, cap));
+        const int n = gdp::server_take_completed(s->s, t.data(), cap);
+        for (int k = 0; k < n; k++) tickets[k] = t[k];
+        *count = n;
+        return GDP_OKAY;
+    }
     const std::vector<int64_t> &c = gdp::server_completed(s->s);
     const int n = static_cast<int>(c.size());
     for (int k = 0; k < n && k < cap; k++) tickets[k] = c[k];
     *count = n;
     return GDP_OKAY;
+}
+
+GDP_RETCODE gdpServerStart(gdp_server s) {
+    if (!s) return GDP_ERROR;
+    try {
+        gdp::server_start(s->s);
+        return GDP_OKAY;
+    } catch (const std::exception &e) {
+        return fail("gdpServerStart", e);
+    }
+}
+
+GDP_RETCODE gdpServerStop(gdp_server s) {
+    if (!s) return GDP_ERROR;
+    try {
+        gdp::server_stop(s->s);
+        return GDP_OKAY;
+    } catch (const std::exception &e) {
+        return fail("gdpServerStop", e);
+    }
 }
 
 GDP_RETCODE gdpServerResultChanges(gdp_server s, const long long ticket, int *cols, double *lbs, double *ubs,

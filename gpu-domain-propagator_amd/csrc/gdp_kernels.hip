@@ -1268,6 +1268,42 @@ __global__ __launch_bounds__(64) void measure_total_kernel(int parts, const doub
 }
 
 
+// Packs the changed columns that diff_kernel found for nd vectors (vector
+// k's cnt[k] entries at [k * cap, ...), none when cnt[k] > cap) into
+// consecutive positions: vector k's start into off[k], the total into
+// off[nd].  One workgroup per vector; each sums the counts before its own.
+__global__ __launch_bounds__(256) void compact_changes_kernel(int nd, int cap, const int *cnt, const int *cols,
+                                                              const double *vl, const double *vu, int *off,
+                                                              int *kcols, double *kl, double *ku) {
+    __shared__ int part[256];
+    for (int k = blockIdx.x; k < nd; k += gridDim.x) {
+        int sum = 0;
+        for (int j = threadIdx.x; j < k; j += blockDim.x) {
+            const int c = cnt[j];
+            sum += c <= cap ? c : 0;
+        }
+        part[threadIdx.x] = sum;
+        __syncthreads();
+        for (int h = 128; h > 0; h >>= 1) {
+            if (static_cast<int>(threadIdx.x) < h) part[threadIdx.x] += part[threadIdx.x + h];
+            __syncthreads();
+        }
+        const int o = part[0];
+        const int c = cnt[k] <= cap ? cnt[k] : 0;
+        if (threadIdx.x == 0) {
+            off[k] = o;
+            if (k == nd - 1) off[nd] = o + c;
+        }
+        const size_t src = static_cast<size_t>(k) * cap;
+        for (int q = threadIdx.x; q < c; q += blockDim.x) {
+            kcols[o + q] = cols[src + q];
+            kl[o + q] = vl[src + q];
+            ku[o + q] = vu[src + q];
+        }
+        __syncthreads();
+    }
+}
+
 // ---------------------------------------------------------------------------
 // problem layout on the device (single instance): the passes over the
 // nonzeros.  The host orders the rows (tree-width bins), pads them and

@@ -76,6 +76,8 @@ EXT_SYMBOLS = (
     "gdpServerCompleted",
     "gdpServerResultChanges",
     "gdpServerResult",
+    "gdpServerStart",
+    "gdpServerStop",
     "gdpSequentialPropagate",
     "gdpOMPPropagate",
     "gdpPropagateWithMeasure",
@@ -219,6 +221,10 @@ def load_library(path: Optional[str] = None) -> ctypes.CDLL:
     lib.gdpServerCompleted.restype = c_int
     lib.gdpServerResult.argtypes = [vp, c_longlong, vp, vp, POINTER(RunInfo), POINTER(c_int)]
     lib.gdpServerResult.restype = c_int
+    lib.gdpServerStart.argtypes = [vp]
+    lib.gdpServerStart.restype = c_int
+    lib.gdpServerStop.argtypes = [vp]
+    lib.gdpServerStop.restype = c_int
     lib.gdpSequentialPropagate.argtypes = [c_int, c_int, c_int, vp, vp, vp, vp, vp, vp, vp, vp, c_int,
                                            POINTER(RunInfo)]
     lib.gdpSequentialPropagate.restype = c_int
@@ -730,6 +736,18 @@ class Server:
         if self.lib.gdpServerStep(self.handle, ctypes.byref(c)) != GDP_OKAY:
             raise RuntimeError(f"gdpServerStep failed: {last_error()}")
         return c.value
+
+    def start(self):
+        """Run the server's own stepping thread (gdpServerStart): requests
+        are stepped without calls to step(); completed() then returns the
+        tickets finished since its last call."""
+        if self.lib.gdpServerStart(self.handle) != GDP_OKAY:
+            raise RuntimeError(f"gdpServerStart failed: {last_error()}")
+
+    def stop(self):
+        """End the stepping thread after its current step (gdpServerStop)."""
+        if self.lib.gdpServerStop(self.handle) != GDP_OKAY:
+            raise RuntimeError(f"gdpServerStop failed: {last_error()}")
 
     def pending(self) -> int:
         c = c_int()

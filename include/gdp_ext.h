@@ -160,7 +160,8 @@ GDP_RETCODE gdpBatchPropagateDevice(gdp_problem p, double *d_lbs, double *d_ubs,
 /* ---- serving node requests with continuous batching ----
  * A server admits submitted requests into free slots of the problem's
  * device state (as many slots as hbm_budget bytes allow, at most max_slots;
- * a slot takes the unit state of gdpProblemGetStats plus 32 bytes a column),
+ * a slot takes the unit state of gdpProblemGetStats plus 48 bytes a column:
+ * its bound vectors, the input of an in-place request and a staging entry),
  * and each gdpServerStep advances every running request by one round in
  * shared launches: requests that arrive while others run join at the next
  * step.  A request's result is bit-identical to propagating it alone. */
@@ -191,12 +192,23 @@ GDP_RETCODE gdpServerSetBase(gdp_server s, const double *lbs, const double *ubs)
  * the rows of the changed columns. */
 GDP_RETCODE gdpServerSubmitDelta(gdp_server s, const int n_changes, const int *cols, const double *lbs,
                                  const double *ubs, const int max_rounds, long long *ticket);
-/* One step: admit, one round of every running request, retire. */
+/* One step: admit, one round of every running request, retire.  An error
+ * while the server's stepping thread runs. */
 GDP_RETCODE gdpServerStep(gdp_server s, int *completed);
 GDP_RETCODE gdpServerPending(gdp_server s, int *pending);
 /* Tickets of the requests completed by the last gdpServerStep: up to cap of
- * them into tickets, their number into *count. */
+ * them into tickets, their number into *count.  While the stepping thread
+ * runs: up to cap of the tickets it completed since the last call, taken
+ * (the rest stay for the next call), their number into *count. */
 GDP_RETCODE gdpServerCompleted(gdp_server s, long long *tickets, const int cap, int *count);
+/* The server's own stepping thread: while it runs, every queued or running
+ * request is stepped without the caller, whose threads only submit and take
+ * results (the submit, completed and result calls may come from any
+ * thread).  Stop ends it after its current step and returns GDP_ERROR if the
+ * thread met an error (message in gdpLastError).  No counterpart in the
+ * reference, which has no server. */
+GDP_RETCODE gdpServerStart(gdp_server s);
+GDP_RETCODE gdpServerStop(gdp_server s);
 /* *ready = 1 and the result copied out (and forgotten) if the request has
  * completed; *ready = 0 otherwise.  lbs and ubs may both be NULL: only the
  * outcome is returned (zero-copy requests, or results not wanted).

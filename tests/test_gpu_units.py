@@ -252,7 +252,7 @@ def test_server_budget_limits_slots(gpu):
     inst = small_mip(seed=1)
     p = I.Problem(inst)
     try:
-        per = p.stats().unit_state_bytes + 32 * inst.n  # state, bound vectors, inputs of in-place requests
+        per = p.stats().unit_state_bytes + 48 * inst.n  # state, bound vectors, in-place inputs, staging
         srv = I.Server(p, hbm_budget=3 * per + 1, max_slots=100)
         assert srv.slots == 3
         srv.close()
@@ -383,3 +383,84 @@ def test_autotune_returns_a_valid_tuning(gpu, oracle):
             assert_same(got, oracle.jacobi(x), x.name)
     finally:
         b.close()
+
+
+def test_server_stepping_thread(gpu, oracle):
+    """The server's own stepping thread: requests of every form submitted
+    from two caller threads while it steps, completions taken as they come;
+    every result bit-identical to the oracle and every ticket completed
+    once.  Stepping by hand is refused while the thread runs; the thread
+    stops and restarts."""
+    import threading
+    import time
+
+    inst = powerlaw_mip(seed=6, m=20000, n=10000, len_hi=5000, s=1.8)
+    base = oracle.jacobi(inst)
+    nodes = node_vectors(inst, oracle, 40, seed=12)
+    p = I.Problem(inst)
+    pins = []
+    try:
+        srv = I.Server(p, hbm_budget=256 << 20, max_slots=5)
+        srv.set_base(base[0], base[1])
+        for k in range(40):
+            if k % 4 == 2:
+                a, b = I.PinnedArray(inst.n), I.PinnedArray(inst.n)
+                a.array[:] = nodes[k][0]
+                b.array[:] = nodes[k][1]
+                pins.append((k, a, b))
+        pinned = {k: (a, b) for k, a, b in pins}
+        tickets, lock = {}, threading.Lock()
+
+        def submit(ks):
+            for k in ks:
+                lb, ub, cols = nodes[k]
+                if k % 4 == 0:
+                    t = srv.submit(lb, ub, hints=cols)
+                elif k % 4 == 1:
+                    t = srv.submit_delta(cols, lb[cols], ub[cols])
+                elif k % 4 == 2:
+                    a, b = pinned[k]
+                    t = srv.submit_pinned(a, b, a, b, hints=cols)  # in place
+                else:
+                    t = srv.submit(lb, ub)
+                with lock:
+                    tickets[t] = k
+                time.sleep(0.0005)
+
+        for rep in range(2):
+            for k, a, b in pins:
+                a.array[:] = nodes[k][0]
+                b.array[:] = nodes[k][1]
+            tickets.clear()
+            srv.start()
+            with pytest.raises(RuntimeError, match="own thread"):
+                srv.step()
+            th = [threading.Thread(target=submit, args=(list(range(h, 40, 2)),)) for h in (0, 1)]
+            for t_ in th:
+                t_.start()
+            seen = []
+            t0 = time.time()
+            while len(seen) < 40 and time.time() - t0 < 60:
+                seen += srv.completed()
+                time.sleep(0.001)
+            for t_ in th:
+                t_.join()
+            srv.stop()
+            assert sorted(seen) == sorted(tickets), "every ticket completed exactly once"
+            assert srv.pending() == 0
+            for t, k in tickets.items():
+                exp = oracle.jacobi(inst, lb=nodes[k][0], ub=nodes[k][1])
+                if k % 4 == 2:
+                    got = srv.result(t, copy=False)
+                    a, b = pinned[k]
+                    assert got is not None and got.rounds == exp[2], f"rep {rep} request {k}"
+                    np.testing.assert_array_equal(a.array, exp[0], err_msg=f"rep {rep} request {k}")
+                    np.testing.assert_array_equal(b.array, exp[1], err_msg=f"rep {rep} request {k}")
+                else:
+                    assert_same(srv.result(t), exp, f"rep {rep} request {k}")
+        srv.close()
+    finally:
+        p.close()
+        for _, a, b in pins:
+            a.free()
+            b.free()
