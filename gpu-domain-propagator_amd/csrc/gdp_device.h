@@ -64,6 +64,8 @@ constexpr int kItemLong = 1;   // one row, one wave, all segments (rows not spli
 constexpr int kItemSeg = 2;    // up to kSegsPerItem consecutive segments of a split row
 constexpr int kSegsPerItem = 4;
 constexpr int kItemUnitShift = 12;  // items[].x >> 12: unit of the item (blocks mode)
+// Most units of one persistent launch whose activity its workgroups keep in LDS.
+constexpr int kMaxFilterUnits = 4096;
 
 // Rows start at multiples of kRowAlign entries (padding entries hold column
 // 0 and coefficient 0 and are never used); the low bits of row_ptr[r] hold

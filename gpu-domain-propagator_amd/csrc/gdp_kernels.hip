@@ -449,6 +449,15 @@ struct UnitStep {
     int start;  // step of the unit's round 1
 };
 
+// The rule for a unit working in `step`, from its state words: loaded, not
+// stopped by its load, under its round cap, and in its first round or moved a
+// bound in the previous step without becoming infeasible.
+__device__ __forceinline__ bool unit_works(int step, int start, int infeasible, int max_rounds, int crossed,
+                                           int envelope, int prev_changed) {
+    return start > 0 && step >= start && !crossed && !envelope && step - start < max_rounds &&
+           (step == start || (prev_changed == step - 1 && infeasible == 0));
+}
+
 __device__ __forceinline__ UnitStep unit_step(const DevMatrix &M, const DevState &D, int u, int step) {
     const long long *p = reinterpret_cast<const long long *>(&D.units[u]);
     const long long w0 = coherent_load(p), w1 = coherent_load(p + 1), w2 = coherent_load(p + 2),
@@ -458,11 +467,20 @@ __device__ __forceinline__ UnitStep unit_step(const DevMatrix &M, const DevState
     const int crossed = uniform(static_cast<int>(w2)), envelope = uniform(static_cast<int>(w2 >> 32));
     const int prev_changed = uniform(static_cast<int>((step - 1) & 1 ? (w3 >> 32) : w3));
     UnitStep r;
-    r.active = start > 0 && step >= start && !crossed && !envelope && step - start < max_rounds &&
-               (step == start || (prev_changed == step - 1 && infeasible == 0));
+    r.active = unit_works(step, start, infeasible, max_rounds, crossed, envelope, prev_changed);
     r.all_rows = !M.mark_rows || markall == step;
     r.start = start;
     return r;
+}
+
+// unit_step's `active` for a different unit on every lane.
+__device__ __forceinline__ bool unit_works_lane(const DevState &D, int u, int step) {
+    const long long *p = reinterpret_cast<const long long *>(&D.units[u]);
+    const long long w0 = coherent_load(p), w1 = coherent_load(p + 1), w2 = coherent_load(p + 2),
+                    w3 = coherent_load(p + 3);
+    return unit_works(step, static_cast<int>(w0), static_cast<int>(w0 >> 32), static_cast<int>(w1 >> 32),
+                      static_cast<int>(w2), static_cast<int>(w2 >> 32),
+                      static_cast<int>((step - 1) & 1 ? (w3 >> 32) : w3));
 }
 
 // Counts the wave's accepted candidates (`moved` per lane) of unit u in
@@ -1019,10 +1037,48 @@ __global__ __launch_bounds__(kThreads) void mega_kernel(DevMatrix M, DevState D,
     const int wave = uniform(blockIdx.x * waves + (threadIdx.x >> 6));
     const int nwaves = gridDim.x * waves;
     const int n_work = M.replicas ? M.n_items * D.n_active : M.n_items;
+    // Several units: at the start of a step every workgroup notes in LDS
+    // which units work in it (their state words do not change during the
+    // sweep), so that a wave tests 64 of its items at once against the LDS
+    // bits and visits only those of working units.  Late steps of a batch
+    // have few units left, and most items would otherwise each pay the
+    // device-coherent loads of their unit's state to find it idle.
+    __shared__ unsigned works[kMaxFilterUnits / 32];
+    const int n_slots = M.replicas ? D.n_active : n_units;  // replicas: positions in the launch's list
+    const bool filter = n_slots > 1 && n_slots <= kMaxFilterUnits;
+    const int lane = threadIdx.x & 63;
     int steps = 0;
     for (int step = first_step; step < first_step + max_steps; step++) {
         int *st = status_row(D, step);
-        for (int it = wave; it < n_work; it += nwaves) sweep_item(M, D, it, step, st);
+        if (filter) {
+            // the previous step's waves of this workgroup are past its
+            // barriers, so no wave still reads the bits overwritten here
+            for (int g = (threadIdx.x >> 6) * 64; g < n_slots; g += blockDim.x) {
+                const int k = g + lane;
+                const bool on = k < n_slots && unit_works_lane(D, M.replicas ? D.active[k] : k, step);
+                const unsigned long long b = __ballot(on);
+                if (lane == 0) {
+                    works[g >> 5] = static_cast<unsigned>(b);
+                    works[(g >> 5) + 1] = static_cast<unsigned>(b >> 32);
+                }
+            }
+            __syncthreads();
+        }
+        // the wave's items in the order wave, wave + nwaves, ...: 64 at a time
+        for (int base = wave; base < n_work; base += 64 * nwaves) {
+            const int it = base + lane * nwaves;
+            bool on = it < n_work;
+            if (filter && on) {
+                const int k = M.replicas ? it / M.n_items : (M.items[it].x >> kItemUnitShift);
+                on = (works[k >> 5] >> (k & 31)) & 1u;
+            }
+            unsigned long long pending = __ballot(on);
+            while (pending) {
+                const int t = __ffsll(static_cast<long long>(pending)) - 1;
+                pending &= pending - 1;
+                sweep_item(M, D, base + t * nwaves, step, st);
+            }
+        }
         if (!grid_barrier(D, nblocks, false)) { steps = -1; break; }
         if (coherent_load(&st[kStatusApply]) > 0) {
             apply_phase(M, D, step, wave, nwaves);
