@@ -1267,6 +1267,10 @@ struct Server {
             cv.notify_all();
             worker.join();
         }
+        // copies on either stream may still read the pinned entries or
+        // write the device buffers freed below
+        if (cs) (void)hipStreamSynchronize(cs);
+        if (P) (void)hipStreamSynchronize(P->stream);
         if (d_buf) (void)hipFree(d_buf);
         if (d_base) (void)hipFree(d_base);
         if (inbox) (void)hipHostFree(inbox);
@@ -1286,7 +1290,6 @@ struct Server {
             if (q) (void)hipFree(q);
         for (void *q : {static_cast<void *>(d_orig), static_cast<void *>(d_blp), static_cast<void *>(d_bup)})
             if (q) (void)hipFree(q);
-        if (cs) (void)hipStreamSynchronize(cs);
         for (hipEvent_t e : stage_ev) (void)hipEventDestroy(e);
         if (d_stage) (void)hipFree(d_stage);
         if (cs) (void)hipStreamDestroy(cs);
@@ -1393,6 +1396,12 @@ void server_set_base(Server *S, const double *lbs, const double *ubs) {
 }
 
 namespace {
+// With the server's lock held: once the stepping thread has failed, no
+// request it would have to run is accepted.
+void check_worker(const Server *S) {
+    if (!S->worker_error.empty()) throw std::runtime_error("stepping thread: " + S->worker_error);
+}
+
 // A free device staging entry (-1 if none); with the server's lock held.
 int take_stage(Server *S) {
     if (S->stage_free.empty()) return -1;
@@ -1434,6 +1443,7 @@ int64_t server_submit(Server *S, const double *lbs, const double *ubs, const int
     int stage = -1;
     {
         std::lock_guard<std::mutex> lk(S->mu);
+        check_worker(S);
         q.ticket = S->next_ticket++;
         if (!S->inbox_free.empty()) {
             q.inbox = S->inbox_free.back();
@@ -1477,6 +1487,7 @@ int64_t server_submit_pinned(Server *S, const double *lbs, const double *ubs, do
     int stage;
     {
         std::lock_guard<std::mutex> lk(S->mu);
+        check_worker(S);
         q.ticket = S->next_ticket++;
         stage = take_stage(S);
     }
@@ -1502,6 +1513,7 @@ int64_t server_submit_delta(Server *S, int n_changes, const int *cols, const dou
     q.max_rounds = max_rounds;
     {
         std::lock_guard<std::mutex> lk(S->mu);
+        check_worker(S);
         q.ticket = S->next_ticket++;
     }
     return enqueue(S, std::move(q));
@@ -1521,6 +1533,8 @@ int server_take_completed(Server *S, int64_t *tickets, int cap) {
         tickets[k++] = S->completed_acc.front();
         S->completed_acc.pop_front();
     }
+    // the requests it finished are handed out first; then its failure
+    if (k == 0) check_worker(S);
     return k;
 }
 

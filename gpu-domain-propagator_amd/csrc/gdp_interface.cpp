@@ -717,11 +717,15 @@ GDP_RETCODE gdpServerPending(gdp_server s, int *pending) {
 GDP_RETCODE gdpServerCompleted(gdp_server s, long long *tickets, const int cap, int *count) {
     if (!s || !count || (cap > 0 && !tickets)) return GDP_ERROR;
     if (gdp::server_running(s->s)) {
-        std::vector<int64_t> t(std::max(0, cap));
-        const int n = gdp::server_take_completed(s->s, t.data(), cap);
-        for (int k = 0; k < n; k++) tickets[k] = t[k];
-        *count = n;
-        return GDP_OKAY;
+        try {
+            std::vector<int64_t> t(std::max(0, cap));
+            const int n = gdp::server_take_completed(s->s, t.data(), cap);
+            for (int k = 0; k < n; k++) tickets[k] = t[k];
+            *count = n;
+            return GDP_OKAY;
+        } catch (const std::exception &e) {
+            return fail("gdpServerCompleted", e);
+        }
     }
     const std::vector<int64_t> &c = gdp::server_completed(s->s);
     const int n = static_cast<int>(c.size());

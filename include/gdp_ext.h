@@ -205,7 +205,9 @@ GDP_RETCODE gdpServerCompleted(gdp_server s, long long *tickets, const int cap, 
  * request is stepped without the caller, whose threads only submit and take
  * results (the submit, completed and result calls may come from any
  * thread).  Stop ends it after its current step and returns GDP_ERROR if the
- * thread met an error (message in gdpLastError).  No counterpart in the
+ * thread met an error (message in gdpLastError).  Once the thread has failed,
+ * the submit calls return GDP_ERROR, and so does gdpServerCompleted after
+ * handing out the tickets the thread finished before.  No counterpart in the
  * reference, which has no server. */
 GDP_RETCODE gdpServerStart(gdp_server s);
 GDP_RETCODE gdpServerStop(gdp_server s);
