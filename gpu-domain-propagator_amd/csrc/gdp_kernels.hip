@@ -905,8 +905,16 @@ __device__ __forceinline__ void publish_phase(const DevMatrix &M, const DevState
             if (coherent_load(&status_row(D, step)[kStatusChanged]) == 0)
                 atomic_or_i(&status_row(D, step)[kStatusChanged], 1);
         }
-        // replicas: one decision for the chunk's unit
-        const int known = M.replicas ? (decide_all_rows(M, D, u, true, step) ? 1 : 0) : -1;
+        // one decision for the chunk's unit (replicas; blocks: when the
+        // chunk's words all belong to one unit, as in a single problem)
+        int known = -1;
+        if (M.replicas) {
+            known = decide_all_rows(M, D, u, true, step) ? 1 : 0;
+        } else {
+            const int u0 = uniform(M.word_unit[q * kChunk]);
+            const bool other = lane < kChunk && w < M.nwords && M.word_unit[w] != u0;
+            if (!__ballot(other)) known = decide_all_rows(M, D, u0, true, step) ? 1 : 0;
+        }
         while (pending) {
             const int t0 = __ffsll(static_cast<long long>(pending)) - 1;
             pending &= pending - 1;
