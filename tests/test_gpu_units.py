@@ -79,6 +79,24 @@ def test_batch_round_prefixes(gpu, oracle):
         b.close()
 
 
+def test_batch_beyond_the_lds_unit_filter(gpu, oracle):
+    """More instances than the persistent kernel notes in LDS per step
+    (4096): its sweep then visits every item and each item checks its own
+    instance's state.  Tiny instances that stop after different rounds."""
+    insts = [small_mip(seed=s, m=8 + s % 13, n=6 + s % 7, len_lo=2, len_hi=5, name=f"tiny:{s}")
+             for s in range(4100)]
+    b = I.Batch(insts)
+    try:
+        outs = b.propagate(sync_type=I.MEGAKERNEL)
+    finally:
+        b.close()
+    rounds = set()
+    for inst, got in zip(insts, outs):
+        assert_same(got, oracle.jacobi(inst), inst.name)
+        rounds.add(got.rounds)
+    assert len(rounds) > 2
+
+
 def node_vectors(inst, oracle, count, seed, from_fixpoint=True):
     rng = np.random.default_rng(seed)
     base = oracle.jacobi(inst) if from_fixpoint else (inst.lb, inst.ub)
