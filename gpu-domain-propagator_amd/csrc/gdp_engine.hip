@@ -26,6 +26,7 @@
 #include "gdp_arith.h"
 #include "gdp_device.h"
 #include "gdp_engine.h"
+#include "gdp_host.h"
 
 namespace gdp {
 
@@ -81,9 +82,55 @@ T *dev_alloc(size_t count, std::vector<void *> &owned) {
     return static_cast<T *>(p);
 }
 
+// Large copies from pageable host memory (the caller's arrays): the runtime
+// would stage them through its own pinned buffers, copying on one thread.
+// Here several threads copy each 32 MiB chunk into one of two pinned staging
+// buffers while the DMA engine drains the other.  The buffers are allocated
+// on first use and kept for the life of the process (64 MiB); one copy at a
+// time uses them.  Returns when every copy out of them has completed.
+constexpr size_t kStageChunk = 32u << 20;
+constexpr size_t kBulkMin = 4u << 20;
+
+void upload_bulk(void *dst, const void *src, size_t bytes, hipStream_t s) {
+    static std::mutex mu;
+    static char *buf[2] = {nullptr, nullptr};
+    std::lock_guard<std::mutex> lock(mu);
+    for (char *&b : buf)
+        if (!b) check(hipHostMalloc(reinterpret_cast<void **>(&b), kStageChunk, hipHostMallocDefault), "staging");
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    bool used[2] = {false, false};
+    auto release = [&]() {
+        for (int k = 0; k < 2; k++) {
+            if (used[k]) (void)hipEventSynchronize(ev[k]);
+            if (ev[k]) (void)hipEventDestroy(ev[k]);
+        }
+    };
+    try {
+        for (hipEvent_t &e : ev) check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event");
+        const int threads = std::max(1, std::min(8, static_cast<int>(std::thread::hardware_concurrency())));
+        int k = 0;
+        for (size_t off = 0; off < bytes; off += kStageChunk, k ^= 1) {
+            const size_t len = std::min(kStageChunk, bytes - off);
+            if (used[k]) check(hipEventSynchronize(ev[k]), "staging wait");  // its previous chunk has left
+            parallel_copy(buf[k], static_cast<const char *>(src) + off, len, threads);
+            check(hipMemcpyAsync(static_cast<char *>(dst) + off, buf[k], len, hipMemcpyHostToDevice, s), "H2D");
+            check(hipEventRecord(ev[k], s), "event");
+            used[k] = true;
+        }
+    } catch (...) {
+        release();
+        throw;
+    }
+    release();
+}
+
 template <typename T>
 void upload(T *dst, const T *src, size_t count, hipStream_t s) {
-    if (count) check(hipMemcpyAsync(dst, src, count * sizeof(T), hipMemcpyHostToDevice, s), "H2D");
+    if (!count) return;
+    if (count * sizeof(T) >= kBulkMin)
+        upload_bulk(dst, src, count * sizeof(T), s);
+    else
+        check(hipMemcpyAsync(dst, src, count * sizeof(T), hipMemcpyHostToDevice, s), "H2D");
 }
 
 // Storage class of a column from its bound envelope: 0 = 2 x int16,
