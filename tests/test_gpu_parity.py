@@ -222,3 +222,30 @@ def test_column_index_out_of_range_is_an_error(gpu):
     exp = fixtures.analytic("achterberg")
     got = I.gpu_propagate(exp, I.MEGAKERNEL)
     np.testing.assert_allclose(got.lbs, exp.meta["lb_expected"], atol=1e-4)
+
+
+def test_large_instance_through_staged_uploads(gpu, oracle):
+    """9.4M nonzeros: the matrix, its coefficients and the bound vectors
+    cross to the device through the library's pinned staging buffers in
+    several chunks, and the drop-in entry point copies the result back."""
+    import ctypes
+    inst = powerlaw_mip(seed=3, m=500_000, n=250_000)
+    assert inst.nnz * 8 > 64 << 20  # the coefficients take more than two 32 MiB chunks
+    exp = oracle.jacobi(inst)
+    p = I.Problem(inst)
+    try:
+        got = p.propagate(inst.lb, inst.ub, I.MEGAKERNEL)
+    finally:
+        p.close()
+    np.testing.assert_array_equal(got.lbs, exp[0])
+    np.testing.assert_array_equal(got.ubs, exp[1])
+    assert got.rounds == exp[2]
+    lib = I.load_library()
+    lb, ub = inst.lb.copy(), inst.ub.copy()
+    P = lambda a: ctypes.c_void_p(np.ascontiguousarray(a).ctypes.data)  # noqa: E731
+    keep = [np.ascontiguousarray(a) for a in (inst.col, inst.row_ptr, inst.val, inst.lhs, inst.rhs, inst.vartype)]
+    rc = lib.propagateConstraintsGPUAtomicDouble(inst.m, inst.n, inst.nnz, *(P(a) for a in keep[:5]), P(lb), P(ub),
+                                                 P(keep[5]), I.MEGAKERNEL)
+    assert rc == I.GDP_OKAY
+    np.testing.assert_array_equal(lb, exp[0])
+    np.testing.assert_array_equal(ub, exp[1])
