@@ -87,7 +87,8 @@ T *dev_alloc(size_t count, std::vector<void *> &owned) {
 // Here several threads copy each 32 MiB chunk into one of two pinned staging
 // buffers while the DMA engine drains the other.  The buffers are allocated
 // on first use and kept for the life of the process (64 MiB); one copy at a
-// time uses them.  Returns when every copy out of them has completed.
+// time uses them, and without them the copy is left to the runtime.  Returns
+// when every copy out of them has completed.
 constexpr size_t kStageChunk = 32u << 20;
 constexpr size_t kBulkMin = 4u << 20;
 
@@ -96,7 +97,13 @@ void upload_bulk(void *dst, const void *src, size_t bytes, hipStream_t s) {
     static char *buf[2] = {nullptr, nullptr};
     std::lock_guard<std::mutex> lock(mu);
     for (char *&b : buf)
-        if (!b) check(hipHostMalloc(reinterpret_cast<void **>(&b), kStageChunk, hipHostMallocDefault), "staging");
+        if (!b && hipHostMalloc(reinterpret_cast<void **>(&b), kStageChunk, hipHostMallocDefault) != hipSuccess) {
+            // no pinned memory to spare: the runtime's own staged copy
+            b = nullptr;
+            (void)hipGetLastError();
+            check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s), "H2D");
+            return;
+        }
     hipEvent_t ev[2] = {nullptr, nullptr};
     bool used[2] = {false, false};
     auto release = [&]() {
