@@ -82,7 +82,7 @@ T *dev_alloc(size_t count, std::vector<void *> &owned) {
     return static_cast<T *>(p);
 }
 
-// Large copies from pageable host memory (the caller's arrays): the runtime
+// Copies of 1 MiB and more from pageable host memory (the caller's arrays): the runtime
 // would stage them through its own pinned buffers, copying on one thread.
 // Here several threads copy each 32 MiB chunk into one of two pinned staging
 // buffers while the DMA engine drains the other.  The buffers are allocated
@@ -90,20 +90,36 @@ T *dev_alloc(size_t count, std::vector<void *> &owned) {
 // time uses them, and without them the copy is left to the runtime.  Returns
 // when every copy out of them has completed.
 constexpr size_t kStageChunk = 32u << 20;
-constexpr size_t kBulkMin = 4u << 20;
+constexpr size_t kBulkMin = 1u << 20;
 
-void upload_bulk(void *dst, const void *src, size_t bytes, hipStream_t s) {
-    static std::mutex mu;
-    static char *buf[2] = {nullptr, nullptr};
-    std::lock_guard<std::mutex> lock(mu);
-    for (char *&b : buf)
+struct Staging {
+    std::mutex mu;
+    char *buf[2] = {nullptr, nullptr};
+};
+Staging &staging() {
+    static Staging st;
+    return st;
+}
+// With st.mu held: whether both staging buffers exist (allocated here).
+bool staging_ready(Staging &st) {
+    for (char *&b : st.buf)
         if (!b && hipHostMalloc(reinterpret_cast<void **>(&b), kStageChunk, hipHostMallocDefault) != hipSuccess) {
-            // no pinned memory to spare: the runtime's own staged copy
             b = nullptr;
             (void)hipGetLastError();
-            check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s), "H2D");
-            return;
+            return false;
         }
+    return true;
+}
+int staging_threads() { return std::max(1, std::min(8, static_cast<int>(std::thread::hardware_concurrency()))); }
+
+void upload_bulk(void *dst, const void *src, size_t bytes, hipStream_t s) {
+    Staging &st = staging();
+    std::lock_guard<std::mutex> lock(st.mu);
+    if (!staging_ready(st)) {  // no pinned memory to spare: the runtime's own staged copy
+        check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s), "H2D");
+        return;
+    }
+    char *const *buf = st.buf;
     hipEvent_t ev[2] = {nullptr, nullptr};
     bool used[2] = {false, false};
     auto release = [&]() {
@@ -114,7 +130,7 @@ void upload_bulk(void *dst, const void *src, size_t bytes, hipStream_t s) {
     };
     try {
         for (hipEvent_t &e : ev) check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event");
-        const int threads = std::max(1, std::min(8, static_cast<int>(std::thread::hardware_concurrency())));
+        const int threads = staging_threads();
         int k = 0;
         for (size_t off = 0; off < bytes; off += kStageChunk, k ^= 1) {
             const size_t len = std::min(kStageChunk, bytes - off);
@@ -129,6 +145,69 @@ void upload_bulk(void *dst, const void *src, size_t bytes, hipStream_t s) {
         throw;
     }
     release();
+}
+
+// The reverse of upload_bulk, into pageable host memory: chunk i's DMA into
+// one staging buffer runs while chunk i - 1 is copied out of the other.
+// Synchronous.
+void download_bulk(void *dst, const void *src, size_t bytes, hipStream_t s) {
+    Staging &st = staging();
+    std::lock_guard<std::mutex> lock(st.mu);
+    if (!staging_ready(st)) {
+        check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s), "D2H");
+        check(hipStreamSynchronize(s), "D2H");
+        return;
+    }
+    char *const *buf = st.buf;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    bool used[2] = {false, false};
+    auto release = [&]() {
+        for (int k = 0; k < 2; k++) {
+            if (used[k]) (void)hipEventSynchronize(ev[k]);
+            if (ev[k]) (void)hipEventDestroy(ev[k]);
+        }
+    };
+    try {
+        for (hipEvent_t &e : ev) check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event");
+        const int threads = staging_threads();
+        int prev = -1;
+        size_t prev_off = 0, prev_len = 0;
+        int k = 0;
+        // a buffer is filled again only after its previous chunk was copied out
+        for (size_t off = 0; off < bytes; off += kStageChunk, k ^= 1) {
+            const size_t len = std::min(kStageChunk, bytes - off);
+            check(hipMemcpyAsync(buf[k], static_cast<const char *>(src) + off, len, hipMemcpyDeviceToHost, s), "D2H");
+            check(hipEventRecord(ev[k], s), "event");
+            used[k] = true;
+            if (prev >= 0) {
+                check(hipEventSynchronize(ev[prev]), "staging wait");
+                parallel_copy(static_cast<char *>(dst) + prev_off, buf[prev], prev_len, threads);
+            }
+            prev = k;
+            prev_off = off;
+            prev_len = len;
+        }
+        if (prev >= 0) {
+            check(hipEventSynchronize(ev[prev]), "staging wait");
+            parallel_copy(static_cast<char *>(dst) + prev_off, buf[prev], prev_len, threads);
+        }
+    } catch (...) {
+        release();
+        throw;
+    }
+    release();
+}
+
+// Device to pageable host memory; synchronous.
+template <typename T>
+void download(T *dst, const T *src, size_t count, hipStream_t s) {
+    if (!count) return;
+    if (count * sizeof(T) >= kBulkMin) {
+        download_bulk(dst, src, count * sizeof(T), s);
+    } else {
+        check(hipMemcpyAsync(dst, src, count * sizeof(T), hipMemcpyDeviceToHost, s), "D2H");
+        check(hipStreamSynchronize(s), "D2H");
+    }
 }
 
 template <typename T>
@@ -897,11 +976,11 @@ RunResult problem_propagate(Problem *P, double *lbs, double *ubs, int sync_type,
     check(hipMalloc(&d_b, 2 * sizeof(double) * static_cast<size_t>(n)), "hipMalloc bounds");
     RunResult res;
     try {
-        check(hipMemcpy(d_b, lbs, n * sizeof(double), hipMemcpyHostToDevice), "H2D lbs");
-        check(hipMemcpy(d_b + n, ubs, n * sizeof(double), hipMemcpyHostToDevice), "H2D ubs");
+        upload(d_b, lbs, n, P->stream);
+        upload(d_b + n, ubs, n, P->stream);
         res = problem_propagate_device(P, d_b, d_b + n, sync_type, max_rounds);
-        check(hipMemcpy(lbs, d_b, n * sizeof(double), hipMemcpyDeviceToHost), "D2H lbs");
-        check(hipMemcpy(ubs, d_b + n, n * sizeof(double), hipMemcpyDeviceToHost), "D2H ubs");
+        download(lbs, d_b, n, P->stream);
+        download(ubs, d_b + n, n, P->stream);
     } catch (...) {
         (void)hipFree(d_b);
         throw;

@@ -173,7 +173,7 @@ int omp_default_threads() {
 }
 
 void parallel_copy(void *dst, const void *src, size_t bytes, int threads) {
-    constexpr size_t kBlock = 1u << 20;
+    constexpr size_t kBlock = 256u << 10;
     const long long blocks = static_cast<long long>((bytes + kBlock - 1) / kBlock);
     char *d = static_cast<char *>(dst);
     const char *s = static_cast<const char *>(src);
