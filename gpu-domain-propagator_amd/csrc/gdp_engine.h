@@ -13,6 +13,7 @@
 // round-synchronous propagation of that unit alone.
 #pragma once
 
+#include <cstddef>
 #include <cstdint>
 #include <string>
 #include <vector>

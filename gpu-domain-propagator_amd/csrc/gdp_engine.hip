@@ -85,37 +85,133 @@ T *dev_alloc(size_t count, std::vector<void *> &owned) {
 // Copies of 1 MiB and more from pageable host memory (the caller's arrays): the runtime
 // would stage them through its own pinned buffers, copying on one thread.
 // Here several threads copy each 32 MiB chunk into one of two pinned staging
-// buffers while the DMA engine drains the other.  The buffers are allocated
-// on first use and kept for the life of the process (64 MiB); one copy at a
-// time uses them, and without them the copy is left to the runtime.  Returns
-// when every copy out of them has completed.
+// buffers while the DMA engine drains the other.  One copy at a time uses
+// them, and without them the copy is left to the runtime.  Returns when
+// every copy out of them has completed.
+//
+// Lifetime: the buffers (64 MiB of pinned memory) and the copy threads exist
+// while at least one problem does.  The first bulk copy creates them, and
+// the destruction of the last problem joins the threads and frees the
+// buffers, so nothing of them is left for the runtime's (or a profiler's)
+// teardown at process exit.
 constexpr size_t kStageChunk = 32u << 20;
 constexpr size_t kBulkMin = 1u << 20;
+constexpr size_t kCopyBlock = 256u << 10;
 
-struct Staging {
-    std::mutex mu;
+class Staging {
+  public:
+    std::mutex use;  // held for a whole bulk copy
     char *buf[2] = {nullptr, nullptr};
+
+    // A problem starts / ends using the staging.
+    void acquire() {
+        std::lock_guard<std::mutex> lk(life_);
+        users_++;
+    }
+    void release() {
+        std::lock_guard<std::mutex> use_lk(use);
+        std::lock_guard<std::mutex> lk(life_);
+        if (--users_ > 0) return;
+        stop_threads();
+        for (char *&b : buf) {
+            if (b) (void)hipHostFree(b);
+            b = nullptr;
+        }
+    }
+    // With `use` held: whether both buffers exist (allocated here).
+    bool ready() {
+        for (char *&b : buf)
+            if (!b && hipHostMalloc(reinterpret_cast<void **>(&b), kStageChunk, hipHostMallocDefault) != hipSuccess) {
+                b = nullptr;
+                (void)hipGetLastError();
+                return false;
+            }
+        return true;
+    }
+    // With `use` held: copies `bytes` from src to dst on the calling thread
+    // and the copy threads (256 KiB blocks dealt out in turn).
+    void copy(void *dst, const void *src, size_t bytes) {
+        start_threads();
+        {
+            std::lock_guard<std::mutex> lk(job_mu_);
+            job_dst_ = static_cast<char *>(dst);
+            job_src_ = static_cast<const char *>(src);
+            job_bytes_ = bytes;
+            job_left_ = static_cast<int>(workers_.size());
+            job_gen_++;
+        }
+        job_cv_.notify_all();
+        copy_part(static_cast<char *>(dst), static_cast<const char *>(src), bytes, 0);
+        std::unique_lock<std::mutex> lk(job_mu_);
+        done_cv_.wait(lk, [this] { return job_left_ == 0; });
+    }
+
+  private:
+    std::mutex life_;
+    int users_ = 0;
+    std::vector<std::thread> workers_;
+    std::mutex job_mu_;
+    std::condition_variable job_cv_, done_cv_;
+    char *job_dst_ = nullptr;
+    const char *job_src_ = nullptr;
+    size_t job_bytes_ = 0;
+    int job_left_ = 0;
+    unsigned long long job_gen_ = 0;
+    bool quit_ = false;
+
+    int parts() const { return static_cast<int>(workers_.size()) + 1; }
+    void copy_part(char *d, const char *s, size_t bytes, int part) {
+        const size_t blocks = (bytes + kCopyBlock - 1) / kCopyBlock;
+        for (size_t b = static_cast<size_t>(part); b < blocks; b += static_cast<size_t>(parts())) {
+            const size_t off = b * kCopyBlock;
+            std::memcpy(d + off, s + off, std::min(kCopyBlock, bytes - off));
+        }
+    }
+    void start_threads() {
+        if (!workers_.empty()) return;
+        const int n = std::max(0, std::min(8, static_cast<int>(std::thread::hardware_concurrency())) - 1);
+        quit_ = false;
+        for (int t = 0; t < n; t++) workers_.emplace_back([this, t] { worker(t + 1); });
+    }
+    void stop_threads() {
+        {
+            std::lock_guard<std::mutex> lk(job_mu_);
+            quit_ = true;
+        }
+        job_cv_.notify_all();
+        for (auto &w : workers_) w.join();
+        workers_.clear();
+    }
+    void worker(int part) {
+        unsigned long long seen = 0;
+        for (;;) {
+            char *d;
+            const char *s;
+            size_t bytes;
+            {
+                std::unique_lock<std::mutex> lk(job_mu_);
+                job_cv_.wait(lk, [&] { return quit_ || job_gen_ != seen; });
+                if (quit_) return;
+                seen = job_gen_;
+                d = job_dst_;
+                s = job_src_;
+                bytes = job_bytes_;
+            }
+            copy_part(d, s, bytes, part);
+            std::lock_guard<std::mutex> lk(job_mu_);
+            if (--job_left_ == 0) done_cv_.notify_all();
+        }
+    }
 };
 Staging &staging() {
     static Staging st;
     return st;
 }
-// With st.mu held: whether both staging buffers exist (allocated here).
-bool staging_ready(Staging &st) {
-    for (char *&b : st.buf)
-        if (!b && hipHostMalloc(reinterpret_cast<void **>(&b), kStageChunk, hipHostMallocDefault) != hipSuccess) {
-            b = nullptr;
-            (void)hipGetLastError();
-            return false;
-        }
-    return true;
-}
-int staging_threads() { return std::max(1, std::min(8, static_cast<int>(std::thread::hardware_concurrency()))); }
 
 void upload_bulk(void *dst, const void *src, size_t bytes, hipStream_t s) {
     Staging &st = staging();
-    std::lock_guard<std::mutex> lock(st.mu);
-    if (!staging_ready(st)) {  // no pinned memory to spare: the runtime's own staged copy
+    std::lock_guard<std::mutex> lock(st.use);
+    if (!st.ready()) {  // no pinned memory to spare: the runtime's own staged copy
         check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s), "H2D");
         return;
     }
@@ -130,12 +226,11 @@ void upload_bulk(void *dst, const void *src, size_t bytes, hipStream_t s) {
     };
     try {
         for (hipEvent_t &e : ev) check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event");
-        const int threads = staging_threads();
         int k = 0;
         for (size_t off = 0; off < bytes; off += kStageChunk, k ^= 1) {
             const size_t len = std::min(kStageChunk, bytes - off);
             if (used[k]) check(hipEventSynchronize(ev[k]), "staging wait");  // its previous chunk has left
-            parallel_copy(buf[k], static_cast<const char *>(src) + off, len, threads);
+            st.copy(buf[k], static_cast<const char *>(src) + off, len);
             check(hipMemcpyAsync(static_cast<char *>(dst) + off, buf[k], len, hipMemcpyHostToDevice, s), "H2D");
             check(hipEventRecord(ev[k], s), "event");
             used[k] = true;
@@ -152,8 +247,8 @@ void upload_bulk(void *dst, const void *src, size_t bytes, hipStream_t s) {
 // Synchronous.
 void download_bulk(void *dst, const void *src, size_t bytes, hipStream_t s) {
     Staging &st = staging();
-    std::lock_guard<std::mutex> lock(st.mu);
-    if (!staging_ready(st)) {
+    std::lock_guard<std::mutex> lock(st.use);
+    if (!st.ready()) {
         check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s), "D2H");
         check(hipStreamSynchronize(s), "D2H");
         return;
@@ -169,7 +264,6 @@ void download_bulk(void *dst, const void *src, size_t bytes, hipStream_t s) {
     };
     try {
         for (hipEvent_t &e : ev) check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event");
-        const int threads = staging_threads();
         int prev = -1;
         size_t prev_off = 0, prev_len = 0;
         int k = 0;
@@ -181,7 +275,7 @@ void download_bulk(void *dst, const void *src, size_t bytes, hipStream_t s) {
             used[k] = true;
             if (prev >= 0) {
                 check(hipEventSynchronize(ev[prev]), "staging wait");
-                parallel_copy(static_cast<char *>(dst) + prev_off, buf[prev], prev_len, threads);
+                st.copy(static_cast<char *>(dst) + prev_off, buf[prev], prev_len);
             }
             prev = k;
             prev_off = off;
@@ -189,7 +283,7 @@ void download_bulk(void *dst, const void *src, size_t bytes, hipStream_t s) {
         }
         if (prev >= 0) {
             check(hipEventSynchronize(ev[prev]), "staging wait");
-            parallel_copy(static_cast<char *>(dst) + prev_off, buf[prev], prev_len, threads);
+            st.copy(static_cast<char *>(dst) + prev_off, buf[prev], prev_len);
         }
     } catch (...) {
         release();
@@ -280,6 +374,15 @@ int grid_for(long long waves) { return static_cast<int>(std::max(1LL, (waves + 3
 std::string last_error() { return g_last_error; }
 void set_last_error(const std::string &msg) { g_last_error = msg; }
 
+// Launch tables of a run state (filled by the host, read by the kernels).
+struct UnitTables {
+    int *list = nullptr;      // listed units of a launch
+    int4 *params = nullptr;   // unit_reset parameters
+    double **lbp = nullptr;   // bound-vector pointer tables
+    double **ubp = nullptr;
+    int *hint_ptr = nullptr;  // hint starts
+};
+
 struct Problem {
     bool replicas = true;
     int m = 0, n = 0, nnz = 0;       // caller's totals (all instances)
@@ -288,8 +391,8 @@ struct Problem {
     ProblemStats stats;
     DevMatrix M{};
     DevState D{};
-    std::vector<void *> owned;       // matrix layout and fixed-size state
-    std::vector<void *> unit_owned;  // per-unit state (depends on the capacity)
+    std::vector<void *> owned;       // matrix layout
+    void *state_buf = nullptr;       // run state owned by the problem (carve_state), or null
     int capacity = 0;                // units with state arrays (replicas)
     int n_units = 0;                 // entries of D.units
     std::vector<int> inst_ncols;     // blocks: columns per instance
@@ -314,6 +417,8 @@ struct Problem {
     std::vector<int> graph_last;
     long long graph_key = -1;
     std::mutex run_mutex;
+    int servers = 0;                  // servers attached (they own the unit slots)
+    bool staging_held = false;
 
     void drop_graphs() {
         for (auto g : graphs)
@@ -328,64 +433,105 @@ struct Problem {
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         if (stream) (void)hipStreamDestroy(stream);
-        for (void *p : unit_owned) (void)hipFree(p);
+        if (state_buf) (void)hipFree(state_buf);
         for (void *p : hint_owned) (void)hipFree(p);
         for (void *p : owned) (void)hipFree(p);
+        if (staging_held) staging().release();
     }
 };
 
 namespace {
 
-// Device state of `cap` units (replicas) or of the batch (blocks).
+// The run state of `cap` bound vectors (replicas) or of the batch (blocks),
+// in one buffer at `base` with every array on a 256-byte boundary: the unit
+// state the kernels read and write (snapshot, accumulators, bitmap, row and
+// segment marks, split-row partials, apply queue, per-unit words and
+// counters), the problem-wide status ring, global words and barrier, and
+// the launch tables the host fills (listed units, reset parameters,
+// bound-vector pointers, hint starts).  With base null only the size is
+// computed.  Returns the bytes needed.
+size_t carve_state(const DevMatrix &M, bool replicas, int n_inst, int cap, char *base, DevState &D,
+                   UnitTables &T) {
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        char *p = base ? base + off : nullptr;
+        off += (std::max<size_t>(bytes, 1) + 255) / 256 * 256;
+        return p;
+    };
+    const size_t k = replicas ? static_cast<size_t>(cap) : 1;
+    const size_t units = replicas ? static_cast<size_t>(cap) : static_cast<size_t>(n_inst);
+    const size_t nf = static_cast<size_t>(M.n - M.n_int);
+    char *snap = take(k * static_cast<size_t>(M.snap_stride));
+    D.S16 = reinterpret_cast<unsigned *>(snap);
+    D.S32 = reinterpret_cast<int2 *>(snap ? snap + M.snap_off32 : nullptr);
+    D.SF = reinterpret_cast<double2 *>(snap ? snap + M.snap_offF : nullptr);
+    D.AI = reinterpret_cast<int2 *>(take(k * M.n_int * sizeof(int2)));
+    D.AX = reinterpret_cast<double2 *>(take(k * M.n_int * sizeof(double2)));
+    D.AF = reinterpret_cast<double2 *>(take(k * nf * sizeof(double2)));
+    D.bm = reinterpret_cast<unsigned *>(take(k * M.nwords * sizeof(unsigned)));
+    D.marks = reinterpret_cast<int *>(take(k * M.m * sizeof(int)));
+    D.seg_parts = reinterpret_cast<SegPart *>(take(k * M.n_seg_parts * sizeof(SegPart)));
+    D.seg_marks = reinterpret_cast<int *>(take(k * M.n_seg_parts * sizeof(int)));
+    D.seg_state = reinterpret_cast<SegState *>(take(k * M.n_seg_rows * sizeof(SegState)));
+    D.seg_count = reinterpret_cast<unsigned *>(take(k * M.n_seg_rows * sizeof(unsigned)));
+    D.apply_tasks = reinterpret_cast<int *>(take(k * M.n_seg_parts * sizeof(int)));
+    D.units = reinterpret_cast<UnitState *>(take(units * sizeof(UnitState)));
+    D.moved = reinterpret_cast<int *>(take(units * kMoveSlots * kMoveLanes * sizeof(int)));
+    D.status = reinterpret_cast<int *>(take(kStatusRows * kStatusWords * sizeof(int)));
+    D.global = reinterpret_cast<int *>(take(kGlobalWords * sizeof(int)));
+    D.barrier = reinterpret_cast<unsigned *>(take(kBarrierWords * sizeof(unsigned)));
+    T.list = reinterpret_cast<int *>(take(units * sizeof(int)));
+    T.params = reinterpret_cast<int4 *>(take(units * sizeof(int4)));
+    T.lbp = reinterpret_cast<double **>(take(units * sizeof(double *)));
+    T.ubp = reinterpret_cast<double **>(take(units * sizeof(double *)));
+    T.hint_ptr = reinterpret_cast<int *>(take((units + 1) * sizeof(int)));
+    D.active = T.list;
+    return off;
+}
+
+// Points the problem's run state at `base` (cap units, see carve_state).
+void bind_state(Problem *P, char *base, int cap) {
+    UnitTables T;
+    carve_state(P->M, P->replicas, P->n_inst, cap, base, P->D, T);
+    P->d_list = T.list;
+    P->d_params = T.params;
+    P->d_lbp = T.lbp;
+    P->d_ubp = T.ubp;
+    P->d_hint_ptr = T.hint_ptr;
+    P->capacity = cap;
+    P->n_units = P->replicas ? cap : P->n_inst;
+}
+
+// Device state of `cap` units (replicas) or of the batch (blocks), owned by
+// the problem.
 void alloc_units(Problem *P, int cap) {
     check(hipStreamSynchronize(P->stream), "sync");
     P->drop_graphs();
-    for (void *p : P->unit_owned) (void)hipFree(p);
-    P->unit_owned.clear();
-    DevMatrix &M = P->M;
-    DevState &D = P->D;
+    if (P->state_buf) (void)hipFree(P->state_buf);
+    P->state_buf = nullptr;
+    P->capacity = 0;
+    P->n_units = 0;
+    DevState D{};
+    UnitTables T;
+    const size_t bytes = carve_state(P->M, P->replicas, P->n_inst, cap, nullptr, D, T);
+    check(hipMalloc(&P->state_buf, bytes), "hipMalloc run state");
+    bind_state(P, static_cast<char *>(P->state_buf), cap);
     const size_t k = P->replicas ? static_cast<size_t>(cap) : 1;
-    const int n32 = M.n_int - M.n16, nf = M.n - M.n_int;
-    auto &o = P->unit_owned;
-    // snapshot: one block of 16-byte chunks per unit
-    M.snap_off32 = (M.n16 * 4 + 15) / 16 * 16;
-    M.snap_offF = M.snap_off32 + (n32 * 8 + 15) / 16 * 16;
-    M.snap_stride = M.snap_offF + 16LL * nf + 16;
-    char *snap = dev_alloc<char>(k * static_cast<size_t>(M.snap_stride), o);
-    D.S16 = reinterpret_cast<unsigned *>(snap);
-    D.S32 = reinterpret_cast<int2 *>(snap + M.snap_off32);
-    D.SF = reinterpret_cast<double2 *>(snap + M.snap_offF);
-    D.AI = dev_alloc<int2>(k * M.n_int, o);
-    D.AX = dev_alloc<double2>(k * M.n_int, o);
-    D.AF = dev_alloc<double2>(k * nf, o);
-    D.bm = dev_alloc<unsigned>(k * M.nwords, o);
-    D.marks = dev_alloc<int>(k * M.m, o);
-    D.seg_parts = dev_alloc<SegPart>(k * M.n_seg_parts, o);
-    D.seg_marks = dev_alloc<int>(k * M.n_seg_parts, o);
-    D.seg_state = dev_alloc<SegState>(k * M.n_seg_rows, o);
-    D.seg_count = dev_alloc<unsigned>(k * M.n_seg_rows, o);
-    const int units = P->replicas ? cap : P->n_inst;
-    D.units = dev_alloc<UnitState>(units, o);
-    D.moved = dev_alloc<int>(static_cast<size_t>(units) * kMoveSlots * kMoveLanes, o);
-    D.apply_tasks = dev_alloc<int>(k * M.n_seg_parts, o);
-    P->d_list = dev_alloc<int>(units, o);
-    P->d_params = dev_alloc<int4>(units, o);
-    P->d_lbp = dev_alloc<double *>(units, o);
-    P->d_ubp = dev_alloc<double *>(units, o);
-    P->d_hint_ptr = dev_alloc<int>(static_cast<size_t>(units) + 1, o);
-    check(hipMemsetAsync(D.units, 0, sizeof(UnitState) * units, P->stream), "memset units");
+    const DevMatrix &M = P->M;
+    check(hipMemsetAsync(P->D.units, 0, sizeof(UnitState) * P->n_units, P->stream), "memset units");
     if (M.n_seg_rows > 0)
-        check(hipMemsetAsync(D.seg_count, 0, sizeof(unsigned) * k * M.n_seg_rows, P->stream), "memset");
-    check(hipMemsetAsync(D.bm, 0, sizeof(unsigned) * k * M.nwords, P->stream), "memset");
+        check(hipMemsetAsync(P->D.seg_count, 0, sizeof(unsigned) * k * M.n_seg_rows, P->stream), "memset");
+    check(hipMemsetAsync(P->D.bm, 0, sizeof(unsigned) * k * M.nwords, P->stream), "memset");
+    check(hipMemsetAsync(P->D.barrier, 0, sizeof(unsigned) * kBarrierWords, P->stream), "memset");
+    check(hipMemsetAsync(P->D.global, 0, sizeof(int) * kGlobalWords, P->stream), "memset");
     check(hipStreamSynchronize(P->stream), "alloc sync");
-    P->capacity = cap;
-    P->n_units = units;
-    D.active = P->d_list;
-    const long long per_unit = static_cast<long long>(M.n16) * 4 + n32 * 8LL + nf * 16LL + M.n_int * 24LL +
-                               nf * 16LL + M.nwords * 8LL + M.m * 4LL + M.n_seg_parts * 44LL +
-                               M.n_seg_rows * 28LL + sizeof(UnitState) + kMoveSlots * kMoveLanes * 4;
-    P->stats.unit_state_bytes = per_unit;
-    P->stats.n_units = units;
+    P->stats.n_units = P->n_units;
+}
+
+// The run state of unit 0 exists (allocated on first use: a problem that is
+// only run through the scratch entry point never allocates it).
+void ensure_units(Problem *P) {
+    if (P->capacity < 1) alloc_units(P, 1);
 }
 
 // The passes over the nonzeros, on the device: the instances' CSR arrays are
@@ -454,6 +600,8 @@ Problem *build(const std::vector<InstanceView> &inst, bool replicas, const Tunin
     // device, which also checks the column indices.
     for (const auto &v : inst) validate(v);
     auto *P = new Problem();
+    staging().acquire();
+    P->staging_held = true;
     try {
         P->replicas = replicas;
         P->n_inst = static_cast<int>(inst.size());
@@ -710,12 +858,23 @@ Problem *build(const std::vector<InstanceView> &inst, bool replicas, const Tunin
         M.perm = d_perm;
         M.word_unit = d_word_unit;
 
-        DevState &D = P->D;
-        D.status = dev_alloc<int>(kStatusRows * kStatusWords, o);
-        D.global = dev_alloc<int>(kGlobalWords, o);
-        D.barrier = dev_alloc<unsigned>(kBarrierWords, o);
+        // snapshot block of a unit: int16 records, then int32, then FP64,
+        // each class from a 16-byte boundary
+        M.snap_off32 = (M.n16 * 4 + 15) / 16 * 16;
+        M.snap_offF = M.snap_off32 + ((M.n_int - M.n16) * 8 + 15) / 16 * 16;
+        M.snap_stride = M.snap_offF + 16LL * (M.n - M.n_int) + 16;
         check(hipStreamSynchronize(s), "upload");
-        alloc_units(P, 1);
+        {
+            // device bytes of one more bound vector's state (the run state
+            // itself is allocated on first use)
+            DevState D{};
+            UnitTables T;
+            if (replicas)
+                P->stats.unit_state_bytes =
+                    static_cast<long long>(carve_state(M, true, NI, 2, nullptr, D, T) -
+                                           carve_state(M, true, NI, 1, nullptr, D, T));
+            P->stats.n_units = replicas ? 0 : NI;
+        }
 
         P->mega_threads = T.mega_threads == 256 || T.mega_threads == 1024 ? T.mega_threads : 512;
         P->mega_fn = P->mega_threads == 256    ? reinterpret_cast<const void *>(mega_kernel<256>)
@@ -752,7 +911,11 @@ void load_units(Problem *P, int n_list, const int *list, double *const *lbp, dou
     hipStream_t s = P->stream;
     DevMatrix &M = P->M;
     DevState &D = P->D;
-    std::vector<int4> params(n_list);
+    // a few units' parameters on the stack: a single-vector run touches no heap
+    int4 few[8];
+    std::vector<int4> many;
+    if (n_list > 8) many.resize(n_list);
+    int4 *params = n_list > 8 ? many.data() : few;
     for (int k = 0; k < n_list; k++) {
         const int u = list[k];
         const bool hinted = hint_ptr != nullptr && hint_ptr[k + 1] > hint_ptr[k] && M.mark_rows;
@@ -760,7 +923,7 @@ void load_units(Problem *P, int n_list, const int *list, double *const *lbp, dou
         params[k] = make_int4(u, limit, hinted ? 0 : 1, max_rounds[k]);
     }
     upload(P->d_list, list, n_list, s);
-    upload(P->d_params, params.data(), n_list, s);
+    upload(P->d_params, params, n_list, s);
     const int n_tables = P->replicas ? n_list : 1;
     upload(P->d_lbp, lbp, n_tables, s);
     upload(P->d_ubp, ubp, n_tables, s);
@@ -893,6 +1056,40 @@ std::vector<UnitState> read_units(Problem *P, int n_list, const int *list) {
     return out;
 }
 
+// State words of unit u (synchronous).
+UnitState read_unit(Problem *P, int u) {
+    UnitState st;
+    check(hipMemcpyAsync(&st, P->D.units + u, sizeof(UnitState), hipMemcpyDeviceToHost, P->stream), "unit D2H");
+    check(hipStreamSynchronize(P->stream), "unit sync");
+    return st;
+}
+
+// A complete run of one bound vector (device pointers) as unit 0, from
+// round 1.  Host buffers stay on the stack.
+RunResult run_single(Problem *P, double *d_lbs, double *d_ubs, int sync_type, int max_rounds) {
+    if (sync_type < CPU_LOOP_ID || sync_type > MEGAKERNEL_ID) throw std::runtime_error("Unknown sync type");
+    if (max_rounds < 1 || max_rounds > kMaxRounds) throw std::runtime_error("max_rounds out of range");
+    const int list[1] = {0};
+    const int caps[1] = {max_rounds};
+    P->D.n_active = 1;
+    load_units(P, 1, list, &d_lbs, &d_ubs, nullptr, nullptr, caps, 1, true);
+    UnitState st = read_unit(P, 0);
+    if (st.envelope) throw std::runtime_error("bounds outside the problem's bound envelope");
+    double ms = 0.0;
+    if (!st.crossed) {
+        ms = run_steps(P, sync_type, max_rounds);
+        st = read_unit(P, 0);
+    }
+    RunResult r = outcome(st);
+    r.kernel_ms = ms;
+    const int gx = std::max(1, std::min(1024, (P->M.n + 255) / 256));
+    hipLaunchKernelGGL(unpack_kernel, dim3(gx, 1), dim3(256), 0, P->stream, P->M, P->D, 1, P->d_list, P->d_lbp,
+                       P->d_ubp);
+    check(hipGetLastError(), "unpack launch");
+    check(hipStreamSynchronize(P->stream), "unpack sync");
+    return r;
+}
+
 // A complete run of the listed units from round 1.
 void run_units(Problem *P, int n_list, const int *list, double *const *lbp, double *const *ubp, const int *hint_ptr,
                const int *hints, int sync_type, int max_rounds, RunResult *out) {
@@ -950,27 +1147,53 @@ Problem *batch_create(const std::vector<InstanceView> &instances, const Tuning *
     return build(instances, false, tuning);
 }
 
-void problem_destroy(Problem *p) { delete p; }
+void problem_destroy(Problem *p) {
+    {
+        std::lock_guard<std::mutex> lock(p->run_mutex);
+        if (p->servers > 0) throw std::runtime_error("a server is attached to this problem: destroy the server first");
+    }
+    delete p;
+}
 
 ProblemStats problem_stats(const Problem *p) { return p->stats; }
+
+namespace {
+// With P->run_mutex held.  A server owns the problem's unit slots (slot k
+// is unit k) for its whole life: nothing else may run units or reallocate
+// their state meanwhile.
+void check_no_server(const Problem *P) {
+    if (P->servers > 0)
+        throw std::runtime_error("a server is attached to this problem: its unit slots belong to the server");
+}
+
+// With P->run_mutex held: one bound vector (device pointers), unit 0.
+RunResult propagate_device_locked(Problem *P, double *d_lbs, double *d_ubs, int sync_type, int max_rounds) {
+    if (!P->replicas) throw std::runtime_error("use the batch entry points for a batch of instances");
+    check_no_server(P);
+    ensure_units(P);
+    return run_single(P, d_lbs, d_ubs, sync_type, max_rounds);
+}
+}  // namespace
 
 void problem_reserve(Problem *P, int capacity) {
     if (!P->replicas) throw std::runtime_error("a batch of instances has no node capacity");
     if (capacity < 1) throw std::runtime_error("capacity must be positive");
     std::lock_guard<std::mutex> lock(P->run_mutex);
+    check_no_server(P);
     if (capacity != P->capacity) alloc_units(P, capacity);
 }
 
 RunResult problem_propagate_device(Problem *P, double *d_lbs, double *d_ubs, int sync_type, int max_rounds) {
     std::lock_guard<std::mutex> lock(P->run_mutex);
-    const int list[1] = {0};
-    RunResult r;
-    if (!P->replicas) throw std::runtime_error("use the batch entry points for a batch of instances");
-    run_units(P, 1, list, &d_lbs, &d_ubs, nullptr, nullptr, sync_type, max_rounds, &r);
-    return r;
+    return propagate_device_locked(P, d_lbs, d_ubs, sync_type, max_rounds);
 }
 
+// The whole call holds the problem's lock: the bound vectors travel on the
+// problem's stream, which another thread's call may otherwise be capturing
+// into a graph (GPU_LOOP) or waiting on.
 RunResult problem_propagate(Problem *P, double *lbs, double *ubs, int sync_type, int max_rounds) {
+    std::lock_guard<std::mutex> lock(P->run_mutex);
+    check_no_server(P);
     const int n = P->n;
     double *d_b = nullptr;
     check(hipMalloc(&d_b, 2 * sizeof(double) * static_cast<size_t>(n)), "hipMalloc bounds");
@@ -978,10 +1201,11 @@ RunResult problem_propagate(Problem *P, double *lbs, double *ubs, int sync_type,
     try {
         upload(d_b, lbs, n, P->stream);
         upload(d_b + n, ubs, n, P->stream);
-        res = problem_propagate_device(P, d_b, d_b + n, sync_type, max_rounds);
+        res = propagate_device_locked(P, d_b, d_b + n, sync_type, max_rounds);
         download(lbs, d_b, n, P->stream);
         download(ubs, d_b + n, n, P->stream);
     } catch (...) {
+        (void)hipStreamSynchronize(P->stream);
         (void)hipFree(d_b);
         throw;
     }
@@ -993,6 +1217,8 @@ RunResult problem_work_profile(Problem *P, double *lbs, double *ubs, int max_rou
     if (!P->replicas) throw std::runtime_error("work profiles are for a single instance");
     if (max_rounds < 1 || max_rounds > kMaxRounds) throw std::runtime_error("max_rounds out of range");
     std::lock_guard<std::mutex> lock(P->run_mutex);
+    check_no_server(P);
+    ensure_units(P);
     const DevMatrix &M = P->M;
     hipStream_t s = P->stream;
     const int n = P->n;
@@ -1100,6 +1326,8 @@ RunResult problem_propagate_measured(Problem *P, double *lbs, double *ubs, int m
     if (!P->replicas) throw std::runtime_error("progress measures are for a single instance");
     if (max_rounds < 1 || max_rounds > kMaxRounds) throw std::runtime_error("max_rounds out of range");
     std::lock_guard<std::mutex> lock(P->run_mutex);
+    check_no_server(P);
+    ensure_units(P);
     hipStream_t s = P->stream;
     const int n = P->n;
     const size_t nb = sizeof(double) * static_cast<size_t>(n);
@@ -1178,8 +1406,9 @@ void nodes_propagate_device(Problem *P, int count, double *const *d_lbs, double 
         for (int h = 0; h < hint_ptr[count]; h++)
             if (!hints || hints[h] < 0 || hints[h] >= P->n) throw std::runtime_error("hint column out of range");
     }
-    if (count > P->capacity) problem_reserve(P, count);
     std::lock_guard<std::mutex> lock(P->run_mutex);
+    check_no_server(P);
+    if (count > P->capacity) alloc_units(P, count);
     std::vector<int> list(count);
     for (int k = 0; k < count; k++) list[k] = k;
     run_units(P, count, list.data(), d_lbs, d_ubs, hint_ptr, hints, sync_type, max_rounds, out);
@@ -1189,6 +1418,7 @@ void batch_propagate_device(Problem *P, double *d_lbs, double *d_ubs, int sync_t
                             RunResult *out) {
     if (P->replicas) throw std::runtime_error("not a batch of instances");
     std::lock_guard<std::mutex> lock(P->run_mutex);
+    ensure_units(P);
     std::vector<int> list(P->n_inst);
     for (int k = 0; k < P->n_inst; k++) list[k] = k;
     run_units(P, P->n_inst, list.data(), &d_lbs, &d_ubs, nullptr, nullptr, sync_type, max_rounds, out);
@@ -1324,6 +1554,7 @@ struct Server {
     };
     static constexpr int kDiffCap = 4096;  // changed columns returned without the full vector
     Problem *P = nullptr;
+    bool attached = false;               // counted in P->servers
     int slots = 0;
     size_t n = 0;
     std::deque<Request> queue;
@@ -1426,6 +1657,10 @@ struct Server {
         for (hipEvent_t e : stage_ev) (void)hipEventDestroy(e);
         if (d_stage) (void)hipFree(d_stage);
         if (cs) (void)hipStreamDestroy(cs);
+        if (attached) {
+            std::lock_guard<std::mutex> lock(P->run_mutex);
+            P->servers--;
+        }
     }
     double *orig_lb(int k) { return d_orig + 2 * n * k; }
     double *orig_ub(int k) { return d_orig + 2 * n * k + n; }
@@ -1447,7 +1682,6 @@ Server *server_create(Problem *P, int64_t hbm_budget, int max_slots) {
         S->P = P;
         S->slots = slots;
         S->n = static_cast<size_t>(P->n);
-        problem_reserve(P, slots);
         check(hipMalloc(&S->d_buf, sizeof(double) * 2 * S->n * slots), "hipMalloc slots");
         check(hipMalloc(&S->d_base, sizeof(double) * 2 * S->n), "hipMalloc base");
         for (int k = 0; k < slots; k++) {
@@ -1503,8 +1737,13 @@ Server *server_create(Problem *P, int64_t hbm_budget, int max_slots) {
             S->stage_free.push_back(e);
         }
         std::reverse(S->stage_ev.begin(), S->stage_ev.end());
-                // fresh status ring for the server's steps
+        // the problem's unit slots become the server's until it is destroyed
         std::lock_guard<std::mutex> lock(P->run_mutex);
+        check_no_server(P);
+        if (P->capacity != slots) alloc_units(P, slots);
+        P->servers++;
+        S->attached = true;
+        // fresh status ring for the server's steps
         hipLaunchKernelGGL(unit_reset_kernel, dim3(1), dim3(256), 0, P->stream, P->D, 0, P->d_params, 0, 1);
         check(hipStreamSynchronize(P->stream), "server init");
     } catch (...) {

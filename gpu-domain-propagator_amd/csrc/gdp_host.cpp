@@ -172,18 +172,6 @@ int omp_default_threads() {
     return 8;  // SHARED_MEM_THREADS of the reference (src/params.h:13)
 }
 
-void parallel_copy(void *dst, const void *src, size_t bytes, int threads) {
-    constexpr size_t kBlock = 256u << 10;
-    const long long blocks = static_cast<long long>((bytes + kBlock - 1) / kBlock);
-    char *d = static_cast<char *>(dst);
-    const char *s = static_cast<const char *>(src);
-#pragma omp parallel for num_threads(threads) schedule(static)
-    for (long long b = 0; b < blocks; b++) {
-        const size_t off = static_cast<size_t>(b) * kBlock;
-        std::memcpy(d + off, s + off, std::min(kBlock, bytes - off));
-    }
-}
-
 HostResult omp_propagate(int m, int n, const int *col, const int *row_ptr, const double *val,
                          const double *lhs, const double *rhs, double *lb, double *ub, const int *vartype,
                          int max_rounds, int threads) {

@@ -40,10 +40,6 @@ HostResult omp_propagate(int m, int n, const int *col, const int *row_ptr, const
 
 int omp_default_threads();
 
-// Copies `bytes` from src to dst with up to `threads` host threads (256 KiB
-// blocks): fills pinned staging memory faster than one thread can.
-void parallel_copy(void *dst, const void *src, size_t bytes, int threads);
-
 // ---- progress measure (src/progressMeasureMethods/, gdp_measure.cpp) ----
 
 // computeWeakestBounds (weakest_bounds.h:772-829): lb/ub in/out; bounds that

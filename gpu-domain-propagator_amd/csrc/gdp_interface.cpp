@@ -227,6 +227,68 @@ GDP_RETCODE with_measure(bool gpu, int m, int n, int nnz, const int *col, const 
 
 }  // namespace
 
+namespace {
+
+gdp::Tuning tuning_from(const gdp_tuning *tuning) {
+    gdp::Tuning T;
+    if (tuning) {
+        if (tuning->item_nnz > 0) T.item_nnz = tuning->item_nnz;
+        if (tuning->mega_blocks_per_cu > 0) T.mega_blocks_per_cu = tuning->mega_blocks_per_cu;
+        if (tuning->mega_threads > 0) T.mega_threads = tuning->mega_threads;
+        if (tuning->graph_rounds > 0) T.graph_rounds = tuning->graph_rounds;
+        if (tuning->max_split_segs >= 0) T.max_split_segs = tuning->max_split_segs;
+        if (tuning->compact_bounds >= 0) T.compact_bounds = tuning->compact_bounds;
+        if (tuning->row_marking >= 0) T.row_marking = tuning->row_marking;
+        if (tuning->mark_limit >= 0) T.mark_limit = tuning->mark_limit;
+        for (int b = 0; b < 10; b++)
+            if (tuning->group_log[b] >= 0) T.group_log[b] = tuning->group_log[b];
+    }
+    return T;
+}
+
+}  // namespace
+
+namespace {
+
+void tuning_to(const gdp::Tuning &T, gdp_tuning *t) {
+    t->item_nnz = T.item_nnz;
+    t->mega_blocks_per_cu = T.mega_blocks_per_cu;
+    t->mega_threads = T.mega_threads;
+    t->graph_rounds = T.graph_rounds;
+    t->max_split_segs = T.max_split_segs;
+    t->compact_bounds = T.compact_bounds;
+    t->row_marking = T.row_marking;
+    t->mark_limit = T.mark_limit;
+    for (int b = 0; b < 10; b++) t->group_log[b] = T.group_log[b];
+}
+
+GDP_RETCODE autotune_views(const std::vector<gdp::InstanceView> &v, const double *lbs, const double *ubs,
+                           const int sync_type, gdp_tuning *tuning, double *best_ms, const char *who) {
+    try {
+        const gdp::Tuning start = tuning_from(tuning);
+        std::vector<gdp::AutotuneTrial> trials;
+        const gdp::Tuning best = gdp::autotune(v, lbs, ubs, sync_type, &start, &trials);
+        double ms = 0.0;
+        for (const auto &t : trials)
+            if (std::memcmp(&t.tuning, &best, sizeof(best)) == 0) ms = t.kernel_ms;
+        tuning_to(best, tuning);
+        if (best_ms) *best_ms = ms;
+        return GDP_OKAY;
+    } catch (const std::exception &e) {
+        return fail(who, e);
+    }
+}
+
+}  // namespace
+
+namespace {
+GDP_RETCODE hip_rc(hipError_t e, const char *what) {
+    if (e == hipSuccess) return GDP_OKAY;
+    gdp::set_last_error(std::string(what) + ": " + hipGetErrorString(e));
+    return GDP_ERROR;
+}
+}  // namespace
+
 extern "C" {
 
 // ---- reference entry points (include/interface.h) ----
@@ -382,27 +444,6 @@ void gdpTuningDefaults(gdp_tuning *t) {
     for (int b = 0; b < 10; b++) t->group_log[b] = d.group_log[b];
 }
 
-namespace {
-
-gdp::Tuning tuning_from(const gdp_tuning *tuning) {
-    gdp::Tuning T;
-    if (tuning) {
-        if (tuning->item_nnz > 0) T.item_nnz = tuning->item_nnz;
-        if (tuning->mega_blocks_per_cu > 0) T.mega_blocks_per_cu = tuning->mega_blocks_per_cu;
-        if (tuning->mega_threads > 0) T.mega_threads = tuning->mega_threads;
-        if (tuning->graph_rounds > 0) T.graph_rounds = tuning->graph_rounds;
-        if (tuning->max_split_segs >= 0) T.max_split_segs = tuning->max_split_segs;
-        if (tuning->compact_bounds >= 0) T.compact_bounds = tuning->compact_bounds;
-        if (tuning->row_marking >= 0) T.row_marking = tuning->row_marking;
-        if (tuning->mark_limit >= 0) T.mark_limit = tuning->mark_limit;
-        for (int b = 0; b < 10; b++)
-            if (tuning->group_log[b] >= 0) T.group_log[b] = tuning->group_log[b];
-    }
-    return T;
-}
-
-}  // namespace
-
 GDP_RETCODE gdpProblemCreate(const int n_cons, const int n_vars, const int nnz, const int *col_indices,
                              const int *row_ptrs, const double *vals, const double *lhss, const double *rhss,
                              const int *vartypes, const double *env_lbs, const double *env_ubs,
@@ -422,7 +463,11 @@ GDP_RETCODE gdpProblemCreate(const int n_cons, const int n_vars, const int nnz, 
 
 GDP_RETCODE gdpProblemDestroy(gdp_problem p) {
     if (!p) return GDP_OKAY;
-    gdp::problem_destroy(p->p);
+    try {
+        gdp::problem_destroy(p->p);
+    } catch (const std::exception &e) {
+        return fail("gdpProblemDestroy", e);
+    }
     delete p;
     return GDP_OKAY;
 }
@@ -546,39 +591,6 @@ GDP_RETCODE gdpBatchPropagate(gdp_problem p, double *lbs, double *ubs, const int
     (void)hipFree(d);
     return rc;
 }
-
-namespace {
-
-void tuning_to(const gdp::Tuning &T, gdp_tuning *t) {
-    t->item_nnz = T.item_nnz;
-    t->mega_blocks_per_cu = T.mega_blocks_per_cu;
-    t->mega_threads = T.mega_threads;
-    t->graph_rounds = T.graph_rounds;
-    t->max_split_segs = T.max_split_segs;
-    t->compact_bounds = T.compact_bounds;
-    t->row_marking = T.row_marking;
-    t->mark_limit = T.mark_limit;
-    for (int b = 0; b < 10; b++) t->group_log[b] = T.group_log[b];
-}
-
-GDP_RETCODE autotune_views(const std::vector<gdp::InstanceView> &v, const double *lbs, const double *ubs,
-                           const int sync_type, gdp_tuning *tuning, double *best_ms, const char *who) {
-    try {
-        const gdp::Tuning start = tuning_from(tuning);
-        std::vector<gdp::AutotuneTrial> trials;
-        const gdp::Tuning best = gdp::autotune(v, lbs, ubs, sync_type, &start, &trials);
-        double ms = 0.0;
-        for (const auto &t : trials)
-            if (std::memcmp(&t.tuning, &best, sizeof(best)) == 0) ms = t.kernel_ms;
-        tuning_to(best, tuning);
-        if (best_ms) *best_ms = ms;
-        return GDP_OKAY;
-    } catch (const std::exception &e) {
-        return fail(who, e);
-    }
-}
-
-}  // namespace
 
 GDP_RETCODE gdpAutotune(const int n_cons, const int n_vars, const int nnz, const int *col_indices, const int *row_ptrs,
                         const double *vals, const double *lhss, const double *rhss, const int *vartypes,
@@ -846,14 +858,6 @@ GDP_RETCODE gdpOMPPropagate(const int n_cons, const int n_vars, const int nnz, c
     return host_double(2, n_cons, n_vars, nnz, col_indices, row_ptrs, vals, lhss, rhss, lbs, ubs, vartypes,
                        max_rounds, threads, info);
 }
-
-namespace {
-GDP_RETCODE hip_rc(hipError_t e, const char *what) {
-    if (e == hipSuccess) return GDP_OKAY;
-    gdp::set_last_error(std::string(what) + ": " + hipGetErrorString(e));
-    return GDP_ERROR;
-}
-}  // namespace
 
 GDP_RETCODE gdpDeviceCount(int *count) {
     if (!count) return GDP_ERROR;

@@ -32,6 +32,12 @@ def test_exports_every_declared_symbol(gdp):
     lib = gdp.load_library()
     for name in declared:
         assert hasattr(lib, name)
+    # nothing else with C linkage: internal helpers would collide with a
+    # client's symbols of the same name in a drop-in library
+    unmangled = {s for s in exported if not s.startswith("_Z")}
+    stray = {s for s in unmangled if s not in I.LEGACY_SYMBOLS and not s.startswith("gdp")}
+    assert not stray, stray
+    assert unmangled == declared, unmangled ^ declared
 
 
 def test_library_has_gfx950_code(gdp):
