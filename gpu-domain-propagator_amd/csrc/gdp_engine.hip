@@ -24,6 +24,7 @@
 #include <vector>
 
 #include "gdp_arith.h"
+#include "gdp_copy_pool.h"
 #include "gdp_device.h"
 #include "gdp_engine.h"
 #include "gdp_host.h"
@@ -96,7 +97,6 @@ T *dev_alloc(size_t count, std::vector<void *> &owned) {
 // teardown at process exit.
 constexpr size_t kStageChunk = 32u << 20;
 constexpr size_t kBulkMin = 1u << 20;
-constexpr size_t kCopyBlock = 256u << 10;
 
 class Staging {
   public:
@@ -112,7 +112,7 @@ class Staging {
         std::lock_guard<std::mutex> use_lk(use);
         std::lock_guard<std::mutex> lk(life_);
         if (--users_ > 0) return;
-        stop_threads();
+        pool_.stop();
         for (char *&b : buf) {
             if (b) (void)hipHostFree(b);
             b = nullptr;
@@ -128,92 +128,17 @@ class Staging {
             }
         return true;
     }
-    // With `use` held: copies `bytes` from src to dst on the calling thread
-    // and the copy threads (256 KiB blocks dealt out in turn).  Every copy is
-    // a new job generation; each copy thread takes each generation once.
+    // With `use` held: copies `bytes` from src to dst with up to 8 host threads.
     void copy(void *dst, const void *src, size_t bytes) {
-        start_threads();
-        int parts;
-        {
-            std::lock_guard<std::mutex> lk(job_mu_);
-            job_dst_ = static_cast<char *>(dst);
-            job_src_ = static_cast<const char *>(src);
-            job_bytes_ = bytes;
-            job_parts_ = parts = static_cast<int>(workers_.size()) + 1;
-            job_left_ = parts - 1;
-            job_gen_++;
-        }
-        job_cv_.notify_all();
-        copy_part(static_cast<char *>(dst), static_cast<const char *>(src), bytes, 0, parts);
-        std::unique_lock<std::mutex> lk(job_mu_);
-        done_cv_.wait(lk, [this] { return job_left_ == 0; });
+        pool_.copy(dst, src, bytes, std::max(1, std::min(8, static_cast<int>(std::thread::hardware_concurrency()))));
     }
 
   private:
     std::mutex life_;
     int users_ = 0;
-    std::vector<std::thread> workers_;
-    std::mutex job_mu_;
-    std::condition_variable job_cv_, done_cv_;
-    char *job_dst_ = nullptr;
-    const char *job_src_ = nullptr;
-    size_t job_bytes_ = 0;
-    int job_parts_ = 1;
-    int job_left_ = 0;
-    unsigned long long job_gen_ = 0;
-    bool quit_ = false;
-
-    static void copy_part(char *d, const char *s, size_t bytes, int part, int parts) {
-        const size_t blocks = (bytes + kCopyBlock - 1) / kCopyBlock;
-        for (size_t b = static_cast<size_t>(part); b < blocks; b += static_cast<size_t>(parts)) {
-            const size_t off = b * kCopyBlock;
-            std::memcpy(d + off, s + off, std::min(kCopyBlock, bytes - off));
-        }
-    }
-    // Threads start at the current generation: a job issued before they
-    // existed is not theirs (its count was complete without them).
-    void start_threads() {
-        if (!workers_.empty()) return;
-        const int n = std::max(0, std::min(8, static_cast<int>(std::thread::hardware_concurrency())) - 1);
-        unsigned long long gen;
-        {
-            std::lock_guard<std::mutex> lk(job_mu_);
-            quit_ = false;
-            gen = job_gen_;
-        }
-        for (int t = 0; t < n; t++) workers_.emplace_back([this, t, gen] { worker(t + 1, gen); });
-    }
-    void stop_threads() {
-        {
-            std::lock_guard<std::mutex> lk(job_mu_);
-            quit_ = true;
-        }
-        job_cv_.notify_all();
-        for (auto &w : workers_) w.join();
-        workers_.clear();
-    }
-    void worker(int part, unsigned long long seen) {
-        for (;;) {
-            char *d;
-            const char *s;
-            size_t bytes;
-            int parts;
-            {
-                std::unique_lock<std::mutex> lk(job_mu_);
-                job_cv_.wait(lk, [&] { return quit_ || job_gen_ != seen; });
-                if (quit_) return;
-                seen = job_gen_;
-                d = job_dst_;
-                s = job_src_;
-                bytes = job_bytes_;
-                parts = job_parts_;
-            }
-            copy_part(d, s, bytes, part, parts);
-            std::lock_guard<std::mutex> lk(job_mu_);
-            if (--job_left_ == 0) done_cv_.notify_all();
-        }
-    }
+    CopyPool pool_;
 };
+
 Staging &staging() {
     static Staging st;
     return st;
