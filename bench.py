@@ -79,31 +79,84 @@ def processed_bytes(rows, nnz, cols):
     return int(12 * sum(nnz) + 20 * sum(rows) + 20 * sum(cols))
 
 
+def host_facts():
+    """The host this run's CPU baselines ran on."""
+    model, cores = None, set()
+    phys = core = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "model name" and model is None:
+                    model = v
+                elif k == "physical id":
+                    phys = v
+                elif k == "core id":
+                    core = v
+                elif not k and phys is not None:
+                    cores.add((phys, core))
+                    phys = core = None
+        if phys is not None:
+            cores.add((phys, core))
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count()
+    return {"nproc": os.cpu_count(), "usable_cpus": usable, "model_name": model,
+            "physical_cores": len(cores) or None, "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS")}
+
+
 def cpu_baseline(inst, runs):
+    """cpu_seq (the oracle's C restatement), one thread, timed like the
+    reference: the round loop alone, without the column index build and the
+    allocations (sequential_propagator.h:248-281)."""
     import oracle_binding
-    times, rounds = [], 0
+    loops, walls, rounds, res = [], [], 0, None
     for _ in range(runs):
         t0 = time.perf_counter()
-        r = oracle_binding.seq(inst)
-        times.append(time.perf_counter() - t0)
-        rounds = r[2]
-    t = statistics.median(times)
+        res, loop_s = oracle_binding.seq_timed(inst)
+        walls.append(time.perf_counter() - t0)
+        loops.append(loop_s)
+        rounds = res[2]
+    t = statistics.median(loops)
     return {"value": inst.nnz * rounds / t, "unit": "nnz/s", "cores": 1, "kind": "port",
             "sample": f"cpu_seq (the oracle's C restatement: Gauss-Seidel rounds with row marking) on the full "
-                      f"workload, median of {runs} runs; {rounds} rounds to the fixpoint in {t:.3f} s "
+                      f"workload, median of {runs} runs of the round loop alone (no CSC build or allocation, as the "
+                      f"reference times it); {rounds} rounds to the fixpoint in {t:.3f} s "
                       f"(value = nnz x its rounds / time)",
-            "time_to_fixpoint_ms": t * 1e3, "rounds": rounds}
+            "time_to_fixpoint_ms": t * 1e3, "wall_with_setup_ms": statistics.median(walls) * 1e3,
+            "rounds": rounds}, res
+
+
+def deviation_vs_seq(inst, lbs, ubs, seq):
+    """Integer mismatches and continuous deviation (relative or absolute) of
+    the GPU fixpoint against cpu_seq's."""
+    import numpy as np
+    is_int = inst.vartype != 3
+    bad_int = int(np.sum(lbs[is_int] != seq[0][is_int]) + np.sum(ubs[is_int] != seq[1][is_int]))
+    worst, beyond = 0.0, 0
+    for g, sv in ((lbs, seq[0]), (ubs, seq[1])):
+        d = np.abs(g - sv)[~is_int] / np.maximum(1.0, np.abs(sv[~is_int]))
+        d[g[~is_int] == sv[~is_int]] = 0.0
+        worst = max(worst, float(d.max()) if d.size else 0.0)
+        beyond += int(np.sum(d > 1e-9))
+    return bad_int, worst, beyond
 
 
 def batched_parity(out):
     """Every batched instance against the round-synchronous oracle (bit for
     bit: bounds, rounds, flags) and, where neither run hit the round cap,
-    against its own cpu_seq fixpoint (integer bounds exact, continuous within
-    1e-9 relative or absolute)."""
+    against its own cpu_seq fixpoint: integer bounds exact, continuous within
+    1e-9 relative or absolute, and each fixpoint one of the other
+    propagator (a round of either started from the other's accepts no
+    candidate)."""
     import numpy as np
     import oracle_binding
     insts, o, out_l, out_u, infos = out["insts"], out["offsets"], out["lb"], out["ub"], out["infos"]
-    exact, bad_int, worst, feas, compared, capped, outside = 0, 0, 0.0, 0, 0, 0, 0
+    exact, bad_int, worst, feas, compared, capped, outside, mutual = 0, 0, 0.0, 0, 0, [], [], 0
     for i, x in enumerate(insts):
         gl, gu = out_l[o[i]:o[i + 1]], out_u[o[i]:o[i + 1]]
         rounds, infeasible, is_capped = infos[i]
@@ -113,20 +166,25 @@ def batched_parity(out):
         seq = oracle_binding.seq(x)
         feas += infeasible == bool(seq[3])
         if is_capped or seq[4] or seq[3]:
-            capped += bool(is_capped or seq[4])
+            if is_capped or seq[4]:
+                capped.append(i)
             continue
         compared += 1
-        is_int = x.vartype != 3
-        bad_int += int(np.sum(gl[is_int] != seq[0][is_int]) + np.sum(gu[is_int] != seq[1][is_int]))
-        for g, sv in ((gl, seq[0]), (gu, seq[1])):
-            d = np.abs(g - sv)[~is_int] / np.maximum(1.0, np.abs(sv[~is_int]))
-            worst = max(worst, float(d.max()) if d.size else 0.0)
-            outside += int(np.sum(d > 1e-9))
+        b, w, beyond = deviation_vs_seq(x, gl, gu, seq)
+        bad_int += b
+        worst = max(worst, w)
+        if beyond:
+            outside.append({"instance": i, "bounds": beyond, "max_dev": w})
+        again = oracle_binding.seq(x, lb=gl, ub=gu)
+        back = oracle_binding.jacobi(x, lb=seq[0], ub=seq[1])
+        mutual += bool(again[2] == 1 and np.array_equal(again[0], gl) and np.array_equal(again[1], gu) and
+                       back[2] == 1 and np.array_equal(back[0], seq[0]) and np.array_equal(back[1], seq[1]))
     return {"bitwise_equal_to_oracle": exact, "of": len(insts),
             "vs_cpu_seq": {"compared": compared, "capped_at_100_rounds": capped,
                            "integer_bound_mismatches": bad_int,
                            "max_continuous_rel_or_abs_dev": worst,
-                           "continuous_bounds_beyond_1e-9": outside,
+                           "instances_with_continuous_bounds_beyond_1e-9": outside,
+                           "mutual_epsilon_fixpoints": mutual,
                            "infeasibility_agrees": feas == len(insts)}}
 
 
@@ -140,18 +198,26 @@ def cpu_leg(inst, out_lb, out_ub, rounds, runs, batched_out):
     parity = {"bitwise_equal_to_oracle": bool(np.array_equal(out_lb, exp[0]) and np.array_equal(out_ub, exp[1])),
               "rounds_equal": exp[2] == rounds}
     print("bench: timing cpu baselines", file=sys.stderr, flush=True)
-    base = cpu_baseline(inst, runs)
+    base, seq = cpu_baseline(inst, runs)
+    bad_int, worst, beyond = deviation_vs_seq(inst, out_lb, out_ub, seq)
+    parity["vs_cpu_seq"] = {"integer_bound_mismatches": bad_int, "max_continuous_rel_or_abs_dev": worst,
+                            "continuous_bounds_beyond_1e-9": beyond, "infeasibility_agrees": bool(seq[3]) is False,
+                            "rounds_cpu_seq": seq[2], "rounds_gpu": rounds}
+    print(f"bench: vs cpu_seq: {bad_int} integer mismatches, max continuous deviation {worst:.3e}",
+          file=sys.stderr, flush=True)
     batched = batched_parity(batched_out) if batched_out is not None else None
     return parity, base, batched
 
 
 def omp_baseline(inst, threads):
+    """The library's cpu_omp; time of the round loop alone (wall time with
+    the column index build beside it)."""
     from gdp_amd import interface as I
     t0 = time.perf_counter()
     out = I.host_propagate(inst, "omp", threads=threads)
     t = time.perf_counter() - t0
-    return {"threads": threads, "rounds": out.rounds, "time_to_fixpoint_ms": t * 1e3,
-            "value": inst.nnz * out.rounds / t, "unit": "nnz/s"}
+    return {"threads": threads, "rounds": out.rounds, "time_to_fixpoint_ms": out.kernel_ms,
+            "wall_with_setup_ms": t * 1e3, "value": inst.nnz * out.rounds / (out.kernel_ms * 1e-3), "unit": "nnz/s"}
 
 
 def load_traffic(inst, rounds, kernel):
@@ -520,6 +586,55 @@ def measured_copy_gbs(nbytes=1 << 30, reps=5):
     return 2.0 * nbytes / best / 1e9
 
 
+def workload_leg(inst, sync, steps, label, fixed_cost_rounds=0):
+    """One more workload propagated to its fixpoint: kernel time per
+    propagation (mean over `steps` after a warmup, inputs resident), rounds,
+    nnz x rounds / s and the bytes of the rows each round processes over the
+    kernel time.  With fixed_cost_rounds = k, also the time of each of the
+    last k rounds, from runs capped at R - k, ..., R rounds (R: the rounds to
+    the fixpoint) and differenced: those rounds process few rows, so their
+    time is mostly the fixed cost of a round (barriers, publish, item scan)."""
+    from gdp_amd import interface as I
+    print(f"bench: {label}: m={inst.m} n={inst.n} nnz={inst.nnz}", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    prob = I.Problem(inst)
+    create_ms = (time.perf_counter() - t0) * 1e3
+    root_l, root_u = I.DeviceArray(inst.lb), I.DeviceArray(inst.ub)
+    dl, du = I.DeviceArray(inst.lb), I.DeviceArray(inst.ub)
+
+    def run(max_rounds=100):
+        dl.copy_from(root_l)
+        du.copy_from(root_u)
+        return prob.propagate_device(dl.ptr, du.ptr, sync, max_rounds)
+
+    run()
+    infos = [run() for _ in range(steps)]
+    rounds = infos[0].rounds
+    ms = statistics.mean(i.kernel_ms for i in infos)
+    wp, w_rows, w_nnz, w_cols, _ = prob.work_profile(inst.lb, inst.ub)
+    alg = processed_bytes(w_rows, w_nnz, w_cols)
+    out = {"workload": label, "m": inst.m, "n": inst.n, "nnz": inst.nnz, "sync": SYNC_NAMES[sync],
+           "rounds": rounds, "infeasible": bool(infos[0].infeasible), "kernel_ms": ms, "ms_per_round": ms / rounds,
+           "nnz_rounds_per_s": inst.nnz * rounds / (ms * 1e-3), "problem_create_ms": create_ms,
+           "processed_bytes_per_launch": alg, "achieved_gbs": alg / (ms * 1e-3) / 1e9,
+           "frac": alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+           "rows_processed_per_round": [int(x) for x in w_rows]}
+    if fixed_cost_rounds and rounds > fixed_cost_rounds + 1:
+        caps = list(range(rounds - fixed_cost_rounds - 1, rounds + 1))
+        t = {c: statistics.median(run(c).kernel_ms for _ in range(3)) for c in caps}
+        per = [(t[c] - t[c - 1]) * 1e3 for c in caps[1:]]
+        out["last_rounds_us"] = per
+        out["last_rounds_rows"] = [int(w_rows[c - 1]) for c in caps[1:]]
+        out["fixed_round_cost_us"] = statistics.median(per)
+    for a in (root_l, root_u, dl, du):
+        a.free()
+    prob.close()
+    print(f"bench: {label}: {rounds} rounds, {ms:.3f} ms, {out['ms_per_round'] * 1e3:.1f} us/round"
+          + (f", last rounds {out['fixed_round_cost_us']:.1f} us each" if "fixed_round_cost_us" in out else ""),
+          file=sys.stderr, flush=True)
+    return out
+
+
 def end_to_end_leg(inst, out_lb, out_ub, sync, runs=3):
     """The drop-in call as a caller of the reference's C interface makes it
     (propagateConstraintsGPUAtomicDouble): host arrays in, host arrays out,
@@ -566,6 +681,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the host baselines and the parity check")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-instances leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the other single-instance workloads (literal Zipf(1.8) shape, cover/knapsack, chains)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB")
     ap.add_argument("--summarize", default=None, metavar="DIR")
     ap.add_argument("--meta-out", default=None, help="write workload facts for --summarize")
@@ -680,6 +797,25 @@ def main():
     result["end_to_end"] = end_to_end_leg(inst, out_lb, out_ub, args.sync)
     prob.close()
     del prob, lb, ub, root_lb, root_ub
+    if not args.no_extra:
+        from gdp_amd.instances import chain_mip, cover_knapsack
+        extra = {}
+        x = powerlaw_mip(seed=args.seed, s=1.8)
+        extra["powerlaw_zipf_1_8"] = workload_leg(
+            x, args.sync, 5, f"powerlaw_mip m=1e6 n=5e5 zipf_s=1.8 row_len=[2,50000] seed={args.seed} "
+                             f"(the literal shape: long rows carry most nonzeros)")
+        del x
+        x = cover_knapsack(one_frac=0.0)
+        extra["cover_knapsack"] = workload_leg(
+            x, args.sync, 5, "cover_knapsack m=4e5 n=2e6, 2% of the binaries fixed to 0 (as specified it "
+                             "cannot chain: 2 rounds)")
+        del x
+        x = chain_mip()
+        extra["chains"] = workload_leg(
+            x, args.sync, 5, "chain_mip m=4e5 n=2e6 row_len=[2,60] 2% pre-fixed (the many-round workload)",
+            fixed_cost_rounds=8)
+        del x
+        result["other_workloads"] = extra
     batched_out = None
     if not args.no_batch:
         result["batched"], batched_out = batched_leg(args.sync)
@@ -689,9 +825,15 @@ def main():
         parity, base, bparity = cpu_leg(inst, out_lb, out_ub, rounds, args.cpu_runs, batched_out)
         result["parity"] = parity
         result["cpu_baseline"] = base
-        # the host cores of this GPU's share, and the reference's compile-time
-        # 8 threads (SHARED_MEM_THREADS)
-        result["cpu_omp"] = omp_baseline(inst, min(16, os.cpu_count() or 8))
+        host = host_facts()
+        result["host"] = host
+        # cpu_omp on the CPUs this process may use (the GPU's share of the
+        # host, at most its physical cores), and on the reference's
+        # compile-time 8 threads (SHARED_MEM_THREADS)
+        share = min(x for x in (host["usable_cpus"], host["physical_cores"], int(host["omp_num_threads_env"] or 0)
+                                or None) if x)
+        result["cpu_omp"] = omp_baseline(inst, share)
+        result["cpu_omp"]["threads_are"] = "the usable CPUs of this process (the GPU's share of the host)"
         result["cpu_omp_8_threads"] = omp_baseline(inst, 8)
         if bparity is not None:
             result["batched"]["parity"] = bparity

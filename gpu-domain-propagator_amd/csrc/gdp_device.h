@@ -47,6 +47,9 @@
 //                  bounds cross)
 //   bm[nwords]     uint32  columns moved in the current step (scanned by publish)
 //   marks[m']      int32   step in which the row must be processed
+//   item_mark[]    int32   per group item: step in which one of its rows must be
+//                          processed (set with the row marks), so that a
+//                          step with few marked rows visits few items
 //   split rows     seg_parts, seg_state, seg_count, seg_marks: a marked split
 //                  row recomputes only the segments holding a column that
 //                  moved in the previous step (and segments not yet computed
@@ -151,6 +154,10 @@ struct DevMatrix {
     const int *col_ptr;    // csc of the internal ids (marking)
     const int *col_rows;
     const int *col_seg;    // per csc entry: its segment of a split row (-1: other rows)
+    const int *col_item;   // per csc entry: the group item holding its row (-1: a split or
+                           // unsplit long row, whose items test the row mark itself)
+    const int *item_row;   // per item: the row whose mark a seg / long item tests (-1: group
+                           // item, which tests its own item mark)
     const int *perm;       // internal id -> caller's column (-1: padding id)
     const int *word_unit;  // blocks mode: unit of each bitmap word
     int snap_off32;        // byte offset of S32 in a unit's snapshot block
@@ -167,6 +174,7 @@ struct DevState {
     double2 *AF;
     unsigned *bm;
     int *marks;
+    int *item_mark;        // per group item: step in which one of its rows must be processed
     SegPart *seg_parts;
     int *seg_marks;        // step in which a split row's segment must be recomputed
     SegState *seg_state;
@@ -196,6 +204,7 @@ __device__ __forceinline__ DevState unit_view(const DevMatrix &M, const DevState
     V.AF += k * (M.n - M.n_int);
     V.bm += k * M.nwords;
     V.marks += k * M.m;
+    V.item_mark += k * M.n_items;
     V.seg_parts += k * M.n_seg_parts;
     V.seg_marks += k * M.n_seg_parts;
     V.seg_state += k * M.n_seg_rows;

@@ -45,6 +45,7 @@ __global__ void unpack_kernel(DevMatrix M, DevState D, int n_list, const int *li
                               double *const *ubp);
 __global__ void delta_kernel(int n_list, double *const *lbp, double *const *ubp, const int *ptr, const int *cols,
                              const double *lv, const double *uv);
+__global__ void copy_pairs_kernel(long long n2, int count, double *const *ptrs);
 __global__ void diff_kernel(int n, int n_list, double *const *lbp, double *const *ubp, const double *const *blp,
                             const double *const *bup, int cap, int *cnt, int *cols, double *vl, double *vu);
 __global__ void compact_changes_kernel(int nd, int cap, const int *cnt, const int *cols, const double *vl,
@@ -53,8 +54,8 @@ __global__ void layout_rows_kernel(int mp, const long long *src_start, const int
                                    const int *col_in, const double *val_in, const int *row_ptr_out,
                                    const int *internal, const int *vartype, int *col_out, double *val_out,
                                    int *col_cnt, int *bad);
-__global__ void layout_csc_kernel(int mp, const int *row_ptr_out, const int *col_out, const int *row_part, int *fill,
-                                  int *col_rows, int *col_seg);
+__global__ void layout_csc_kernel(int mp, const int *row_ptr_out, const int *col_out, const int *row_part,
+                                  const int *row_item, int *fill, int *col_rows, int *col_seg, int *col_item);
 __global__ void measure_kernel(int n, const double *lb, const double *ub, const double *wl, const double *wu,
                                const double *ll, const double *lu, double *pl, double *pu, double *fin_part,
                                int *inf_part);
@@ -347,6 +348,7 @@ struct Problem {
     int cu_count = 0;
     int mega_grid = 0;
     int mega_threads = 0;             // workgroup size of the persistent kernel
+    int sweep_grid = 0;               // workgroups of the step sweep kernel (all resident)
     const void *mega_fn = nullptr;
     // captured GPU_LOOP graphs, one per chunk of steps, for one launch shape
     std::vector<hipGraphExec_t> graphs;
@@ -406,6 +408,7 @@ size_t carve_state(const DevMatrix &M, bool replicas, int n_inst, int cap, char 
     D.AF = reinterpret_cast<double2 *>(take(k * nf * sizeof(double2)));
     D.bm = reinterpret_cast<unsigned *>(take(k * M.nwords * sizeof(unsigned)));
     D.marks = reinterpret_cast<int *>(take(k * M.m * sizeof(int)));
+    D.item_mark = reinterpret_cast<int *>(take(k * M.n_items * sizeof(int)));
     D.seg_parts = reinterpret_cast<SegPart *>(take(k * M.n_seg_parts * sizeof(SegPart)));
     D.seg_marks = reinterpret_cast<int *>(take(k * M.n_seg_parts * sizeof(int)));
     D.seg_state = reinterpret_cast<SegState *>(take(k * M.n_seg_rows * sizeof(SegState)));
@@ -464,8 +467,7 @@ void alloc_units(Problem *P, int cap) {
     P->stats.n_units = P->n_units;
 }
 
-// The run state of unit 0 exists (allocated on first use: a problem that is
-// only run through the scratch entry point never allocates it).
+// The run state of unit 0 exists (allocated on first use).
 void ensure_units(Problem *P) {
     if (P->capacity < 1) alloc_units(P, 1);
 }
@@ -478,8 +480,9 @@ void ensure_units(Problem *P) {
 // range are reported here.  Synchronous.
 void device_layout_build(const std::vector<InstanceView> &inst, const std::vector<int> &col_off, int mp, int n_ids,
                          int n_padded, const std::vector<long long> &src_start, const std::vector<int> &row_inst,
-                         const std::vector<int> &row_part, const int *d_rp, const int *d_internal, int *d_col,
-                         double *d_val, int *d_cptr, int *d_crows, int *d_cseg, hipStream_t s) {
+                         const std::vector<int> &row_part, const std::vector<int> &row_item, const int *d_rp,
+                         const int *d_internal, int *d_col, double *d_val, int *d_cptr, int *d_crows, int *d_cseg,
+                         int *d_citem, hipStream_t s) {
     const int NI = static_cast<int>(inst.size());
     long long nnz = 0;
     for (const auto &v : inst) nnz += v.nnz;
@@ -491,6 +494,7 @@ void device_layout_build(const std::vector<InstanceView> &inst, const std::vecto
         int *t_inst = NI > 1 ? dev_alloc<int>(mp, tmp) : nullptr;
         int *t_coff = dev_alloc<int>(NI + 1, tmp);
         int *t_part = dev_alloc<int>(mp, tmp);
+        int *t_item = dev_alloc<int>(mp, tmp);
         int *t_vt = dev_alloc<int>(col_off[NI], tmp);
         int *t_cnt = dev_alloc<int>(n_ids + 1, tmp);  // counts, then the fill positions
         long long at = 0;
@@ -504,6 +508,7 @@ void device_layout_build(const std::vector<InstanceView> &inst, const std::vecto
         if (t_inst) upload(t_inst, row_inst.data(), mp, s);
         upload(t_coff, col_off.data(), NI + 1, s);
         upload(t_part, row_part.data(), mp, s);
+        upload(t_item, row_item.data(), mp, s);
         check(hipMemsetAsync(d_col, 0, sizeof(int) * n_padded, s), "memset layout");
         check(hipMemsetAsync(d_val, 0, sizeof(double) * n_padded, s), "memset layout");
         check(hipMemsetAsync(t_cnt, 0, sizeof(int) * (n_ids + 1), s), "memset counts");
@@ -519,8 +524,8 @@ void device_layout_build(const std::vector<InstanceView> &inst, const std::vecto
         for (int c = 0; c < n_ids; c++) cptr[c + 1] = cptr[c] + cnt[c];
         upload(d_cptr, cptr.data(), n_ids + 1, s);
         upload(t_cnt, cptr.data(), n_ids, s);
-        hipLaunchKernelGGL(layout_csc_kernel, dim3(grid), dim3(256), 0, s, mp, d_rp, d_col, t_part, t_cnt, d_crows,
-                           d_cseg);
+        hipLaunchKernelGGL(layout_csc_kernel, dim3(grid), dim3(256), 0, s, mp, d_rp, d_col, t_part, t_item, t_cnt,
+                           d_crows, d_cseg, d_citem);
         check(hipGetLastError(), "layout launch");
         check(hipStreamSynchronize(s), "layout");
     } catch (...) {
@@ -728,6 +733,19 @@ Problem *build(const std::vector<InstanceView> &inst, bool replicas, const Tunin
         // of every column-to-row entry
         std::vector<int> row_part(mp, -1);
         for (const int4 &sr : h_seg_rows) row_part[sr.x] = sr.y;
+        // per item: the row whose mark a seg / long item tests (-1: group
+        // item); per row: the group item that holds it (-1: split or long row)
+        std::vector<int> h_item_row(h_items.size(), -1), row_item(mp, -1);
+        for (size_t q = 0; q < h_items.size(); q++) {
+            const int4 it = h_items[q];
+            const int kind = it.x & 15;
+            if (kind == kItemSeg)
+                h_item_row[q] = h_seg_rows[it.y].x;
+            else if (kind == kItemLong)
+                h_item_row[q] = it.y;
+            else
+                for (int t = 0; t < it.z; t++) row_item[it.y + t] = static_cast<int>(q);
+        }
 
         P->stats.m = P->m;
         P->stats.n = P->n;
@@ -768,19 +786,22 @@ Problem *build(const std::vector<InstanceView> &inst, bool replicas, const Tunin
         const size_t n_csc = std::max<long long>(1, nnz_total);
         int *d_crows = dev_alloc<int>(n_csc, o);
         int *d_cseg = dev_alloc<int>(n_csc, o);
+        int *d_citem = dev_alloc<int>(n_csc, o);
+        int *d_item_row = dev_alloc<int>(h_item_row.size(), o);
         int *d_perm = dev_alloc<int>(n_ids, o);
         int *d_word_unit = dev_alloc<int>(word_unit.size(), o);
         P->d_inv_perm = dev_alloc<int>(P->n, o);
         upload(d_rp, h_rp.data(), mp + 1, s);
         upload(d_sides, h_sides.data(), mp, s);
         upload(d_items, h_items.data(), h_items.size(), s);
+        upload(d_item_row, h_item_row.data(), h_item_row.size(), s);
         upload(d_seg_rows, h_seg_rows.data(), h_seg_rows.size(), s);
         upload(d_part_row, h_part_row.data(), h_part_row.size(), s);
         upload(d_perm, perm.data(), n_ids, s);
         upload(d_word_unit, word_unit.data(), word_unit.size(), s);
         upload(P->d_inv_perm, internal.data(), P->n, s);
-        device_layout_build(inst, col_off, mp, n_ids, n_padded, src_start, h_unit, row_part, d_rp, P->d_inv_perm, d_col,
-                            d_val, d_cptr, d_crows, d_cseg, s);
+        device_layout_build(inst, col_off, mp, n_ids, n_padded, src_start, h_unit, row_part, row_item, d_rp,
+                            P->d_inv_perm, d_col, d_val, d_cptr, d_crows, d_cseg, d_citem, s);
         M.row_ptr = d_rp;
         M.sides = d_sides;
         M.col = d_col;
@@ -791,6 +812,8 @@ Problem *build(const std::vector<InstanceView> &inst, bool replicas, const Tunin
         M.col_ptr = d_cptr;
         M.col_rows = d_crows;
         M.col_seg = d_cseg;
+        M.col_item = d_citem;
+        M.item_row = d_item_row;
         M.perm = d_perm;
         M.word_unit = d_word_unit;
 
@@ -820,6 +843,11 @@ Problem *build(const std::vector<InstanceView> &inst, bool replicas, const Tunin
         check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, P->mega_fn, P->mega_threads, 0), "occupancy");
         if (T.mega_blocks_per_cu > 0) per_cu = std::min(per_cu, T.mega_blocks_per_cu);
         P->mega_grid = std::max(1, per_cu) * P->cu_count;
+        int sweep_per_cu = 0;
+        check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&sweep_per_cu, reinterpret_cast<const void *>(sweep_kernel),
+                                                           256, 0),
+              "occupancy");
+        P->sweep_grid = std::max(1, sweep_per_cu) * P->cu_count;
         P->stats.mega_threads = P->mega_threads;
         P->stats.mega_grid = P->mega_grid;
     } catch (...) {
@@ -887,7 +915,8 @@ void launch_step(Problem *P, int step, int guard, int first_step) {
     const long long n_work = P->replicas ? static_cast<long long>(M.n_items) * D.n_active : M.n_items;
     const int apply_grid = std::max(1, std::min(1024, grid_for(static_cast<long long>(M.n_seg_parts) *
                                                                  (P->replicas ? D.n_active : 1))));
-    hipLaunchKernelGGL(sweep_kernel, dim3(grid_for(n_work)), dim3(256), 0, s, M, D, step, guard, first_step);
+    const int sweep_grid = static_cast<int>(std::min<long long>(grid_for(n_work), P->sweep_grid));
+    hipLaunchKernelGGL(sweep_kernel, dim3(sweep_grid), dim3(256), 0, s, M, D, step, guard, first_step);
     if (M.n_seg_parts > 0)
         hipLaunchKernelGGL(apply_split_kernel, dim3(apply_grid), dim3(256), 0, s, M, D, step, guard, first_step);
     const long long pub_waves = static_cast<long long>(P->replicas ? D.n_active : 1) * ((M.nwords + 7) / 8);
@@ -1500,6 +1529,12 @@ struct Server {
     // tickets.  The device work of a step runs without it.
     std::mutex mu;
     std::condition_variable cv;
+    // Back-pressure while the stepping thread runs: submitters wait here for
+    // room in the queue (queue_cap requests) and, for copied requests, for a
+    // free pinned inbox entry; woken when a step admits requests or frees
+    // entries.
+    std::condition_variable space_cv;
+    size_t queue_cap = 0;
     std::thread worker;
     bool running = false, stop = false;
     std::string worker_error;
@@ -1520,6 +1555,7 @@ struct Server {
                                          // only the changed columns come back
     double *d_orig = nullptr;            // slots x (lb, ub): inputs of in-place requests
     double **d_blp = nullptr, **d_bup = nullptr;  // base vector of each diffed slot
+    double **d_cp = nullptr;             // admission copies: sources, then destinations
     // changed columns of finished delta requests (device and pinned host)
     int *d_ccnt = nullptr, *d_ccols = nullptr;
     double *d_cl = nullptr, *d_cu = nullptr;
@@ -1536,6 +1572,7 @@ struct Server {
     // prefetch: request vectors from pinned memory are copied at submission
     // into device staging entries on a copy stream, overlapping the steps
     hipStream_t cs = nullptr;
+    hipEvent_t cs_tail = nullptr;        // recorded at the copy stream's tail by a step
     double *d_stage = nullptr;
     std::vector<hipEvent_t> stage_ev;
     std::vector<int> stage_free;
@@ -1588,9 +1625,11 @@ struct Server {
         for (void *q : {static_cast<void *>(d_dptr), static_cast<void *>(d_dcols), static_cast<void *>(d_dlb),
                         static_cast<void *>(d_dub), static_cast<void *>(d_dl), static_cast<void *>(d_du)})
             if (q) (void)hipFree(q);
-        for (void *q : {static_cast<void *>(d_orig), static_cast<void *>(d_blp), static_cast<void *>(d_bup)})
+        for (void *q : {static_cast<void *>(d_orig), static_cast<void *>(d_blp), static_cast<void *>(d_bup),
+                        static_cast<void *>(d_cp)})
             if (q) (void)hipFree(q);
         for (hipEvent_t e : stage_ev) (void)hipEventDestroy(e);
+        if (cs_tail) (void)hipEventDestroy(cs_tail);
         if (d_stage) (void)hipFree(d_stage);
         if (cs) (void)hipStreamDestroy(cs);
         if (attached) {
@@ -1617,6 +1656,7 @@ Server *server_create(Problem *P, int64_t hbm_budget, int max_slots) {
     try {
         S->P = P;
         S->slots = slots;
+        S->queue_cap = static_cast<size_t>(slots);
         S->n = static_cast<size_t>(P->n);
         check(hipMalloc(&S->d_buf, sizeof(double) * 2 * S->n * slots), "hipMalloc slots");
         check(hipMalloc(&S->d_base, sizeof(double) * 2 * S->n), "hipMalloc base");
@@ -1632,6 +1672,7 @@ Server *server_create(Problem *P, int64_t hbm_budget, int max_slots) {
         check(hipMalloc(&S->d_orig, sizeof(double) * 2 * S->n * slots), "hipMalloc inputs");
         check(hipMalloc(&S->d_blp, sizeof(double *) * slots), "hipMalloc");
         check(hipMalloc(&S->d_bup, sizeof(double *) * slots), "hipMalloc");
+        check(hipMalloc(&S->d_cp, sizeof(double *) * 4 * slots), "hipMalloc");
         // changed-column regions for finished delta requests (kDiffCap each)
         S->diff_slots = slots;
         const size_t dc = static_cast<size_t>(slots) * Server::kDiffCap;
@@ -1665,6 +1706,7 @@ Server *server_create(Problem *P, int64_t hbm_budget, int max_slots) {
         const int n_stage = static_cast<int>(
             std::max<int64_t>(1, std::min<int64_t>(slots, (4LL << 30) / static_cast<int64_t>(16 * S->n))));
         check(hipStreamCreateWithFlags(&S->cs, hipStreamNonBlocking), "hipStreamCreate");
+        check(hipEventCreateWithFlags(&S->cs_tail, hipEventDisableTiming), "hipEventCreate");
         check(hipMalloc(&S->d_stage, sizeof(double) * 2 * S->n * n_stage), "hipMalloc staging");
         for (int e = n_stage - 1; e >= 0; e--) {
             hipEvent_t ev;
@@ -1710,6 +1752,19 @@ void check_worker(const Server *S) {
     if (!S->worker_error.empty()) throw std::runtime_error("stepping thread: " + S->worker_error);
 }
 
+// With the server's lock held (lk): while the stepping thread runs, waits
+// until the queue has room and, for a copied request (inbox), a pinned
+// inbox entry is free.  Bounded admission: an overloaded server holds its
+// submitters back instead of queueing without limit.  Returns at once when
+// the thread is not running (a caller that steps by hand must not wait for
+// itself) or has stopped or failed.
+void wait_for_room(Server *S, std::unique_lock<std::mutex> &lk, bool inbox) {
+    S->space_cv.wait(lk, [S, inbox] {
+        return !S->running || S->stop || !S->worker_error.empty() ||
+               (S->queue.size() < S->queue_cap && (!inbox || !S->inbox_free.empty()));
+    });
+}
+
 // A free device staging entry (-1 if none); with the server's lock held.
 int take_stage(Server *S) {
     if (S->stage_free.empty()) return -1;
@@ -1750,7 +1805,11 @@ int64_t server_submit(Server *S, const double *lbs, const double *ubs, const int
     q.max_rounds = max_rounds;
     int stage = -1;
     {
-        std::lock_guard<std::mutex> lk(S->mu);
+        std::unique_lock<std::mutex> lk(S->mu);
+        check_worker(S);
+        // while the stepping thread runs, wait for room instead of holding the
+        // vectors in pageable memory, whose copies would stall the steps
+        wait_for_room(S, lk, true);
         check_worker(S);
         q.ticket = S->next_ticket++;
         if (!S->inbox_free.empty()) {
@@ -1794,7 +1853,9 @@ int64_t server_submit_pinned(Server *S, const double *lbs, const double *ubs, do
     q.max_rounds = max_rounds;
     int stage;
     {
-        std::lock_guard<std::mutex> lk(S->mu);
+        std::unique_lock<std::mutex> lk(S->mu);
+        check_worker(S);
+        wait_for_room(S, lk, false);
         check_worker(S);
         q.ticket = S->next_ticket++;
         stage = take_stage(S);
@@ -1820,7 +1881,9 @@ int64_t server_submit_delta(Server *S, int n_changes, const int *cols, const dou
     q.dub.assign(ubs, ubs + n_changes);
     q.max_rounds = max_rounds;
     {
-        std::lock_guard<std::mutex> lk(S->mu);
+        std::unique_lock<std::mutex> lk(S->mu);
+        check_worker(S);
+        wait_for_room(S, lk, false);
         check_worker(S);
         q.ticket = S->next_ticket++;
     }
@@ -1864,6 +1927,22 @@ void grow(T *&p, size_t count) {
 
 namespace {
 int step_impl(Server *S);
+
+// Copies bound-vector pairs (2n doubles each, lb then ub) from src[k] to
+// dst[k] on the problem's stream in one launch; the pointer table goes to
+// d_cp from entry `at`.
+void copy_pairs(Server *S, const std::vector<double *> &src, const std::vector<double *> &dst, int at) {
+    const int count = static_cast<int>(src.size());
+    if (!count) return;
+    std::vector<double *> table(src);
+    table.insert(table.end(), dst.begin(), dst.end());
+    hipStream_t s = S->P->stream;
+    upload(S->d_cp + at, table.data(), table.size(), s);
+    const long long n2 = 2LL * static_cast<long long>(S->n);
+    const int gx = static_cast<int>(std::max(1LL, std::min(256LL, (n2 / 2 + 255) / 256)));
+    hipLaunchKernelGGL(copy_pairs_kernel, dim3(gx, std::min(count, 1024)), dim3(256), 0, s, n2, count, S->d_cp + at);
+    check(hipGetLastError(), "copy launch");
+}
 }  // namespace
 
 int server_step(Server *S) {
@@ -1885,9 +1964,12 @@ void worker_loop(Server *S) {
             step_impl(S);
         }
     } catch (const std::exception &e) {
-        std::lock_guard<std::mutex> lk(S->mu);
-        S->worker_error = e.what();
-        S->stop = true;
+        {
+            std::lock_guard<std::mutex> lk(S->mu);
+            S->worker_error = e.what();
+            S->stop = true;
+        }
+        S->space_cv.notify_all();
     }
 }
 }  // namespace
@@ -1909,6 +1991,7 @@ void server_stop(Server *S) {
         S->stop = true;
     }
     S->cv.notify_all();
+    S->space_cv.notify_all();
     S->worker.join();
     std::string err;
     {
@@ -1944,13 +2027,20 @@ int step_impl(Server *S) {
             S->queue.pop_front();
         }
     }
+    if (!taken.empty()) S->space_cv.notify_all();  // room in the queue
+    // Device-side fills of the admitted slots go into two batched copies:
+    // (1) the base vector into delta requests' slots and prefetched staging
+    // entries into their slots, (2) in-place requests' inputs, kept for the
+    // diff at retirement, once every slot is filled.
+    std::vector<double *> cp1_src, cp1_dst, cp2_src, cp2_dst;
+    bool staged = false;
     for (auto &kq : taken) {
         const int k = kq.first;
         Server::Request &q = kq.second;
+        const bool in_place = q.pin_lb && q.out_lb == q.pin_lb && q.out_ub == q.pin_ub;
         if (q.delta) {
-            check(hipMemcpyAsync(S->slot_lb[k], S->d_base, n * sizeof(double), hipMemcpyDeviceToDevice, s), "D2D");
-            check(hipMemcpyAsync(S->slot_ub[k], S->d_base + n, n * sizeof(double), hipMemcpyDeviceToDevice, s),
-                  "D2D");
+            cp1_src.push_back(S->d_base);
+            cp1_dst.push_back(S->slot_lb[k]);
             del_lb.push_back(S->slot_lb[k]);
             del_ub.push_back(S->slot_ub[k]);
             dcols.insert(dcols.end(), q.cols.begin(), q.cols.end());
@@ -1959,23 +2049,14 @@ int step_impl(Server *S) {
             dptr.push_back(static_cast<int>(dcols.size()));
         } else if (q.stage >= 0) {
             // prefetched: the slot's (lb, ub) pair is contiguous, like the entry's
-            check(hipStreamWaitEvent(s, S->stage_ev[q.stage], 0), "wait");
-            check(hipMemcpyAsync(S->slot_lb[k], S->stage_lb(q.stage), 2 * n * sizeof(double), hipMemcpyDeviceToDevice,
-                                 s),
-                  "D2D");
+            cp1_src.push_back(S->stage_lb(q.stage));
+            cp1_dst.push_back(S->slot_lb[k]);
+            staged = true;
             used_stage.push_back(q.stage);
             if (q.inbox >= 0) used_inbox.push_back(q.inbox);
-            if (q.pin_lb && q.out_lb == q.pin_lb && q.out_ub == q.pin_ub)
-                check(hipMemcpyAsync(S->orig_lb(k), S->slot_lb[k], 2 * n * sizeof(double), hipMemcpyDeviceToDevice,
-                                     s),
-                      "D2D");
         } else if (q.pin_lb) {
             check(hipMemcpyAsync(S->slot_lb[k], q.pin_lb, n * sizeof(double), hipMemcpyHostToDevice, s), "H2D");
             check(hipMemcpyAsync(S->slot_ub[k], q.pin_ub, n * sizeof(double), hipMemcpyHostToDevice, s), "H2D");
-            if (q.out_lb == q.pin_lb && q.out_ub == q.pin_ub)
-                check(hipMemcpyAsync(S->orig_lb(k), S->slot_lb[k], 2 * n * sizeof(double), hipMemcpyDeviceToDevice,
-                                     s),
-                      "D2D");
         } else if (q.inbox >= 0) {
             check(hipMemcpyAsync(S->slot_lb[k], S->in_lb(q.inbox), n * sizeof(double), hipMemcpyHostToDevice, s),
                   "H2D");
@@ -1988,6 +2069,10 @@ int step_impl(Server *S) {
             keep.push_back(std::move(q.lb));
             keep.push_back(std::move(q.ub));
         }
+        if (in_place) {
+            cp2_src.push_back(S->slot_lb[k]);
+            cp2_dst.push_back(S->orig_lb(k));
+        }
         adm.push_back(k);
         adm_lb.push_back(S->slot_lb[k]);
         adm_ub.push_back(S->slot_ub[k]);
@@ -1997,8 +2082,15 @@ int step_impl(Server *S) {
         S->slot_delta[k] = q.delta ? 1 : 0;
         S->slot_out_lb[k] = q.out_lb;
         S->slot_out_ub[k] = q.out_ub;
-        S->slot_inplace[k] = q.pin_lb && q.out_lb == q.pin_lb && q.out_ub == q.pin_ub ? 1 : 0;
+        S->slot_inplace[k] = in_place ? 1 : 0;
     }
+    // every prefetch of an admitted request was issued on the copy stream
+    // before the request was queued: one event at the copy stream's tail
+    if (staged) {
+        check(hipEventRecord(S->cs_tail, S->cs), "event");
+        check(hipStreamWaitEvent(s, S->cs_tail, 0), "wait");
+    }
+    copy_pairs(S, cp1_src, cp1_dst, 0);
     if (!del_lb.empty()) {
         const size_t nd = del_lb.size(), nc = dcols.size();
         if (nc > S->dcap || !S->d_dl) {
@@ -2020,6 +2112,7 @@ int step_impl(Server *S) {
                            static_cast<int>(nd), S->d_dl, S->d_du, S->d_dptr, S->d_dcols, S->d_dlb, S->d_dub);
         check(hipGetLastError(), "delta launch");
     }
+    copy_pairs(S, cp2_src, cp2_dst, 2 * S->slots);
     if (!adm.empty())
         load_units(P, static_cast<int>(adm.size()), adm.data(), adm_lb.data(), adm_ub.data(), hint_ptr.data(),
                    hints.data(), caps.data(), S->step, false);
@@ -2039,6 +2132,7 @@ int step_impl(Server *S) {
         for (int e : used_stage) S->stage_free.push_back(e);
         for (auto &v : keep) S->give_vec(std::move(v));
     }
+    if (!used_inbox.empty()) S->space_cv.notify_all();
     keep.clear();
     int err = 0;
     check(hipMemcpy(&err, P->D.global + kGlobalError, sizeof(int), hipMemcpyDeviceToHost), "D2H");

@@ -20,6 +20,7 @@
 #include <omp.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstdlib>
 #include <cstring>
 #include <memory>
@@ -105,6 +106,11 @@ bool seq_round(int m, const int *col, const int *row_ptr, const double *val, con
     return change;
 }
 
+using Clock = std::chrono::steady_clock;
+double ms_since(Clock::time_point t0) {
+    return std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
+}
+
 }  // namespace
 
 HostResult seq_propagate(int m, int n, const int *col, const int *row_ptr, const double *val,
@@ -117,6 +123,7 @@ HostResult seq_propagate(int m, int n, const int *col, const int *row_ptr, const
     std::vector<unsigned char> chg(n, 0);
     bool change = true;
     int round;
+    const Clock::time_point t0 = Clock::now();
     for (round = 1; round <= max_rounds && change; round++) {
         std::fill(chg.begin(), chg.end(), 0);
         change = seq_round(m, col, row_ptr, val, lhs, rhs, lb, ub, vartype, csc, marked, next, chg, nullptr);
@@ -124,14 +131,17 @@ HostResult seq_propagate(int m, int n, const int *col, const int *row_ptr, const
         if (change && crossed_any(n, lb, ub, chg.data())) {
             res.infeasible = 1;
             res.rounds = round;
+            res.loop_ms = ms_since(t0);
             return res;
         }
         if (!go_on) {
             res.rounds = round;
             res.capped = change ? 1 : 0;
+            res.loop_ms = ms_since(t0);
             return res;
         }
     }
+    res.loop_ms = ms_since(t0);
     res.rounds = round - 1;
     res.capped = change ? 1 : 0;
     return res;
@@ -148,6 +158,7 @@ HostResult seq_dis_propagate(int m, int n, const int *col, const int *row_ptr, c
     std::vector<RowActs> acts(m);
     bool change = true;
     int round;
+    const Clock::time_point t0 = Clock::now();
     for (round = 1; round <= max_rounds && change; round++) {
         for (int i = 0; i < m; i++) acts[i] = row_activities(i, col, row_ptr, val, lb, ub);
         std::fill(chg.begin(), chg.end(), 0);
@@ -155,9 +166,11 @@ HostResult seq_dis_propagate(int m, int n, const int *col, const int *row_ptr, c
         if (change && crossed_any(n, lb, ub, chg.data())) {
             res.infeasible = 1;
             res.rounds = round;
+            res.loop_ms = ms_since(t0);
             return res;
         }
     }
+    res.loop_ms = ms_since(t0);
     res.rounds = round - 1;
     res.capped = change ? 1 : 0;
     return res;
@@ -190,6 +203,7 @@ HostResult omp_propagate(int m, int n, const int *col, const int *row_ptr, const
     // round fewer than cpu_seq's cap; the cap passed in is honoured as given.
     bool change = true;
     int round;
+    const Clock::time_point t0 = Clock::now();
     for (round = 1; round <= max_rounds && change; round++) {
         std::memcpy(lb_snap.data(), lb, n * sizeof(double));
         std::memcpy(ub_snap.data(), ub, n * sizeof(double));
@@ -230,6 +244,7 @@ HostResult omp_propagate(int m, int n, const int *col, const int *row_ptr, const
             break;
         }
     }
+    res.loop_ms = ms_since(t0);
     if (!res.infeasible) {
         res.rounds = round - 1;
         res.capped = change ? 1 : 0;

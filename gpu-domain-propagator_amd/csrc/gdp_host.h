@@ -19,6 +19,8 @@ struct HostResult {
     int rounds = 0;
     int infeasible = 0;
     int capped = 0;
+    double loop_ms = 0.0;  // wall time of the round loop alone (no CSC build or
+                           // allocation: sequential_propagator.h:248-281)
 };
 
 // Called after every round with the round number and the bounds at its

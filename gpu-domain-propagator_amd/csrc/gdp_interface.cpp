@@ -118,7 +118,7 @@ GDP_RETCODE host_double(int which, int m, int n, int nnz, const int *col, const 
             r = gdp::seq_dis_propagate(m, n, col, rp, val, lhs, rhs, lb, ub, vt, max_rounds);
         else
             r = gdp::omp_propagate(m, n, col, rp, val, lhs, rhs, lb, ub, vt, max_rounds, threads);
-        fill_info(info, r.rounds, r.infeasible, r.capped, 0.0);
+        fill_info(info, r.rounds, r.infeasible, r.capped, r.loop_ms);
         return GDP_OKAY;
     } catch (const std::exception &e) {
         return fail("host propagator", e);

@@ -458,14 +458,15 @@ __device__ __forceinline__ bool unit_works(int step, int start, int infeasible, 
            (step == start || (prev_changed == step - 1 && infeasible == 0));
 }
 
-__device__ __forceinline__ UnitStep unit_step(const DevMatrix &M, const DevState &D, int u, int step) {
+// unit_step for a different unit on every lane.
+__device__ __forceinline__ UnitStep unit_step_lane(const DevMatrix &M, const DevState &D, int u, int step) {
     const long long *p = reinterpret_cast<const long long *>(&D.units[u]);
     const long long w0 = coherent_load(p), w1 = coherent_load(p + 1), w2 = coherent_load(p + 2),
                     w3 = coherent_load(p + 3);
-    const int start = uniform(static_cast<int>(w0)), infeasible = uniform(static_cast<int>(w0 >> 32));
-    const int markall = uniform(static_cast<int>(w1)), max_rounds = uniform(static_cast<int>(w1 >> 32));
-    const int crossed = uniform(static_cast<int>(w2)), envelope = uniform(static_cast<int>(w2 >> 32));
-    const int prev_changed = uniform(static_cast<int>((step - 1) & 1 ? (w3 >> 32) : w3));
+    const int start = static_cast<int>(w0), infeasible = static_cast<int>(w0 >> 32);
+    const int markall = static_cast<int>(w1), max_rounds = static_cast<int>(w1 >> 32);
+    const int crossed = static_cast<int>(w2), envelope = static_cast<int>(w2 >> 32);
+    const int prev_changed = static_cast<int>((step - 1) & 1 ? (w3 >> 32) : w3);
     UnitStep r;
     r.active = unit_works(step, start, infeasible, max_rounds, crossed, envelope, prev_changed);
     r.all_rows = !M.mark_rows || markall == step;
@@ -473,14 +474,27 @@ __device__ __forceinline__ UnitStep unit_step(const DevMatrix &M, const DevState
     return r;
 }
 
-// unit_step's `active` for a different unit on every lane.
-__device__ __forceinline__ bool unit_works_lane(const DevState &D, int u, int step) {
-    const long long *p = reinterpret_cast<const long long *>(&D.units[u]);
-    const long long w0 = coherent_load(p), w1 = coherent_load(p + 1), w2 = coherent_load(p + 2),
-                    w3 = coherent_load(p + 3);
-    return unit_works(step, static_cast<int>(w0), static_cast<int>(w0 >> 32), static_cast<int>(w1 >> 32),
-                      static_cast<int>(w2), static_cast<int>(w2 >> 32),
-                      static_cast<int>((step - 1) & 1 ? (w3 >> 32) : w3));
+// The same unit on every lane of the wave (result in scalar registers).
+__device__ __forceinline__ UnitStep unit_step(const DevMatrix &M, const DevState &D, int u, int step) {
+    const UnitStep l = unit_step_lane(M, D, uniform(u), step);
+    UnitStep r;
+    r.active = uniform(l.active ? 1 : 0) != 0;
+    r.all_rows = uniform(l.all_rows ? 1 : 0) != 0;
+    r.start = uniform(l.start);
+    return r;
+}
+
+// Whether item b of a working unit (state V) has work in `step`: every item
+// when the unit processes all rows; otherwise a group item whose item mark,
+// or a seg / long item whose row mark, names the step.  Items still test
+// their rows one by one, so an item visited needlessly costs time, never a
+// different result.
+__device__ __forceinline__ bool item_has_work(const DevMatrix &M, const DevState &V, int b, int step,
+                                              bool all_rows) {
+    if (all_rows) return true;
+    const int row = M.item_row[b];
+    const int mark = row >= 0 ? coherent_load(&V.marks[row]) : coherent_load(&V.item_mark[b]);
+    return mark == step;
 }
 
 // Counts the wave's accepted candidates (`moved` per lane) of unit u in
@@ -709,20 +723,27 @@ __device__ __forceinline__ void seg_item(const DevMatrix &M, const DevState &D, 
     for (int q = lane; q < info.z; q += 64) coherent_store(&D.apply_tasks[pos + q], base + q);
 }
 
-// Sweep item `it` of `step`: decodes the unit, skips inactive units and
-// unmarked rows, runs the item and records the unit's change.
-__device__ __forceinline__ void sweep_item(const DevMatrix &M, const DevState &D, int it, int step, int *st) {
-    int u, b;
-    it = uniform(it);
+// Unit and item of work index `it` (replicas: the launch's listed units,
+// n_items each; blocks: the item, whose unit it names).
+__device__ __forceinline__ void decode_work(const DevMatrix &M, const DevState &D, int it, int &u, int &b) {
     if (M.replicas) {
-        u = uniform(D.active[it / M.n_items]);
-        b = it % M.n_items;
+        const int k = it / M.n_items;
+        u = D.active[k];
+        b = it - k * M.n_items;
     } else {
         b = it;
-        u = uniform(M.items[b].x >> kItemUnitShift);
+        u = M.items[b].x >> kItemUnitShift;
     }
-    const UnitStep us = unit_step(M, D, u, step);
-    if (!us.active) return;
+}
+
+// Sweep item `it` of `step` for a unit working in the step (us: its state):
+// skips unmarked rows, runs the item and records the unit's change.
+__device__ __forceinline__ void sweep_item(const DevMatrix &M, const DevState &D, int it, int step, int *st,
+                                           const UnitStep &us) {
+    int u, b;
+    decode_work(M, D, uniform(it), u, b);
+    u = uniform(u);
+    b = uniform(b);
     const DevState V = unit_view(M, D, u);
     const Sink S = make_sink(V);
     int4 d = M.items[b];
@@ -872,6 +893,8 @@ __device__ __forceinline__ void publish_word(const DevMatrix &M, const DevState 
             coherent_store(&V.marks[M.col_rows[k]], step + 1);
             const int sp = M.col_seg[k];
             if (sp >= 0) coherent_store(&V.seg_marks[sp], step + 1);
+            const int ci = M.col_item[k];
+            if (ci >= 0) coherent_store(&V.item_mark[ci], step + 1);
         }
     }
 }
@@ -950,11 +973,37 @@ __device__ __forceinline__ bool step_skipped(const DevState &D, int step, int fi
     return coherent_load(&status_row(D, step - 1)[kStatusChanged]) == 0;
 }
 
+// The sweep of one step over a grid sized to the device (grid stride): each
+// wave tests 64 of its work items at once (one lane each: the unit's state
+// and the item's mark) and visits those with work, as the persistent kernel
+// does.  Many units' items then cost a few loads each instead of a wave.
 __global__ __launch_bounds__(256) void sweep_kernel(DevMatrix M, DevState D, int step, int guard, int first_step) {
     if (guard && step_skipped(D, step, first_step)) return;
-    const int it = uniform(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+    const int waves = blockDim.x >> 6;
+    const int wave = uniform(blockIdx.x * waves + (threadIdx.x >> 6));
+    const int nwaves = gridDim.x * waves;
     const int n_work = M.replicas ? M.n_items * D.n_active : M.n_items;
-    if (it < n_work) sweep_item(M, D, it, step, status_row(D, step));
+    const int lane = threadIdx.x & 63;
+    int *st = status_row(D, step);
+    for (int base = wave; base < n_work; base += 64 * nwaves) {
+        const int it = base + lane * nwaves;
+        bool on = false;
+        if (it < n_work) {
+            int u, b;
+            decode_work(M, D, it, u, b);
+            const UnitStep us = unit_step_lane(M, D, u, step);
+            on = us.active && item_has_work(M, unit_view(M, D, u), b, step, us.all_rows);
+        }
+        unsigned long long pending = __ballot(on);
+        while (pending) {
+            const int t = __ffsll(static_cast<long long>(pending)) - 1;
+            pending &= pending - 1;
+            const int itt = uniform(base + t * nwaves);
+            int uu, bb;
+            decode_work(M, D, itt, uu, bb);
+            sweep_item(M, D, itt, step, st, unit_step(M, D, uu, step));
+        }
+    }
 }
 
 __global__ __launch_bounds__(256) void apply_split_kernel(DevMatrix M, DevState D, int step, int guard,
@@ -1046,45 +1095,84 @@ __global__ __launch_bounds__(kThreads) void mega_kernel(DevMatrix M, DevState D,
     const int nwaves = gridDim.x * waves;
     const int n_work = M.replicas ? M.n_items * D.n_active : M.n_items;
     // Several units: at the start of a step every workgroup notes in LDS
-    // which units work in it (their state words do not change during the
-    // sweep), so that a wave tests 64 of its items at once against the LDS
-    // bits and visits only those of working units.  Late steps of a batch
-    // have few units left, and most items would otherwise each pay the
-    // device-coherent loads of their unit's state to find it idle.
+    // which units work in it, which of them process all rows, and their
+    // first steps (the state words do not change during the sweep), so that
+    // an item's unit costs no device-coherent loads.  One unit: every wave
+    // reads its state once per step.  Then each wave tests 64 of its items at
+    // once (lanes): items of idle units, and items none of whose rows are
+    // marked when their unit processes marked rows only, are not visited.
     __shared__ unsigned works[kMaxFilterUnits / 32];
+    __shared__ unsigned all_rows_of[kMaxFilterUnits / 32];
+    __shared__ int start_of[kMaxFilterUnits];
     const int n_slots = M.replicas ? D.n_active : n_units;  // replicas: positions in the launch's list
     const bool filter = n_slots > 1 && n_slots <= kMaxFilterUnits;
     const int lane = threadIdx.x & 63;
     int steps = 0;
     for (int step = first_step; step < first_step + max_steps; step++) {
         int *st = status_row(D, step);
-        if (filter) {
+        UnitStep one{};
+        if (n_slots == 1) {
+            one = unit_step(M, D, M.replicas ? D.active[0] : 0, step);
+        } else if (filter) {
             // the previous step's waves of this workgroup are past its
-            // barriers, so no wave still reads the bits overwritten here
+            // barriers, so no wave still reads the entries overwritten here
             for (int g = (threadIdx.x >> 6) * 64; g < n_slots; g += blockDim.x) {
                 const int k = g + lane;
-                const bool on = k < n_slots && unit_works_lane(D, M.replicas ? D.active[k] : k, step);
-                const unsigned long long b = __ballot(on);
+                UnitStep us{};
+                if (k < n_slots) us = unit_step_lane(M, D, M.replicas ? D.active[k] : k, step);
+                const unsigned long long bw = __ballot(k < n_slots && us.active);
+                const unsigned long long ba = __ballot(k < n_slots && us.all_rows);
                 if (lane == 0) {
-                    works[g >> 5] = static_cast<unsigned>(b);
-                    works[(g >> 5) + 1] = static_cast<unsigned>(b >> 32);
+                    works[g >> 5] = static_cast<unsigned>(bw);
+                    works[(g >> 5) + 1] = static_cast<unsigned>(bw >> 32);
+                    all_rows_of[g >> 5] = static_cast<unsigned>(ba);
+                    all_rows_of[(g >> 5) + 1] = static_cast<unsigned>(ba >> 32);
                 }
+                if (k < n_slots) start_of[k] = us.start;
             }
             __syncthreads();
         }
         // the wave's items in the order wave, wave + nwaves, ...: 64 at a time
-        for (int base = wave; base < n_work; base += 64 * nwaves) {
-            const int it = base + lane * nwaves;
-            bool on = it < n_work;
-            if (filter && on) {
-                const int k = M.replicas ? it / M.n_items : (M.items[it].x >> kItemUnitShift);
-                on = (works[k >> 5] >> (k & 31)) & 1u;
-            }
-            unsigned long long pending = __ballot(on);
-            while (pending) {
-                const int t = __ffsll(static_cast<long long>(pending)) - 1;
-                pending &= pending - 1;
-                sweep_item(M, D, base + t * nwaves, step, st);
+        if (n_slots != 1 || one.active) {
+            for (int base = wave; base < n_work; base += 64 * nwaves) {
+                const int it = base + lane * nwaves;
+                bool on = it < n_work;
+                int u = 0, b = 0;
+                UnitStep us = one;
+                if (on) {
+                    decode_work(M, D, it, u, b);
+                    if (filter) {
+                        const int k = M.replicas ? it / M.n_items : u;
+                        us.active = (works[k >> 5] >> (k & 31)) & 1u;
+                        us.all_rows = (all_rows_of[k >> 5] >> (k & 31)) & 1u;
+                        us.start = start_of[k];
+                    } else if (n_slots != 1) {
+                        us = unit_step_lane(M, D, u, step);
+                    }
+                    on = us.active && item_has_work(M, unit_view(M, D, u), b, step, us.all_rows);
+                }
+                unsigned long long pending = __ballot(on);
+                while (pending) {
+                    const int t = __ffsll(static_cast<long long>(pending)) - 1;
+                    pending &= pending - 1;
+                    // the picked item's unit state again, wave-uniform (no
+                    // per-lane state stays live across the item)
+                    const int itt = uniform(base + t * nwaves);
+                    UnitStep ut = one;
+                    if (n_slots != 1) {
+                        int uu, bb;
+                        decode_work(M, D, itt, uu, bb);
+                        if (filter) {
+                            const int k = uniform(M.replicas ? itt / M.n_items : uu);
+                            ut.active = true;
+                            ut.all_rows = (all_rows_of[k >> 5] >> (k & 31)) & 1u;
+                            ut.start = start_of[k];
+                        } else {
+                            ut = unit_step(M, D, uu, step);
+                        }
+                    }
+                    sweep_item(M, D, itt, step, st, ut);
+                }
             }
         }
         if (!grid_barrier(D, nblocks, false)) { steps = -1; break; }
@@ -1175,6 +1263,7 @@ __global__ __launch_bounds__(256) void load_kernel(DevMatrix M, DevState D, int 
         {
             for (int w = tid; w < M.nwords; w += stride) V.bm[w] = 0u;
             for (int r = tid; r < M.m; r += stride) V.marks[r] = 0;
+            for (int q = tid; q < M.n_items; q += stride) V.item_mark[q] = 0;
             for (int q = tid; q < M.n_seg_rows; q += stride) V.seg_count[q] = 0u;
             for (int q = tid; q < M.n_seg_parts; q += stride) V.seg_parts[q].stamp = 0;
         }
@@ -1192,7 +1281,10 @@ __global__ __launch_bounds__(256) void hint_kernel(DevMatrix M, DevState D, int 
         for (int h = hint_ptr[k] + blockIdx.x * blockDim.x + threadIdx.x; h < hint_ptr[k + 1];
              h += gridDim.x * blockDim.x) {
             const int id = inv_perm[hints[h]];
-            for (int q = M.col_ptr[id]; q < M.col_ptr[id + 1]; q++) V.marks[M.col_rows[q]] = start_step;
+            for (int q = M.col_ptr[id]; q < M.col_ptr[id + 1]; q++) {
+                V.marks[M.col_rows[q]] = start_step;
+                if (M.col_item[q] >= 0) V.item_mark[M.col_item[q]] = start_step;
+            }
         }
     }
 }
@@ -1241,6 +1333,19 @@ __global__ __launch_bounds__(256) void delta_kernel(int n_list, double *const *l
             lbp[k][cols[h]] = lv[h];
             ubp[k][cols[h]] = uv[h];
         }
+}
+
+// Copies `count` bound-vector pairs of n2 doubles: ptrs[k] to ptrs[count + k]
+// (16-byte accesses; n2 is even).  Grid: x over elements, y over pairs.
+__global__ __launch_bounds__(256) void copy_pairs_kernel(long long n2, int count, double *const *ptrs) {
+    const long long h = n2 / 2;
+    for (int k = blockIdx.y; k < count; k += gridDim.y) {
+        const double2 *src = reinterpret_cast<const double2 *>(ptrs[k]);
+        double2 *dst = reinterpret_cast<double2 *>(ptrs[count + k]);
+        for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < h;
+             i += static_cast<long long>(gridDim.x) * blockDim.x)
+            dst[i] = src[i];
+    }
 }
 
 // Columns in which vector k (lbp[k], ubp[k]) differs from the base vector,
@@ -1412,12 +1517,13 @@ __global__ __launch_bounds__(256) void layout_rows_kernel(int mp, const long lon
 }
 
 // Column-to-row index: every entry of layout row r appends r to its column's
-// list (fill[] starts at col_ptr[]) and, for split rows (row_part[r] >= 0),
-// the entry's segment.  The order inside a column's list is not fixed; the
-// lists are used as sets (row and segment marking).  One wave per row.
+// list (fill[] starts at col_ptr[]), for split rows (row_part[r] >= 0) the
+// entry's segment, and the group item holding the row (row_item[r], -1 for
+// split and long rows).  The order inside a column's list is not fixed; the
+// lists are used as sets (row, segment and item marking).  One wave per row.
 __global__ __launch_bounds__(256) void layout_csc_kernel(int mp, const int *row_ptr_out, const int *col_out,
-                                                         const int *row_part, int *fill, int *col_rows,
-                                                         int *col_seg) {
+                                                         const int *row_part, const int *row_item, int *fill,
+                                                         int *col_rows, int *col_seg, int *col_item) {
     const int lane = threadIdx.x & 63;
     const int waves = gridDim.x * (blockDim.x >> 6);
     for (int r = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); r < mp; r += waves) {
@@ -1425,11 +1531,13 @@ __global__ __launch_bounds__(256) void layout_csc_kernel(int mp, const int *row_
         const int st = a & ~(kRowAlign - 1);
         const int len = (b & ~(kRowAlign - 1)) - st - (a & (kRowAlign - 1));
         const int part = row_part[r];
+        const int item = row_item[r];
         for (int e = lane; e < len; e += 64) {
             const int id = col_out[st + e] & kIdMask;
             const int slot = atomic_add_i(&fill[id], 1);
             col_rows[slot] = r;
             col_seg[slot] = part >= 0 ? part + e / kSegLen : -1;
+            col_item[slot] = item;
         }
     }
 }

@@ -16,6 +16,9 @@ Generators (all deterministic in their seed, numpy PCG64):
                    with random sign; 50% binary, 20% integer [0,100], 30%
                    continuous [-1e4,1e4] with 5% infinite bounds; sides from a
                    feasible point plus slack U[0, 0.1*activity range].
+  chain_mip        m=4e5, n=2e6, about 12.4M nonzeros, 2% of the variables
+                   pre-fixed: small_mip's family at the cover/knapsack sizes,
+                   49 rounds to the fixpoint (the many-round workload).
   cover_knapsack   m=4e5 half covering rows (0/1, length U{3..40}, >= 1), half
                    knapsack rows (weights U{1..1000}, length U{10..200},
                    capacity 0.3*sum of weights); n=2e6 binaries, 2% fixed
@@ -120,7 +123,7 @@ def _finalize(m, n, row_ptr, col, val, lhs, rhs, lb, ub, vartype, name, meta) ->
 
 
 def small_mip(seed: int = 0, m: int = 2000, n: int = 1000, len_lo: int = 2, len_hi: int = 20,
-              int_frac: float = 0.6, name: str = "small_mip") -> Instance:
+              int_frac: float = 0.6, fixed_frac: float = 0.0, name: str = "small_mip") -> Instance:
     rng = np.random.default_rng(seed)
     lengths = rng.integers(len_lo, len_hi + 1, size=m)
     lengths = np.minimum(lengths, n)
@@ -144,8 +147,15 @@ def small_mip(seed: int = 0, m: int = 2000, n: int = 1000, len_lo: int = 2, len_
     drop_lhs = rng.random(m) < 0.5
     lhs[one_sided & drop_lhs] = -INF
     rhs[one_sided & ~drop_lhs] = INF
-    return _finalize(m, n, row_ptr, c, val, lhs, rhs, lb, ub, vartype, name,
-                     {"generator": "small_mip", "seed": seed})
+    meta = {"generator": "small_mip", "seed": seed}
+    if fixed_frac > 0:
+        # pre-fixed variables, at the feasible point the sides were built
+        # from (drawn last: the rest of the instance is that of fixed_frac=0)
+        fix = rng.random(n) < fixed_frac
+        lb[fix] = x0[fix]
+        ub[fix] = x0[fix]
+        meta["fixed_frac"] = fixed_frac
+    return _finalize(m, n, row_ptr, c, val, lhs, rhs, lb, ub, vartype, name, meta)
 
 
 def zipf_lengths(rng, m: int, s: float, lo: int, hi: int) -> np.ndarray:
@@ -219,6 +229,20 @@ def cover_knapsack(seed: int = 2, m: int = 400_000, n: int = 2_000_000, fixed_fr
     ub[fixed & ~to_one] = 0.0
     return _finalize(m, n, row_ptr, c, val, lhs, rhs, lb, ub, vartype, name,
                      {"generator": "cover_knapsack", "seed": seed})
+
+
+def chain_mip(seed: int = 1, m: int = 400_000, n: int = 2_000_000, len_hi: int = 60,
+              fixed_frac: float = 0.02, name: str = "chain_mip") -> Instance:
+    """The many-round workload: the structured binary instance's sizes
+    (m = 400,000, n = 2,000,000, about 12.4M nonzeros) and its 2% of
+    pre-fixed variables, built from the general family (small_mip: row
+    lengths U{2..60}, 60% integer columns, continuous bounds converging over
+    many rounds).  Seed 1 takes 49 round-synchronous rounds (cpu_seq 33).
+    cover_knapsack itself cannot chain: a zero propagates only through a
+    covering row whose other members are all zero, a one only through a
+    knapsack row filled to 30% of its weight, and every fixing fraction that
+    makes either likely makes some row infeasible in round 1."""
+    return small_mip(seed=seed, m=m, n=n, len_hi=len_hi, fixed_frac=fixed_frac, name=name)
 
 
 def batch_family(seed: int = 3, count: int = 256, m_lo: int = 1000, m_hi: int = 200_000,

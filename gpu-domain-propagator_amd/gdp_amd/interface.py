@@ -585,7 +585,8 @@ class Problem:
 
     def close(self):
         if getattr(self, "handle", None):
-            self.lib.gdpProblemDestroy(self.handle)
+            if self.lib.gdpProblemDestroy(self.handle) != GDP_OKAY:
+                raise RuntimeError(f"gdpProblemDestroy failed: {last_error()}")
             self.handle = None
 
     def __del__(self):

@@ -26,7 +26,9 @@ typedef struct {
     int rounds;        /* executed rounds, counting the final no-change round */
     int infeasible;    /* 1 if some variable ended with lb > ub + GDP_EPS */
     int capped;        /* 1 if the round cap stopped a run that still changed */
-    double kernel_ms;  /* device time of the round loop (host paths: wall time) */
+    double kernel_ms;  /* device time of the round loop (host propagators: wall
+                          time of the round loop alone, without the column index
+                          build and the allocations, as the reference times it) */
 } gdp_run_info;
 
 typedef struct {

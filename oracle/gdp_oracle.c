@@ -8,11 +8,19 @@
  *
  * Each function cites the reference file:line it restates.
  */
+#define _POSIX_C_SOURCE 199309L
 #include "gdp_oracle.h"
 
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
+
+static double now_seconds(void) {
+    struct timespec t;
+    clock_gettime(CLOCK_MONOTONIC, &t);
+    return (double)t.tv_sec + 1e-9 * (double)t.tv_nsec;
+}
 
 #define OR_INF 1e20
 #define OR_EPS 1e-9
@@ -306,7 +314,7 @@ int oracle_seq_propagate(int m, int n, int nnz, const int *col, const int *rowpt
                          const double *val, const double *lhs, const double *rhs,
                          double *lb, double *ub, const int *vartype,
                          int max_rounds, int detect_infeasible, oracle_result *res) {
-    res->rounds = 0; res->infeasible = 0; res->capped = 0;
+    res->rounds = 0; res->infeasible = 0; res->capped = 0; res->loop_seconds = 0.0;
     if (m == 0 || n == 0 || nnz == 0) return 1;
     if (detect_infeasible && any_crossed(n, lb, ub, NULL)) { res->infeasible = 1; return 1; }
     work_t w;
@@ -314,6 +322,7 @@ int oracle_seq_propagate(int m, int n, int nnz, const int *col, const int *rowpt
     oracle_csr_to_csc(m, n, nnz, col, rowptr, val, w.colptr, w.rowidx, NULL);
     for (int i = 0; i < m; i++) w.marked[i] = 1;
     int change = 1, round;
+    const double t0 = now_seconds();
     for (round = 1; round <= max_rounds && change; round++) {
         memset(w.chg, 0, (size_t)n);
         change = seq_round(m, col, rowptr, val, lhs, rhs, lb, ub, vartype, w.colptr, w.rowidx,
@@ -324,6 +333,7 @@ int oracle_seq_propagate(int m, int n, int nnz, const int *col, const int *rowpt
             break;
         }
     }
+    res->loop_seconds = now_seconds() - t0;
     res->rounds = round - 1;
     res->capped = change && !res->infeasible;
     work_free(&w);
@@ -335,7 +345,7 @@ int oracle_seq_disjoint_propagate(int m, int n, int nnz, const int *col, const i
                                   const double *val, const double *lhs, const double *rhs,
                                   double *lb, double *ub, const int *vartype,
                                   int max_rounds, oracle_result *res) {
-    res->rounds = 0; res->infeasible = 0; res->capped = 0;
+    res->rounds = 0; res->infeasible = 0; res->capped = 0; res->loop_seconds = 0.0;
     if (m == 0 || n == 0 || nnz == 0) return 1;
     work_t w;
     if (!work_alloc(&w, m, n, nnz)) { work_free(&w); return 0; }
@@ -372,7 +382,7 @@ int oracle_jacobi_propagate(int m, int n, int nnz, const int *col, const int *ro
                             double *lb, double *ub, const int *vartype,
                             int max_rounds, int detect_infeasible, int seg_len,
                             oracle_result *res) {
-    res->rounds = 0; res->infeasible = 0; res->capped = 0;
+    res->rounds = 0; res->infeasible = 0; res->capped = 0; res->loop_seconds = 0.0;
     if (m == 0 || n == 0 || nnz == 0) return 1;
     if (detect_infeasible && any_crossed(n, lb, ub, NULL)) { res->infeasible = 1; return 1; }
     work_t w;

@@ -34,6 +34,9 @@ typedef struct {
     int rounds;      /* executed rounds, counting the final no-change round */
     int infeasible;  /* 1 if bounds crossed */
     int capped;      /* 1 if the round cap stopped a run that still changed */
+    double loop_seconds; /* monotonic-clock time of the round loop alone (the
+                            reference's timing scope: no CSC build, no
+                            allocation; sequential_propagator.h:248-281) */
 } oracle_result;
 
 /* ---- scalar arithmetic (reference src/propagation_methods.cuh) ---- */

@@ -14,7 +14,7 @@ SEG_LEN = 256  # canonical segment length shared with the kernels (kSegLen)
 
 
 class Result(ctypes.Structure):
-    _fields_ = [("rounds", c_int), ("infeasible", c_int), ("capped", c_int)]
+    _fields_ = [("rounds", c_int), ("infeasible", c_int), ("capped", c_int), ("loop_seconds", c_double)]
 
 
 _lib = None
@@ -71,13 +71,19 @@ def _arrays(inst):
 
 def seq(inst, max_rounds=100, detect_infeasible=True, lb=None, ub=None):
     """cpu_seq fixpoint: returns (lb, ub, rounds, infeasible, capped)."""
+    return seq_timed(inst, max_rounds, detect_infeasible, lb, ub)[0]
+
+
+def seq_timed(inst, max_rounds=100, detect_infeasible=True, lb=None, ub=None):
+    """seq() and the time of its round loop alone in seconds (no CSC build,
+    no allocation: the reference's timing scope)."""
     col, rp, val, lhs, rhs, vt = _arrays(inst)
     lb = np.array(inst.lb if lb is None else lb, np.float64)
     ub = np.array(inst.ub if ub is None else ub, np.float64)
     r = Result()
     lib().oracle_seq_propagate(inst.m, inst.n, len(col), _p(col), _p(rp), _p(val), _p(lhs), _p(rhs), _p(lb),
                                _p(ub), _p(vt), max_rounds, int(detect_infeasible), ctypes.byref(r))
-    return lb, ub, r.rounds, bool(r.infeasible), bool(r.capped)
+    return (lb, ub, r.rounds, bool(r.infeasible), bool(r.capped)), r.loop_seconds
 
 
 def jacobi(inst, max_rounds=100, detect_infeasible=True, seg_len=SEG_LEN, lb=None, ub=None):

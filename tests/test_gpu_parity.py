@@ -249,3 +249,17 @@ def test_large_instance_through_staged_uploads(gpu, oracle):
     assert rc == I.GDP_OKAY
     np.testing.assert_array_equal(lb, exp[0])
     np.testing.assert_array_equal(ub, exp[1])
+
+
+@pytest.mark.parametrize("sync", [I.CPU_LOOP, I.GPU_LOOP, I.MEGAKERNEL])
+@pytest.mark.parametrize("index", [56, 136, 150, 156])
+def test_batch_tolerance_cases_alone(gpu, oracle, index, sync):
+    """The batched workload's instances whose fixpoint lies more than 1e-9
+    from cpu_seq's in one continuous bound (56, 136) and whose propagation
+    reaches the round cap (150, 156), each propagated alone: bit for bit the
+    oracle, the cap reported as such."""
+    from gdp_amd.instances import batch_family
+    inst = small_mip(**batch_family(count=256)[index])
+    exp = oracle.jacobi(inst)
+    assert exp[4] == (index in (150, 156))
+    assert_same(I.gpu_propagate(inst, sync), exp, f"instance {index} sync={sync}")

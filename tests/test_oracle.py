@@ -307,3 +307,46 @@ def test_gate_ties_discriminate_exact_maximum():
     exceeds R by more than the tolerance."""
     differ = [p for p, passes in fixtures.GATE_TIE_PATTERNS if (max(p) > 1.0) != passes]
     assert len(differ) >= 3
+
+
+# The two propagators stop once no candidate beats a bound by more than eps
+# (isLbBetter / isUbBetter, propagation_methods.cuh:54-64).  Where a
+# continuous bound converges geometrically they can stop at different points
+# of that window, along different acceptance paths.  Pinned here on the two
+# instances of the batched workload where the gap exceeds 1e-9, and the
+# property that holds everywhere: each fixpoint is one of the other
+# propagator (a round started from it accepts no candidate) when the row
+# sums are exact in FP64, as for these integer-coefficient instances.
+@pytest.mark.parametrize("index,col,gap", [(56, 5533, 1.301e-9), (136, 10810, -1.156e-9)])
+def test_epsilon_fixpoints_along_different_paths(oracle, index, col, gap):
+    from gdp_amd.instances import batch_family
+    inst = small_mip(**batch_family(count=256)[index])
+    jac, seq = oracle.jacobi(inst), oracle.seq(inst)
+    assert not (jac[4] or seq[4] or jac[3] or seq[3])
+    assert jac[0][col] - seq[0][col] == pytest.approx(gap, rel=1e-3)
+    rel = lambda a, b: np.abs(a - b) / np.maximum(1.0, np.abs(b))  # noqa: E731
+    beyond = np.flatnonzero((rel(jac[0], seq[0]) > 1e-9) | (rel(jac[1], seq[1]) > 1e-9))
+    assert beyond.tolist() == [col]
+    again = oracle.seq(inst, lb=jac[0], ub=jac[1])
+    assert again[2] == 1 and np.array_equal(again[0], jac[0]) and np.array_equal(again[1], jac[1])
+    back = oracle.jacobi(inst, lb=seq[0], ub=seq[1])
+    assert back[2] == 1 and np.array_equal(back[0], seq[0]) and np.array_equal(back[1], seq[1])
+    if index == 136:
+        # cpu_seq passes through the round-synchronous fixpoint's value in its
+        # round 16 and then accepts one more step of 1.156e-9 (> eps) there,
+        # which the round-synchronous order is never offered
+        r16 = oracle.seq(inst, max_rounds=16)
+        assert r16[0][col] == jac[0][col]
+        assert seq[0][col] - r16[0][col] > 1e-9
+
+
+@pytest.mark.parametrize("index,seq_rounds,seq_capped", [(150, 84, False), (156, 100, True)])
+def test_round_cap_instances_of_the_batch(oracle, index, seq_rounds, seq_capped):
+    """Instances 150 and 156 of the batched workload: the round-synchronous
+    propagation still moves bounds after 100 rounds (capped); cpu_seq
+    reaches its fixpoint in 84 rounds on 150 and is capped too on 156."""
+    from gdp_amd.instances import batch_family
+    inst = small_mip(**batch_family(count=256)[index])
+    jac, seq = oracle.jacobi(inst), oracle.seq(inst)
+    assert (jac[2], jac[4]) == (100, True)
+    assert (seq[2], seq[4]) == (seq_rounds, seq_capped)
