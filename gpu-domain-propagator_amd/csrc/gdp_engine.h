@@ -100,6 +100,16 @@ RunResult problem_propagate(Problem *p, double *lbs, double *ubs, int sync_type,
 RunResult problem_propagate_device(Problem *p, double *d_lbs, double *d_ubs, int sync_type,
                                    int max_rounds);
 
+// Same as problem_propagate_device with the run state (unit state, status
+// words, launch tables) in a fixed device array of kScratchBytes that the
+// library holds per device, instead of the problem's own: no device memory
+// is allocated.  Calls on one device are serialised.  Throws when the
+// problem's run state (problem_run_state_bytes) exceeds the array.
+constexpr size_t kScratchBytes = 64u << 20;
+RunResult problem_propagate_no_alloc(Problem *p, double *d_lbs, double *d_ubs, int sync_type, int max_rounds);
+// Device bytes of the run state of one bound vector.
+size_t problem_run_state_bytes(const Problem *p);
+
 // Work of each round of one propagation of host vectors lbs/ubs (in/out,
 // the CPU_LOOP launches, same result): the rows whose activities the round
 // computes (rows marked by the previous round, or every row), their

@@ -356,6 +356,7 @@ struct Problem {
     long long graph_key = -1;
     std::mutex run_mutex;
     int servers = 0;                  // servers attached (they own the unit slots)
+    bool graphs_enabled = true;       // GPU_LOOP may capture graphs of the current run state
     bool staging_held = false;
 
     void drop_graphs() {
@@ -467,7 +468,8 @@ void alloc_units(Problem *P, int cap) {
     P->stats.n_units = P->n_units;
 }
 
-// The run state of unit 0 exists (allocated on first use).
+// The run state of unit 0 exists (allocated on first use: a problem that is
+// only run through the scratch entry point never allocates it).
 void ensure_units(Problem *P) {
     if (P->capacity < 1) alloc_units(P, 1);
 }
@@ -946,6 +948,24 @@ double run_steps(Problem *P, int sync_type, int max_rounds) {
             if (err) throw std::runtime_error("device wait timed out");
             if (!flags[kStatusChanged]) break;
         }
+    } else if (sync_type == GPU_LOOP_ID && !P->graphs_enabled) {
+        // the same chunks of self-stopping launches, issued without capture
+        // (no graph is created, so nothing is allocated)
+        const int chunk = std::max(1, P->tuning.graph_rounds);
+        for (int first = 1; first <= max_rounds; first += chunk) {
+            const int last = std::min(max_rounds, first + chunk - 1);
+            for (int step = first; step <= last; step++) launch_step(P, step, 1, 1);
+            check(hipGetLastError(), "step launch");
+            int flags[kStatusWords];
+            int err = 0;
+            check(hipMemcpyAsync(flags, D.status + kStatusWords * (last % kStatusRows), sizeof(flags),
+                                 hipMemcpyDeviceToHost, s),
+                  "flags D2H");
+            check(hipMemcpyAsync(&err, D.global + kGlobalError, sizeof(int), hipMemcpyDeviceToHost, s), "D2H");
+            check(hipStreamSynchronize(s), "step sync");
+            if (err) throw std::runtime_error("device wait timed out");
+            if (!flags[kStatusChanged]) break;
+        }
     } else if (sync_type == GPU_LOOP_ID) {
         // captured chunks of steps whose launches stop themselves on the device
         const int chunk = std::max(1, P->tuning.graph_rounds);
@@ -1387,6 +1407,103 @@ void batch_propagate_device(Problem *P, double *d_lbs, double *d_ubs, int sync_t
     std::vector<int> list(P->n_inst);
     for (int k = 0; k < P->n_inst; k++) list[k] = k;
     run_units(P, P->n_inst, list.data(), &d_lbs, &d_ubs, nullptr, nullptr, sync_type, max_rounds, out);
+}
+
+// ---------------------------------------------------------------------------
+// propagation that allocates nothing
+// ---------------------------------------------------------------------------
+
+// The run state of the allocation-free entry point lives in this fixed array
+// of device memory, which every device the library runs on holds until the
+// process ends.  In host code `scratch` names a host-side stand-in, not the
+// device array: its device address comes from hipGetSymbolAddress (once per
+// device, with that device current), and only that address is handed to
+// kernels.
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;                 // the address table
+char *g_scratch_ptr[kMaxDevices] = {};   // device address of `scratch`, per device
+std::mutex g_scratch_use[kMaxDevices];   // one call at a time uses a device's array
+
+// Device address of `scratch` on device `dev` (the current device): looked
+// up on the first call on that device.
+char *scratch_address(int dev) {
+    if (dev < 0 || dev >= kMaxDevices) throw std::runtime_error("device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        check(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        if (!ptr) throw std::runtime_error("hipGetSymbolAddress(scratch) returned no address");
+        g_scratch_ptr[dev] = static_cast<char *>(ptr);
+    }
+    return g_scratch_ptr[dev];
+}
+
+// Runs the problem on a borrowed run state for the lifetime of the object:
+// the owned state (if any), its launch tables and its captured graphs are
+// set aside and restored afterwards.
+class BorrowedState {
+  public:
+    BorrowedState(Problem *P, char *base)
+        : P_(P), D_(P->D), list_(P->d_list), params_(P->d_params), lbp_(P->d_lbp), ubp_(P->d_ubp),
+          hint_ptr_(P->d_hint_ptr), capacity_(P->capacity), n_units_(P->n_units), graphs_(P->graphs_enabled) {
+        bind_state(P, base, 1);
+        P->graphs_enabled = false;
+    }
+    ~BorrowedState() {
+        P_->D = D_;
+        P_->d_list = list_;
+        P_->d_params = params_;
+        P_->d_lbp = lbp_;
+        P_->d_ubp = ubp_;
+        P_->d_hint_ptr = hint_ptr_;
+        P_->capacity = capacity_;
+        P_->n_units = n_units_;
+        P_->graphs_enabled = graphs_;
+    }
+    BorrowedState(const BorrowedState &) = delete;
+    BorrowedState &operator=(const BorrowedState &) = delete;
+
+  private:
+    Problem *P_;
+    DevState D_;
+    int *list_;
+    int4 *params_;
+    double **lbp_, **ubp_;
+    int *hint_ptr_;
+    int capacity_, n_units_;
+    bool graphs_;
+};
+}  // namespace
+
+size_t problem_run_state_bytes(const Problem *P) {
+    DevState D{};
+    UnitTables T;
+    return carve_state(P->M, P->replicas, P->n_inst, 1, nullptr, D, T);
+}
+
+RunResult problem_propagate_no_alloc(Problem *P, double *d_lbs, double *d_ubs, int sync_type, int max_rounds) {
+    if (!P->replicas) throw std::runtime_error("use the batch entry points for a batch of instances");
+    const size_t need = problem_run_state_bytes(P);
+    if (need > kScratchBytes)
+        throw std::runtime_error("the problem needs " + std::to_string(need) +
+                                 " bytes of run state, more than the " + std::to_string(kScratchBytes) +
+                                 " bytes of the allocation-free entry point's device array");
+    int dev = -1;
+    check(hipGetDevice(&dev), "hipGetDevice");
+    if (dev != P->device) throw std::runtime_error("the problem's device is not the current device");
+    char *base = scratch_address(dev);
+    std::lock_guard<std::mutex> use(g_scratch_use[dev]);
+    std::lock_guard<std::mutex> lock(P->run_mutex);
+    BorrowedState borrowed(P, base);
+    // The array keeps the previous call's contents.  Everything a run reads
+    // before writing is reset by its loading launches: the unit's state words
+    // and candidate counters, the status ring, global words and barrier
+    // (unit_reset_kernel); the snapshot, accumulators, bitmap, row marks,
+    // split-row counters and partial stamps (load_kernel).
+    return run_single(P, d_lbs, d_ubs, sync_type, max_rounds);
 }
 
 // ---------------------------------------------------------------------------

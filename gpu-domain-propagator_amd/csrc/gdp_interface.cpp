@@ -492,6 +492,7 @@ GDP_RETCODE gdpProblemGetStats(gdp_problem p, gdp_problem_stats *out) {
     out->unit_state_bytes = s.unit_state_bytes;
     out->mega_threads = s.mega_threads;
     out->mega_grid = s.mega_grid;
+    out->run_state_bytes = static_cast<long long>(gdp::problem_run_state_bytes(p->p));
     return GDP_OKAY;
 }
 
@@ -840,6 +841,20 @@ GDP_RETCODE gdpPropagateDevice(gdp_problem p, double *d_lbs, double *d_ubs, cons
         return GDP_OKAY;
     } catch (const std::exception &e) {
         return fail("gdpPropagateDevice", e);
+    }
+}
+
+GDP_RETCODE gdpPropagateDeviceNoAlloc(gdp_problem p, double *d_lbs, double *d_ubs, const int sync_type,
+                                      const int max_rounds, gdp_run_info *info) {
+    static_assert(GDP_NOALLOC_SCRATCH_BYTES == static_cast<long long>(gdp::kScratchBytes),
+                  "the header states the size of the device array");
+    if (!p || !d_lbs || !d_ubs) return GDP_ERROR;
+    try {
+        const RunResult r = gdp::problem_propagate_no_alloc(p->p, d_lbs, d_ubs, sync_type, max_rounds);
+        fill_info(info, r.rounds, r.infeasible, r.capped, r.kernel_ms);
+        return GDP_OKAY;
+    } catch (const std::exception &e) {
+        return fail("gdpPropagateDeviceNoAlloc", e);
     }
 }
 

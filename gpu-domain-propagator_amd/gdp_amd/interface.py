@@ -56,6 +56,7 @@ EXT_SYMBOLS = (
     "gdpProblemGetStats",
     "gdpPropagate",
     "gdpPropagateDevice",
+    "gdpPropagateDeviceNoAlloc",
     "gdpProblemReserve",
     "gdpProblemWorkProfile",
     "gdpPropagateNodesDevice",
@@ -141,6 +142,7 @@ class ProblemStats(ctypes.Structure):
         ("unit_state_bytes", c_longlong),
         ("mega_threads", c_int),
         ("mega_grid", c_int),
+        ("run_state_bytes", c_longlong),
     ]
 
 
@@ -178,6 +180,8 @@ def load_library(path: Optional[str] = None) -> ctypes.CDLL:
     lib.gdpPropagate.restype = c_int
     lib.gdpPropagateDevice.argtypes = [vp, vp, vp, c_int, c_int, POINTER(RunInfo)]
     lib.gdpPropagateDevice.restype = c_int
+    lib.gdpPropagateDeviceNoAlloc.argtypes = [vp, vp, vp, c_int, c_int, POINTER(RunInfo)]
+    lib.gdpPropagateDeviceNoAlloc.restype = c_int
     lib.gdpProblemWorkProfile.argtypes = [vp, vp, vp, c_int, vp, vp, vp, vp, POINTER(c_int), POINTER(RunInfo)]
     lib.gdpProblemWorkProfile.restype = c_int
     lib.gdpProblemReserve.argtypes = [vp, c_int]
@@ -541,6 +545,17 @@ class Problem:
                                          max_rounds, ctypes.byref(info))
         if rc != GDP_OKAY:
             raise RuntimeError(f"gdpPropagateDevice failed: {last_error()}")
+        return info
+
+    def propagate_device_no_alloc(self, d_lbs_ptr: int, d_ubs_ptr: int, sync_type: int = CPU_LOOP,
+                                  max_rounds: int = MAX_NUM_ROUNDS) -> RunInfo:
+        """propagate_device with the run state in the library's fixed device
+        array: no device memory is allocated (gdpPropagateDeviceNoAlloc)."""
+        info = RunInfo()
+        rc = self.lib.gdpPropagateDeviceNoAlloc(self.handle, c_void_p(d_lbs_ptr), c_void_p(d_ubs_ptr), sync_type,
+                                                max_rounds, ctypes.byref(info))
+        if rc != GDP_OKAY:
+            raise RuntimeError(f"gdpPropagateDeviceNoAlloc failed: {last_error()}")
         return info
 
     def reserve(self, capacity: int):

@@ -61,6 +61,10 @@ typedef struct {
     long long unit_state_bytes; /* device bytes of one bound vector's state */
     int mega_threads;           /* persistent kernel: workgroup size */
     int mega_grid;              /* persistent kernel: workgroups */
+    long long run_state_bytes;  /* device bytes of the run state of one propagation:
+                                   allocated by the problem on its first propagation,
+                                   taken from the fixed device array by
+                                   gdpPropagateDeviceNoAlloc */
 } gdp_problem_stats;
 
 /* Fills *t with the defaults. */
@@ -105,6 +109,24 @@ GDP_RETCODE gdpProblemWorkProfile(gdp_problem p, double *lbs, double *ubs, const
  * has finished. */
 GDP_RETCODE gdpPropagateDevice(gdp_problem p, double *d_lbs, double *d_ubs, const int sync_type,
                                const int max_rounds, gdp_run_info *info);
+
+/* Same as gdpPropagateDevice, allocating no device memory: no counterpart in
+ * the reference.  For callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The run
+ * state (unit state, split-row partials, status words, counters, launch
+ * tables; gdp_problem_stats.run_state_bytes) lives in one fixed array of
+ * GDP_NOALLOC_SCRATCH_BYTES (64 MiB) of device memory that the library holds
+ * on every device it runs on until the process ends, instead of the
+ * problem's own state, which is never allocated for a problem run only this
+ * way.  Problems whose run state exceeds the array return GDP_ERROR.  The
+ * problem's device must be the current device.  The array is shared by all
+ * problems on a device and keeps the last call's contents; every call resets
+ * what it reads.  Calls on one device are serialised by the library (calls
+ * that overlapped on one array would race).  Supports every sync type;
+ * GPU_LOOP issues its self-stopping launches without capturing a graph. */
+#define GDP_NOALLOC_SCRATCH_BYTES (64LL << 20)
+GDP_RETCODE gdpPropagateDeviceNoAlloc(gdp_problem p, double *d_lbs, double *d_ubs, const int sync_type,
+                                      const int max_rounds, gdp_run_info *info);
 
 /* ---- many bound vectors against one matrix ----
  * The problem keeps device state for `capacity` bound vectors (grown on
