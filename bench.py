@@ -330,124 +330,47 @@ def serving_leg(sync):
         d.free()
     srv = I.Server(p, hbm_budget=16 << 30, max_slots=1024)
     srv.set_base(root.lbs, root.ubs)
-    # zero-copy requests: node vectors produced in pinned memory, results
-    # written into a ring of pinned output buffers (a request waits for a free
-    # one: back-pressure)
-    n_in, n_out = 200, 512
-    pin_in = [(I.PinnedArray(inst.n), I.PinnedArray(inst.n)) for _ in range(n_in)]
-    for q, (a, c) in enumerate(pin_in):
-        a.array[:] = pool[q][0]
-        c.array[:] = pool[q][1]
-    pin_out = [(I.PinnedArray(inst.n), I.PinnedArray(inst.n)) for _ in range(n_out)]
-    # The server steps on its own thread.  Four driver threads produce the
-    # requests at their Poisson arrival times (request k on thread k mod 4),
-    # submit them and take completed results, whoever sees them first: the
-    # vector copies run inside the library and numpy, which release the GIL,
-    # so the threads overlap where one thread would bound the rate.
-    import threading
-    n_drv = 4
+    # The load generator (gpu-domain-propagator_amd/bench/serve_load.cpp) is
+    # native: 4 submitting threads issue the requests at their Poisson
+    # arrival times and one thread takes the results, through the C API,
+    # while the server steps on its own thread.
+    load = ctypes.CDLL(os.path.join(REPO, "gpu-domain-propagator_amd", "lib", "libgdp_serveload.so"))
+    load.gdpServeLoad.restype = ctypes.c_int
+    dp = ctypes.POINTER(ctypes.c_double)
+    pool_l = (dp * len(pool))(*[x[0].ctypes.data_as(dp) for x in pool])
+    pool_u = (dp * len(pool))(*[x[1].ctypes.data_as(dp) for x in pool])
+    cols = [np.ascontiguousarray(x[2], np.int32) for x in pool]
+    ip = ctypes.POINTER(ctypes.c_int)
+    pool_c = (ip * len(pool))(*[c.ctypes.data_as(ip) for c in cols])
+    pool_n = (ctypes.c_int * len(pool))(*[len(c) for c in cols])
     srv.start()
     rates = []
+    modes = {"full": 0, "pinned": 1, "inplace": 2, "delta": 3}
     for mode in ("full", "pinned", "inplace", "delta"):
         for rate in (1000, 2000, 5000, 10000, 20000, 50000):
-            N = int(min(4000, max(600, rate)))
+            N = int(max(1000, min(10000, rate // 4)))
             arr = np.cumsum(rng.exponential(1.0 / rate, size=N))
-            # per ticket: request index, completion time; output pairs of
-            # zero-copy requests (a result may be taken by another thread
-            # before the submitting thread has recorded its ticket)
-            st = {"idx": {}, "done_t": {}, "got": 0, "free_out": list(range(n_out)), "out_of": {},
-                  "early": set()}
-            lock = threading.Lock()
-            start = time.perf_counter()
-
-            def take_results():
-                comp = srv.completed()
-                tnow = time.perf_counter() - start
-                for tk in comp:
-                    # full vectors back for full requests, in place for zero-copy
-                    # requests, changed columns for delta requests
-                    if mode == "full":
-                        srv.result(tk)
-                    elif mode in ("pinned", "inplace"):
-                        srv.result(tk, copy=False)
-                    else:
-                        srv.result_changes(tk)
-                    with lock:
-                        st["done_t"][tk] = tnow
-                        st["got"] += 1
-                        o = st["out_of"].pop(tk, None)
-                        if o is not None:
-                            st["free_out"].append(o)
-                        elif mode in ("pinned", "inplace"):
-                            st["early"].add(tk)
-                return len(comp)
-
-            def drive(h):
-                ks = list(range(h, N, n_drv))
-                i = 0
-                while True:
-                    with lock:
-                        if st["got"] >= N:
-                            return
-                    now = time.perf_counter() - start
-                    sub = 0
-                    while i < len(ks) and arr[ks[i]] <= now:
-                        k = ks[i]
-                        lb, ub, cols = pool[k % len(pool)]
-                        o = None
-                        if mode in ("pinned", "inplace"):
-                            with lock:
-                                o = st["free_out"].pop() if st["free_out"] else None
-                            if o is None:
-                                break  # back-pressure: wait for a free output pair
-                        if mode == "full":
-                            t_ = srv.submit(lb, ub, hints=cols)
-                        elif mode == "pinned":
-                            src = pin_in[k % n_in]
-                            t_ = srv.submit_pinned(src[0], src[1], pin_out[o][0], pin_out[o][1],
-                                                   hints=pool[k % n_in][2])
-                        elif mode == "inplace":
-                            # the node vector is produced in a free pinned buffer
-                            # pair, which the result then replaces
-                            pin_out[o][0].array[:] = lb
-                            pin_out[o][1].array[:] = ub
-                            t_ = srv.submit_pinned(pin_out[o][0], pin_out[o][1], pin_out[o][0], pin_out[o][1],
-                                                   hints=cols)
-                        else:
-                            t_ = srv.submit_delta(cols, lb[cols], ub[cols])
-                        with lock:
-                            st["idx"][t_] = k
-                            if o is not None:
-                                if t_ in st["early"]:
-                                    st["early"].discard(t_)
-                                    st["free_out"].append(o)
-                                else:
-                                    st["out_of"][t_] = o
-                        i += 1
-                        sub += 1
-                    if not take_results() and not sub:
-                        time.sleep(0.00005)
-
-            drivers = [threading.Thread(target=drive, args=(h,)) for h in range(n_drv)]
-            for d in drivers:
-                d.start()
-            for d in drivers:
-                d.join()
+            arr -= arr[0]
             done = np.zeros(N)
-            for tk, k in st["idx"].items():
-                done[k] = st["done_t"][tk]
+            ok = load.gdpServeLoad(srv.handle, modes[mode], N, arr.ctypes.data_as(dp), len(pool), inst.n, pool_l,
+                                   pool_u, pool_c, pool_n, 4, 256, done.ctypes.data_as(dp))
+            if not ok:
+                raise RuntimeError(f"serving load {mode} {rate}/s failed: {I.last_error()}")
             lat = (done - arr) * 1e3
             rates.append({"requests": mode, "offered_per_s": rate, "n": N,
-                          "achieved_per_s": N / (done.max() - arr[0]),
+                          "achieved_per_s": N / done.max(),
                           "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99))})
             print(f"bench: serving {mode} {rate}/s -> {rates[-1]['achieved_per_s']:.0f}/s "
                   f"p50 {rates[-1]['p50_ms']:.2f} ms p99 {rates[-1]['p99_ms']:.2f} ms", file=sys.stderr, flush=True)
     srv.stop()
     srv.close()
-    for a, c in pin_in + pin_out:
-        a.free()
-        c.free()
     p.close()
+    plateau = {}
+    for mode in modes:
+        rs = [r for r in rates if r["requests"] == mode]
+        top = max(r["achieved_per_s"] for r in rs)
+        at_max = next(r for r in rs if r["offered_per_s"] == 50000)["achieved_per_s"]
+        plateau[mode] = {"best_per_s": top, "at_50k_offered_per_s": at_max, "ratio": at_max / top}
     best = max(r["achieved_per_s"] for r in rates)
     best_full = max(r["achieved_per_s"] for r in rates if r["requests"] in ("full", "pinned", "inplace"))
     return {"workload": f"powerlaw_mip m={inst.m} n={inst.n} nnz={inst.nnz} (zipf 2.08, rows <= 5000); nodes = "
@@ -456,7 +379,7 @@ def serving_leg(sync):
             "plain_nodes_per_s": plain, "shared_launch_1024_nodes_per_s": shared,
             "shared_speedup": shared / plain, "served_best_per_s": best, "served_speedup": best / plain,
             "served_full_vectors_best_per_s": best_full, "served_full_vectors_speedup": best_full / plain,
-            "rates": rates}
+            "plateau_under_overload": plateau, "rates": rates}
 
 
 # ---------------------------------------------------------------------------
