@@ -489,6 +489,51 @@ def summarize(d, kernel="mega_kernel"):
     print(json.dumps(out))
 
 
+def summarize_steps(d, out_md):
+    """Per-round durations of the step kernels (sweep, apply, publish) of the
+    CPU_LOOP propagations in a rocprofv3 kernel trace under d: one row per
+    round of the last propagation of each round count (the timed ones come
+    after the warmup), written as a markdown table to out_md."""
+    trace = _csv_rows(f"{d}/**/*kernel_trace.csv")
+    trace.sort(key=lambda r: int(r["Start_Timestamp"]))
+    props, cur = [], None
+    for r in trace:
+        name = r["Kernel_Name"].split("(")[0].split("::")[-1]
+        dur = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3
+        if name == "load_kernel":
+            cur = {"steps": [], "start": int(r["Start_Timestamp"])}
+        elif cur is None:
+            continue
+        elif name == "sweep_kernel":
+            cur["steps"].append([dur, 0.0, 0.0])
+        elif name == "apply_split_kernel" and cur["steps"]:
+            cur["steps"][-1][1] = dur
+        elif name == "publish_kernel" and cur["steps"]:
+            cur["steps"][-1][2] = dur
+        elif name == "unpack_kernel":
+            cur["wall"] = (int(r["End_Timestamp"]) - cur["start"]) / 1e3
+            props.append(cur)
+            cur = None
+    last = {}  # the fastest propagation of each round count (a timed one)
+    for p in props:
+        k = len(p["steps"])
+        if k not in last or p["wall"] < last[k]["wall"]:
+            last[k] = p
+    with open(out_md, "w") as fh:
+        fh.write("# Step kernels per round (CPU_LOOP launches)\n\n")
+        fh.write("Durations in microseconds from the kernel trace, one propagation per round count "
+                 "(the fastest one from its load to its unpack: a timed step).\n\n")
+        for k in sorted(last):
+            p = last[k]
+            tot = [sum(x[i] for x in p["steps"]) for i in range(3)]
+            fh.write(f"## {k} rounds: sweep {tot[0]:.0f} us, apply {tot[1]:.0f} us, publish {tot[2]:.0f} us, "
+                     f"load to unpack {p['wall']:.0f} us\n\n| round | sweep | apply | publish |\n|---|---|---|---|\n")
+            for i, x in enumerate(p["steps"]):
+                fh.write(f"| {i + 1} | {x[0]:.1f} | {x[1]:.1f} | {x[2]:.1f} |\n")
+            fh.write("\n")
+    print(f"wrote {out_md}")
+
+
 # ---------------------------------------------------------------------------
 
 def measured_copy_gbs(nbytes=1 << 30, reps=5):
@@ -604,14 +649,20 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the host baselines and the parity check")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-instances leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    ap.add_argument("--no-tune", action="store_true", help="the default tuning instead of the autotuner's pick")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the other single-instance workloads (literal Zipf(1.8) shape, cover/knapsack, chains)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB")
     ap.add_argument("--summarize", default=None, metavar="DIR")
+    ap.add_argument("--summarize-steps", default=None, nargs=2, metavar=("DIR", "OUT_MD"),
+                    help="per-round step-kernel durations from a CPU_LOOP kernel trace under DIR into OUT_MD")
     ap.add_argument("--meta-out", default=None, help="write workload facts for --summarize")
     args = ap.parse_args()
     if args.summarize:
         summarize(args.summarize)
+        return
+    if args.summarize_steps:
+        summarize_steps(*args.summarize_steps)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the propagation path runs on a single MI355X "
@@ -646,7 +697,25 @@ def main():
     root_ub = I.DeviceArray(inst.ub)
     lb = I.DeviceArray(inst.lb)
     ub = I.DeviceArray(inst.ub)
-    prob = I.Problem(inst)
+    tuning, default_ms = None, None
+    if not args.no_tune:
+        # the library's autotuner picks the layout and the persistent grid for
+        # this instance (every candidate gives the same bits); the default
+        # tuning's time is reported beside the tuned one
+        prob = I.Problem(inst)
+        dt = []
+        for _ in range(4):
+            lb.copy_from(root_lb)
+            ub.copy_from(root_ub)
+            dt.append(prob.propagate_device(lb.ptr, ub.ptr, args.sync).kernel_ms)
+        default_ms = statistics.median(dt[1:])
+        prob.close()
+        print("bench: autotuning", file=sys.stderr, flush=True)
+        t_tune = time.perf_counter()
+        tuning, _ = I.autotune(inst, args.sync)
+        tune_s = time.perf_counter() - t_tune
+        print(f"bench: tuned in {tune_s:.2f} s: {tuning}", file=sys.stderr, flush=True)
+    prob = I.Problem(inst, tuning)
 
     def step():
         lb.copy_from(root_lb)   # synchronous device-to-device copies, outside the kernel timing
@@ -705,7 +774,9 @@ def main():
         "config": {"workload": f"powerlaw_mip single MIP m={inst.m} n={inst.n} nnz={inst.nnz} "
                                f"zipf_s={inst.meta['zipf_s']} row_len=[2,50000] seed={args.seed}, "
                                f"root bounds to the fixpoint",
-                   "sync": SYNC_NAMES[args.sync], "rounds": rounds, "global_batch": 1},
+                   "sync": SYNC_NAMES[args.sync], "rounds": rounds, "global_batch": 1,
+                   "tuning": tuning if tuning is not None else "default",
+                   "default_tuning_ms": default_ms},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": kernel, "launch_ms": kernel_ms, "algorithmic_bytes_per_launch": alg,

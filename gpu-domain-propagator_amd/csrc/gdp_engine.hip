@@ -12,7 +12,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
@@ -250,6 +253,34 @@ void upload(T *dst, const T *src, size_t count, hipStream_t s) {
         check(hipMemcpyAsync(dst, src, count * sizeof(T), hipMemcpyHostToDevice, s), "H2D");
 }
 
+// Phase times of a problem's creation, printed to stderr when the
+// environment variable GDP_TRACE_CREATE is set (a diagnostic of where the
+// creation time goes: host ordering, allocations, copies, device layout).
+class CreateTrace {
+  public:
+    CreateTrace() : on_(std::getenv("GDP_TRACE_CREATE") != nullptr), t0_(Clock::now()), last_(t0_) {}
+    ~CreateTrace() {
+        if (on_) std::fprintf(stderr, "create:%s total=%.2f ms\n", line_.c_str(), ms(t0_, Clock::now()));
+    }
+    void mark(const char *phase) {
+        if (!on_) return;
+        const Clock::time_point t = Clock::now();
+        char buf[96];
+        std::snprintf(buf, sizeof(buf), " %s=%.2f", phase, ms(last_, t));
+        line_ += buf;
+        last_ = t;
+    }
+
+  private:
+    using Clock = std::chrono::steady_clock;
+    static double ms(Clock::time_point a, Clock::time_point b) {
+        return std::chrono::duration<double, std::milli>(b - a).count();
+    }
+    bool on_;
+    Clock::time_point t0_, last_;
+    std::string line_;
+};
+
 // Storage class of a column from its bound envelope: 0 = 2 x int16,
 // 1 = 2 x int32, 2 = 2 x double.
 int column_class(int vartype, const double *env_lb, const double *env_ub, int j) {
@@ -484,7 +515,7 @@ void device_layout_build(const std::vector<InstanceView> &inst, const std::vecto
                          int n_padded, const std::vector<long long> &src_start, const std::vector<int> &row_inst,
                          const std::vector<int> &row_part, const std::vector<int> &row_item, const int *d_rp,
                          const int *d_internal, int *d_col, double *d_val, int *d_cptr, int *d_crows, int *d_cseg,
-                         int *d_citem, hipStream_t s) {
+                         int *d_citem, hipStream_t s, CreateTrace &trace) {
     const int NI = static_cast<int>(inst.size());
     long long nnz = 0;
     for (const auto &v : inst) nnz += v.nnz;
@@ -514,6 +545,8 @@ void device_layout_build(const std::vector<InstanceView> &inst, const std::vecto
         check(hipMemsetAsync(d_col, 0, sizeof(int) * n_padded, s), "memset layout");
         check(hipMemsetAsync(d_val, 0, sizeof(double) * n_padded, s), "memset layout");
         check(hipMemsetAsync(t_cnt, 0, sizeof(int) * (n_ids + 1), s), "memset counts");
+        check(hipStreamSynchronize(s), "upload");
+        trace.mark("upload_csr");
         const int grid = std::max(1, std::min(2048, (mp + 3) / 4));
         hipLaunchKernelGGL(layout_rows_kernel, dim3(grid), dim3(256), 0, s, mp, t_src, t_inst, t_coff, t_col, t_val,
                            d_rp, d_internal, t_vt, d_col, d_val, t_cnt, t_cnt + n_ids);
@@ -521,6 +554,7 @@ void device_layout_build(const std::vector<InstanceView> &inst, const std::vecto
         std::vector<int> cnt(n_ids + 1);
         check(hipMemcpyAsync(cnt.data(), t_cnt, sizeof(int) * (n_ids + 1), hipMemcpyDeviceToHost, s), "D2H counts");
         check(hipStreamSynchronize(s), "layout");
+        trace.mark("layout_rows");
         if (cnt[n_ids]) throw std::runtime_error("column index out of range");
         std::vector<int> cptr(n_ids + 1, 0);
         for (int c = 0; c < n_ids; c++) cptr[c + 1] = cptr[c] + cnt[c];
@@ -530,18 +564,22 @@ void device_layout_build(const std::vector<InstanceView> &inst, const std::vecto
                            d_crows, d_cseg, d_citem);
         check(hipGetLastError(), "layout launch");
         check(hipStreamSynchronize(s), "layout");
+        trace.mark("layout_csc");
     } catch (...) {
         for (void *p : tmp) (void)hipFree(p);
         throw;
     }
     for (void *p : tmp) check(hipFree(p), "hipFree");
+    trace.mark("free_temp");
 }
 
 Problem *build(const std::vector<InstanceView> &inst, bool replicas, const Tuning *tuning) {
     if (inst.empty()) throw std::runtime_error("no instance");
+    CreateTrace trace;
     // The host orders rows and columns; the nonzeros are laid out on the
     // device, which also checks the column indices.
     for (const auto &v : inst) validate(v);
+    trace.mark("validate");
     auto *P = new Problem();
     staging().acquire();
     P->staging_held = true;
@@ -618,6 +656,7 @@ Problem *build(const std::vector<InstanceView> &inst, bool replicas, const Tunin
         P->stats.n32 = class_start[2] - class_start[1];
         P->stats.nf = n_ids - class_start[2];
 
+        trace.mark("columns");
         // ---- rows grouped by tree width: long rows first, then wide to
         // narrow; instance-major inside a bin so that items never mix units ----
         constexpr int kLongBin = 10;
@@ -667,6 +706,7 @@ Problem *build(const std::vector<InstanceView> &inst, bool replicas, const Tunin
         }
         const int n_padded = h_rp[mp];
 
+        trace.mark("rows");
         // ---- sweep items ----
         // larger group items for large instances (fewer, longer waves)
         int max_inst_nnz = 0;
@@ -755,6 +795,7 @@ Problem *build(const std::vector<InstanceView> &inst, bool replicas, const Tunin
         P->stats.m_nonempty = mp;
         P->stats.n_items = static_cast<int>(h_items.size());
 
+        trace.mark("items");
         // ---- device ----
         check(hipGetDevice(&P->device), "hipGetDevice");
         hipDeviceProp_t prop;
@@ -793,6 +834,7 @@ Problem *build(const std::vector<InstanceView> &inst, bool replicas, const Tunin
         int *d_perm = dev_alloc<int>(n_ids, o);
         int *d_word_unit = dev_alloc<int>(word_unit.size(), o);
         P->d_inv_perm = dev_alloc<int>(P->n, o);
+        trace.mark("device_setup_alloc");
         upload(d_rp, h_rp.data(), mp + 1, s);
         upload(d_sides, h_sides.data(), mp, s);
         upload(d_items, h_items.data(), h_items.size(), s);
@@ -802,8 +844,10 @@ Problem *build(const std::vector<InstanceView> &inst, bool replicas, const Tunin
         upload(d_perm, perm.data(), n_ids, s);
         upload(d_word_unit, word_unit.data(), word_unit.size(), s);
         upload(P->d_inv_perm, internal.data(), P->n, s);
+        check(hipStreamSynchronize(s), "upload");
+        trace.mark("upload_order");
         device_layout_build(inst, col_off, mp, n_ids, n_padded, src_start, h_unit, row_part, row_item, d_rp,
-                            P->d_inv_perm, d_col, d_val, d_cptr, d_crows, d_cseg, d_citem, s);
+                            P->d_inv_perm, d_col, d_val, d_cptr, d_crows, d_cseg, d_citem, s, trace);
         M.row_ptr = d_rp;
         M.sides = d_sides;
         M.col = d_col;
@@ -852,6 +896,7 @@ Problem *build(const std::vector<InstanceView> &inst, bool replicas, const Tunin
         P->sweep_grid = std::max(1, sweep_per_cu) * P->cu_count;
         P->stats.mega_threads = P->mega_threads;
         P->stats.mega_grid = P->mega_grid;
+        trace.mark("finish");
     } catch (...) {
         delete P;
         throw;

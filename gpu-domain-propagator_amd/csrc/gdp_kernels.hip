@@ -911,6 +911,19 @@ __device__ __forceinline__ void publish_phase(const DevMatrix &M, const DevState
     const int half = lane >> 5;
     const int chunks = (M.nwords + kChunk - 1) / kChunk;
     const int n_chunks = (M.replicas ? D.n_active : 1) * chunks;
+    // One unit in the launch (a single problem): the all-rows decision is
+    // taken once per workgroup (its first wave reads the unit's counters,
+    // final after the sweep's barrier) and shared through LDS instead of
+    // once per chunk after the chunk's words arrive.
+    __shared__ int single_all_rows;
+    const bool single = M.replicas && D.n_active == 1;
+    if (single) {
+        if (threadIdx.x < 64) {
+            const bool all_rows = decide_all_rows(M, D, uniform(D.active[0]), true, step);
+            if (threadIdx.x == 0) single_all_rows = all_rows ? 1 : 0;
+        }
+        __syncthreads();
+    }
     for (int c = wave; c < n_chunks; c += nwaves) {
         const int k = c / chunks;
         const int q = c - k * chunks;
@@ -931,7 +944,9 @@ __device__ __forceinline__ void publish_phase(const DevMatrix &M, const DevState
         // one decision for the chunk's unit (replicas; blocks: when the
         // chunk's words all belong to one unit, as in a single problem)
         int known = -1;
-        if (M.replicas) {
+        if (single) {
+            known = uniform(single_all_rows);
+        } else if (M.replicas) {
             known = decide_all_rows(M, D, u, true, step) ? 1 : 0;
         } else {
             const int u0 = uniform(M.word_unit[q * kChunk]);
