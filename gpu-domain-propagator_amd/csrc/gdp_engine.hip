@@ -31,6 +31,7 @@
 #include "gdp_device.h"
 #include "gdp_engine.h"
 #include "gdp_host.h"
+#include "gdp_order.h"
 
 namespace gdp {
 
@@ -70,12 +71,6 @@ thread_local std::string g_last_error;
 
 void check(hipError_t e, const char *what) {
     if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
-}
-
-int next_pow2_log(int len) {
-    int lg = 0;
-    while ((1 << lg) < len) lg++;
-    return lg;
 }
 
 template <typename T>
@@ -508,14 +503,15 @@ void ensure_units(Problem *P) {
 // The passes over the nonzeros, on the device: the instances' CSR arrays are
 // uploaded as given (concatenated), the entries are moved into the padded
 // layout (layout row r from source position src_start[r] of instance
-// row_inst[r], at the starts of d_rp), the columns' entries are counted and
-// the column-to-row index is filled.  Column indices outside an instance's
-// range are reported here.  Synchronous.
+// row_inst[r], at the starts of d_rp), the columns' entries are counted, the
+// counts scanned into the column starts and the column-to-row index filled.
+// Column indices outside an instance's range are reported here.  src_start,
+// row_inst (null for one instance), row_part and row_item are device arrays
+// of the mp layout rows.  Synchronous.
 void device_layout_build(const std::vector<InstanceView> &inst, const std::vector<int> &col_off, int mp, int n_ids,
-                         int n_padded, const std::vector<long long> &src_start, const std::vector<int> &row_inst,
-                         const std::vector<int> &row_part, const std::vector<int> &row_item, const int *d_rp,
-                         const int *d_internal, int *d_col, double *d_val, int *d_cptr, int *d_crows, int *d_cseg,
-                         int *d_citem, hipStream_t s, CreateTrace &trace) {
+                         int n_padded, const long long *src_start, const int *row_inst, const int *row_part,
+                         const int *row_item, const int *d_rp, const int *d_internal, int *d_col, double *d_val,
+                         int *d_cptr, int *d_crows, int *d_cseg, int *d_citem, hipStream_t s, CreateTrace &trace) {
     const int NI = static_cast<int>(inst.size());
     long long nnz = 0;
     for (const auto &v : inst) nnz += v.nnz;
@@ -523,13 +519,9 @@ void device_layout_build(const std::vector<InstanceView> &inst, const std::vecto
     try {
         int *t_col = dev_alloc<int>(nnz, tmp);
         double *t_val = dev_alloc<double>(nnz, tmp);
-        long long *t_src = dev_alloc<long long>(mp, tmp);
-        int *t_inst = NI > 1 ? dev_alloc<int>(mp, tmp) : nullptr;
         int *t_coff = dev_alloc<int>(NI + 1, tmp);
-        int *t_part = dev_alloc<int>(mp, tmp);
-        int *t_item = dev_alloc<int>(mp, tmp);
         int *t_vt = dev_alloc<int>(col_off[NI], tmp);
-        int *t_cnt = dev_alloc<int>(n_ids + 1, tmp);  // counts, then the fill positions
+        int *t_cnt = dev_alloc<int>(n_ids + 2, tmp);  // counts (n_ids + 1: the scan's total), the out-of-range flag
         long long at = 0;
         for (int i = 0; i < NI; i++) {
             upload(t_col + at, inst[i].col, inst[i].nnz, s);
@@ -537,30 +529,24 @@ void device_layout_build(const std::vector<InstanceView> &inst, const std::vecto
             upload(t_vt + col_off[i], inst[i].vartype, inst[i].n, s);
             at += inst[i].nnz;
         }
-        upload(t_src, src_start.data(), mp, s);
-        if (t_inst) upload(t_inst, row_inst.data(), mp, s);
         upload(t_coff, col_off.data(), NI + 1, s);
-        upload(t_part, row_part.data(), mp, s);
-        upload(t_item, row_item.data(), mp, s);
         check(hipMemsetAsync(d_col, 0, sizeof(int) * n_padded, s), "memset layout");
         check(hipMemsetAsync(d_val, 0, sizeof(double) * n_padded, s), "memset layout");
-        check(hipMemsetAsync(t_cnt, 0, sizeof(int) * (n_ids + 1), s), "memset counts");
+        check(hipMemsetAsync(t_cnt, 0, sizeof(int) * (n_ids + 2), s), "memset counts");
         check(hipStreamSynchronize(s), "upload");
         trace.mark("upload_csr");
         const int grid = std::max(1, std::min(2048, (mp + 3) / 4));
-        hipLaunchKernelGGL(layout_rows_kernel, dim3(grid), dim3(256), 0, s, mp, t_src, t_inst, t_coff, t_col, t_val,
-                           d_rp, d_internal, t_vt, d_col, d_val, t_cnt, t_cnt + n_ids);
+        hipLaunchKernelGGL(layout_rows_kernel, dim3(grid), dim3(256), 0, s, mp, src_start, row_inst, t_coff, t_col,
+                           t_val, d_rp, d_internal, t_vt, d_col, d_val, t_cnt, t_cnt + n_ids + 1);
         check(hipGetLastError(), "layout launch");
-        std::vector<int> cnt(n_ids + 1);
-        check(hipMemcpyAsync(cnt.data(), t_cnt, sizeof(int) * (n_ids + 1), hipMemcpyDeviceToHost, s), "D2H counts");
+        int bad = 0;
+        check(hipMemcpyAsync(&bad, t_cnt + n_ids + 1, sizeof(int), hipMemcpyDeviceToHost, s), "D2H flag");
         check(hipStreamSynchronize(s), "layout");
         trace.mark("layout_rows");
-        if (cnt[n_ids]) throw std::runtime_error("column index out of range");
-        std::vector<int> cptr(n_ids + 1, 0);
-        for (int c = 0; c < n_ids; c++) cptr[c + 1] = cptr[c] + cnt[c];
-        upload(d_cptr, cptr.data(), n_ids + 1, s);
-        upload(t_cnt, cptr.data(), n_ids, s);
-        hipLaunchKernelGGL(layout_csc_kernel, dim3(grid), dim3(256), 0, s, mp, d_rp, d_col, t_part, t_item, t_cnt,
+        if (bad) throw std::runtime_error("column index out of range");
+        order_exclusive_sum(t_cnt, d_cptr, n_ids + 1, s);
+        check(hipMemcpyAsync(t_cnt, d_cptr, sizeof(int) * n_ids, hipMemcpyDeviceToDevice, s), "fill positions");
+        hipLaunchKernelGGL(layout_csc_kernel, dim3(grid), dim3(256), 0, s, mp, d_rp, d_col, row_part, row_item, t_cnt,
                            d_crows, d_cseg, d_citem);
         check(hipGetLastError(), "layout launch");
         check(hipStreamSynchronize(s), "layout");
@@ -571,6 +557,95 @@ void device_layout_build(const std::vector<InstanceView> &inst, const std::vecto
     }
     for (void *p : tmp) check(hipFree(p), "hipFree");
     trace.mark("free_temp");
+}
+
+// Layout rows as ordered on the device (gdp_order.hip): host copies of the
+// totals per bin, of the first layout row and row count of each (bin key,
+// instance) run, and of the long rows' lengths; device arrays of the source
+// position and instance of each layout row.
+struct RowOrder {
+    int mp = 0;            // nonempty rows
+    int n_padded = 0;      // entries of the padded layout
+    std::vector<int> start, count;  // [key * NI + i]; key 0: long rows, key k: tree-width bin 10 - k
+    std::vector<int> long_len;      // lengths of layout rows [0, bin_rows[10])
+    long long bin_rows[11] = {0}, bin_nnz[11] = {0};
+    long long *src = nullptr;  // device, per layout row
+    int *inst = nullptr;       // device, per layout row (null for one instance)
+};
+
+// Orders the rows on the device.  Allocates the row starts d_rp (mp + 1
+// entries, padding count in the low bits) and sides d_sides (mp) into
+// `owned`, the per-row temporaries into `tmp`.
+RowOrder device_row_order(const std::vector<InstanceView> &inst, const std::vector<int> &row_off, long long nnz_total,
+                          int **d_rp, double2 **d_sides, std::vector<void *> &owned, std::vector<void *> &tmp,
+                          hipStream_t s) {
+    const int NI = static_cast<int>(inst.size());
+    const int M = row_off[NI];
+    RowOrder R;
+    std::vector<long long> nnz_off(NI + 1, 0);
+    for (int i = 0; i < NI; i++) nnz_off[i + 1] = nnz_off[i] + inst[i].nnz;
+    int *t_rp = dev_alloc<int>(static_cast<size_t>(M) + NI, tmp);
+    double *t_lhs = dev_alloc<double>(M, tmp);
+    double *t_rhs = dev_alloc<double>(M, tmp);
+    int *t_roff = dev_alloc<int>(NI + 1, tmp);
+    long long *t_noff = dev_alloc<long long>(NI + 1, tmp);
+    unsigned char *t_key = dev_alloc<unsigned char>(2 * static_cast<size_t>(M), tmp);  // keys, sorted keys
+    int *t_row = dev_alloc<int>(2 * static_cast<size_t>(M), tmp);                     // rows, sorted rows
+    unsigned long long *t_tot = dev_alloc<unsigned long long>(48, tmp);
+    for (int i = 0; i < NI; i++) {
+        upload(t_rp + row_off[i] + i, inst[i].row_ptr, static_cast<size_t>(inst[i].m) + 1, s);
+        upload(t_lhs + row_off[i], inst[i].lhs, inst[i].m, s);
+        upload(t_rhs + row_off[i], inst[i].rhs, inst[i].m, s);
+    }
+    upload(t_roff, row_off.data(), NI + 1, s);
+    upload(t_noff, nnz_off.data(), NI + 1, s);
+    check(hipMemsetAsync(t_tot, 0, sizeof(unsigned long long) * 48, s), "memset totals");
+    order_row_keys(M, NI, t_roff, t_rp, t_key, t_row, t_tot, s);
+    order_sort_rows(t_key, t_key + M, t_row, t_row + M, M, s);
+    unsigned long long tot[48];
+    check(hipMemcpyAsync(tot, t_tot, sizeof(tot), hipMemcpyDeviceToHost, s), "D2H totals");
+    check(hipStreamSynchronize(s), "row keys");
+    long long mp = 0, pad = 0;
+    for (int key = 0; key <= 10; key++) {
+        R.bin_rows[10 - key] = static_cast<long long>(tot[key]);
+        R.bin_nnz[10 - key] = static_cast<long long>(tot[16 + key]);
+        mp += static_cast<long long>(tot[key]);
+        pad += static_cast<long long>(tot[32 + key]);
+    }
+    if (nnz_total + pad > 0x7fffff00LL) throw std::runtime_error("too many nonzeros");
+    R.mp = static_cast<int>(mp);
+    const int n_long = static_cast<int>(R.bin_rows[10]);
+    *d_rp = dev_alloc<int>(static_cast<size_t>(R.mp) + 1, owned);
+    *d_sides = dev_alloc<double2>(R.mp, owned);
+    int *t_len = dev_alloc<int>(R.mp, tmp);
+    int *t_plen = dev_alloc<int>(static_cast<size_t>(R.mp) + 1, tmp);
+    int *t_start = dev_alloc<int>(11 * static_cast<size_t>(NI), tmp);
+    R.src = dev_alloc<long long>(R.mp, tmp);
+    R.inst = NI > 1 ? dev_alloc<int>(R.mp, tmp) : nullptr;
+    check(hipMemsetAsync(t_start, 0xff, sizeof(int) * 11 * NI, s), "memset starts");
+    check(hipMemsetAsync(t_plen + R.mp, 0, sizeof(int), s), "memset total");
+    order_place_rows(R.mp, NI, t_roff, t_noff, t_rp, t_lhs, t_rhs, t_key + M, t_row + M, t_len, t_plen, R.src,
+                     R.inst, *d_sides, t_start, s);
+    order_exclusive_sum(t_plen, *d_rp, R.mp + 1, s);
+    order_pad_rows(R.mp, t_len, t_plen, *d_rp, s);
+    R.start.resize(11 * static_cast<size_t>(NI));
+    R.long_len.resize(n_long);
+    check(hipMemcpyAsync(R.start.data(), t_start, sizeof(int) * 11 * NI, hipMemcpyDeviceToHost, s), "D2H starts");
+    if (n_long)
+        check(hipMemcpyAsync(R.long_len.data(), t_len, sizeof(int) * n_long, hipMemcpyDeviceToHost, s), "D2H lens");
+    check(hipMemcpyAsync(&R.n_padded, *d_rp + R.mp, sizeof(int), hipMemcpyDeviceToHost, s), "D2H total");
+    check(hipStreamSynchronize(s), "row order");
+    // runs are contiguous and ordered by (key, instance): a run ends where
+    // the next present one starts
+    R.count.assign(R.start.size(), 0);
+    int prev = -1;
+    for (size_t k = 0; k < R.start.size(); k++) {
+        if (R.start[k] < 0) continue;
+        if (prev >= 0) R.count[prev] = R.start[k] - R.start[prev];
+        prev = static_cast<int>(k);
+    }
+    if (prev >= 0) R.count[prev] = R.mp - R.start[prev];
+    return R;
 }
 
 Problem *build(const std::vector<InstanceView> &inst, bool replicas, const Tuning *tuning) {
@@ -613,98 +688,87 @@ Problem *build(const std::vector<InstanceView> &inst, bool replicas, const Tunin
 
         // ---- columns: internal ids by storage class, instance-major within a
         // class; blocks mode pads each (class, instance) range to 32 ids ----
-        std::vector<int> cls(P->n);
+        std::vector<unsigned char> cls(P->n);
+        std::vector<int> cnt_ci(3 * static_cast<size_t>(NI), 0);
         for (int i = 0; i < NI; i++)
-            for (int j = 0; j < inst[i].n; j++)
-                cls[col_off[i] + j] = T.compact_bounds ? column_class(inst[i].vartype[j], inst[i].env_lb,
-                                                                      inst[i].env_ub, j)
-                                                       : 2;
-        std::vector<int> perm, internal(P->n), word_unit;
+            for (int j = 0; j < inst[i].n; j++) {
+                const int c =
+                    T.compact_bounds ? column_class(inst[i].vartype[j], inst[i].env_lb, inst[i].env_ub, j) : 2;
+                cls[col_off[i] + j] = static_cast<unsigned char>(c);
+                cnt_ci[c * NI + i]++;
+            }
+        std::vector<int> start_ci(3 * static_cast<size_t>(NI));
         int class_start[3];
+        long long n_ids_l = 0;
         for (int c = 0; c < 3; c++) {
-            class_start[c] = static_cast<int>(perm.size());
+            class_start[c] = static_cast<int>(n_ids_l);
             for (int i = 0; i < NI; i++) {
-                for (int j = 0; j < inst[i].n; j++) {
-                    const int g = col_off[i] + j;
-                    if (cls[g] != c) continue;
-                    internal[g] = static_cast<int>(perm.size());
-                    perm.push_back(g);
-                }
-                if (!replicas)
-                    while (perm.size() % 32) perm.push_back(-1);
+                start_ci[c * NI + i] = static_cast<int>(n_ids_l);
+                n_ids_l += cnt_ci[c * NI + i];
+                if (!replicas) n_ids_l = (n_ids_l + 31) / 32 * 32;
+                if (n_ids_l > kIdMask) throw std::runtime_error("too many columns");
             }
         }
-        const int n_ids = static_cast<int>(perm.size());
-        if (n_ids > kIdMask) throw std::runtime_error("too many columns");
+        const int n_ids = static_cast<int>(n_ids_l);
+        std::vector<int> perm(n_ids, -1), internal(P->n), word_unit;
+        {
+            std::vector<int> pos = start_ci;
+            for (int i = 0; i < NI; i++)
+                for (int j = 0; j < inst[i].n; j++) {
+                    const int g = col_off[i] + j;
+                    const int id = pos[cls[g] * NI + i]++;
+                    internal[g] = id;
+                    perm[id] = g;
+                }
+        }
         const int nwords = (n_ids + 31) / 32;
         if (!replicas) {
+            // every word lies in one (class, instance) range
             word_unit.assign(nwords, 0);
-            std::vector<int> unit_of_col(P->n);
-            for (int i = 0; i < NI; i++)
-                for (int j = 0; j < inst[i].n; j++) unit_of_col[col_off[i] + j] = i;
-            for (int w = 0; w < nwords; w++) {
-                int u = -1;
-                for (int b = 0; b < 32 && w * 32 + b < n_ids; b++)
-                    if (perm[w * 32 + b] >= 0) {
-                        u = unit_of_col[perm[w * 32 + b]];
-                        break;
-                    }
-                word_unit[w] = u < 0 ? 0 : u;
-            }
+            for (int c = 0; c < 3; c++)
+                for (int i = 0; i < NI; i++) {
+                    const int a0 = start_ci[c * NI + i], cnt = cnt_ci[c * NI + i];
+                    for (int w = a0 / 32; w < (a0 + cnt + 31) / 32; w++) word_unit[w] = i;
+                }
         }
         P->stats.n16 = class_start[1] - class_start[0];
         P->stats.n32 = class_start[2] - class_start[1];
         P->stats.nf = n_ids - class_start[2];
 
         trace.mark("columns");
-        // ---- rows grouped by tree width: long rows first, then wide to
-        // narrow; instance-major inside a bin so that items never mix units ----
-        constexpr int kLongBin = 10;
-        std::vector<std::vector<std::pair<int, int>>> rows_in_bin(11);  // (instance, row)
-        for (int i = 0; i < NI; i++)
-            for (int r = 0; r < inst[i].m; r++) {
-                const int len = inst[i].row_ptr[r + 1] - inst[i].row_ptr[r];
-                if (len == 0) continue;
-                const int b = len > kSegLen ? kLongBin : next_pow2_log(len);
-                rows_in_bin[b].push_back({i, r});
-                P->stats.bin_rows[b]++;
-                P->stats.bin_nnz[b] += len;
+        // ---- device ----
+        check(hipGetDevice(&P->device), "hipGetDevice");
+        hipDeviceProp_t prop;
+        check(hipGetDeviceProperties(&prop, P->device), "hipGetDeviceProperties");
+        P->cu_count = prop.multiProcessorCount;
+        check(hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking), "hipStreamCreate");
+        check(hipEventCreate(&P->ev0), "hipEventCreate");
+        check(hipEventCreate(&P->ev1), "hipEventCreate");
+        hipStream_t s = P->stream;
+        auto &o = P->owned;
+        std::vector<void *> tmp;  // per-row temporaries of the ordering
+        struct FreeTmp {
+            std::vector<void *> &v;
+            ~FreeTmp() {
+                for (void *p : v) (void)hipFree(p);
             }
-        std::vector<std::pair<int, int>> order;
-        for (int b = kLongBin; b >= 0; b--)
-            for (auto &x : rows_in_bin[b]) order.push_back(x);
-        const int mp = static_cast<int>(order.size());
+        } free_tmp{tmp};
 
-        // rows padded to multiples of kRowAlign entries (padding: column 0,
-        // coefficient 0); the padding count sits in the low bits of h_rp
-        std::vector<int> h_len(mp), h_unit(mp);
-        long long padded = 0;
-        for (int r = 0; r < mp; r++) {
-            const auto &v = inst[order[r].first];
-            h_len[r] = v.row_ptr[order[r].second + 1] - v.row_ptr[order[r].second];
-            h_unit[r] = order[r].first;
-            padded += (h_len[r] + kRowAlign - 1) / kRowAlign * kRowAlign;
+        // ---- rows grouped by tree width: long rows first, then wide to
+        // narrow; instance-major inside a bin so that items never mix units.
+        // Ordered on the device; rows padded to multiples of kRowAlign entries
+        // (padding: column 0, coefficient 0), the padding count in the low
+        // bits of the row starts ----
+        constexpr int kLongBin = 10;
+        int *d_rp = nullptr;
+        double2 *d_sides = nullptr;
+        const RowOrder R = device_row_order(inst, row_off, nnz_total, &d_rp, &d_sides, o, tmp, s);
+        const int mp = R.mp;
+        const int n_padded = R.n_padded;
+        for (int b = 0; b <= kLongBin; b++) {
+            P->stats.bin_rows[b] = R.bin_rows[b];
+            P->stats.bin_nnz[b] = R.bin_nnz[b];
         }
-        if (padded > 0x7fffff00LL) throw std::runtime_error("too many nonzeros");
-        std::vector<int> h_rp(mp + 1);
-        std::vector<double2> h_sides(mp);
-        std::vector<long long> src_start(mp);  // position of the row's entries in the concatenated inputs
-        {
-            std::vector<long long> nnz_off(NI + 1, 0);
-            for (int i = 0; i < NI; i++) nnz_off[i + 1] = nnz_off[i] + inst[i].nnz;
-            int pos = 0;
-            for (int r = 0; r < mp; r++) {
-                const int i = order[r].first, row = order[r].second;
-                const auto &v = inst[i];
-                const int pad = (kRowAlign - h_len[r] % kRowAlign) % kRowAlign;
-                h_rp[r] = pos | pad;
-                h_sides[r] = make_double2(v.lhs[row], v.rhs[row]);
-                src_start[r] = nnz_off[i] + v.row_ptr[row];
-                pos += h_len[r] + pad;
-            }
-            h_rp[mp] = pos;
-        }
-        const int n_padded = h_rp[mp];
 
         trace.mark("rows");
         // ---- sweep items ----
@@ -718,66 +782,52 @@ Problem *build(const std::vector<InstanceView> &inst, bool replicas, const Tunin
         std::vector<int4> h_seg_rows;
         std::vector<int> h_part_row;
         int n_parts = 0;
-        int r = 0;
         {
-            const int cnt = static_cast<int>(rows_in_bin[kLongBin].size());
-            std::vector<int> giant;
-            for (int q = 0; q < cnt; q++) {
-                const int len = h_len[r + q];
-                const int nseg = (len + kSegLen - 1) / kSegLen;
-                const int unit_bits = h_unit[r + q] << kItemUnitShift;
-                if (T.max_split_segs < 0 || nseg <= T.max_split_segs) {
-                    const int lr = static_cast<int>(h_seg_rows.size());
-                    h_seg_rows.push_back(make_int4(r + q, n_parts, nseg, h_unit[r + q]));
-                    for (int sg = 0; sg < nseg; sg += kSegsPerItem)
-                        h_items.push_back(
-                            make_int4(kItemSeg | unit_bits, lr, sg, std::min(kSegsPerItem, nseg - sg)));
-                    for (int sg = 0; sg < nseg; sg++) h_part_row.push_back(lr);
-                    n_parts += nseg;
-                } else {
-                    giant.push_back(r + q);
+            std::vector<std::pair<int, int>> giant;  // (layout row, instance)
+            for (int i = 0; i < NI; i++) {
+                const int st = R.start[i], cnt = R.count[i];  // key 0: the long rows
+                for (int q = 0; q < cnt; q++) {
+                    const int row = st + q;
+                    const int nseg = (R.long_len[row] + kSegLen - 1) / kSegLen;
+                    const int unit_bits = i << kItemUnitShift;
+                    if (T.max_split_segs < 0 || nseg <= T.max_split_segs) {
+                        const int lr = static_cast<int>(h_seg_rows.size());
+                        h_seg_rows.push_back(make_int4(row, n_parts, nseg, i));
+                        for (int sg = 0; sg < nseg; sg += kSegsPerItem)
+                            h_items.push_back(
+                                make_int4(kItemSeg | unit_bits, lr, sg, std::min(kSegsPerItem, nseg - sg)));
+                        for (int sg = 0; sg < nseg; sg++) h_part_row.push_back(lr);
+                        n_parts += nseg;
+                    } else {
+                        giant.push_back({row, i});
+                    }
                 }
             }
-            for (int row : giant) h_items.push_back(make_int4(kItemLong | (h_unit[row] << kItemUnitShift), row, 1, 0));
-            P->stats.n_long_rows = cnt;
-            r += cnt;
+            for (const auto &g : giant)
+                h_items.push_back(make_int4(kItemLong | (g.second << kItemUnitShift), g.first, 1, 0));
+            P->stats.n_long_rows = static_cast<int>(R.bin_rows[kLongBin]);
         }
         for (int b = kLongBin - 1; b >= 0; b--) {
-            const int cnt = static_cast<int>(rows_in_bin[b].size());
-            if (cnt > 0) {
-                const int width = 1 << b;
-                int logg = std::min(T.group_log[b], b);
-                logg = std::max(logg, b - 2);  // at most 4 entries per lane
-                logg = std::min(logg, 6);
-                const int logk = b - logg;
-                const int per_pass = 64 >> logg;
-                int rows_per_item = std::max(1, item_nnz / width);
-                rows_per_item = ((rows_per_item + per_pass - 1) / per_pass) * per_pass;
-                int q = 0;
-                while (q < cnt) {
-                    const int u = h_unit[r + q];
-                    int nr = std::min(rows_per_item, cnt - q);
-                    for (int t = 1; t < nr; t++)
-                        if (h_unit[r + q + t] != u) {
-                            nr = t;
-                            break;
-                        }
-                    h_items.push_back(make_int4(kItemGroup | (logg << 4) | (logk << 8) | (u << kItemUnitShift),
-                                                r + q, nr, 0));
-                    q += nr;
-                }
+            const int key = kLongBin - b;
+            const int width = 1 << b;
+            int logg = std::min(T.group_log[b], b);
+            logg = std::max(logg, b - 2);  // at most 4 entries per lane
+            logg = std::min(logg, 6);
+            const int logk = b - logg;
+            const int per_pass = 64 >> logg;
+            int rows_per_item = std::max(1, item_nnz / width);
+            rows_per_item = ((rows_per_item + per_pass - 1) / per_pass) * per_pass;
+            for (int i = 0; i < NI; i++) {
+                const int st = R.start[key * NI + i], cnt = R.count[key * NI + i];
+                for (int q = 0; q < cnt; q += rows_per_item)
+                    h_items.push_back(make_int4(kItemGroup | (logg << 4) | (logk << 8) | (i << kItemUnitShift),
+                                                st + q, std::min(rows_per_item, cnt - q), 0));
             }
-            r += cnt;
         }
         if (NI >= (1 << (31 - kItemUnitShift))) throw std::runtime_error("too many instances in a batch");
 
-        // first segment of each split row (-1: not split), for the segment
-        // of every column-to-row entry
-        std::vector<int> row_part(mp, -1);
-        for (const int4 &sr : h_seg_rows) row_part[sr.x] = sr.y;
-        // per item: the row whose mark a seg / long item tests (-1: group
-        // item); per row: the group item that holds it (-1: split or long row)
-        std::vector<int> h_item_row(h_items.size(), -1), row_item(mp, -1);
+        // per item: the row whose mark a seg / long item tests (-1: group item)
+        std::vector<int> h_item_row(h_items.size(), -1);
         for (size_t q = 0; q < h_items.size(); q++) {
             const int4 it = h_items[q];
             const int kind = it.x & 15;
@@ -785,8 +835,6 @@ Problem *build(const std::vector<InstanceView> &inst, bool replicas, const Tunin
                 h_item_row[q] = h_seg_rows[it.y].x;
             else if (kind == kItemLong)
                 h_item_row[q] = it.y;
-            else
-                for (int t = 0; t < it.z; t++) row_item[it.y + t] = static_cast<int>(q);
         }
 
         P->stats.m = P->m;
@@ -796,16 +844,6 @@ Problem *build(const std::vector<InstanceView> &inst, bool replicas, const Tunin
         P->stats.n_items = static_cast<int>(h_items.size());
 
         trace.mark("items");
-        // ---- device ----
-        check(hipGetDevice(&P->device), "hipGetDevice");
-        hipDeviceProp_t prop;
-        check(hipGetDeviceProperties(&prop, P->device), "hipGetDeviceProperties");
-        P->cu_count = prop.multiProcessorCount;
-        check(hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking), "hipStreamCreate");
-        check(hipEventCreate(&P->ev0), "hipEventCreate");
-        check(hipEventCreate(&P->ev1), "hipEventCreate");
-        hipStream_t s = P->stream;
-
         DevMatrix &M = P->M;
         M.m = mp;
         M.n = n_ids;
@@ -817,9 +855,6 @@ Problem *build(const std::vector<InstanceView> &inst, bool replicas, const Tunin
         M.mark_rows = T.row_marking ? 1 : 0;
         M.n_seg_rows = static_cast<int>(h_seg_rows.size());
         M.n_seg_parts = n_parts;
-        auto &o = P->owned;
-        int *d_rp = dev_alloc<int>(mp + 1, o);
-        double2 *d_sides = dev_alloc<double2>(mp, o);
         int *d_col = dev_alloc<int>(n_padded, o);
         double *d_val = dev_alloc<double>(n_padded, o);
         int4 *d_items = dev_alloc<int4>(h_items.size(), o);
@@ -834,9 +869,10 @@ Problem *build(const std::vector<InstanceView> &inst, bool replicas, const Tunin
         int *d_perm = dev_alloc<int>(n_ids, o);
         int *d_word_unit = dev_alloc<int>(word_unit.size(), o);
         P->d_inv_perm = dev_alloc<int>(P->n, o);
+        // per layout row: first segment of a split row, group item (-1: none)
+        int *t_row_part = dev_alloc<int>(mp, tmp);
+        int *t_row_item = dev_alloc<int>(mp, tmp);
         trace.mark("device_setup_alloc");
-        upload(d_rp, h_rp.data(), mp + 1, s);
-        upload(d_sides, h_sides.data(), mp, s);
         upload(d_items, h_items.data(), h_items.size(), s);
         upload(d_item_row, h_item_row.data(), h_item_row.size(), s);
         upload(d_seg_rows, h_seg_rows.data(), h_seg_rows.size(), s);
@@ -844,9 +880,12 @@ Problem *build(const std::vector<InstanceView> &inst, bool replicas, const Tunin
         upload(d_perm, perm.data(), n_ids, s);
         upload(d_word_unit, word_unit.data(), word_unit.size(), s);
         upload(P->d_inv_perm, internal.data(), P->n, s);
+        check(hipMemsetAsync(t_row_part, 0xff, sizeof(int) * mp, s), "memset row tables");
+        check(hipMemsetAsync(t_row_item, 0xff, sizeof(int) * mp, s), "memset row tables");
+        order_row_tables(M.n_items, d_items, M.n_seg_rows, d_seg_rows, t_row_item, t_row_part, s);
         check(hipStreamSynchronize(s), "upload");
         trace.mark("upload_order");
-        device_layout_build(inst, col_off, mp, n_ids, n_padded, src_start, h_unit, row_part, row_item, d_rp,
+        device_layout_build(inst, col_off, mp, n_ids, n_padded, R.src, R.inst, t_row_part, t_row_item, d_rp,
                             P->d_inv_perm, d_col, d_val, d_cptr, d_crows, d_cseg, d_citem, s, trace);
         M.row_ptr = d_rp;
         M.sides = d_sides;
