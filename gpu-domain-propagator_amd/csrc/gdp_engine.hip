@@ -19,6 +19,8 @@
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <exception>
+#include <functional>
 #include <mutex>
 #include <thread>
 #include <stdexcept>
@@ -60,6 +62,12 @@ __global__ void layout_rows_kernel(int mp, const long long *src_start, const int
                                    int *col_cnt, int *bad);
 __global__ void layout_csc_kernel(int mp, const int *row_ptr_out, const int *col_out, const int *row_part,
                                   const int *row_item, int *fill, int *col_rows, int *col_seg, int *col_item);
+__global__ void layout_rows_short_kernel(int r_begin, int mp, const long long *src_start, const int *row_inst,
+                                         const int *inst_col_off, const int *col_in, const double *val_in,
+                                         const int *row_ptr_out, const int *internal, const int *vartype,
+                                         int *col_out, double *val_out, int *col_cnt, int *bad);
+__global__ void layout_csc_short_kernel(int r_begin, int mp, const int *row_ptr_out, const int *col_out,
+                                        const int *row_item, int *fill, int *col_rows, int *col_seg, int *col_item);
 __global__ void measure_kernel(int n, const double *lb, const double *ub, const double *wl, const double *wu,
                                const double *ll, const double *lu, double *pl, double *pu, double *fin_part,
                                int *inf_part);
@@ -500,44 +508,102 @@ void ensure_units(Problem *P) {
     if (P->capacity < 1) alloc_units(P, 1);
 }
 
-// The passes over the nonzeros, on the device: the instances' CSR arrays are
-// uploaded as given (concatenated), the entries are moved into the padded
+// The instances' CSR arrays (column indices, coefficients) and variable
+// types, concatenated on the device.  The copy runs on its own stream from
+// its own host thread, started once every other bulk copy of the creation
+// has been issued (bulk copies take the staging buffers one at a time), so
+// that it overlaps the ordering of the rows on the problem's stream and the
+// host's building of the work items.
+class CsrUpload {
+  public:
+    int *col = nullptr;
+    double *val = nullptr;
+    int *vartype = nullptr;
+
+    CsrUpload(const std::vector<InstanceView> &inst, const std::vector<int> &col_off) : inst_(inst), off_(col_off) {
+        long long nnz = 0;
+        for (const auto &v : inst) nnz += v.nnz;
+        col = dev_alloc<int>(nnz, mem_);
+        val = dev_alloc<double>(nnz, mem_);
+        vartype = dev_alloc<int>(col_off.back(), mem_);
+        check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+    }
+    CsrUpload(const CsrUpload &) = delete;
+    CsrUpload &operator=(const CsrUpload &) = delete;
+    ~CsrUpload() {
+        if (th_.joinable()) th_.join();
+        if (stream_) (void)hipStreamDestroy(stream_);
+        for (void *p : mem_) (void)hipFree(p);
+    }
+
+    void start(int device) {
+        th_ = std::thread([this, device] {
+            try {
+                check(hipSetDevice(device), "hipSetDevice");
+                long long at = 0;
+                for (size_t i = 0; i < inst_.size(); i++) {
+                    upload(col + at, inst_[i].col, inst_[i].nnz, stream_);
+                    upload(val + at, inst_[i].val, inst_[i].nnz, stream_);
+                    upload(vartype + off_[i], inst_[i].vartype, inst_[i].n, stream_);
+                    at += inst_[i].nnz;
+                }
+                check(hipStreamSynchronize(stream_), "CSR upload");
+            } catch (...) {
+                err_ = std::current_exception();
+            }
+        });
+    }
+    // Waits for the copy; rethrows its error.
+    void finish() {
+        if (th_.joinable()) th_.join();
+        if (err_) std::rethrow_exception(err_);
+    }
+
+  private:
+    const std::vector<InstanceView> &inst_;
+    const std::vector<int> &off_;
+    std::vector<void *> mem_;
+    hipStream_t stream_ = nullptr;
+    std::thread th_;
+    std::exception_ptr err_;
+};
+
+// The passes over the nonzeros, on the device: from the CSR arrays as
+// uploaded (concatenated), the entries are moved into the padded
 // layout (layout row r from source position src_start[r] of instance
 // row_inst[r], at the starts of d_rp), the columns' entries are counted, the
 // counts scanned into the column starts and the column-to-row index filled.
 // Column indices outside an instance's range are reported here.  src_start,
 // row_inst (null for one instance), row_part and row_item are device arrays
 // of the mp layout rows.  Synchronous.
-void device_layout_build(const std::vector<InstanceView> &inst, const std::vector<int> &col_off, int mp, int n_ids,
-                         int n_padded, const long long *src_start, const int *row_inst, const int *row_part,
-                         const int *row_item, const int *d_rp, const int *d_internal, int *d_col, double *d_val,
-                         int *d_cptr, int *d_crows, int *d_cseg, int *d_citem, hipStream_t s, CreateTrace &trace) {
-    const int NI = static_cast<int>(inst.size());
-    long long nnz = 0;
-    for (const auto &v : inst) nnz += v.nnz;
+void device_layout_build(CsrUpload &csr, const std::vector<int> &col_off, int mp, int n_wide, int n_ids, int n_padded,
+                         const long long *src_start, const int *row_inst, const int *row_part, const int *row_item,
+                         const int *d_rp, const int *d_internal, int *d_col, double *d_val, int *d_cptr,
+                         int *d_crows, int *d_cseg, int *d_citem, hipStream_t s, CreateTrace &trace) {
+    const int NI = static_cast<int>(col_off.size()) - 1;
     std::vector<void *> tmp;
     try {
-        int *t_col = dev_alloc<int>(nnz, tmp);
-        double *t_val = dev_alloc<double>(nnz, tmp);
         int *t_coff = dev_alloc<int>(NI + 1, tmp);
-        int *t_vt = dev_alloc<int>(col_off[NI], tmp);
         int *t_cnt = dev_alloc<int>(n_ids + 2, tmp);  // counts (n_ids + 1: the scan's total), the out-of-range flag
-        long long at = 0;
-        for (int i = 0; i < NI; i++) {
-            upload(t_col + at, inst[i].col, inst[i].nnz, s);
-            upload(t_val + at, inst[i].val, inst[i].nnz, s);
-            upload(t_vt + col_off[i], inst[i].vartype, inst[i].n, s);
-            at += inst[i].nnz;
-        }
         upload(t_coff, col_off.data(), NI + 1, s);
         check(hipMemsetAsync(d_col, 0, sizeof(int) * n_padded, s), "memset layout");
         check(hipMemsetAsync(d_val, 0, sizeof(double) * n_padded, s), "memset layout");
         check(hipMemsetAsync(t_cnt, 0, sizeof(int) * (n_ids + 2), s), "memset counts");
-        check(hipStreamSynchronize(s), "upload");
-        trace.mark("upload_csr");
-        const int grid = std::max(1, std::min(2048, (mp + 3) / 4));
-        hipLaunchKernelGGL(layout_rows_kernel, dim3(grid), dim3(256), 0, s, mp, src_start, row_inst, t_coff, t_col,
-                           t_val, d_rp, d_internal, t_vt, d_col, d_val, t_cnt, t_cnt + n_ids + 1);
+        csr.finish();
+        trace.mark("wait_csr");
+        const int *t_col = csr.col;
+        const double *t_val = csr.val;
+        const int *t_vt = csr.vartype;
+        // rows [0, n_wide) one wave per row, the short rows after them 64 per wave
+        const int grid = std::max(1, std::min(2048, (n_wide + 3) / 4));
+        const int grid_short = std::max(1, std::min(2048, (mp - n_wide + 255) / 256));
+        if (n_wide > 0)
+            hipLaunchKernelGGL(layout_rows_kernel, dim3(grid), dim3(256), 0, s, n_wide, src_start, row_inst, t_coff,
+                               t_col, t_val, d_rp, d_internal, t_vt, d_col, d_val, t_cnt, t_cnt + n_ids + 1);
+        if (mp > n_wide)
+            hipLaunchKernelGGL(layout_rows_short_kernel, dim3(grid_short), dim3(256), 0, s, n_wide, mp, src_start,
+                               row_inst, t_coff, t_col, t_val, d_rp, d_internal, t_vt, d_col, d_val, t_cnt,
+                               t_cnt + n_ids + 1);
         check(hipGetLastError(), "layout launch");
         int bad = 0;
         check(hipMemcpyAsync(&bad, t_cnt + n_ids + 1, sizeof(int), hipMemcpyDeviceToHost, s), "D2H flag");
@@ -546,8 +612,12 @@ void device_layout_build(const std::vector<InstanceView> &inst, const std::vecto
         if (bad) throw std::runtime_error("column index out of range");
         order_exclusive_sum(t_cnt, d_cptr, n_ids + 1, s);
         check(hipMemcpyAsync(t_cnt, d_cptr, sizeof(int) * n_ids, hipMemcpyDeviceToDevice, s), "fill positions");
-        hipLaunchKernelGGL(layout_csc_kernel, dim3(grid), dim3(256), 0, s, mp, d_rp, d_col, row_part, row_item, t_cnt,
-                           d_crows, d_cseg, d_citem);
+        if (n_wide > 0)
+            hipLaunchKernelGGL(layout_csc_kernel, dim3(grid), dim3(256), 0, s, n_wide, d_rp, d_col, row_part, row_item,
+                               t_cnt, d_crows, d_cseg, d_citem);
+        if (mp > n_wide)
+            hipLaunchKernelGGL(layout_csc_short_kernel, dim3(grid_short), dim3(256), 0, s, n_wide, mp, d_rp, d_col,
+                               row_item, t_cnt, d_crows, d_cseg, d_citem);
         check(hipGetLastError(), "layout launch");
         check(hipStreamSynchronize(s), "layout");
         trace.mark("layout_csc");
@@ -575,10 +645,11 @@ struct RowOrder {
 
 // Orders the rows on the device.  Allocates the row starts d_rp (mp + 1
 // entries, padding count in the low bits) and sides d_sides (mp) into
-// `owned`, the per-row temporaries into `tmp`.
+// `owned`, the per-row temporaries into `tmp`.  Calls after_copies once its
+// bulk copies are issued.
 RowOrder device_row_order(const std::vector<InstanceView> &inst, const std::vector<int> &row_off, long long nnz_total,
                           int **d_rp, double2 **d_sides, std::vector<void *> &owned, std::vector<void *> &tmp,
-                          hipStream_t s) {
+                          hipStream_t s, const std::function<void()> &after_copies) {
     const int NI = static_cast<int>(inst.size());
     const int M = row_off[NI];
     RowOrder R;
@@ -599,6 +670,7 @@ RowOrder device_row_order(const std::vector<InstanceView> &inst, const std::vect
     }
     upload(t_roff, row_off.data(), NI + 1, s);
     upload(t_noff, nnz_off.data(), NI + 1, s);
+    after_copies();
     check(hipMemsetAsync(t_tot, 0, sizeof(unsigned long long) * 48, s), "memset totals");
     order_row_keys(M, NI, t_roff, t_rp, t_key, t_row, t_tot, s);
     order_sort_rows(t_key, t_key + M, t_row, t_row + M, M, s);
@@ -760,9 +832,15 @@ Problem *build(const std::vector<InstanceView> &inst, bool replicas, const Tunin
         // (padding: column 0, coefficient 0), the padding count in the low
         // bits of the row starts ----
         constexpr int kLongBin = 10;
+        int *d_perm = dev_alloc<int>(n_ids, o);
+        P->d_inv_perm = dev_alloc<int>(P->n, o);
+        upload(d_perm, perm.data(), n_ids, s);
+        upload(P->d_inv_perm, internal.data(), P->n, s);
+        CsrUpload csr(inst, col_off);
         int *d_rp = nullptr;
         double2 *d_sides = nullptr;
-        const RowOrder R = device_row_order(inst, row_off, nnz_total, &d_rp, &d_sides, o, tmp, s);
+        const RowOrder R =
+            device_row_order(inst, row_off, nnz_total, &d_rp, &d_sides, o, tmp, s, [&] { csr.start(P->device); });
         const int mp = R.mp;
         const int n_padded = R.n_padded;
         for (int b = 0; b <= kLongBin; b++) {
@@ -866,9 +944,7 @@ Problem *build(const std::vector<InstanceView> &inst, bool replicas, const Tunin
         int *d_cseg = dev_alloc<int>(n_csc, o);
         int *d_citem = dev_alloc<int>(n_csc, o);
         int *d_item_row = dev_alloc<int>(h_item_row.size(), o);
-        int *d_perm = dev_alloc<int>(n_ids, o);
         int *d_word_unit = dev_alloc<int>(word_unit.size(), o);
-        P->d_inv_perm = dev_alloc<int>(P->n, o);
         // per layout row: first segment of a split row, group item (-1: none)
         int *t_row_part = dev_alloc<int>(mp, tmp);
         int *t_row_item = dev_alloc<int>(mp, tmp);
@@ -877,15 +953,17 @@ Problem *build(const std::vector<InstanceView> &inst, bool replicas, const Tunin
         upload(d_item_row, h_item_row.data(), h_item_row.size(), s);
         upload(d_seg_rows, h_seg_rows.data(), h_seg_rows.size(), s);
         upload(d_part_row, h_part_row.data(), h_part_row.size(), s);
-        upload(d_perm, perm.data(), n_ids, s);
         upload(d_word_unit, word_unit.data(), word_unit.size(), s);
-        upload(P->d_inv_perm, internal.data(), P->n, s);
         check(hipMemsetAsync(t_row_part, 0xff, sizeof(int) * mp, s), "memset row tables");
         check(hipMemsetAsync(t_row_item, 0xff, sizeof(int) * mp, s), "memset row tables");
         order_row_tables(M.n_items, d_items, M.n_seg_rows, d_seg_rows, t_row_item, t_row_part, s);
         check(hipStreamSynchronize(s), "upload");
         trace.mark("upload_order");
-        device_layout_build(inst, col_off, mp, n_ids, n_padded, R.src, R.inst, t_row_part, t_row_item, d_rp,
+        // rows wider than 64 entries: the long rows and the bins of widths 2^9..2^7
+        int n_wide = 0;
+        for (int key = 0; key < 4; key++)
+            for (int i = 0; i < NI; i++) n_wide += R.count[key * NI + i];
+        device_layout_build(csr, col_off, mp, n_wide, n_ids, n_padded, R.src, R.inst, t_row_part, t_row_item, d_rp,
                             P->d_inv_perm, d_col, d_val, d_cptr, d_crows, d_cseg, d_citem, s, trace);
         M.row_ptr = d_rp;
         M.sides = d_sides;

@@ -64,10 +64,8 @@ GDP_RETCODE gpu_double(int m, int n, int nnz, const int *col, const int *rp, con
     gdp::Problem *p = nullptr;
     try {
         p = gdp::problem_create(m, n, nnz, col, rp, val, lhs, rhs, vt, lb, ub, nullptr);
-        std::vector<double> l(lb, lb + n), u(ub, ub + n);
-        const RunResult r = gdp::problem_propagate(p, l.data(), u.data(), sync_type, MAX_NUM_ROUNDS);
-        std::memcpy(lb, l.data(), n * sizeof(double));
-        std::memcpy(ub, u.data(), n * sizeof(double));
+        // the results are copied into lb / ub after the propagation succeeded
+        const RunResult r = gdp::problem_propagate(p, lb, ub, sync_type, MAX_NUM_ROUNDS);
         fill_info(info, r.rounds, r.infeasible, r.capped, r.kernel_ms);
         gdp::problem_destroy(p);
         return GDP_OKAY;

@@ -1557,4 +1557,98 @@ __global__ __launch_bounds__(256) void layout_csc_kernel(int mp, const int *row_
     }
 }
 
+// Rows of at most 64 entries (the tail of the layout: rows are ordered by
+// decreasing tree width), 64 rows per wave.  Lane t holds the start, length,
+// source position and instance of row r0 + t; the wave walks the padded
+// entries of its rows 64 at a time, and each lane finds the row of its entry
+// by a binary search over the starts the lanes hold.  The padding entries
+// (zeroed beforehand) are skipped.  Same results as the kernels above.
+struct ShortRows {
+    int nrows, st, len, inst, S0, S1;
+    long long k0;
+};
+
+__device__ __forceinline__ ShortRows short_rows(int r0, int mp, const int *row_ptr_out, const long long *src_start,
+                                                const int *row_inst) {
+    const int lane = threadIdx.x & 63;
+    ShortRows R;
+    R.nrows = min(64, mp - r0);
+    const int r = r0 + min(lane, R.nrows - 1);
+    const int a = row_ptr_out[r], b = row_ptr_out[r + 1];
+    R.st = a & ~(kRowAlign - 1);
+    R.len = (b & ~(kRowAlign - 1)) - R.st - (a & (kRowAlign - 1));
+    R.k0 = src_start ? src_start[r] : 0;
+    R.inst = row_inst ? row_inst[r] : 0;
+    R.S0 = __shfl(R.st, 0, 64);
+    R.S1 = __shfl(b & ~(kRowAlign - 1), R.nrows - 1, 64);
+    return R;
+}
+
+// Lane of the row holding padded entry e (S0 <= e): the largest j < nrows
+// with start_j <= e.  Every lane of the wave takes part.
+__device__ __forceinline__ int short_row_of(const ShortRows &R, int e) {
+    int lo = 0;
+    for (int step = 32; step > 0; step >>= 1) {
+        const int j = lo + step;
+        const int sj = __shfl(R.st, j < R.nrows ? j : R.nrows - 1, 64);
+        if (j < R.nrows && sj <= e) lo = j;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(256) void layout_rows_short_kernel(int r_begin, int mp, const long long *src_start,
+                                                                const int *row_inst, const int *inst_col_off,
+                                                                const int *col_in, const double *val_in,
+                                                                const int *row_ptr_out, const int *internal,
+                                                                const int *vartype, int *col_out, double *val_out,
+                                                                int *col_cnt, int *bad) {
+    const int lane = threadIdx.x & 63;
+    const int waves = gridDim.x * (blockDim.x >> 6);
+    for (int r0 = r_begin + 64 * (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)); r0 < mp; r0 += 64 * waves) {
+        const ShortRows R = short_rows(r0, mp, row_ptr_out, src_start, row_inst);
+        for (int base = R.S0; base < R.S1; base += 64) {
+            const int e = base + lane;
+            const int j = short_row_of(R, e);
+            const int rs = __shfl(R.st, j, 64), rl = __shfl(R.len, j, 64), inst = __shfl(R.inst, j, 64);
+            const long long k0 = __shfl(R.k0, j, 64);
+            const int k = e - rs;
+            if (e >= R.S1 || k >= rl) continue;
+            const int cbase = inst_col_off[inst], lim = inst_col_off[inst + 1] - cbase;
+            const int c = col_in[k0 + k];
+            if (c < 0 || c >= lim) {
+                atomic_or_i(bad, 1);
+                continue;
+            }
+            const int g = cbase + c;
+            const int id = internal[g];
+            col_out[e] = id | (vartype[g] == kVarContinuous ? 0 : kIntFlag);
+            val_out[e] = val_in[k0 + k];
+            atomic_add_i(&col_cnt[id], 1);
+        }
+    }
+}
+
+// Column-to-row index entries of the short rows (never split: segment -1).
+__global__ __launch_bounds__(256) void layout_csc_short_kernel(int r_begin, int mp, const int *row_ptr_out,
+                                                               const int *col_out, const int *row_item, int *fill,
+                                                               int *col_rows, int *col_seg, int *col_item) {
+    const int lane = threadIdx.x & 63;
+    const int waves = gridDim.x * (blockDim.x >> 6);
+    for (int r0 = r_begin + 64 * (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)); r0 < mp; r0 += 64 * waves) {
+        const ShortRows R = short_rows(r0, mp, row_ptr_out, nullptr, nullptr);
+        const int item = row_item[r0 + min(lane, R.nrows - 1)];
+        for (int base = R.S0; base < R.S1; base += 64) {
+            const int e = base + lane;
+            const int j = short_row_of(R, e);
+            const int rs = __shfl(R.st, j, 64), rl = __shfl(R.len, j, 64), it = __shfl(item, j, 64);
+            if (e >= R.S1 || e - rs >= rl) continue;
+            const int id = col_out[e] & kIdMask;
+            const int slot = atomic_add_i(&fill[id], 1);
+            col_rows[slot] = r0 + j;
+            col_seg[slot] = -1;
+            col_item[slot] = it;
+        }
+    }
+}
+
 }  // namespace gdp
