@@ -521,6 +521,10 @@ def summarize_steps(d, out_md):
             last[k] = p
     with open(out_md, "w") as fh:
         fh.write("# Step kernels per round (CPU_LOOP launches)\n\n")
+        fh.write("Commands (from the repository root, `cd /tmp && export TMPDIR=/tmp` first):\n\n")
+        fh.write(f"    rocprofv3 --kernel-trace --stats --output-format csv -d {d} -o steps -- python3 bench.py "
+                 "--sync 0 --steps 3 --warmup 1 --no-cpu --no-batch --no-serve --no-extra --no-tune\n")
+        fh.write(f"    python3 bench.py --summarize-steps {d} {out_md}\n\n")
         fh.write("Durations in microseconds from the kernel trace, one propagation per round count "
                  "(the fastest one from its load to its unpack: a timed step).\n\n")
         for k in sorted(last):
