@@ -353,7 +353,7 @@ def serving_leg(sync):
             arr -= arr[0]
             done = np.zeros(N)
             ok = load.gdpServeLoad(srv.handle, modes[mode], N, arr.ctypes.data_as(dp), len(pool), inst.n, pool_l,
-                                   pool_u, pool_c, pool_n, 4, 256, done.ctypes.data_as(dp))
+                                   pool_u, pool_c, pool_n, 4, 256, done.ctypes.data_as(dp), None)
             if not ok:
                 raise RuntimeError(f"serving load {mode} {rate}/s failed: {I.last_error()}")
             lat = (done - arr) * 1e3

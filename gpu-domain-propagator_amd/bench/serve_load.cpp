@@ -7,7 +7,8 @@
 // each waiting until its arrival time), and one collecting thread takes
 // completed tickets and their results.  Requests cycle through a pool of
 // node vectors.  Returns each request's completion time, seconds from the
-// start, so the caller computes latencies against the arrival times.
+// start, so the caller computes latencies against the arrival times, and
+// (submitted_s, may be null) the time its submit call returned.
 //
 // Request forms (mode):
 //   0  full vectors copied by the server (gdpServerSubmit / gdpServerResult)
@@ -46,7 +47,7 @@ struct Pool {
 extern "C" int gdpServeLoad(gdp_server s, int mode, int n_requests, const double *arrival_s, int pool_size,
                             int n_vars, const double *const *pool_lb, const double *const *pool_ub,
                             const int *const *pool_cols, const int *pool_ncols, int threads, int pinned_pairs,
-                            double *done_s) {
+                            double *done_s, double *submitted_s) {
     if (!s || n_requests < 1 || pool_size < 1 || threads < 1 || mode < 0 || mode > 3) return 0;
     const size_t n = static_cast<size_t>(n_vars);
     Pool P{n_vars, pool_size, pool_lb, pool_ub, pool_cols, pool_ncols};
@@ -136,6 +137,7 @@ extern "C" int gdpServeLoad(gdp_server s, int mode, int n_requests, const double
                 }
                 rc = gdpServerSubmitDelta(s, P.ncols[v], P.cols[v], l.data(), u.data(), MAX_NUM_ROUNDS, &t);
             }
+            if (submitted_s) submitted_s[k] = now_s();
             std::lock_guard<std::mutex> lk(mu);
             if (rc != GDP_OKAY) {
                 failed = true;

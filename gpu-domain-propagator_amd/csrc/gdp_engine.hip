@@ -1789,6 +1789,7 @@ struct Server {
     };
     struct Done {
         int outbox = -1;                 // pinned outbox entry holding the result, or -1
+        int park = -1;                   // device parking entry holding the result, or -1
         std::vector<double> lb, ub;      // the result (full vector), or empty
         bool changes = false;            // the result as columns that differ from the base
         std::vector<int> cols;
@@ -1818,6 +1819,11 @@ struct Server {
     bool running = false, stop = false;
     std::string worker_error;
     int busy = 0;                         // slots holding a request
+    long long parked_results = 0;         // results retired into device parking (no free outbox entry)
+    bool trace = std::getenv("GDP_TRACE_SERVER") != nullptr;
+    double slowest_ms = 0.0, slowest_admit_ms = 0.0, slowest_copies_ms = 0.0, slowest_run_ms = 0.0;
+    int slowest_step = 0;
+    long long pageable_results = 0;       // results retired into host vectors (parking full too)
     std::deque<int64_t> completed_acc;    // completed by the stepping thread, not yet taken
     std::vector<int64_t> slot_ticket;    // -1: free
     std::unordered_map<int64_t, Done> done;
@@ -1825,6 +1831,13 @@ struct Server {
     int64_t next_ticket = 1;
     int step = 1;
     double *d_buf = nullptr;             // slots x (lb, ub) bound vectors
+    // Device parking for finished results that find no free pinned outbox
+    // entry (the callers collect more slowly than the server finishes): a
+    // device-to-device copy in the step, the copy to the host in the caller's
+    // result call, so that the stepping thread neither waits for a copy into
+    // pageable memory nor allocates host memory for it.  One entry per slot.
+    double *d_park = nullptr;
+    std::vector<int> park_free;
     double *d_base = nullptr;            // base vector of delta requests
     bool has_base = false;
     std::vector<double> base_lb, base_ub;  // host copy of the base
@@ -1888,6 +1901,7 @@ struct Server {
         if (cs) (void)hipStreamSynchronize(cs);
         if (P) (void)hipStreamSynchronize(P->stream);
         if (d_buf) (void)hipFree(d_buf);
+        if (d_park) (void)hipFree(d_park);
         if (d_base) (void)hipFree(d_base);
         if (inbox) (void)hipHostFree(inbox);
         if (outbox) (void)hipHostFree(outbox);
@@ -1922,13 +1936,16 @@ struct Server {
     double *in_ub(int e) { return inbox + 2 * n * e + n; }
     double *out_lb(int e) { return outbox + 2 * n * e; }
     double *out_ub(int e) { return outbox + 2 * n * e + n; }
+    double *park_lb(int e) { return d_park + 2 * n * e; }
+    double *park_ub(int e) { return d_park + 2 * n * e + n; }
 };
 
 Server *server_create(Problem *P, int64_t hbm_budget, int max_slots) {
     if (!P->replicas) throw std::runtime_error("serving needs a single matrix");
     if (max_slots < 1) throw std::runtime_error("max_slots must be positive");
-    // state, bound vectors, inputs of in-place requests, device staging
-    const int64_t per_slot = P->stats.unit_state_bytes + 48LL * P->n;
+    // state, bound vectors, inputs of in-place requests, device staging,
+    // result parking
+    const int64_t per_slot = P->stats.unit_state_bytes + 64LL * P->n;
     int slots = static_cast<int>(std::min<int64_t>(max_slots, hbm_budget / std::max<int64_t>(1, per_slot)));
     if (slots < 1) throw std::runtime_error("HBM budget below one node's state");
     auto *S = new Server();
@@ -1939,6 +1956,8 @@ Server *server_create(Problem *P, int64_t hbm_budget, int max_slots) {
         S->n = static_cast<size_t>(P->n);
         check(hipMalloc(&S->d_buf, sizeof(double) * 2 * S->n * slots), "hipMalloc slots");
         check(hipMalloc(&S->d_base, sizeof(double) * 2 * S->n), "hipMalloc base");
+        check(hipMalloc(&S->d_park, sizeof(double) * 2 * S->n * slots), "hipMalloc parking");
+        for (int e = slots - 1; e >= 0; e--) S->park_free.push_back(e);
         for (int k = 0; k < slots; k++) {
             S->slot_lb.push_back(S->d_buf + 2 * k * S->n);
             S->slot_ub.push_back(S->d_buf + (2 * k + 1) * S->n);
@@ -2010,7 +2029,17 @@ Server *server_create(Problem *P, int64_t hbm_budget, int max_slots) {
     return S;
 }
 
-void server_destroy(Server *s) { delete s; }
+void server_destroy(Server *s) {
+    // GDP_TRACE_SERVER: where results that found no free pinned outbox entry went
+    if (std::getenv("GDP_TRACE_SERVER"))
+        std::fprintf(stderr,
+                     "server: %d steps, results retired into device parking %lld, into host vectors %lld; slowest "
+                     "step %d: %.2f ms (admission %.2f of which copies %.2f, step %.2f, retirement %.2f)\n",
+                     s->step, s->parked_results, s->pageable_results, s->slowest_step, s->slowest_ms,
+                     s->slowest_admit_ms, s->slowest_copies_ms, s->slowest_run_ms,
+                     s->slowest_ms - s->slowest_admit_ms - s->slowest_run_ms);
+    delete s;
+}
 
 int server_slots(const Server *s) { return s->slots; }
 
@@ -2283,6 +2312,26 @@ void server_stop(Server *S) {
 
 namespace {
 int step_impl(Server *S) {
+    using Clock = std::chrono::steady_clock;
+    const Clock::time_point t_begin = Clock::now();
+    Clock::time_point t_run = t_begin, t_retire = t_begin, t_copies = t_begin;
+    // GDP_TRACE_SERVER: the slowest step and where its time went
+    struct StepClock {
+        Server *S;
+        const Clock::time_point &b, &r, &t, &c;
+        ~StepClock() {
+            if (!S->trace) return;
+            const Clock::time_point e = Clock::now();
+            const double ms = std::chrono::duration<double, std::milli>(e - b).count();
+            if (ms > S->slowest_ms) {
+                S->slowest_ms = ms;
+                S->slowest_step = S->step;
+                S->slowest_admit_ms = std::chrono::duration<double, std::milli>(r - b).count();
+                S->slowest_copies_ms = std::chrono::duration<double, std::milli>(c - b).count();
+                S->slowest_run_ms = std::chrono::duration<double, std::milli>(t - r).count();
+            }
+        }
+    } step_clock{S, t_begin, t_run, t_retire, t_copies};
     Problem *P = S->P;
     std::lock_guard<std::mutex> lock(P->run_mutex);
     S->last_completed.clear();
@@ -2392,9 +2441,12 @@ int step_impl(Server *S) {
         check(hipGetLastError(), "delta launch");
     }
     copy_pairs(S, cp2_src, cp2_dst, 2 * S->slots);
+    t_copies = Clock::now();
     if (!adm.empty())
         load_units(P, static_cast<int>(adm.size()), adm.data(), adm_lb.data(), adm_ub.data(), hint_ptr.data(),
                    hints.data(), caps.data(), S->step, false);
+    t_run = Clock::now();
+    t_retire = t_run;
     // ---- every active slot advances one round ----
     std::vector<int> active;
     for (int k = 0; k < S->slots; k++)
@@ -2416,6 +2468,7 @@ int step_impl(Server *S) {
     int err = 0;
     check(hipMemcpy(&err, P->D.global + kGlobalError, sizeof(int), hipMemcpyDeviceToHost), "D2H");
     if (err) throw std::runtime_error("device wait timed out");
+    t_retire = Clock::now();
     // ---- retire finished slots: unpack and copy the results out ----
     std::vector<int> fin;
     std::vector<RunResult> fin_r;
@@ -2520,15 +2573,23 @@ int step_impl(Server *S) {
                     S->outbox_free.pop_back();
                     hl = S->out_lb(d.outbox);
                     hu = S->out_ub(d.outbox);
+                } else if (!S->park_free.empty()) {
+                    d.park = S->park_free.back();
+                    S->park_free.pop_back();
+                    S->parked_results++;
+                    hl = S->park_lb(d.park);
+                    hu = S->park_ub(d.park);
                 } else {
                     d.lb = S->take_vec();
                     d.ub = S->take_vec();
+                    S->pageable_results++;
                     hl = d.lb.data();
                     hu = d.ub.data();
                 }
             }
-            check(hipMemcpyAsync(hl, S->slot_lb[k], n * sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
-            check(hipMemcpyAsync(hu, S->slot_ub[k], n * sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
+            const hipMemcpyKind kind = d.park >= 0 ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+            check(hipMemcpyAsync(hl, S->slot_lb[k], n * sizeof(double), kind, s), "result copy");
+            check(hipMemcpyAsync(hu, S->slot_ub[k], n * sizeof(double), kind, s), "result copy");
         }
         check(hipStreamSynchronize(s), "retire sync");
         std::lock_guard<std::mutex> lk(S->mu);
@@ -2564,8 +2625,24 @@ bool take_done(Server *S, int64_t ticket, Server::Done &d) {
 void release_done(Server *S, Server::Done &d) {
     std::lock_guard<std::mutex> lk(S->mu);
     if (d.outbox >= 0) S->outbox_free.push_back(d.outbox);
+    if (d.park >= 0) S->park_free.push_back(d.park);
     S->give_vec(std::move(d.lb));
     S->give_vec(std::move(d.ub));
+}
+
+// Copies a parked result to the caller's host arrays on the server's copy
+// stream (the steps run on the problem's stream meanwhile), each vector
+// through the library's pinned staging: a DMA into a staging buffer, then
+// the copy threads into the caller's memory.  On failure the entry is
+// returned to the server before the error is passed on.
+void download_parked(Server *S, Server::Done &d, double *lbs, double *ubs) {
+    try {
+        download_bulk(lbs, S->park_lb(d.park), sizeof(double) * S->n, S->cs);
+        download_bulk(ubs, S->park_ub(d.park), sizeof(double) * S->n, S->cs);
+    } catch (...) {
+        release_done(S, d);
+        throw;
+    }
 }
 }  // namespace
 
@@ -2584,6 +2661,8 @@ bool server_result(Server *S, int64_t ticket, double *lbs, double *ubs, RunResul
     } else if (d.outbox >= 0) {
         std::memcpy(lbs, S->out_lb(d.outbox), sizeof(double) * S->n);
         std::memcpy(ubs, S->out_ub(d.outbox), sizeof(double) * S->n);
+    } else if (d.park >= 0) {
+        download_parked(S, d, lbs, ubs);
     } else {
         std::memcpy(lbs, d.lb.data(), sizeof(double) * S->n);
         std::memcpy(ubs, d.ub.data(), sizeof(double) * S->n);
@@ -2606,6 +2685,11 @@ int server_result_changes(Server *S, int64_t ticket, std::vector<int> &cols, std
         lbs = std::move(d.clb);
         ubs = std::move(d.cub);
     } else {
+        if (d.park >= 0) {
+            d.lb = S->take_vec();
+            d.ub = S->take_vec();
+            download_parked(S, d, d.lb.data(), d.ub.data());
+        }
         const double *fl = d.outbox >= 0 ? S->out_lb(d.outbox) : d.lb.data();
         const double *fu = d.outbox >= 0 ? S->out_ub(d.outbox) : d.ub.data();
         for (size_t j = 0; j < S->n; j++)

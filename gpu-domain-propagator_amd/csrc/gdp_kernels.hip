@@ -1280,7 +1280,13 @@ __global__ __launch_bounds__(256) void load_kernel(DevMatrix M, DevState D, int 
             for (int r = tid; r < M.m; r += stride) V.marks[r] = 0;
             for (int q = tid; q < M.n_items; q += stride) V.item_mark[q] = 0;
             for (int q = tid; q < M.n_seg_rows; q += stride) V.seg_count[q] = 0u;
-            for (int q = tid; q < M.n_seg_parts; q += stride) V.seg_parts[q].stamp = 0;
+            // segment marks too: their step stamps would otherwise match a
+            // step of this run (steps restart with every run) and make it
+            // recompute segments whose columns did not move
+            for (int q = tid; q < M.n_seg_parts; q += stride) {
+                V.seg_parts[q].stamp = 0;
+                V.seg_marks[q] = 0;
+            }
         }
     }
 }

@@ -184,8 +184,9 @@ GDP_RETCODE gdpBatchPropagateDevice(gdp_problem p, double *d_lbs, double *d_ubs,
 /* ---- serving node requests with continuous batching ----
  * A server admits submitted requests into free slots of the problem's
  * device state (as many slots as hbm_budget bytes allow, at most max_slots;
- * a slot takes the unit state of gdpProblemGetStats plus 48 bytes a column:
- * its bound vectors, the input of an in-place request and a staging entry),
+ * a slot takes the unit state of gdpProblemGetStats plus 64 bytes a column:
+ * its bound vectors, the input of an in-place request, a staging entry and
+ * a parking entry for a finished result its caller has not collected yet),
  * and each gdpServerStep advances every running request by one round in
  * shared launches: requests that arrive while others run join at the next
  * step.  A request's result is bit-identical to propagating it alone. */

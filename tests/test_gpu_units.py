@@ -178,7 +178,9 @@ def test_work_profile(gpu, oracle, case):
     column the previous round changed.  A row longer than 256 entries is
     processed in 256-entry segments, and in a marked round only its segments
     holding a changed column (or not computed before in the run) are summed
-    again.  The run itself is the propagation."""
+    again.  The run itself is the propagation.  Runs of other bound vectors
+    on the same problem come first: the marks they leave carry the same
+    step numbers and must not make this run do more."""
     from test_gpu_parity import _case
     inst = _case(case)
     exp = oracle.jacobi(inst)
@@ -186,6 +188,8 @@ def test_work_profile(gpu, oracle, case):
     nonempty = lens > 0
     p = I.Problem(inst)
     try:
+        for lb, ub, _ in node_vectors(inst, oracle, 3, seed=21, from_fixpoint=False):
+            p.propagate(lb, ub, I.CPU_LOOP)
         out, rows, nnz, cols, acc = p.work_profile(inst.lb, inst.ub)
     finally:
         p.close()
@@ -270,7 +274,7 @@ def test_server_budget_limits_slots(gpu):
     inst = small_mip(seed=1)
     p = I.Problem(inst)
     try:
-        per = p.stats().unit_state_bytes + 48 * inst.n  # state, bound vectors, in-place inputs, staging
+        per = p.stats().unit_state_bytes + 64 * inst.n  # state, bound vectors, in-place inputs, staging, parking
         srv = I.Server(p, hbm_budget=3 * per + 1, max_slots=100)
         assert srv.slots == 3
         srv.close()
@@ -401,6 +405,38 @@ def test_autotune_returns_a_valid_tuning(gpu, oracle):
             assert_same(got, oracle.jacobi(x), x.name)
     finally:
         b.close()
+
+
+def test_server_results_waiting_for_collection(gpu, oracle):
+    """A running server whose caller collects nothing until every request has
+    completed: 12 copied requests with 4 slots, so 4 results go to the pinned
+    outbox, 4 are parked in device memory and 4 go to host vectors; every
+    result bit-identical to the oracle, each handed out once, and the entries
+    reused by a second round of requests."""
+    import time
+
+    inst = powerlaw_mip(seed=7, m=20000, n=10000, len_hi=5000, s=1.8)
+    nodes = node_vectors(inst, oracle, 24, seed=14)
+    p = I.Problem(inst)
+    try:
+        srv = I.Server(p, hbm_budget=1 << 30, max_slots=4)
+        assert srv.slots == 4
+        srv.start()
+        for rep in range(2):
+            batch = nodes[12 * rep:12 * rep + 12]
+            tickets = [srv.submit(lb, ub) for lb, ub, _ in batch]
+            seen, t0 = [], time.time()
+            while len(seen) < len(tickets) and time.time() - t0 < 60:
+                seen += srv.completed()
+                time.sleep(0.001)
+            assert sorted(seen) == sorted(tickets), "every request completed before any result was taken"
+            for t, (lb, ub, _) in zip(tickets, batch):
+                assert_same(srv.result(t), oracle.jacobi(inst, lb=lb, ub=ub), f"rep {rep} ticket {t}")
+                assert srv.result(t) is None
+        srv.stop()
+        srv.close()
+    finally:
+        p.close()
 
 
 def test_server_stepping_thread(gpu, oracle):
