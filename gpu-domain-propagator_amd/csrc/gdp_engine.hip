@@ -1940,6 +1940,29 @@ struct Server {
     double *park_ub(int e) { return d_park + 2 * n * e + n; }
 };
 
+// Writes zeros to every page of a fresh host allocation, split between up to
+// 8 threads.
+void prefault(void *p, size_t bytes) {
+    const int k = std::max(1, std::min(8, static_cast<int>(std::thread::hardware_concurrency())));
+    const size_t part = (bytes + k - 1) / k;
+    std::vector<std::thread> th;
+    int done_by_caller = 0;  // parts this thread takes when no more threads start
+    try {
+        for (int t = 1; t < k; t++) {
+            const size_t off = std::min(bytes, part * t), len = std::min(part, bytes - off);
+            th.emplace_back([p, off, len] { std::memset(static_cast<char *>(p) + off, 0, len); });
+        }
+    } catch (...) {
+        done_by_caller = k - 1 - static_cast<int>(th.size());
+    }
+    for (int t = k - done_by_caller; t < k; t++) {
+        const size_t off = std::min(bytes, part * t);
+        std::memset(static_cast<char *>(p) + off, 0, std::min(part, bytes - off));
+    }
+    std::memset(p, 0, std::min(part, bytes));
+    for (auto &x : th) x.join();
+}
+
 Server *server_create(Problem *P, int64_t hbm_budget, int max_slots) {
     if (!P->replicas) throw std::runtime_error("serving needs a single matrix");
     if (max_slots < 1) throw std::runtime_error("max_slots must be positive");
@@ -1998,6 +2021,11 @@ Server *server_create(Problem *P, int64_t hbm_budget, int max_slots) {
             S->inbox_free.push_back(e);
             S->outbox_free.push_back(e);
         }
+        // The host maps pinned pages on their first access, about 0.4 s for
+        // these 2 x 1 GiB at serving sizes; taken here, not by the first
+        // burst of requests (it stalled their submitters and collectors).
+        prefault(S->inbox, sizeof(double) * 2 * S->n * entries);
+        prefault(S->outbox, sizeof(double) * 2 * S->n * entries);
         // device staging for prefetched request vectors: one per slot, up to
         // 4 GiB (a small share of the device), so that requests from pinned
         // memory cross PCIe on the copy stream while steps run
