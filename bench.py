@@ -538,6 +538,58 @@ def summarize_steps(d, out_md):
     print(f"wrote {out_md}")
 
 
+def summarize_batch(d, out_md):
+    """The batched leg from a rocprofv3 kernel trace under d of a bench run
+    whose stdout (the JSON line) is in d/bench.out: the last 260 mega_kernel
+    dispatches are the 4 launches of the tuned batch and the 256 plain
+    launches, in that order."""
+    trace = [r for r in _csv_rows(f"{d}/**/*kernel_trace.csv") if "mega_kernel" in r["Kernel_Name"]]
+    trace.sort(key=lambda r: int(r["Start_Timestamp"]))
+    dur = [(int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e6 for r in trace]
+    grid = [int(r.get("Grid_Size_X") or r.get("Grid_Size") or 0) // max(1, int(r.get("Workgroup_Size_X") or
+                                                                             r.get("Workgroup_Size") or 1))
+            for r in trace]
+    wg = [int(r.get("Workgroup_Size_X") or r.get("Workgroup_Size") or 0) for r in trace]
+    if len(dur) < 260:
+        raise SystemExit(f"{len(dur)} mega_kernel dispatches, expected at least 260")
+    batch, plain = dur[-260:-256], dur[-256:]
+    line = None
+    with open(f"{d}/bench.out") as fh:
+        for ln in fh:
+            if ln.startswith("{") and '"metric"' in ln:
+                line = json.loads(ln)
+    bt = line["batched"]
+    tuned = statistics.median(batch[1:])
+    with open(out_md, "w") as fh:
+        fh.write("# mega_kernel on the batch of 256 MIPs, against 256 plain launches\n\n")
+        fh.write(f"Workload: {bt['workload']}. All instances share one persistent launch of the library's batch; "
+                 "the baseline runs one launch per instance, back to back, each on its own device-resident "
+                 "problem.\n\n")
+        fh.write("Commands (from the repository root on one MI355X, `cd /tmp && export TMPDIR=/tmp` first):\n\n")
+        fh.write(f"    rocprofv3 --kernel-trace --stats --output-format csv -d {d} -o batch -- python3 bench.py "
+                 f"--no-serve --no-cpu --no-extra --no-tune > {d}/bench.out\n")
+        fh.write(f"    python3 bench.py --summarize-batch {d} {out_md}\n\n")
+        fh.write(f"The kernel trace holds {len(dur)} `mega_kernel` dispatches in the bench's order (the 1M-row "
+                 "problem, the drop-in call, the batch with the default layout, the autotuner's trials on a "
+                 "sample of 32 instances, the batch with the tuned layout, the 256 plain launches); the figures "
+                 "are the last 260 (end minus start timestamp).\n\n")
+        fh.write("| quantity | value |\n|---|---|\n")
+        fh.write(f"| tuned batch, 4 launches | {', '.join(f'{x:.3f}' for x in batch)} ms |\n")
+        fh.write(f"| tuned batch, median of the 3 timed launches (the first warms up) | {tuned:.2f} ms |\n")
+        fh.write(f"| bench.py's own figure for the same launches (HIP events, median) | {bt['batched_ms']:.2f} ms |\n")
+        fh.write(f"| grid of the batch launch | {grid[-257]} workgroups x {wg[-257]} threads |\n")
+        fh.write(f"| 256 plain launches, sum of kernel durations | {sum(plain):.1f} ms ({min(plain):.2f} ms to "
+                 f"{max(plain):.2f} ms each) |\n")
+        fh.write(f"| bench.py's own sum for the plain launches (HIP events around each) | {bt['plain_ms']:.1f} ms |\n")
+        fh.write(f"| grids of the plain launches | from {min(grid[-256:])} to {max(grid[-256:])} workgroups, sized "
+                 "to each instance |\n")
+        fh.write(f"| speedup, kernel trace | {sum(plain):.1f} / {tuned:.2f} = {sum(plain) / tuned:.1f}x |\n")
+        fh.write(f"| speedup, bench.py | {bt['plain_ms']:.1f} / {bt['batched_ms']:.2f} = {bt['speedup']:.1f}x |\n\n")
+        fh.write("`bench.py` asserts that the tuned layout's results equal the default layout's; its full run "
+                 "(without `--no-cpu`) compares all 256 results with the oracle bit for bit.\n")
+    print(f"wrote {out_md}")
+
+
 # ---------------------------------------------------------------------------
 
 def measured_copy_gbs(nbytes=1 << 30, reps=5):
@@ -658,6 +710,8 @@ def main():
                     help="skip the other single-instance workloads (literal Zipf(1.8) shape, cover/knapsack, chains)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB")
     ap.add_argument("--summarize", default=None, metavar="DIR")
+    ap.add_argument("--summarize-batch", default=None, nargs=2, metavar=("DIR", "OUT_MD"),
+                    help="write the batched leg's kernel-trace summary (no GPU)")
     ap.add_argument("--summarize-steps", default=None, nargs=2, metavar=("DIR", "OUT_MD"),
                     help="per-round step-kernel durations from a CPU_LOOP kernel trace under DIR into OUT_MD")
     ap.add_argument("--meta-out", default=None, help="write workload facts for --summarize")
@@ -667,6 +721,9 @@ def main():
         return
     if args.summarize_steps:
         summarize_steps(*args.summarize_steps)
+        return
+    if args.summarize_batch:
+        summarize_batch(*args.summarize_batch)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the propagation path runs on a single MI355X "
