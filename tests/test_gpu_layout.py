@@ -120,3 +120,39 @@ def test_wide_batch_layout_summary():
         check_summary(b.stats(), insts, batch=True)
     finally:
         b.close()
+
+
+def singleton_rows(m=17_000_000, n=1_000_000, seed=0):
+    """m rows of one entry each (a x_j <= b or >= b) over n columns: more
+    rows than one pass of the scans' tile totals holds (4096 tiles of 4096
+    entries), so the row order's scans run in several chunks."""
+    from gdp_amd.instances import Instance
+    rng = np.random.default_rng(seed)
+    col = rng.integers(0, n, size=m).astype(np.int32)
+    val = np.where(rng.random(m) < 0.5, -1.0, 1.0) * rng.integers(1, 5, size=m)
+    is_int = rng.random(n) < 0.6
+    lb = np.where(is_int, 0.0, -100.0)
+    ub = np.where(is_int, 50.0, 100.0)
+    # each row cuts its column's domain, keeping a point x0 feasible
+    x0 = np.where(is_int, rng.integers(10, 40, size=n), rng.uniform(-50, 50, size=n))
+    act = val * x0[col]
+    slack = rng.uniform(0, 30, size=m)
+    upper = rng.random(m) < 0.5
+    lhs = np.where(upper, -INF, act - slack)
+    rhs = np.where(upper, act + slack, INF)
+    return Instance(m, n, col, np.arange(m + 1, dtype=np.int32), val, lhs, rhs, lb, ub,
+                    np.where(is_int, 1, 3).astype(np.int32), "singletons")
+
+
+def test_seventeen_million_singleton_rows(oracle):
+    inst = singleton_rows()
+    p = I.Problem(inst)
+    try:
+        check_summary(p.stats(), [inst], batch=False)
+        got = p.propagate(inst.lb, inst.ub, I.MEGAKERNEL)
+    finally:
+        p.close()
+    exp = oracle.jacobi(inst)
+    np.testing.assert_array_equal(got.lbs, exp[0])
+    np.testing.assert_array_equal(got.ubs, exp[1])
+    assert got.rounds == exp[2] and got.infeasible == exp[3]
