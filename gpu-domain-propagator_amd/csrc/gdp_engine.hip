@@ -31,6 +31,7 @@
 #include "gdp_arith.h"
 #include "gdp_copy_pool.h"
 #include "gdp_device.h"
+#include "gdp_devmem.h"
 #include "gdp_engine.h"
 #include "gdp_host.h"
 #include "gdp_order.h"
@@ -85,7 +86,7 @@ template <typename T>
 T *dev_alloc(size_t count, std::vector<void *> &owned) {
     void *p = nullptr;
     if (count == 0) count = 1;
-    check(hipMalloc(&p, count * sizeof(T)), "hipMalloc");
+    check(cached_malloc(&p, count * sizeof(T)), "hipMalloc");
     owned.push_back(p);
     return static_cast<T *>(p);
 }
@@ -289,11 +290,13 @@ class CreateTrace {
 int column_class(int vartype, const double *env_lb, const double *env_ub, int j) {
     if (vartype == kVarContinuous || !env_lb || !env_ub) return 2;
     const double l = env_lb[j], u = env_ub[j];
-    if (!(std::isfinite(l) && std::isfinite(u)) || std::floor(l) != l || std::floor(u) != u) return 2;
-    if (is_neg_inf(l) || is_pos_inf(u)) return 2;
-    if (l >= -32767.0 && u <= 32767.0 && l <= 32767.0 && u >= -32767.0) return 0;
-    if (l >= -2147483647.0 && u <= 2147483647.0 && l <= 2147483647.0 && u >= -2147483647.0) return 1;
-    return 2;
+    // both within the int32 range (false for NaN and infinities), then
+    // integral: exact through a 64-bit integer round trip in that range,
+    // without the library calls of floor()
+    if (!(l >= -2147483647.0 && l <= 2147483647.0 && u >= -2147483647.0 && u <= 2147483647.0)) return 2;
+    if (static_cast<double>(static_cast<long long>(l)) != l || static_cast<double>(static_cast<long long>(u)) != u)
+        return 2;
+    return l >= -32767.0 && l <= 32767.0 && u >= -32767.0 && u <= 32767.0 ? 0 : 1;
 }
 
 // Input contract of the row structure (the column indices are checked on
@@ -406,9 +409,9 @@ struct Problem {
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         if (stream) (void)hipStreamDestroy(stream);
-        if (state_buf) (void)hipFree(state_buf);
-        for (void *p : hint_owned) (void)hipFree(p);
-        for (void *p : owned) (void)hipFree(p);
+        if (state_buf) (void)cached_free(state_buf);
+        for (void *p : hint_owned) (void)cached_free(p);
+        for (void *p : owned) (void)cached_free(p);
         if (staging_held) staging().release();
     }
 };
@@ -481,14 +484,14 @@ void bind_state(Problem *P, char *base, int cap) {
 void alloc_units(Problem *P, int cap) {
     check(hipStreamSynchronize(P->stream), "sync");
     P->drop_graphs();
-    if (P->state_buf) (void)hipFree(P->state_buf);
+    if (P->state_buf) (void)cached_free(P->state_buf);
     P->state_buf = nullptr;
     P->capacity = 0;
     P->n_units = 0;
     DevState D{};
     UnitTables T;
     const size_t bytes = carve_state(P->M, P->replicas, P->n_inst, cap, nullptr, D, T);
-    check(hipMalloc(&P->state_buf, bytes), "hipMalloc run state");
+    check(cached_malloc(&P->state_buf, bytes), "hipMalloc run state");
     bind_state(P, static_cast<char *>(P->state_buf), cap);
     const size_t k = P->replicas ? static_cast<size_t>(cap) : 1;
     const DevMatrix &M = P->M;
@@ -533,7 +536,7 @@ class CsrUpload {
     ~CsrUpload() {
         if (th_.joinable()) th_.join();
         if (stream_) (void)hipStreamDestroy(stream_);
-        for (void *p : mem_) (void)hipFree(p);
+        for (void *p : mem_) (void)cached_free(p);
     }
 
     void start(int device) {
@@ -622,10 +625,10 @@ void device_layout_build(CsrUpload &csr, const std::vector<int> &col_off, int mp
         check(hipStreamSynchronize(s), "layout");
         trace.mark("layout_csc");
     } catch (...) {
-        for (void *p : tmp) (void)hipFree(p);
+        for (void *p : tmp) (void)cached_free(p);
         throw;
     }
-    for (void *p : tmp) check(hipFree(p), "hipFree");
+    for (void *p : tmp) check(cached_free(p), "hipFree");
     trace.mark("free_temp");
 }
 
@@ -822,7 +825,7 @@ Problem *build(const std::vector<InstanceView> &inst, bool replicas, const Tunin
         struct FreeTmp {
             std::vector<void *> &v;
             ~FreeTmp() {
-                for (void *p : v) (void)hipFree(p);
+                for (void *p : v) (void)cached_free(p);
             }
         } free_tmp{tmp};
 
@@ -1025,7 +1028,7 @@ Problem *build(const std::vector<InstanceView> &inst, bool replicas, const Tunin
 void reserve_hints(Problem *P, size_t count) {
     if (count <= P->hints_cap) return;
     check(hipStreamSynchronize(P->stream), "sync");
-    for (void *p : P->hint_owned) (void)hipFree(p);
+    for (void *p : P->hint_owned) (void)cached_free(p);
     P->hint_owned.clear();
     P->hints_cap = std::max<size_t>(count, 1024);
     P->d_hints = dev_alloc<int>(P->hints_cap, P->hint_owned);
@@ -1343,7 +1346,7 @@ RunResult problem_propagate(Problem *P, double *lbs, double *ubs, int sync_type,
     check_no_server(P);
     const int n = P->n;
     double *d_b = nullptr;
-    check(hipMalloc(&d_b, 2 * sizeof(double) * static_cast<size_t>(n)), "hipMalloc bounds");
+    check(cached_malloc(&d_b, 2 * sizeof(double) * static_cast<size_t>(n)), "hipMalloc bounds");
     RunResult res;
     try {
         upload(d_b, lbs, n, P->stream);
@@ -1353,10 +1356,10 @@ RunResult problem_propagate(Problem *P, double *lbs, double *ubs, int sync_type,
         download(ubs, d_b + n, n, P->stream);
     } catch (...) {
         (void)hipStreamSynchronize(P->stream);
-        (void)hipFree(d_b);
+        (void)cached_free(d_b);
         throw;
     }
-    check(hipFree(d_b), "hipFree");
+    check(cached_free(d_b), "hipFree");
     return res;
 }
 
@@ -1388,7 +1391,7 @@ RunResult problem_work_profile(Problem *P, double *lbs, double *ubs, int max_rou
     for (int q = 0; q < M.n_seg_rows; q++) split_of[seg_rows[q].x] = q;
     std::vector<int> seg_marks(std::max(1, M.n_seg_parts)), seg_stamp(std::max(1, M.n_seg_parts), 0);
     double *d_b = nullptr;
-    check(hipMalloc(&d_b, 2 * sizeof(double) * static_cast<size_t>(n)), "hipMalloc bounds");
+    check(cached_malloc(&d_b, 2 * sizeof(double) * static_cast<size_t>(n)), "hipMalloc bounds");
     RunResult res;
     work.clear();
     try {
@@ -1460,10 +1463,10 @@ RunResult problem_work_profile(Problem *P, double *lbs, double *ubs, int max_rou
         check(hipMemcpy(lbs, d_b, n * sizeof(double), hipMemcpyDeviceToHost), "D2H lbs");
         check(hipMemcpy(ubs, d_b + n, n * sizeof(double), hipMemcpyDeviceToHost), "D2H ubs");
     } catch (...) {
-        (void)hipFree(d_b);
+        (void)cached_free(d_b);
         throw;
     }
-    check(hipFree(d_b), "hipFree");
+    check(cached_free(d_b), "hipFree");
     return res;
 }
 
@@ -1483,7 +1486,7 @@ RunResult problem_propagate_measured(Problem *P, double *lbs, double *ubs, int m
     // then the workgroup partials and the per-round sums
     const size_t bytes = 8 * nb + gx * (sizeof(double) + sizeof(int)) + kMaxRounds * (sizeof(double) + sizeof(int));
     char *buf = nullptr;
-    check(hipMalloc(&buf, bytes), "hipMalloc measure buffers");
+    check(cached_malloc(&buf, bytes), "hipMalloc measure buffers");
     double *d_b = reinterpret_cast<double *>(buf);
     double *d_prev = d_b + 2 * static_cast<size_t>(n);
     double *d_w = d_prev + 2 * static_cast<size_t>(n);
@@ -1535,10 +1538,10 @@ RunResult problem_propagate_measured(Problem *P, double *lbs, double *ubs, int m
         check(hipMemcpy(lbs, d_b, nb, hipMemcpyDeviceToHost), "D2H lbs");
         check(hipMemcpy(ubs, d_b + n, nb, hipMemcpyDeviceToHost), "D2H ubs");
     } catch (...) {
-        (void)hipFree(buf);
+        (void)cached_free(buf);
         throw;
     }
-    check(hipFree(buf), "hipFree");
+    check(cached_free(buf), "hipFree");
     return res;
 }
 
@@ -1680,7 +1683,7 @@ Tuning autotune(const std::vector<InstanceView> &instances, const double *lbs, c
     for (const auto &v : instances) n_total += v.n;
     // device copies of the bounds every trial starts from
     double *d = nullptr;
-    check(hipMalloc(&d, sizeof(double) * 4 * n_total), "hipMalloc");
+    check(cached_malloc(&d, sizeof(double) * 4 * n_total), "hipMalloc");
     double *root_l = d, *root_u = d + n_total, *work_l = d + 2 * n_total, *work_u = d + 3 * n_total;
     Tuning best = start ? *start : Tuning();
     double best_ms = 1e300;
@@ -1761,10 +1764,10 @@ Tuning autotune(const std::vector<InstanceView> &instances, const double *lbs, c
             }
         }
     } catch (...) {
-        (void)hipFree(d);
+        (void)cached_free(d);
         throw;
     }
-    check(hipFree(d), "hipFree");
+    check(cached_free(d), "hipFree");
     return best;
 }
 
@@ -1900,30 +1903,30 @@ struct Server {
         // write the device buffers freed below
         if (cs) (void)hipStreamSynchronize(cs);
         if (P) (void)hipStreamSynchronize(P->stream);
-        if (d_buf) (void)hipFree(d_buf);
-        if (d_park) (void)hipFree(d_park);
-        if (d_base) (void)hipFree(d_base);
+        if (d_buf) (void)cached_free(d_buf);
+        if (d_park) (void)cached_free(d_park);
+        if (d_base) (void)cached_free(d_base);
         if (inbox) (void)hipHostFree(inbox);
         if (outbox) (void)hipHostFree(outbox);
         for (void *q : {static_cast<void *>(d_kcols), static_cast<void *>(d_off), static_cast<void *>(d_kl),
                         static_cast<void *>(d_ku)})
-            if (q) (void)hipFree(q);
+            if (q) (void)cached_free(q);
         if (h_off) (void)hipHostFree(h_off);
         for (void *q : {static_cast<void *>(h_ccnt), static_cast<void *>(h_ccols), static_cast<void *>(h_cl),
                         static_cast<void *>(h_cu)})
             if (q) (void)hipHostFree(q);
         for (void *q : {static_cast<void *>(d_ccnt), static_cast<void *>(d_ccols), static_cast<void *>(d_cl),
                         static_cast<void *>(d_cu)})
-            if (q) (void)hipFree(q);
+            if (q) (void)cached_free(q);
         for (void *q : {static_cast<void *>(d_dptr), static_cast<void *>(d_dcols), static_cast<void *>(d_dlb),
                         static_cast<void *>(d_dub), static_cast<void *>(d_dl), static_cast<void *>(d_du)})
-            if (q) (void)hipFree(q);
+            if (q) (void)cached_free(q);
         for (void *q : {static_cast<void *>(d_orig), static_cast<void *>(d_blp), static_cast<void *>(d_bup),
                         static_cast<void *>(d_cp)})
-            if (q) (void)hipFree(q);
+            if (q) (void)cached_free(q);
         for (hipEvent_t e : stage_ev) (void)hipEventDestroy(e);
         if (cs_tail) (void)hipEventDestroy(cs_tail);
-        if (d_stage) (void)hipFree(d_stage);
+        if (d_stage) (void)cached_free(d_stage);
         if (cs) (void)hipStreamDestroy(cs);
         if (attached) {
             std::lock_guard<std::mutex> lock(P->run_mutex);
@@ -1977,9 +1980,9 @@ Server *server_create(Problem *P, int64_t hbm_budget, int max_slots) {
         S->slots = slots;
         S->queue_cap = static_cast<size_t>(slots);
         S->n = static_cast<size_t>(P->n);
-        check(hipMalloc(&S->d_buf, sizeof(double) * 2 * S->n * slots), "hipMalloc slots");
-        check(hipMalloc(&S->d_base, sizeof(double) * 2 * S->n), "hipMalloc base");
-        check(hipMalloc(&S->d_park, sizeof(double) * 2 * S->n * slots), "hipMalloc parking");
+        check(cached_malloc(&S->d_buf, sizeof(double) * 2 * S->n * slots), "hipMalloc slots");
+        check(cached_malloc(&S->d_base, sizeof(double) * 2 * S->n), "hipMalloc base");
+        check(cached_malloc(&S->d_park, sizeof(double) * 2 * S->n * slots), "hipMalloc parking");
         for (int e = slots - 1; e >= 0; e--) S->park_free.push_back(e);
         for (int k = 0; k < slots; k++) {
             S->slot_lb.push_back(S->d_buf + 2 * k * S->n);
@@ -1990,25 +1993,25 @@ Server *server_create(Problem *P, int64_t hbm_budget, int max_slots) {
         S->slot_out_lb.assign(slots, nullptr);
         S->slot_out_ub.assign(slots, nullptr);
         S->slot_inplace.assign(slots, 0);
-        check(hipMalloc(&S->d_orig, sizeof(double) * 2 * S->n * slots), "hipMalloc inputs");
-        check(hipMalloc(&S->d_blp, sizeof(double *) * slots), "hipMalloc");
-        check(hipMalloc(&S->d_bup, sizeof(double *) * slots), "hipMalloc");
-        check(hipMalloc(&S->d_cp, sizeof(double *) * 4 * slots), "hipMalloc");
+        check(cached_malloc(&S->d_orig, sizeof(double) * 2 * S->n * slots), "hipMalloc inputs");
+        check(cached_malloc(&S->d_blp, sizeof(double *) * slots), "hipMalloc");
+        check(cached_malloc(&S->d_bup, sizeof(double *) * slots), "hipMalloc");
+        check(cached_malloc(&S->d_cp, sizeof(double *) * 4 * slots), "hipMalloc");
         // changed-column regions for finished delta requests (kDiffCap each)
         S->diff_slots = slots;
         const size_t dc = static_cast<size_t>(slots) * Server::kDiffCap;
-        check(hipMalloc(&S->d_ccnt, sizeof(int) * slots), "hipMalloc");
-        check(hipMalloc(&S->d_ccols, sizeof(int) * dc), "hipMalloc");
-        check(hipMalloc(&S->d_cl, sizeof(double) * dc), "hipMalloc");
-        check(hipMalloc(&S->d_cu, sizeof(double) * dc), "hipMalloc");
+        check(cached_malloc(&S->d_ccnt, sizeof(int) * slots), "hipMalloc");
+        check(cached_malloc(&S->d_ccols, sizeof(int) * dc), "hipMalloc");
+        check(cached_malloc(&S->d_cl, sizeof(double) * dc), "hipMalloc");
+        check(cached_malloc(&S->d_cu, sizeof(double) * dc), "hipMalloc");
         check(hipHostMalloc(reinterpret_cast<void **>(&S->h_ccnt), sizeof(int) * slots, 0), "hipHostMalloc");
         check(hipHostMalloc(reinterpret_cast<void **>(&S->h_ccols), sizeof(int) * dc, 0), "hipHostMalloc");
         check(hipHostMalloc(reinterpret_cast<void **>(&S->h_cl), sizeof(double) * dc, 0), "hipHostMalloc");
         check(hipHostMalloc(reinterpret_cast<void **>(&S->h_cu), sizeof(double) * dc, 0), "hipHostMalloc");
-        check(hipMalloc(&S->d_kcols, sizeof(int) * dc), "hipMalloc");
-        check(hipMalloc(&S->d_kl, sizeof(double) * dc), "hipMalloc");
-        check(hipMalloc(&S->d_ku, sizeof(double) * dc), "hipMalloc");
-        check(hipMalloc(&S->d_off, sizeof(int) * (slots + 1)), "hipMalloc");
+        check(cached_malloc(&S->d_kcols, sizeof(int) * dc), "hipMalloc");
+        check(cached_malloc(&S->d_kl, sizeof(double) * dc), "hipMalloc");
+        check(cached_malloc(&S->d_ku, sizeof(double) * dc), "hipMalloc");
+        check(cached_malloc(&S->d_off, sizeof(int) * (slots + 1)), "hipMalloc");
         check(hipHostMalloc(reinterpret_cast<void **>(&S->h_off), sizeof(int) * (slots + 1), 0), "hipHostMalloc");
         // pinned staging for request and result vectors (up to 1 GiB each)
         const int entries = static_cast<int>(
@@ -2033,7 +2036,7 @@ Server *server_create(Problem *P, int64_t hbm_budget, int max_slots) {
             std::max<int64_t>(1, std::min<int64_t>(slots, (4LL << 30) / static_cast<int64_t>(16 * S->n))));
         check(hipStreamCreateWithFlags(&S->cs, hipStreamNonBlocking), "hipStreamCreate");
         check(hipEventCreateWithFlags(&S->cs_tail, hipEventDisableTiming), "hipEventCreate");
-        check(hipMalloc(&S->d_stage, sizeof(double) * 2 * S->n * n_stage), "hipMalloc staging");
+        check(cached_malloc(&S->d_stage, sizeof(double) * 2 * S->n * n_stage), "hipMalloc staging");
         for (int e = n_stage - 1; e >= 0; e--) {
             hipEvent_t ev;
             check(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "hipEventCreate");
@@ -2254,9 +2257,9 @@ namespace {
 
 template <typename T>
 void grow(T *&p, size_t count) {
-    if (p) (void)hipFree(p);
+    if (p) (void)cached_free(p);
     p = nullptr;
-    check(hipMalloc(&p, sizeof(T) * std::max<size_t>(1, count)), "hipMalloc");
+    check(cached_malloc(&p, sizeof(T) * std::max<size_t>(1, count)), "hipMalloc");
 }
 
 }  // namespace

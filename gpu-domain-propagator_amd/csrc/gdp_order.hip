@@ -28,6 +28,7 @@
 
 #include "gdp_arith.h"
 #include "gdp_device.h"
+#include "gdp_devmem.h"
 #include "gdp_order.h"
 
 namespace gdp {
@@ -281,7 +282,7 @@ void order_sort_rows(const unsigned char *keys, unsigned char *keys_sorted, cons
     if (tiles == 0) return;
     const int nh = kSortKeys * tiles;
     int *tmp = nullptr;  // histogram, then the scan's tile totals
-    check_order(hipMalloc(&tmp, sizeof(int) * (static_cast<size_t>(nh) + (nh + kTile - 1) / kTile)), "hipMalloc");
+    check_order(cached_malloc(&tmp, sizeof(int) * (static_cast<size_t>(nh) + (nh + kTile - 1) / kTile)), "hipMalloc");
     hipError_t e = hipSuccess;
     try {
         hipLaunchKernelGGL(key_hist_kernel, dim3(tiles), dim3(kTileThreads), 0, s, static_cast<long long>(M), keys,
@@ -294,10 +295,10 @@ void order_sort_rows(const unsigned char *keys, unsigned char *keys_sorted, cons
         e = hipStreamSynchronize(s);
     } catch (...) {
         (void)hipStreamSynchronize(s);
-        (void)hipFree(tmp);
+        (void)cached_free(tmp);
         throw;
     }
-    (void)hipFree(tmp);
+    (void)cached_free(tmp);
     check_order(e, "row sort");
 }
 
@@ -313,17 +314,17 @@ void order_exclusive_sum(const int *in, int *out, int n, hipStream_t s) {
     const int tiles = (n + kTile - 1) / kTile;
     if (tiles == 0) return;
     int *partial = nullptr;
-    check_order(hipMalloc(&partial, sizeof(int) * tiles), "hipMalloc");
+    check_order(cached_malloc(&partial, sizeof(int) * tiles), "hipMalloc");
     hipError_t e = hipSuccess;
     try {
         exclusive_scan_async(in, out, n, partial, s);
         e = hipStreamSynchronize(s);
     } catch (...) {
         (void)hipStreamSynchronize(s);
-        (void)hipFree(partial);
+        (void)cached_free(partial);
         throw;
     }
-    (void)hipFree(partial);
+    (void)cached_free(partial);
     check_order(e, "scan");
 }
 
