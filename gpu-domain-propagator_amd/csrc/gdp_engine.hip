@@ -11,6 +11,8 @@
 // several bound vectors or instances can share the launches (units).
 #include <hip/hip_runtime.h>
 
+#include <sched.h>
+
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -136,9 +138,22 @@ class Staging {
             }
         return true;
     }
-    // With `use` held: copies `bytes` from src to dst with up to 8 host threads.
-    void copy(void *dst, const void *src, size_t bytes) {
-        pool_.copy(dst, src, bytes, std::max(1, std::min(8, static_cast<int>(std::thread::hardware_concurrency()))));
+    // With `use` held: copies `bytes` from src to dst with the copy threads.
+    void copy(void *dst, const void *src, size_t bytes) { pool_.copy(dst, src, bytes, copy_threads()); }
+
+    // Threads of a bulk copy: GDP_COPY_THREADS if set (1..64), else 8, at
+    // most the CPUs this process may run on.
+    static int copy_threads() {
+        static const int k = [] {
+            int usable = static_cast<int>(std::thread::hardware_concurrency());
+            cpu_set_t set;
+            if (sched_getaffinity(0, sizeof(set), &set) == 0) usable = CPU_COUNT(&set);
+            int want = 8;
+            if (const char *e = std::getenv("GDP_COPY_THREADS")) want = std::atoi(e);
+            want = std::max(1, std::min(64, want));
+            return std::max(1, std::min(want, std::max(1, usable)));
+        }();
+        return k;
     }
 
   private:
