@@ -518,3 +518,28 @@ def test_server_stepping_thread(gpu, oracle):
         for _, a, b in pins:
             a.free()
             b.free()
+
+
+def test_problems_of_changing_sizes_reuse_device_memory(gpu, oracle):
+    """Problems of different sizes created and destroyed in an interleaved
+    order, some kept alive across others, each propagated through the
+    persistent kernel and through host-driven steps: the device blocks that
+    freed problems leave behind are handed to later ones, and every result
+    stays bit-identical to the oracle."""
+    rng = np.random.default_rng(17)
+    alive = []
+    for it in range(12):
+        m = int(rng.choice([300, 2000, 9000, 30000]))
+        inst = small_mip(seed=200 + it, m=m, n=max(2, m // 2), len_hi=int(rng.choice([5, 30, 400])))
+        p = I.Problem(inst)
+        exp = oracle.jacobi(inst)
+        for sync in (I.MEGAKERNEL, I.CPU_LOOP):
+            assert_same(p.propagate(inst.lb, inst.ub, sync), exp, f"iteration {it} sync {sync}")
+        alive.append((p, inst, exp))
+        if len(alive) > 3 or rng.random() < 0.4:
+            q, qi, qe = alive.pop(int(rng.integers(0, len(alive))))
+            assert_same(q.propagate(qi.lb, qi.ub, I.MEGAKERNEL), qe, f"iteration {it}, an older problem")
+            q.close()
+    for q, qi, qe in alive:
+        assert_same(q.propagate(qi.lb, qi.ub, I.GPU_LOOP), qe, "the last problems")
+        q.close()
