@@ -1048,7 +1048,9 @@ __global__ __launch_bounds__(256) void publish_kernel(DevMatrix M, DevState D, i
 // stores, so no cache write-back is needed.  With `invalidate`, each
 // workgroup invalidates its caches after the barrier so that its next
 // gathers of the snapshot cannot hit lines cached before the publish.
-__device__ bool grid_barrier(const DevState &D, unsigned nblocks, bool invalidate) {
+// Always inlined: as a call it makes the caller spill its DevState argument
+// to scratch and lose a wave of occupancy in the persistent kernel.
+__device__ __forceinline__ bool grid_barrier(const DevState &D, unsigned nblocks, bool invalidate) {
     __shared__ int ok_sh;
     mem_wait();
     __syncthreads();
