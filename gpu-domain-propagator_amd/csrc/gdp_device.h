@@ -69,6 +69,18 @@ constexpr int kSegsPerItem = 4;
 constexpr int kItemUnitShift = 12;  // items[].x >> 12: unit of the item (blocks mode)
 // Most units of one persistent launch whose activity its workgroups keep in LDS.
 constexpr int kMaxFilterUnits = 4096;
+// Bytes of the persistent kernel's dynamic LDS in a launch of several units:
+// the working / all-rows bits and first steps of up to kMaxFilterUnits units.
+constexpr int kFilterLdsBytes = (kMaxFilterUnits / 32) * 4 * 2 + kMaxFilterUnits * 4;
+// Binary columns (integer columns whose bound envelope lies in [0, 1]) have
+// internal ids [0, n_bin) in replicas mode, and each unit keeps their bounds
+// also as 2-bit codes, 16 columns to a 32-bit word: bit 0 = (lb >= 1), bit
+// 1 = (ub < 1).  As bounds only tighten during a run, a code's bits are only
+// ever set, so the publish of a step ORs new bits in.  A persistent launch of
+// one unit copies the codes of the first n_lds binary columns into LDS at
+// the start of every step and reads their bounds there instead of gathering
+// their records.  At most kMaxLdsBinBytes of LDS per workgroup.
+constexpr int kMaxLdsBinBytes = 64 * 1024;
 
 // Rows start at multiples of kRowAlign entries (padding entries hold column
 // 0 and coefficient 0 and are never used); the low bits of row_ptr[r] hold
@@ -163,6 +175,8 @@ struct DevMatrix {
     int snap_off32;        // byte offset of S32 in a unit's snapshot block
     int snap_offF;         // byte offset of SF
     long long snap_stride; // bytes between the snapshot blocks of two units
+    int n_bin;             // binary columns: internal ids [0, n_bin) (replicas mode; blocks: 0)
+    int sb_words;          // words of a unit's binary codes: ceil(n_bin / 64) * 4
 };
 
 struct DevState {
@@ -188,6 +202,8 @@ struct DevState {
     int *apply_tasks;      // unit * n_seg_parts + segment, per step
     const int *active;     // units of the launch (replicas mode)
     int n_active;
+    int n_lds;             // binary columns read from LDS (persistent launch of one unit; else 0)
+    unsigned *SB;          // 2-bit codes of the binary columns (sb_words per unit)
 };
 
 // State arrays of unit u (replicas mode); the shared arrays are unchanged.
@@ -209,6 +225,7 @@ __device__ __forceinline__ DevState unit_view(const DevMatrix &M, const DevState
     V.seg_marks += k * M.n_seg_parts;
     V.seg_state += k * M.n_seg_rows;
     V.seg_count += k * M.n_seg_rows;
+    V.SB += k * M.sb_words;
     return V;
 }
 

@@ -63,6 +63,8 @@ struct ProblemStats {
     int64_t unit_state_bytes = 0;  // device bytes of one unit's state (replicas)
     int mega_threads = 0;          // persistent kernel: workgroup size
     int mega_grid = 0;             // and its workgroups
+    int n_bin = 0;                 // binary columns (envelope in [0, 1]; single problems)
+    int n_lds = 0;                 // of them, read from LDS by one-unit persistent launches
 };
 
 // One instance of a batch (caller's arrays, CSR, zero-based).

@@ -43,7 +43,7 @@ namespace gdp {
 __global__ void sweep_kernel(DevMatrix M, DevState D, int step, int guard, int first_step);
 __global__ void apply_split_kernel(DevMatrix M, DevState D, int step, int guard, int first_step);
 __global__ void publish_kernel(DevMatrix M, DevState D, int step, int guard, int first_step, int n_units);
-template <int kThreads>
+template <int kThreads, bool kLds>
 __global__ void mega_kernel(DevMatrix M, DevState D, int first_step, int max_steps, int n_units);
 __global__ void unit_reset_kernel(DevState D, int n_list, const int4 *params, int start_step, int clear_status);
 __global__ void load_kernel(DevMatrix M, DevState D, int n_list, const int *list, const double *const *lbp,
@@ -399,9 +399,11 @@ struct Problem {
     int device = 0;
     int cu_count = 0;
     int mega_grid = 0;
+    int n_lds = 0;                    // binary columns a one-unit persistent launch reads from LDS
     int mega_threads = 0;             // workgroup size of the persistent kernel
     int sweep_grid = 0;               // workgroups of the step sweep kernel (all resident)
     const void *mega_fn = nullptr;
+    const void *mega_fn_lds = nullptr;  // the same reading binary columns from LDS (one unit)
     // captured GPU_LOOP graphs, one per chunk of steps, for one launch shape
     std::vector<hipGraphExec_t> graphs;
     std::vector<int> graph_last;
@@ -460,6 +462,7 @@ size_t carve_state(const DevMatrix &M, bool replicas, int n_inst, int cap, char 
     D.AX = reinterpret_cast<double2 *>(take(k * M.n_int * sizeof(double2)));
     D.AF = reinterpret_cast<double2 *>(take(k * nf * sizeof(double2)));
     D.bm = reinterpret_cast<unsigned *>(take(k * M.nwords * sizeof(unsigned)));
+    D.SB = reinterpret_cast<unsigned *>(take(k * M.sb_words * sizeof(unsigned)));
     D.marks = reinterpret_cast<int *>(take(k * M.m * sizeof(int)));
     D.item_mark = reinterpret_cast<int *>(take(k * M.n_items * sizeof(int)));
     D.seg_parts = reinterpret_cast<SegPart *>(take(k * M.n_seg_parts * sizeof(SegPart)));
@@ -800,13 +803,27 @@ Problem *build(const std::vector<InstanceView> &inst, bool replicas, const Tunin
             }
         }
         const int n_ids = static_cast<int>(n_ids_l);
+        // replicas mode: the binary columns (int16 class, envelope in [0, 1])
+        // take the first ids of their class
+        std::vector<unsigned char> binary;
+        int n_bin = 0;
+        if (replicas && NI == 1) {
+            binary.assign(P->n, 0);
+            for (int j = 0; j < inst[0].n; j++)
+                if (cls[j] == 0 && inst[0].env_lb[j] >= 0.0 && inst[0].env_ub[j] <= 1.0) {
+                    binary[j] = 1;
+                    n_bin++;
+                }
+        }
         std::vector<int> perm(n_ids, -1), internal(P->n), word_unit;
         {
             std::vector<int> pos = start_ci;
+            int bin_pos = start_ci[0];
+            if (n_bin) pos[0] += n_bin;
             for (int i = 0; i < NI; i++)
                 for (int j = 0; j < inst[i].n; j++) {
                     const int g = col_off[i] + j;
-                    const int id = pos[cls[g] * NI + i]++;
+                    const int id = n_bin && binary[g] ? bin_pos++ : pos[cls[g] * NI + i]++;
                     internal[g] = id;
                     perm[id] = g;
                 }
@@ -999,7 +1016,10 @@ Problem *build(const std::vector<InstanceView> &inst, bool replicas, const Tunin
         M.word_unit = d_word_unit;
 
         // snapshot block of a unit: int16 records, then int32, then FP64,
-        // each class from a 16-byte boundary
+        // each class from a 16-byte boundary; the binary columns' codes in
+        // 16-byte groups of 64 columns
+        M.n_bin = n_bin;
+        M.sb_words = (n_bin + 63) / 64 * 4;
         M.snap_off32 = (M.n16 * 4 + 15) / 16 * 16;
         M.snap_offF = M.snap_off32 + ((M.n_int - M.n16) * 8 + 15) / 16 * 16;
         M.snap_stride = M.snap_offF + 16LL * (M.n - M.n_int) + 16;
@@ -1017,13 +1037,30 @@ Problem *build(const std::vector<InstanceView> &inst, bool replicas, const Tunin
         }
 
         P->mega_threads = T.mega_threads == 256 || T.mega_threads == 1024 ? T.mega_threads : 512;
-        P->mega_fn = P->mega_threads == 256    ? reinterpret_cast<const void *>(mega_kernel<256>)
-                     : P->mega_threads == 1024 ? reinterpret_cast<const void *>(mega_kernel<1024>)
-                                               : reinterpret_cast<const void *>(mega_kernel<512>);
-        int per_cu = 0;
-        check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, P->mega_fn, P->mega_threads, 0), "occupancy");
-        if (T.mega_blocks_per_cu > 0) per_cu = std::min(per_cu, T.mega_blocks_per_cu);
-        P->mega_grid = std::max(1, per_cu) * P->cu_count;
+        P->mega_fn = P->mega_threads == 256    ? reinterpret_cast<const void *>(mega_kernel<256, false>)
+                     : P->mega_threads == 1024 ? reinterpret_cast<const void *>(mega_kernel<1024, false>)
+                                               : reinterpret_cast<const void *>(mega_kernel<512, false>);
+        P->mega_fn_lds = P->mega_threads == 256    ? reinterpret_cast<const void *>(mega_kernel<256, true>)
+                         : P->mega_threads == 1024 ? reinterpret_cast<const void *>(mega_kernel<1024, true>)
+                                                   : reinterpret_cast<const void *>(mega_kernel<512, true>);
+        auto mega_per_cu = [&](const void *fn, size_t lds_bytes) {
+            int per_cu = 0;
+            check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, P->mega_threads, lds_bytes),
+                  "occupancy");
+            if (T.mega_blocks_per_cu > 0) per_cu = std::min(per_cu, T.mega_blocks_per_cu);
+            return std::max(1, per_cu);
+        };
+        // the grid with the unit filter's LDS (launches of several units);
+        // launches of one unit hold as many binary columns' codes in LDS as
+        // keep that many workgroups resident
+        const int per_cu = mega_per_cu(P->mega_fn, kFilterLdsBytes);
+        P->mega_grid = per_cu * P->cu_count;
+        int n_lds = std::min(n_bin, kMaxLdsBinBytes * 4) / 64 * 64;
+        while (n_lds > 0 && mega_per_cu(P->mega_fn_lds, static_cast<size_t>(n_lds) / 4) < per_cu)
+            n_lds = n_lds / 2 / 64 * 64;
+        P->n_lds = n_lds;
+        P->stats.n_bin = n_bin;
+        P->stats.n_lds = n_lds;
         int sweep_per_cu = 0;
         check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&sweep_per_cu, reinterpret_cast<const void *>(sweep_kernel),
                                                            256, 0),
@@ -1185,7 +1222,17 @@ double run_steps(Problem *P, int sync_type, int max_rounds) {
         }
     } else {
         int first = 1, steps = max_rounds, n_units = P->n_units;
-        void *args[] = {const_cast<DevMatrix *>(&M), &D, &first, &steps, &n_units};
+        // dynamic LDS: the unit filter's arrays for several units, else the
+        // binary columns' codes
+        DevState Dm = D;
+        size_t lds_bytes = kFilterLdsBytes;
+        const void *fn = P->mega_fn;
+        if ((P->replicas ? D.n_active : n_units) == 1) {
+            lds_bytes = static_cast<size_t>(P->n_lds) / 4;
+            Dm.n_lds = P->n_lds;
+            if (P->n_lds > 0) fn = P->mega_fn_lds;
+        }
+        void *args[] = {const_cast<DevMatrix *>(&M), &Dm, &first, &steps, &n_units};
         // as many workgroups as the largest phase has waves of work (sweep
         // items, bitmap chunks, split-row segments), up to the resident
         // maximum: a small problem then pays its grid barriers over a few
@@ -1197,7 +1244,7 @@ double run_steps(Problem *P, int sync_type, int max_rounds) {
         const int waves_per_group = P->mega_threads / 64;
         const int grid = static_cast<int>(
             std::max(1LL, std::min<long long>(P->mega_grid, (need + waves_per_group - 1) / waves_per_group)));
-        check(hipLaunchCooperativeKernel(P->mega_fn, dim3(grid), dim3(P->mega_threads), args, 0, s),
+        check(hipLaunchCooperativeKernel(fn, dim3(grid), dim3(P->mega_threads), args, lds_bytes, s),
               "mega_kernel launch");
         int g[kGlobalWords];
         check(hipMemcpyAsync(g, D.global, sizeof(g), hipMemcpyDeviceToHost, s), "D2H");

@@ -491,6 +491,8 @@ GDP_RETCODE gdpProblemGetStats(gdp_problem p, gdp_problem_stats *out) {
     out->mega_threads = s.mega_threads;
     out->mega_grid = s.mega_grid;
     out->run_state_bytes = static_cast<long long>(gdp::problem_run_state_bytes(p->p));
+    out->n_binary = s.n_bin;
+    out->n_lds_binary = s.n_lds;
     return GDP_OKAY;
 }
 

@@ -108,6 +108,18 @@ __device__ __forceinline__ unsigned record_offset(const DevMatrix &M, int id) {
     return id < M.n_int ? a : af;
 }
 
+// Dynamic LDS of the persistent kernel: the unit filter's arrays in a launch
+// of several units, else the 2-bit codes of the first D.n_lds binary columns
+// (other kernels run with D.n_lds = 0 and never touch it).
+extern __shared__ __align__(16) unsigned dyn_lds[];
+
+// Bounds of binary column `id` < D.n_lds from its 2-bit code in LDS.
+__device__ __forceinline__ void lds_binary_bounds(int id, double &l, double &u) {
+    const unsigned code = dyn_lds[id >> 4] >> ((id & 15) * 2);
+    l = (code & 1u) ? 1.0 : 0.0;
+    u = (code & 2u) ? 0.0 : 1.0;
+}
+
 __device__ __forceinline__ uint4 load_chunk(const DevState &D, unsigned off) {
     return *reinterpret_cast<const uint4 *>(reinterpret_cast<const char *>(D.S16) + (off & ~15u));
 }
@@ -135,8 +147,12 @@ __device__ __forceinline__ void decode_bounds(const DevMatrix &M, int id, unsign
 
 // Bounds of internal column `id` at the start of the round.
 __device__ __forceinline__ void load_bounds(const DevMatrix &M, const DevState &D, int id, double &l, double &u) {
-    const unsigned off = record_offset(M, id);
-    decode_bounds(M, id, off, load_chunk(D, off), l, u);
+    if (id < D.n_lds) {
+        lds_binary_bounds(id, l, u);
+    } else {
+        const unsigned off = record_offset(M, id);
+        decode_bounds(M, id, off, load_chunk(D, off), l, u);
+    }
 }
 
 __device__ __forceinline__ bool fits_int32(double v) { return v >= -2147483647.0 && v <= 2147483647.0; }
@@ -349,8 +365,9 @@ struct LaneEntries {
                 }
             }
         }
-        // every gather first (entries past the end read column 0's record and
-        // are discarded), then the decoding and the contributions
+        // every gather first (entries past the end read column 0's bounds and
+        // are discarded), then the decoding and the contributions; binary
+        // columns held in LDS are read there instead
         unsigned off[K];
         uint4 chunk[K];
 #pragma unroll
@@ -360,10 +377,15 @@ struct LaneEntries {
                 a[k] = 0.0;
             }
             off[k] = record_offset(M, jp[k] & kIdMask);
-            chunk[k] = load_chunk(D, off[k]);
+            chunk[k] = make_uint4(0u, 0u, 0u, 0u);
+            if ((jp[k] & kIdMask) >= D.n_lds) chunk[k] = load_chunk(D, off[k]);
         }
 #pragma unroll
-        for (int k = 0; k < K; k++) decode_bounds(M, jp[k] & kIdMask, off[k], chunk[k], l[k], u[k]);
+        for (int k = 0; k < K; k++) {
+            const int id = jp[k] & kIdMask;
+            if (id < D.n_lds) lds_binary_bounds(id, l[k], u[k]);
+            else decode_bounds(M, id, off[k], chunk[k], l[k], u[k]);
+        }
 #pragma unroll
         for (int k = 0; k < K; k++) {
             smin[k] = 0.0;
@@ -860,11 +882,22 @@ __device__ __forceinline__ void publish_word(const DevMatrix &M, const DevState 
     const bool all_rows = all_rows_known >= 0 ? all_rows_known != 0 : decide_all_rows(M, D, u, valid, step);
     const bool moved = valid && ((bits >> b) & 1u);
     const int id = w * 32 + b;
+    unsigned code = 0u;
     if (moved) {
         double l, uu;
         accumulated(M, V, id, l, uu);
         store_snapshot(M, V, id, l, uu);
         if (eps_gt(l, uu)) atomic_max_i(&D.units[u].infeasible, step);
+        if (id < D.n_lds) code = (l >= 1.0 ? 1u : 0u) | (uu < 1.0 ? 2u : 0u);
+    }
+    if (D.n_lds > 0) {
+        // the launch reads these columns from LDS (one unit, reloaded at the
+        // start of every propagation): the new code bits of each 16
+        // columns, ORed into their table word
+        unsigned cw = code << ((id & 15) * 2);
+#pragma unroll
+        for (int h = 1; h < 16; h <<= 1) cw |= static_cast<unsigned>(__shfl_xor(static_cast<int>(cw), h, 64));
+        if ((b & 15) == 0 && cw != 0u) atomic_or_u(&V.SB[id >> 4], cw);
     }
     if (!M.mark_rows || !valid || all_rows) return;
     // rows of the moved columns, flattened over the 32 lanes of the half:
@@ -992,7 +1025,11 @@ __device__ __forceinline__ bool step_skipped(const DevState &D, int step, int fi
 // wave tests 64 of its work items at once (one lane each: the unit's state
 // and the item's mark) and visits those with work, as the persistent kernel
 // does.  Many units' items then cost a few loads each instead of a wave.
-__global__ __launch_bounds__(256) void sweep_kernel(DevMatrix M, DevState D, int step, int guard, int first_step) {
+__global__ __launch_bounds__(256) void sweep_kernel(DevMatrix M, DevState D_arg, int step, int guard,
+                                                    int first_step) {
+    // no LDS in the step kernels (a constant, so that the LDS reads compile out)
+    DevState D = D_arg;
+    D.n_lds = 0;
     if (guard && step_skipped(D, step, first_step)) return;
     const int waves = blockDim.x >> 6;
     const int wave = uniform(blockIdx.x * waves + (threadIdx.x >> 6));
@@ -1021,15 +1058,19 @@ __global__ __launch_bounds__(256) void sweep_kernel(DevMatrix M, DevState D, int
     }
 }
 
-__global__ __launch_bounds__(256) void apply_split_kernel(DevMatrix M, DevState D, int step, int guard,
+__global__ __launch_bounds__(256) void apply_split_kernel(DevMatrix M, DevState D_arg, int step, int guard,
                                                           int first_step) {
+    DevState D = D_arg;
+    D.n_lds = 0;
     if (guard && step_skipped(D, step, first_step)) return;
     const int waves = blockDim.x >> 6;
     apply_phase(M, D, step, uniform(blockIdx.x * waves + (threadIdx.x >> 6)), gridDim.x * waves);
 }
 
-__global__ __launch_bounds__(256) void publish_kernel(DevMatrix M, DevState D, int step, int guard, int first_step,
-                                                      int n_units) {
+__global__ __launch_bounds__(256) void publish_kernel(DevMatrix M, DevState D_arg, int step, int guard,
+                                                      int first_step, int n_units) {
+    DevState D = D_arg;
+    D.n_lds = 0;
     if (guard && step_skipped(D, step, first_step)) return;
     const int waves = blockDim.x >> 6;
     publish_phase(M, D, step, uniform(blockIdx.x * waves + (threadIdx.x >> 6)), gridDim.x * waves, n_units);
@@ -1103,9 +1144,13 @@ __device__ __forceinline__ bool grid_barrier(const DevState &D, unsigned nblocks
 // unit moved a bound.  Phases of a step: sweep; apply of the split rows when
 // the sweep queued any; publish.  The number of steps run goes to the global
 // status block (-1 after a failed wait).
-template <int kThreads>
-__global__ __launch_bounds__(kThreads) void mega_kernel(DevMatrix M, DevState D, int first_step, int max_steps,
+template <int kThreads, bool kLds>
+__global__ __launch_bounds__(kThreads) void mega_kernel(DevMatrix M, DevState D_arg, int first_step, int max_steps,
                                                         int n_units) {
+    // kLds: a launch of one unit that reads binary columns from LDS; without
+    // it n_lds is a constant 0, and the LDS reads and code updates compile out
+    DevState D = D_arg;
+    if constexpr (!kLds) D.n_lds = 0;
     const unsigned nblocks = gridDim.x;
     const int waves = blockDim.x >> 6;
     const int wave = uniform(blockIdx.x * waves + (threadIdx.x >> 6));
@@ -1118,9 +1163,10 @@ __global__ __launch_bounds__(kThreads) void mega_kernel(DevMatrix M, DevState D,
     // reads its state once per step.  Then each wave tests 64 of its items at
     // once (lanes): items of idle units, and items none of whose rows are
     // marked when their unit processes marked rows only, are not visited.
-    __shared__ unsigned works[kMaxFilterUnits / 32];
-    __shared__ unsigned all_rows_of[kMaxFilterUnits / 32];
-    __shared__ int start_of[kMaxFilterUnits];
+    // (in the dynamic LDS, kFilterLdsBytes of it in launches of several units)
+    unsigned *works = dyn_lds;
+    unsigned *all_rows_of = dyn_lds + kMaxFilterUnits / 32;
+    int *start_of = reinterpret_cast<int *>(dyn_lds + 2 * (kMaxFilterUnits / 32));
     const int n_slots = M.replicas ? D.n_active : n_units;  // replicas: positions in the launch's list
     const bool filter = n_slots > 1 && n_slots <= kMaxFilterUnits;
     const int lane = threadIdx.x & 63;
@@ -1129,7 +1175,17 @@ __global__ __launch_bounds__(kThreads) void mega_kernel(DevMatrix M, DevState D,
         int *st = status_row(D, step);
         UnitStep one{};
         if (n_slots == 1) {
-            one = unit_step(M, D, M.replicas ? D.active[0] : 0, step);
+            const int u1 = M.replicas ? uniform(D.active[0]) : 0;
+            one = unit_step(M, D, u1, step);
+            if (D.n_lds > 0) {
+                // the unit's binary codes as published by the previous step
+                // (the waves of this workgroup that read the LDS in the
+                // previous step are past its barriers)
+                const uint4 *src = reinterpret_cast<const uint4 *>(unit_view(M, D, u1).SB);
+                uint4 *dst = reinterpret_cast<uint4 *>(dyn_lds);
+                for (int i = threadIdx.x; i < D.n_lds / 64; i += blockDim.x) dst[i] = src[i];
+                __syncthreads();
+            }
         } else if (filter) {
             // the previous step's waves of this workgroup are past its
             // barriers, so no wave still reads the entries overwritten here
@@ -1205,9 +1261,12 @@ __global__ __launch_bounds__(kThreads) void mega_kernel(DevMatrix M, DevState D,
     if (blockIdx.x == 0 && threadIdx.x == 0) D.global[kGlobalSteps] = steps;
 }
 
-template __global__ void mega_kernel<256>(DevMatrix, DevState, int, int, int);
-template __global__ void mega_kernel<512>(DevMatrix, DevState, int, int, int);
-template __global__ void mega_kernel<1024>(DevMatrix, DevState, int, int, int);
+template __global__ void mega_kernel<256, false>(DevMatrix, DevState, int, int, int);
+template __global__ void mega_kernel<512, false>(DevMatrix, DevState, int, int, int);
+template __global__ void mega_kernel<1024, false>(DevMatrix, DevState, int, int, int);
+template __global__ void mega_kernel<256, true>(DevMatrix, DevState, int, int, int);
+template __global__ void mega_kernel<512, true>(DevMatrix, DevState, int, int, int);
+template __global__ void mega_kernel<1024, true>(DevMatrix, DevState, int, int, int);
 
 // ---------------------------------------------------------------------------
 // loading and reading bound vectors
@@ -1264,9 +1323,12 @@ __global__ __launch_bounds__(256) void load_kernel(DevMatrix M, DevState D, int 
             }
             if (id < M.n_int) {
                 const double lim = id < M.n16 ? 32767.0 : 2147483647.0;
-                const bool fits =
-                    l >= -lim && l <= lim && uu >= -lim && uu <= lim && floor(l) == l && floor(uu) == uu;
-                if (!fits) atomic_or_i(&D.units[cu].envelope, 1);
+                const double lo_lim = id < M.n_bin ? 0.0 : -lim, hi_lim = id < M.n_bin ? 1.0 : lim;
+                const bool fits = l >= lo_lim && l <= hi_lim && uu >= lo_lim && uu <= hi_lim && floor(l) == l &&
+                                  floor(uu) == uu;
+                // crossed input bounds make the vector infeasible, whatever
+                // the column's storage class
+                if (!fits && !eps_gt(l, uu)) atomic_or_i(&D.units[cu].envelope, 1);
                 const int il = fits ? static_cast<int>(l) : 0, iu = fits ? static_cast<int>(uu) : 0;
                 V.AI[id] = make_int2(il, iu);
                 V.AX[id] = make_double2(-INFINITY, INFINITY);
@@ -1276,6 +1338,19 @@ __global__ __launch_bounds__(256) void load_kernel(DevMatrix M, DevState D, int 
                 V.AF[id - M.n_int] = make_double2(l, uu);
                 V.SF[id - M.n_int] = make_double2(l, uu);
             }
+        }
+        // 2-bit codes of the binary columns, padding words zero
+        for (int w = tid; w < M.sb_words; w += stride) {
+            unsigned cw = 0u;
+            for (int t = 0; t < 16; t++) {
+                const int id = w * 16 + t;
+                if (id < M.n_bin) {
+                    const int j = M.perm[id];
+                    const double l = lbs[j], uu = ubs[j];
+                    cw |= ((l >= 1.0 ? 1u : 0u) | (uu < 1.0 ? 2u : 0u)) << (2 * t);
+                }
+            }
+            V.SB[w] = cw;
         }
         {
             for (int w = tid; w < M.nwords; w += stride) V.bm[w] = 0u;

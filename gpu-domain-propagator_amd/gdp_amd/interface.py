@@ -143,6 +143,8 @@ class ProblemStats(ctypes.Structure):
         ("mega_threads", c_int),
         ("mega_grid", c_int),
         ("run_state_bytes", c_longlong),
+        ("n_binary", c_int),
+        ("n_lds_binary", c_int),
     ]
 
 

@@ -65,6 +65,10 @@ typedef struct {
                                    allocated by the problem on its first propagation,
                                    taken from the fixed device array by
                                    gdpPropagateDeviceNoAlloc */
+    int n_binary;               /* integer columns whose envelope lies in [0, 1]
+                                   (single problems; 0 for batches) */
+    int n_lds_binary;           /* of them, the columns whose bounds a persistent
+                                   launch of one bound vector reads from LDS */
 } gdp_problem_stats;
 
 /* Fills *t with the defaults. */

@@ -7,9 +7,11 @@ host builds the sweep items from the (bin, instance) runs that it reads back.
 The summary below is recomputed here from the instances' row lengths alone:
 rows and nonzeros per bin, nonempty and long rows, the number of sweep items
 (which depends on every run's length and so on the sort putting each row in
-its run), and the columns per bound storage class (with the 32-id padding of
-each (class, instance) range in a batch).  The propagation results on the
-same layouts are checked against the oracle by the parity tests.
+its run), the columns per bound storage class (with the 32-id padding of
+each (class, instance) range in a batch), and the binary columns of a single
+problem with the part of them a persistent launch reads from LDS.  The
+propagation results on the same layouts are checked against the oracle by the
+parity tests.
 """
 import numpy as np
 import pytest
@@ -48,6 +50,16 @@ def _class(inst):
     return np.where(small, 0, np.where(wide, 1, 2))
 
 
+# binary columns a persistent launch of 512-thread workgroups holds in LDS:
+# 64 KB of 2-bit codes, in whole 16-byte words
+LDS_BINARY_MAX = 1 << 18
+
+
+def _n_binary(inst):
+    """Integer columns whose bounds (the envelope) lie in [0, 1]."""
+    return int(np.sum((_class(inst) == 0) & (inst.lb >= 0) & (inst.ub <= 1)))
+
+
 def expected_summary(instances, batch):
     max_nnz = max(x.nnz for x in instances)
     item_nnz = 2048 if max_nnz >= 4_000_000 else 1024
@@ -69,8 +81,10 @@ def expected_summary(instances, batch):
     for x in instances:
         cnt = np.bincount(_class(x), minlength=3)
         per_class += -(-cnt // 32) * 32 if batch else cnt
+    n_bin = 0 if batch else _n_binary(instances[0])
     return dict(bin_rows=bin_rows, bin_nnz=bin_nnz, m_nonempty=int(bin_rows.sum()), n_long_rows=int(bin_rows[10]),
-                n_items=n_items, classes=per_class)
+                n_items=n_items, classes=per_class, n_binary=n_bin,
+                n_lds_binary=min(n_bin, LDS_BINARY_MAX) // 64 * 64)
 
 
 def check_summary(stats, instances, batch):
@@ -81,6 +95,8 @@ def check_summary(stats, instances, batch):
     assert stats.n_long_rows == want["n_long_rows"]
     assert stats.n_items == want["n_items"]
     assert [stats.n16, stats.n32, stats.nf] == want["classes"].tolist()
+    assert stats.n_binary == want["n_binary"]
+    assert stats.n_lds_binary == want["n_lds_binary"]
     assert stats.n_cons == sum(x.m for x in instances)
     assert stats.nnz == sum(x.nnz for x in instances)
 
@@ -156,3 +172,25 @@ def test_seventeen_million_singleton_rows(oracle):
     np.testing.assert_array_equal(got.lbs, exp[0])
     np.testing.assert_array_equal(got.ubs, exp[1])
     assert got.rounds == exp[2] and got.infeasible == exp[3]
+
+
+def test_binaries_split_between_lds_and_records(oracle):
+    """More binary columns than LDS holds: the first 2^18 are read from LDS,
+    the rest from their int16 records, in the same rows.  Bit-identical to the
+    oracle, and to the step-kernel loop, which reads every record."""
+    inst = powerlaw_mip(seed=4, m=200_000, n=700_000, s=2.2, len_hi=3000)
+    p = I.Problem(inst)
+    try:
+        st = p.stats()
+        check_summary(st, [inst], batch=False)
+        assert st.n_lds_binary == LDS_BINARY_MAX < st.n_binary
+        mega = p.propagate(inst.lb, inst.ub, I.MEGAKERNEL)
+        loop = p.propagate(inst.lb, inst.ub, I.CPU_LOOP)
+    finally:
+        p.close()
+    exp = oracle.jacobi(inst)
+    assert exp[2] > 2
+    for got in (mega, loop):
+        np.testing.assert_array_equal(got.lbs, exp[0])
+        np.testing.assert_array_equal(got.ubs, exp[1])
+        assert got.rounds == exp[2] and got.infeasible == exp[3]
