@@ -406,11 +406,40 @@ def _counter_per_dispatch(rows, name_part, counter):
     return per
 
 
+PROFILE_TIMED = "--steps 10 --warmup 2 --no-cpu --no-batch --no-serve --no-extra --no-tune"
+PROFILE_PASS = "--steps 2 --warmup 1 --no-cpu --no-batch --no-serve --no-extra --no-tune"
+PROFILE_RDREQ = "TCC_EA0_RDREQ_32B_sum TCC_EA0_RDREQ_64B_sum TCC_EA0_RDREQ_128B_sum TCC_EA0_RDREQ_sum"
+PROFILE_CAL_MB = 256
+
+
+def profile_commands(d):
+    """The runs whose outputs under `d` summarize() reads, each rocprofv3 run
+    in a process of its own."""
+    return [
+        "cd /tmp && export TMPDIR=/tmp && cd <repository root>  (each rocprofv3 run on its own)",
+        f"rocprofv3 --kernel-trace --stats --output-format csv -d {d}/trace -o bench -- python3 bench.py "
+        f"{PROFILE_TIMED} --meta-out {d}/bench_meta.json",
+        f"rocprofv3 --pmc FETCH_SIZE --output-format csv -d {d}/fetch -o bench -- python3 bench.py {PROFILE_PASS}",
+        f"rocprofv3 --pmc WRITE_SIZE --output-format csv -d {d}/write -o bench -- python3 bench.py {PROFILE_PASS}",
+        f"rocprofv3 --pmc FETCH_SIZE --output-format csv -d {d}/cal_fetch -o cal -- python3 bench.py "
+        f"--calibrate-copy {PROFILE_CAL_MB}",
+        f"rocprofv3 --pmc WRITE_SIZE --output-format csv -d {d}/cal_write -o cal -- python3 bench.py "
+        f"--calibrate-copy {PROFILE_CAL_MB}",
+        f"rocprofv3 --pmc {PROFILE_RDREQ} --output-format csv -d {d}/rdreq -o bench -- python3 bench.py {PROFILE_PASS}",
+        f"rocprofv3 --pmc {PROFILE_RDREQ} --output-format csv -d {d}/cal_rdreq -o cal -- python3 bench.py "
+        f"--calibrate-copy {PROFILE_CAL_MB}",
+        f"python3 bench.py --summarize {d}",
+    ]
+
+
 def summarize(d, kernel="mega_kernel"):
     stats = _csv_rows(f"{d}/trace/**/*kernel_stats.csv")
     line = next(r for r in stats if kernel in r["Name"])
-    with open(f"{d}/meta.json") as fh:
+    # workload facts written by the kernel-trace run (--meta-out)
+    with open(f"{d}/bench_meta.json") as fh:
         meta = json.load(fh)
+    meta["calibration_copy_bytes"] = PROFILE_CAL_MB << 20
+    meta["commands"] = profile_commands(d)
     # the timed launches: the kernel trace in dispatch order, less the warmup
     trace = [r for r in _csv_rows(f"{d}/trace/**/*kernel_trace.csv") if kernel in r["Kernel_Name"]]
     trace.sort(key=lambda r: int(r["Start_Timestamp"]))
@@ -811,6 +840,7 @@ def main():
     print(f"bench: {rounds} rounds, kernel {kernel_ms:.3f} ms/step, wall {wall_ms:.3f} ms/step incl. bound restore, "
           f"{achieved:.1f} GB/s on processed rows ({alg / dense:.3f} of dense sweeps)", file=sys.stderr, flush=True)
     if args.meta_out:
+        os.makedirs(os.path.dirname(args.meta_out) or ".", exist_ok=True)
         with open(args.meta_out, "w") as fh:
             json.dump({"workload": {"m": inst.m, "n": inst.n, "nnz": inst.nnz, "rounds": rounds},
                        "algorithmic_bytes_per_launch": alg, "dense_sweep_bytes_per_launch": dense,
