@@ -889,8 +889,12 @@ Problem *build(const std::vector<InstanceView> &inst, bool replicas, const Tunin
         int max_inst_nnz = 0;
         for (const auto &v : inst) max_inst_nnz = std::max(max_inst_nnz, v.nnz);
         const int item_nnz = T.item_nnz > 0 ? T.item_nnz : (max_inst_nnz >= 4000000 ? 2048 : 1024);
-        // Order: segments of split rows, whole very long rows, then the
-        // tree-width bins from wide to narrow, so that long work starts first.
+        // Order per instance: segments of split rows, whole very long rows,
+        // then the tree-width bins from wide to narrow, so that long work
+        // starts first; instance after instance, so that the items the grid
+        // works on at one time (consecutive ones, dealt out with a stride)
+        // gather from the bounds of few instances of a batch.
+        std::vector<std::vector<int4>> inst_items(NI);
         std::vector<int4> h_items;
         std::vector<int4> h_seg_rows;
         std::vector<int> h_part_row;
@@ -907,7 +911,7 @@ Problem *build(const std::vector<InstanceView> &inst, bool replicas, const Tunin
                         const int lr = static_cast<int>(h_seg_rows.size());
                         h_seg_rows.push_back(make_int4(row, n_parts, nseg, i));
                         for (int sg = 0; sg < nseg; sg += kSegsPerItem)
-                            h_items.push_back(
+                            inst_items[i].push_back(
                                 make_int4(kItemSeg | unit_bits, lr, sg, std::min(kSegsPerItem, nseg - sg)));
                         for (int sg = 0; sg < nseg; sg++) h_part_row.push_back(lr);
                         n_parts += nseg;
@@ -917,7 +921,7 @@ Problem *build(const std::vector<InstanceView> &inst, bool replicas, const Tunin
                 }
             }
             for (const auto &g : giant)
-                h_items.push_back(make_int4(kItemLong | (g.second << kItemUnitShift), g.first, 1, 0));
+                inst_items[g.second].push_back(make_int4(kItemLong | (g.second << kItemUnitShift), g.first, 1, 0));
             P->stats.n_long_rows = static_cast<int>(R.bin_rows[kLongBin]);
         }
         for (int b = kLongBin - 1; b >= 0; b--) {
@@ -933,10 +937,11 @@ Problem *build(const std::vector<InstanceView> &inst, bool replicas, const Tunin
             for (int i = 0; i < NI; i++) {
                 const int st = R.start[key * NI + i], cnt = R.count[key * NI + i];
                 for (int q = 0; q < cnt; q += rows_per_item)
-                    h_items.push_back(make_int4(kItemGroup | (logg << 4) | (logk << 8) | (i << kItemUnitShift),
-                                                st + q, std::min(rows_per_item, cnt - q), 0));
+                    inst_items[i].push_back(make_int4(kItemGroup | (logg << 4) | (logk << 8) | (i << kItemUnitShift),
+                                                      st + q, std::min(rows_per_item, cnt - q), 0));
             }
         }
+        for (const auto &v : inst_items) h_items.insert(h_items.end(), v.begin(), v.end());
         if (NI >= (1 << (31 - kItemUnitShift))) throw std::runtime_error("too many instances in a batch");
 
         // per item: the row whose mark a seg / long item tests (-1: group item)
