@@ -123,6 +123,23 @@ def test_nodes_match_oracle(gpu, oracle, case, sync):
         assert_same(hinted[k], exp, f"hinted node {k}")
 
 
+def test_single_hinted_nodes_with_binaries_in_lds(gpu, oracle):
+    """One node vector per launch: the persistent kernel reads the binary
+    columns from LDS, starting from a step that processes only the rows of
+    the hinted columns, after launches of several vectors on the same units."""
+    inst = powerlaw_mip(seed=5, m=20000, n=10000, len_hi=5000, s=1.8)
+    nodes = node_vectors(inst, oracle, 6, seed=12)
+    p = I.Problem(inst)
+    try:
+        assert p.stats().n_lds_binary > 0
+        p.propagate_nodes([x[0] for x in nodes], [x[1] for x in nodes], sync_type=I.MEGAKERNEL)
+        for k, (lb, ub, cols) in enumerate(nodes):
+            got = p.propagate_nodes([lb], [ub], hints=[cols], sync_type=I.MEGAKERNEL)[0]
+            assert_same(got, oracle.jacobi(inst, lb=lb, ub=ub), f"node {k}")
+    finally:
+        p.close()
+
+
 def test_nodes_from_root_and_capacity_growth(gpu, oracle):
     """Vectors that are not near a fixpoint (no hints), more vectors than the
     reserved capacity, then fewer."""
