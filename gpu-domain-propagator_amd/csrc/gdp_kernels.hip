@@ -913,21 +913,35 @@ __device__ __forceinline__ void publish_word(const DevMatrix &M, const DevState 
     }
     const int total = __shfl(pre, 31, 32);
     pre -= cn;  // exclusive
-    for (int base = 0; base < total; base += 32) {  // same trip count on every lane of the half
-        const int i = base + b;
-        int lo = 0;
+    // kMarkBatch entries per lane at a time: their index loads are all issued
+    // before the marks they name are stored
+    constexpr int kMarkBatch = 4;
+    for (int base = 0; base < total; base += 32 * kMarkBatch) {  // same trip count on every lane of the half
+        int row[kMarkBatch], sp[kMarkBatch], ci[kMarkBatch];
 #pragma unroll
-        for (int st = 16; st > 0; st >>= 1) {
-            const int cand = lo + st;
-            if (cand < 32 && __shfl(pre, cand, 32) <= i) lo = cand;
+        for (int q = 0; q < kMarkBatch; q++) {
+            const int i = base + q * 32 + b;
+            int lo = 0;
+#pragma unroll
+            for (int st = 16; st > 0; st >>= 1) {
+                const int cand = lo + st;
+                if (cand < 32 && __shfl(pre, cand, 32) <= i) lo = cand;
+            }
+            const int k = __shfl(c0, lo, 32) + i - __shfl(pre, lo, 32);
+            row[q] = -1;
+            sp[q] = -1;
+            ci[q] = -1;
+            if (i < total) {
+                row[q] = M.col_rows[k];
+                sp[q] = M.col_seg[k];
+                ci[q] = M.col_item[k];
+            }
         }
-        const int k = __shfl(c0, lo, 32) + i - __shfl(pre, lo, 32);
-        if (i < total) {
-            coherent_store(&V.marks[M.col_rows[k]], step + 1);
-            const int sp = M.col_seg[k];
-            if (sp >= 0) coherent_store(&V.seg_marks[sp], step + 1);
-            const int ci = M.col_item[k];
-            if (ci >= 0) coherent_store(&V.item_mark[ci], step + 1);
+#pragma unroll
+        for (int q = 0; q < kMarkBatch; q++) {
+            if (row[q] >= 0) coherent_store(&V.marks[row[q]], step + 1);
+            if (sp[q] >= 0) coherent_store(&V.seg_marks[sp[q]], step + 1);
+            if (ci[q] >= 0) coherent_store(&V.item_mark[ci[q]], step + 1);
         }
     }
 }
