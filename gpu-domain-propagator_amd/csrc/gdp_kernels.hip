@@ -671,16 +671,24 @@ __device__ __forceinline__ void seg_item(const DevMatrix &M, const DevState &D, 
     const int r = info.x;
     int start, len;
     row_extent(M, r, start, len);
+    // a segment computed earlier in this run none of whose columns moved in
+    // the previous step keeps its partial (its columns' bounds have not
+    // changed since it was computed: every change marks it); lane q reads
+    // segment q's stamp and mark
+    unsigned long long keep = 0ull;
+    if (!all_rows) {
+        bool kept = false;
+        if (lane < cnt) {
+            const int part = info.y + s + lane;
+            const int stamp = coherent_load(&V.seg_parts[part].stamp);
+            const int mark = coherent_load(&V.seg_marks[part]);
+            kept = stamp >= unit_start && mark != step;
+        }
+        keep = __ballot(kept);
+    }
     for (int q = 0; q < cnt; q++) {
         const int part = info.y + s + q;
-        // a segment computed earlier in this run none of whose columns moved
-        // in the previous step keeps its partial (its columns' bounds have
-        // not changed since it was computed: every change marks it)
-        if (!all_rows) {
-            const int stamp = uniform(coherent_load(&V.seg_parts[part].stamp));
-            const int mark = uniform(coherent_load(&V.seg_marks[part]));
-            if (stamp >= unit_start && mark != step) continue;
-        }
+        if (!all_rows && ((keep >> q) & 1ull)) continue;
         const int s0 = (s + q) * kSegLen;
         LaneEntries<K> E;
         E.load(M, V, start + s0, lane * K, len - s0);
