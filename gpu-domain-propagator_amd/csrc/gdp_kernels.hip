@@ -881,12 +881,7 @@ __device__ __forceinline__ void publish_word(const DevMatrix &M, const DevState 
     const int b = lane & 31;
     if (valid && !M.replicas) u = M.word_unit[w];
     const DevState V = unit_view(M, D, u);
-    if (valid && b == 0) {
-        coherent_store(&V.bm[w], 0u);
-        // the unit moved a bound (read first: many words of a unit set it)
-        if (!M.replicas && coherent_load(&D.units[u].changed[step & 1]) != step)
-            atomic_max_i(&D.units[u].changed[step & 1], step);
-    }
+    if (valid && b == 0) coherent_store(&V.bm[w], 0u);
     const bool all_rows = all_rows_known >= 0 ? all_rows_known != 0 : decide_all_rows(M, D, u, valid, step);
     const bool moved = valid && ((bits >> b) & 1u);
     const int id = w * 32 + b;
@@ -956,9 +951,9 @@ __device__ __forceinline__ void publish_word(const DevMatrix &M, const DevState 
 
 // Publish phase of `step` over the waves of the grid: every wave scans 8
 // bitmap words of one unit (replicas: the launch's units; blocks: the one
-// shared bitmap) and publishes the nonzero ones two at a time; a nonzero word
-// records that its unit, and the step, changed.  Also clears the next step's
-// status row and the candidate counters two steps ahead.
+// shared bitmap) and publishes the nonzero ones two at a time; a unit with
+// accepted candidates records that it, and the step, changed.  Also clears
+// the next step's status row and the candidate counters two steps ahead.
 __device__ __forceinline__ void publish_phase(const DevMatrix &M, const DevState &D, int step, int wave,
                                               int nwaves, int n_units) {
     constexpr int kChunk = 8;  // bitmap words per wave
@@ -988,14 +983,6 @@ __device__ __forceinline__ void publish_phase(const DevMatrix &M, const DevState
         const unsigned bits = lane < kChunk && w < M.nwords ? coherent_load(&V.bm[w]) : 0u;
         unsigned long long pending = __ballot(bits != 0u);
         if (!pending) continue;
-        if (lane == 0) {  // the chunk's unit (replicas) moved a bound in this step
-            // read first: most chunks find the flags set, and same-address
-            // atomics from every chunk serialise
-            if (M.replicas && coherent_load(&D.units[u].changed[step & 1]) != step)
-                atomic_max_i(&D.units[u].changed[step & 1], step);
-            if (coherent_load(&status_row(D, step)[kStatusChanged]) == 0)
-                atomic_or_i(&status_row(D, step)[kStatusChanged], 1);
-        }
         // one decision for the chunk's unit (replicas; blocks: when the
         // chunk's words all belong to one unit, as in a single problem)
         int known = -1;
@@ -1019,6 +1006,25 @@ __device__ __forceinline__ void publish_phase(const DevMatrix &M, const DevState
             const int t = half ? t1 : t0;
             const unsigned wb = __shfl(bits, t < 0 ? 0 : t, 64);
             publish_word(M, D, u, q * kChunk + (t < 0 ? 0 : t), wb, t >= 0, step, known);
+        }
+    }
+    // The units that accepted candidates in this step moved a bound: their
+    // candidate counters (final since the sweep's barrier) set the unit's
+    // and the step's changed flags, one wave per unit, so that the words
+    // above need no flag reads.
+    const int n_flag_units = M.replicas ? D.n_active : n_units;
+    for (int k = wave; k < n_flag_units; k += nwaves) {
+        const int uu = M.replicas ? uniform(D.active[k]) : k;
+        int cnt = lane < kMoveLanes
+                      ? coherent_load(&D.moved[(uu * kMoveSlots + step % kMoveSlots) * kMoveLanes + lane])
+                      : 0;
+#pragma unroll
+        for (int h = 1; h < kMoveLanes; h <<= 1) cnt += __shfl_xor(cnt, h, 64);
+        if (lane == 0 && cnt > 0) {
+            coherent_store(&D.units[uu].changed[step & 1], step);
+            // read first: many units set it
+            if (coherent_load(&status_row(D, step)[kStatusChanged]) == 0)
+                atomic_or_i(&status_row(D, step)[kStatusChanged], 1);
         }
     }
     const int tid = blockIdx.x * blockDim.x + threadIdx.x;
