@@ -857,9 +857,10 @@ __device__ __forceinline__ void apply_phase(const DevMatrix &M, const DevState &
 // reach its threshold processes all rows next step instead (no marks are
 // pushed then).  Clears the word.
 // Whether unit u processes all rows in the step after `step` (its accepted
-// candidates of `step` reached its limit); sets the unit's all-rows step.
-// Lanes 0..15 of each half-wave read the unit's counters: call with every
-// lane of the wave, `valid` per half.
+// candidates of `step` reached its limit; the unit's all-rows step is set
+// from the same counters by the publish phase's per-unit pass).  Lanes 0..15
+// of each half-wave read the unit's counters: call with every lane of the
+// wave, `valid` per half.
 __device__ __forceinline__ bool decide_all_rows(const DevMatrix &M, const DevState &D, int u, bool valid, int step) {
     const int b = threadIdx.x & 31;
     int cnt = 0;
@@ -869,10 +870,7 @@ __device__ __forceinline__ bool decide_all_rows(const DevMatrix &M, const DevSta
     for (int h = 1; h < kMoveLanes; h <<= 1) cnt += __shfl_xor(cnt, h, 64);
     cnt = __shfl(cnt, threadIdx.x & 32, 64);
     const int limit = valid ? coherent_load(&D.units[u].mark_limit) : 0;
-    const bool all_rows = valid && M.mark_rows && cnt >= limit;
-    if (all_rows && b == 0 && coherent_load(&D.units[u].markall) != step + 1)
-        atomic_max_i(&D.units[u].markall, step + 1);
-    return all_rows;
+    return valid && M.mark_rows && cnt >= limit;
 }
 
 __device__ __forceinline__ void publish_word(const DevMatrix &M, const DevState &D, int u, int w, unsigned bits,
@@ -1010,18 +1008,21 @@ __device__ __forceinline__ void publish_phase(const DevMatrix &M, const DevState
     }
     // The units that accepted candidates in this step moved a bound: their
     // candidate counters (final since the sweep's barrier) set the unit's
-    // and the step's changed flags, one wave per unit, so that the words
-    // above need no flag reads.
+    // and the step's changed flags, and the unit's all-rows step when they
+    // reach its limit, one wave per unit, so that the words above need no
+    // flag reads.
     const int n_flag_units = M.replicas ? D.n_active : n_units;
     for (int k = wave; k < n_flag_units; k += nwaves) {
         const int uu = M.replicas ? uniform(D.active[k]) : k;
         int cnt = lane < kMoveLanes
                       ? coherent_load(&D.moved[(uu * kMoveSlots + step % kMoveSlots) * kMoveLanes + lane])
                       : 0;
+        const int limit = lane == 0 ? coherent_load(&D.units[uu].mark_limit) : 0;
 #pragma unroll
         for (int h = 1; h < kMoveLanes; h <<= 1) cnt += __shfl_xor(cnt, h, 64);
         if (lane == 0 && cnt > 0) {
             coherent_store(&D.units[uu].changed[step & 1], step);
+            if (M.mark_rows && cnt >= limit) coherent_store(&D.units[uu].markall, step + 1);
             // read first: many units set it
             if (coherent_load(&status_row(D, step)[kStatusChanged]) == 0)
                 atomic_or_i(&status_row(D, step)[kStatusChanged], 1);
