@@ -200,6 +200,44 @@ __device__ __forceinline__ void accumulate(const DevMatrix &M, const Sink &K, in
     __hip_atomic_fetch_or(&D.bm[id >> 5], 1u << (id & 31), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// The accumulator words of internal column `id` as read (device-coherent):
+// both int32 sides in one access, or the FP64 pair.  acc_resolve() turns
+// them into bounds; issuing the reads of several columns first lets their
+// latencies overlap.
+struct AccRaw {
+    long long w;
+    double fl, fu;
+};
+__device__ __forceinline__ AccRaw acc_load(const DevMatrix &M, const DevState &D, int id) {
+    AccRaw a{0, 0.0, 0.0};
+    if (id < M.n_int) {
+        a.w = coherent_load(reinterpret_cast<const long long *>(&D.AI[id]));
+    } else {
+        const double2 *p = &D.AF[id - M.n_int];
+        a.fl = coherent_load_d(&p->x);
+        a.fu = coherent_load_d(&p->y);
+    }
+    return a;
+}
+__device__ __forceinline__ void acc_resolve(const DevMatrix &M, const DevState &D, int id, const AccRaw &a,
+                                            double &l, double &u) {
+    if (id < M.n_int) {
+        const int lo = static_cast<int>(static_cast<unsigned long long>(a.w) & 0xffffffffull);
+        const int hi = static_cast<int>(static_cast<unsigned long long>(a.w) >> 32);
+        l = static_cast<double>(lo);
+        u = static_cast<double>(hi);
+        if (lo == kAiHigh || hi == kAiLow) {  // AX only behind a sentinel
+            const double xl = coherent_load_d(&D.AX[id].x);
+            const double xu = coherent_load_d(&D.AX[id].y);
+            l = xl > l ? xl : l;
+            u = xu < u ? xu : u;
+        }
+    } else {
+        l = a.fl;
+        u = a.fu;
+    }
+}
+
 // Current accumulated bounds of internal column `id` (device-coherent).
 __device__ __forceinline__ void accumulated(const DevMatrix &M, const DevState &D, int id, double &l, double &u) {
     if (id < M.n_int) {
@@ -992,6 +1030,53 @@ __device__ __forceinline__ void publish_phase(const DevMatrix &M, const DevState
             const int u0 = uniform(M.word_unit[q * kChunk]);
             const bool other = lane < kChunk && w < M.nwords && M.word_unit[w] != u0;
             if (!__ballot(other)) known = decide_all_rows(M, D, u0, true, step) ? 1 : 0;
+        }
+        if (M.replicas && (known == 1 || !M.mark_rows)) {
+            // no row marks to push: the chunk's nonzero words all at once,
+            // their accumulator reads issued before any is used (lane b of
+            // each half takes bit b of the half's word of every pair)
+            constexpr int kPairs = kChunk / 2;
+            int ids[kPairs];
+            bool mv[kPairs];
+            AccRaw raw[kPairs];
+#pragma unroll
+            for (int i = 0; i < kPairs; i++) {
+                int t0 = -1, t1 = -1;
+                if (pending) {
+                    t0 = __ffsll(static_cast<long long>(pending)) - 1;
+                    pending &= pending - 1;
+                }
+                if (pending) {
+                    t1 = __ffsll(static_cast<long long>(pending)) - 1;
+                    pending &= pending - 1;
+                }
+                const int t = half ? t1 : t0;
+                const int wt = q * kChunk + (t < 0 ? 0 : t);
+                const unsigned wb = __shfl(bits, t < 0 ? 0 : t, 64);
+                if (t >= 0 && (lane & 31) == 0) coherent_store(&V.bm[wt], 0u);
+                ids[i] = wt * 32 + (lane & 31);
+                mv[i] = t >= 0 && ((wb >> (lane & 31)) & 1u);
+                raw[i] = mv[i] ? acc_load(M, V, ids[i]) : AccRaw{0, 0.0, 0.0};
+            }
+#pragma unroll
+            for (int i = 0; i < kPairs; i++) {
+                unsigned code = 0u;
+                if (mv[i]) {
+                    double l, uu;
+                    acc_resolve(M, V, ids[i], raw[i], l, uu);
+                    store_snapshot(M, V, ids[i], l, uu);
+                    if (eps_gt(l, uu)) atomic_max_i(&D.units[u].infeasible, step);
+                    if (ids[i] < D.n_lds) code = (l >= 1.0 ? 1u : 0u) | (uu < 1.0 ? 2u : 0u);
+                }
+                if (D.n_lds > 0) {
+                    unsigned cw = code << ((ids[i] & 15) * 2);
+#pragma unroll
+                    for (int h = 1; h < 16; h <<= 1)
+                        cw |= static_cast<unsigned>(__shfl_xor(static_cast<int>(cw), h, 64));
+                    if ((lane & 15) == 0 && cw != 0u) atomic_or_u(&V.SB[ids[i] >> 4], cw);
+                }
+            }
+            continue;
         }
         while (pending) {
             const int t0 = __ffsll(static_cast<long long>(pending)) - 1;
