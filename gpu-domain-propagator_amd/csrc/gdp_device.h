@@ -121,6 +121,12 @@ struct UnitState {
 // marks (the count bounds the number of moved columns from above).
 constexpr int kMoveSlots = 4;
 constexpr int kMoveLanes = 16;
+// Each counter on its own 128-byte line (ints between two counters).
+constexpr int kMoveStride = 32;
+// Index of counter j of unit u's slot `slot`.
+__host__ __device__ inline int moved_index(int u, int slot, int j) {
+    return ((u * kMoveSlots + slot) * kMoveLanes + j) * kMoveStride;
+}
 
 // Grid barrier words, each counter on its own 128-byte line.
 constexpr int kBarrierTop = 0;
@@ -195,7 +201,7 @@ struct DevState {
     unsigned *seg_count;   // segments reduced in the current step
     // shared by all units
     UnitState *units;
-    int *moved;            // [units][kMoveSlots][kMoveLanes] accepted candidates
+    int *moved;            // [units][kMoveSlots][kMoveLanes] accepted candidates, kMoveStride ints apart
     int *status;           // kStatusRows * kStatusWords ring
     int *global;           // kGlobalWords
     unsigned *barrier;     // kBarrierWords

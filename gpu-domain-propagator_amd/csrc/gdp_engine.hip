@@ -471,7 +471,7 @@ size_t carve_state(const DevMatrix &M, bool replicas, int n_inst, int cap, char 
     D.seg_count = reinterpret_cast<unsigned *>(take(k * M.n_seg_rows * sizeof(unsigned)));
     D.apply_tasks = reinterpret_cast<int *>(take(k * M.n_seg_parts * sizeof(int)));
     D.units = reinterpret_cast<UnitState *>(take(units * sizeof(UnitState)));
-    D.moved = reinterpret_cast<int *>(take(units * kMoveSlots * kMoveLanes * sizeof(int)));
+    D.moved = reinterpret_cast<int *>(take(units * kMoveSlots * kMoveLanes * kMoveStride * sizeof(int)));
     D.status = reinterpret_cast<int *>(take(kStatusRows * kStatusWords * sizeof(int)));
     D.global = reinterpret_cast<int *>(take(kGlobalWords * sizeof(int)));
     D.barrier = reinterpret_cast<unsigned *>(take(kBarrierWords * sizeof(unsigned)));
@@ -1512,12 +1512,12 @@ RunResult problem_work_profile(Problem *P, double *lbs, double *ubs, int max_rou
             }
             launch_step(P, step, 0, 1);
             check(hipGetLastError(), "step launch");
-            int cnt[kMoveLanes];
-            check(hipMemcpyAsync(cnt, P->D.moved + (step % kMoveSlots) * kMoveLanes, sizeof(cnt),
+            int cnt[kMoveLanes * kMoveStride];
+            check(hipMemcpyAsync(cnt, P->D.moved + moved_index(0, step % kMoveSlots, 0), sizeof(cnt),
                                  hipMemcpyDeviceToHost, s),
                   "D2H moved");
             check(hipStreamSynchronize(s), "step sync");
-            for (int j = 0; j < kMoveLanes; j++) w.accepted += cnt[j];
+            for (int j = 0; j < kMoveLanes; j++) w.accepted += cnt[j * kMoveStride];
             work.push_back(w);
             st = read_units(P, 1, list);
         }

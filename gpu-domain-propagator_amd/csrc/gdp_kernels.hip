@@ -568,7 +568,7 @@ __device__ __forceinline__ void note_change(const DevState &D, int u, int step, 
     for (int h = 1; h < 64; h <<= 1) moved += __shfl_xor(moved, h, 64);
     if ((threadIdx.x & 63) == 0) {
         const int lane_slot = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) % kMoveLanes;
-        atomic_add_i(&D.moved[(u * kMoveSlots + step % kMoveSlots) * kMoveLanes + lane_slot], moved);
+        atomic_add_i(&D.moved[moved_index(u, step % kMoveSlots, lane_slot)], moved);
     }
 }
 
@@ -903,7 +903,7 @@ __device__ __forceinline__ bool decide_all_rows(const DevMatrix &M, const DevSta
     const int b = threadIdx.x & 31;
     int cnt = 0;
     if (valid && M.mark_rows && b < kMoveLanes)
-        cnt = coherent_load(&D.moved[(u * kMoveSlots + step % kMoveSlots) * kMoveLanes + b]);
+        cnt = coherent_load(&D.moved[moved_index(u, step % kMoveSlots, b)]);
 #pragma unroll
     for (int h = 1; h < kMoveLanes; h <<= 1) cnt += __shfl_xor(cnt, h, 64);
     cnt = __shfl(cnt, threadIdx.x & 32, 64);
@@ -1100,7 +1100,7 @@ __device__ __forceinline__ void publish_phase(const DevMatrix &M, const DevState
     for (int k = wave; k < n_flag_units; k += nwaves) {
         const int uu = M.replicas ? uniform(D.active[k]) : k;
         int cnt = lane < kMoveLanes
-                      ? coherent_load(&D.moved[(uu * kMoveSlots + step % kMoveSlots) * kMoveLanes + lane])
+                      ? coherent_load(&D.moved[moved_index(uu, step % kMoveSlots, lane)])
                       : 0;
         const int limit = lane == 0 ? coherent_load(&D.units[uu].mark_limit) : 0;
 #pragma unroll
@@ -1118,7 +1118,7 @@ __device__ __forceinline__ void publish_phase(const DevMatrix &M, const DevState
     if (tid < kStatusWords) coherent_store(&status_row(D, step + 1)[tid], 0);
     for (int k = tid; k < n_units * kMoveLanes; k += stride) {
         const int uu = k / kMoveLanes, j = k - uu * kMoveLanes;
-        coherent_store(&D.moved[(uu * kMoveSlots + (step + 2) % kMoveSlots) * kMoveLanes + j], 0);
+        coherent_store(&D.moved[moved_index(uu, (step + 2) % kMoveSlots, j)], 0);
     }
 }
 
@@ -1402,7 +1402,8 @@ __global__ void unit_reset_kernel(DevState D, int n_list, const int4 *params, in
         s.max_rounds = q.w;
         s.mark_limit = q.y;
         D.units[q.x] = s;
-        for (int j = 0; j < kMoveSlots * kMoveLanes; j++) D.moved[q.x * kMoveSlots * kMoveLanes + j] = 0;
+        for (int sl = 0; sl < kMoveSlots; sl++)
+            for (int j = 0; j < kMoveLanes; j++) D.moved[moved_index(q.x, sl, j)] = 0;
     }
     if (clear_status) {
         for (int k = tid; k < kStatusRows * kStatusWords; k += stride) D.status[k] = 0;
