@@ -885,10 +885,7 @@ Problem *build(const std::vector<InstanceView> &inst, bool replicas, const Tunin
 
         trace.mark("rows");
         // ---- sweep items ----
-        // larger group items for large instances (fewer, longer waves)
-        int max_inst_nnz = 0;
-        for (const auto &v : inst) max_inst_nnz = std::max(max_inst_nnz, v.nnz);
-        const int item_nnz = T.item_nnz > 0 ? T.item_nnz : (max_inst_nnz >= 4000000 ? 2048 : 1024);
+        const int item_nnz = T.item_nnz > 0 ? T.item_nnz : 1024;
         // Order per instance: segments of split rows, whole very long rows,
         // then the tree-width bins from wide to narrow, so that long work
         // starts first; instance after instance, so that the items the grid

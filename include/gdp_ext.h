@@ -32,8 +32,7 @@ typedef struct {
 } gdp_run_info;
 
 typedef struct {
-    int item_nnz;            /* target nonzeros per work item (0: default, 2048 for
-                                instances of at least 4M nonzeros, else 1024) */
+    int item_nnz;            /* target nonzeros per work item (0: default, 1024) */
     int mega_blocks_per_cu;  /* persistent grid: workgroups per CU (0: occupancy) */
     int mega_threads;        /* persistent kernel workgroup size: 256, 512 (default) or 1024 */
     int graph_rounds;        /* rounds per captured graph in GPU_LOOP (0: 8) */
