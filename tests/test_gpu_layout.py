@@ -61,8 +61,7 @@ def _n_binary(inst):
 
 
 def expected_summary(instances, batch):
-    max_nnz = max(x.nnz for x in instances)
-    item_nnz = 2048 if max_nnz >= 4_000_000 else 1024
+    item_nnz = 1024
     bin_rows = np.zeros(11, np.int64)
     bin_nnz = np.zeros(11, np.int64)
     n_items = 0
