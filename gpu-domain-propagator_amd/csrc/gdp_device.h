@@ -140,7 +140,7 @@ constexpr long long kWaitTimeoutTicks = 100000000LL;
 // Partial activity of one segment of a split row.
 struct SegPart {
     double smin, smax, dmax;
-    int mininf, maxinf;
+    int mininf, maxinf;  // written and read as one 8-byte word
     int stamp;   // step in which the partial was computed (0: not in this run)
     int pad;
 };
@@ -149,7 +149,7 @@ struct SegPart {
 // when the row gate passes (read by the apply phase of that step).
 struct SegState {
     double minact, maxact;
-    int mininf, maxinf;
+    int mininf, maxinf;  // written and read as one 8-byte word
 };
 
 struct DevMatrix {

@@ -88,6 +88,19 @@ __device__ __forceinline__ double coherent_load_d(const double *p) {
 __device__ __forceinline__ void coherent_store_d(double *p, double v) {
     coherent_store(reinterpret_cast<long long *>(p), __double_as_longlong(v));
 }
+// Two adjacent ints (8-byte aligned, first in the low word) in one
+// device-coherent access.
+__device__ __forceinline__ void coherent_store_2i(int *p, int a, int b) {
+    coherent_store(reinterpret_cast<long long *>(p),
+                   static_cast<long long>(static_cast<unsigned long long>(static_cast<unsigned>(a)) |
+                                          (static_cast<unsigned long long>(static_cast<unsigned>(b)) << 32)));
+}
+__device__ __forceinline__ void coherent_load_2i(const int *p, int &a, int &b) {
+    const unsigned long long w =
+        static_cast<unsigned long long>(coherent_load(reinterpret_cast<const long long *>(p)));
+    a = static_cast<int>(w & 0xffffffffull);
+    b = static_cast<int>(w >> 32);
+}
 // waits until every memory operation this wave issued has completed
 __device__ __forceinline__ void mem_wait() { __builtin_amdgcn_s_waitcnt(0); }
 
@@ -690,8 +703,7 @@ __device__ __forceinline__ void store_part(const DevState &V, int part, const La
     coherent_store_d(&p->smin, E.tmin);
     coherent_store_d(&p->smax, E.tmax);
     coherent_store_d(&p->dmax, E.dmax);
-    coherent_store(&p->mininf, E.mininf);
-    coherent_store(&p->maxinf, E.maxinf);
+    coherent_store_2i(&p->mininf, E.mininf, E.maxinf);
     coherent_store(&p->stamp, step);
 }
 
@@ -754,8 +766,10 @@ __device__ __forceinline__ void seg_item(const DevMatrix &M, const DevState &D, 
             pmin = coherent_load_d(&pq->smin);
             pmax = coherent_load_d(&pq->smax);
             dm = fmax(dm, coherent_load_d(&pq->dmax));
-            ni += coherent_load(&pq->mininf);
-            xi += coherent_load(&pq->maxinf);
+            int pn, px;
+            coherent_load_2i(&pq->mininf, pn, px);
+            ni += pn;
+            xi += px;
         }
         const int n_here = min(64, info.z - b);
         for (int t = 0; t < n_here; t++) {
@@ -780,8 +794,7 @@ __device__ __forceinline__ void seg_item(const DevMatrix &M, const DevState &D, 
             SegState *stp = &V.seg_state[lr];
             coherent_store_d(&stp->minact, am);
             coherent_store_d(&stp->maxact, ax);
-            coherent_store(&stp->mininf, ni);
-            coherent_store(&stp->maxinf, xi);
+            coherent_store_2i(&stp->mininf, ni, xi);
             pos = atomic_add_i(&st[kStatusApply], info.z);
         }
     }
@@ -859,8 +872,8 @@ __device__ __forceinline__ void apply_split_task(const DevMatrix &M, const DevSt
     const SegState *stp = &V.seg_state[lr];
     const double minact = coherent_load_d(&stp->minact);
     const double maxact = coherent_load_d(&stp->maxact);
-    const int ni = coherent_load(&stp->mininf);
-    const int xi = coherent_load(&stp->maxinf);
+    int ni, xi;
+    coherent_load_2i(&stp->mininf, ni, xi);
     const double2 sd = M.sides[r];
     int moved = 0;
 #pragma unroll
