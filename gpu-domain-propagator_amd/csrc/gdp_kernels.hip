@@ -1152,8 +1152,8 @@ __device__ __forceinline__ bool step_skipped(const DevState &D, int step, int fi
 // wave tests 64 of its work items at once (one lane each: the unit's state
 // and the item's mark) and visits those with work, as the persistent kernel
 // does.  Many units' items then cost a few loads each instead of a wave.
-__global__ __launch_bounds__(256) void sweep_kernel(DevMatrix M, DevState D_arg, int step, int guard,
-                                                    int first_step) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void sweep_kernel(
+    DevMatrix M, DevState D_arg, int step, int guard, int first_step) {
     // no LDS in the step kernels (a constant, so that the LDS reads compile out)
     DevState D = D_arg;
     D.n_lds = 0;
