@@ -140,10 +140,12 @@ constexpr long long kWaitTimeoutTicks = 100000000LL;
 // Partial activity of one segment of a split row.
 struct SegPart {
     double smin, smax, dmax;
-    int mininf, maxinf;  // written and read as one 8-byte word
-    int stamp;   // step in which the partial was computed (0: not in this run)
-    int pad;
+    int infs;    // infinite contributions: min side in the low 16 bits, max side in the high
+                 // (at most kSegLen each)
+    int stamp;   // step in which the partial was computed (0: not in this run); written
+                 // with infs as one 8-byte word
 };
+static_assert(kSegLen < (1 << 15), "a segment's infinite contributions fit in 16 bits");
 
 // Activity of a split row, written by the last segment wave of a step
 // when the row gate passes (read by the apply phase of that step).

@@ -703,8 +703,7 @@ __device__ __forceinline__ void store_part(const DevState &V, int part, const La
     coherent_store_d(&p->smin, E.tmin);
     coherent_store_d(&p->smax, E.tmax);
     coherent_store_d(&p->dmax, E.dmax);
-    coherent_store_2i(&p->mininf, E.mininf, E.maxinf);
-    coherent_store(&p->stamp, step);
+    coherent_store_2i(&p->infs, E.mininf | (E.maxinf << 16), step);
 }
 
 // Segments [s, s + cnt) of split row lr (cnt <= kSegsPerItem) for unit u.
@@ -766,10 +765,11 @@ __device__ __forceinline__ void seg_item(const DevMatrix &M, const DevState &D, 
             pmin = coherent_load_d(&pq->smin);
             pmax = coherent_load_d(&pq->smax);
             dm = fmax(dm, coherent_load_d(&pq->dmax));
-            int pn, px;
-            coherent_load_2i(&pq->mininf, pn, px);
-            ni += pn;
-            xi += px;
+            int infs, stamp;
+            coherent_load_2i(&pq->infs, infs, stamp);
+            (void)stamp;
+            ni += infs & 0xffff;
+            xi += infs >> 16;
         }
         const int n_here = min(64, info.z - b);
         for (int t = 0; t < n_here; t++) {
