@@ -8,7 +8,7 @@
 // as hipFree does, so a block is never reused while a kernel may still touch
 // it.  The cache holds at most 4 GiB per device; an allocation that fails
 // empties it and tries again.  Blocks still cached when the process ends are
-// left to the driver.
+// reclaimed with the process's device memory.
 #pragma once
 
 #include <hip/hip_runtime.h>
